@@ -1,0 +1,213 @@
+/*
+ * vlm_c.h -- C ABI of the MI355X free-wake hot path (libvlm.so).
+ *
+ * Drop-in boundary for the unsteady vortex-lattice hot path of
+ * cibinjoseph/VOLCANOR.  The reference has no FFI: the path sits behind
+ * Fortran type-bound procedures.  Each entry point below names the
+ * reference routine it replaces (file:line in cibinjoseph/VOLCANOR);
+ * the Fortran host binds them with ISO_C_BINDING interfaces
+ * (volcanor_amd/fortran/vlm_c_binding.f90).
+ *
+ * Conventions
+ *   - Every function returns VLM_OK (0) or a VLM_ERR_* code; the message of
+ *     the last failure is available from vlm_last_error().  The library
+ *     never aborts the process.
+ *   - Host buffers are plain double arrays in Fortran (column-major) order:
+ *     a point list is 3 x m (x,y,z of point 0 first).
+ *   - All device work is ordered on the context's HIP stream; calls that
+ *     return host data synchronise that stream.  A context is used by one
+ *     host thread at a time.
+ *   - Indices (rotor ir, blade b, rows, spans) are 0-based; cursor values
+ *     rowNear/rowFar are reported 1-based, as in the reference.
+ *
+ * Lattice layout of a near wake (device-resident, exported by
+ * vlm_rotor_get_wake):
+ *   node(b,i,j)   trailing corner 2 of ring (i,j) for j < ns, corner 3 of
+ *                 ring (i,ns-1) for j = ns        -> nodes[((b*nNwake+i)*(ns+1)+j)*3+d]
+ *   le(b,j)       leading corners 1/4 of the newest row (the shed line)
+ *                                                  -> le[(b*(ns+1)+j)*3+d]
+ *   gam(b,i,j)    ring circulation                 -> gam[(b*nNwake+i)*ns+j]
+ *   rc4(b,i,j,f)  core radius of filament f=0..3   -> rc4[((b*nNwake+i)*ns+j)*4+f]
+ * Ring (i,j) has corners c1 = node(i-1,j) (le(j) on the newest row),
+ * c2 = node(i,j), c3 = node(i,j+1), c4 = node(i-1,j+1) (le(j+1)).
+ * A far-wake filament k: far_fc[(b*nFwake+k)*6 + e*3 + d], e=0 TE end
+ * (fc(:,1)), e=1 LE end (fc(:,2)).
+ * Velocity arrays: velN[((b*nNwake+i)*(ns+1)+j)*3+d], velF[(b*nFwake+k)*3+d].
+ */
+#ifndef VLM_C_H
+#define VLM_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vlm_ctx vlm_ctx;
+
+enum {
+    VLM_OK = 0,
+    VLM_ERR_ARG = 1,       /* invalid argument or size */
+    VLM_ERR_HIP = 2,       /* HIP runtime failure */
+    VLM_ERR_STATE = 3,     /* call out of order */
+    VLM_ERR_SINGULAR = 4,  /* zero pivot in the AIC factorisation */
+    VLM_ERR_ALLOC = 5      /* device allocation failed */
+};
+
+/* Precision of the Biot-Savart pair arithmetic. */
+enum {
+    VLM_FP64 = 0,   /* FP64 throughout */
+    VLM_MIXED = 1   /* FP32 pairs, compensated tile sums, FP64 near field */
+};
+
+/* Source sets of a rotor for vlm_rotor_induced_velocity. */
+enum {
+    VLM_SET_WING = 0,        /* rotor_vind_bywing, classdef.f90:4424-4443 */
+    VLM_SET_WING_BOUND = 1,  /* rotor_vind_bywing_boundVortices, 4445-4457 */
+    VLM_SET_WAKE = 2,        /* rotor_vind_bywake, 4459-4479 */
+    VLM_SET_WAKE_PRED = 3    /* rotor_vind_bywake(P, 'P') */
+};
+
+enum { VLM_WAKE_C = 0, VLM_WAKE_P = 1 };     /* current / predicted wake */
+enum { VLM_EDGE_LE = 0, VLM_EDGE_TE = 1 };   /* assignshed edge */
+enum {                                        /* fdScheme, main.f90:844 */
+    VLM_SCHEME_FT1 = 0, VLM_SCHEME_PC2 = 1, VLM_SCHEME_AB2 = 2,
+    VLM_SCHEME_AM2 = 3
+};
+enum {                                        /* vlm_rotor_get_velocity */
+    VLM_VEL = 0, VLM_VEL_PREV = 1, VLM_VEL_PRED = 2, VLM_VEL_STEP = 3
+};
+
+/* ---- context ---------------------------------------------------------- */
+int vlm_create(vlm_ctx **ctx, int device);
+int vlm_destroy(vlm_ctx *ctx);
+const char *vlm_last_error(const vlm_ctx *ctx);
+int vlm_set_precision(vlm_ctx *ctx, int precision);
+/* bug_compat=1 (default) reproduces the predicted-wake row loop
+ * `do i = 1, rowNear, nNwake` (classdef.f90:1552, 1559); 0 convects every
+ * active predicted row. */
+int vlm_set_bug_compat(vlm_ctx *ctx, int bug_compat);
+int vlm_synchronize(vlm_ctx *ctx);
+
+/* ---- a1/a2: direct sums over explicit filament lists ----------------- */
+/* Sum over filaments s of gam[s] * vf_vind(p, a_s -> b_s, rc[s])
+ * (classdef.f90:476-503), at m points.  a, b, p, v are 3 x n arrays.
+ * accumulate != 0 adds into v. */
+int vlm_filament_vind(vlm_ctx *ctx, int64_t nfil, const double *a,
+                      const double *b, const double *gam, const double *rc,
+                      int64_t m, const double *p, double *v, int accumulate);
+
+/* Device-resident variant: upload a filament set and a point set once,
+ * evaluate repeatedly, download the velocities. */
+int vlm_fset_create(vlm_ctx *ctx, int64_t nfil, const double *a,
+                    const double *b, const double *gam, const double *rc,
+                    int *fset);
+int vlm_pset_create(vlm_ctx *ctx, int64_t m, const double *p, int *pset);
+int vlm_fset_vind(vlm_ctx *ctx, int fset, int pset);
+int vlm_pset_velocity(vlm_ctx *ctx, int pset, double *v);
+
+/* ---- rotor objects ----------------------------------------------------- */
+typedef struct {
+    int nb, nc, ns;          /* blades, chordwise and spanwise panels */
+    int nNwake, nFwake;      /* near-wake rows, far-wake filaments per blade */
+    int nbConvect;           /* blades convected (1 with axisymmetrySwitch) */
+    int rollupStart, rollupEnd;  /* 1-based span range, classdef.f90:3135-3136 */
+    int rollupMin;           /* sign(Omega*theta0) > 0: pick min gam, 4546-4554 */
+    int suppressFwake;       /* suppressFwakeSwitch */
+} vlm_rotor_desc;
+
+/* rotor_init wake allocation, classdef.f90:3731-3850.  Wake cores start at
+ * spanwiseCore (vf2, vf4) and streamwiseCoreVec(j)/(j+1) (vf1/vf3); far
+ * filaments at streamwiseCoreVec(ns+1).  streamwiseCoreVec has ns+1
+ * entries, already scaled by chord. */
+int vlm_rotor_create(vlm_ctx *ctx, const vlm_rotor_desc *desc,
+                     double spanwiseCore, const double *streamwiseCoreVec,
+                     int *ir);
+/* Wing vortex rings after rigid motion (classdef.f90:1092-1281):
+ * corners[((b*ns+is)*nc+ic)*12 + k*3 + d], rc[((b*ns+is)*nc+ic)*4 + f],
+ * gam[(b*ns+is)*nc+ic].  Any pointer may be NULL to keep the old value. */
+int vlm_rotor_set_wing(vlm_ctx *ctx, int ir, const double *corners,
+                       const double *rc, const double *gam);
+int vlm_rotor_set_axis(vlm_ctx *ctx, int ir, const double *shaftAxis,
+                       const double *hubCoords);
+
+/* Cursor bookkeeping: main.f90:227-229 and 412-417. */
+int vlm_rotor_init_cursors(vlm_ctx *ctx, int ir);
+int vlm_rotor_begin_step(vlm_ctx *ctx, int ir, int iter);
+int vlm_rotor_cursors(vlm_ctx *ctx, int ir, int *rowNear, int *rowFar);
+
+/* Wake bookkeeping. */
+int vlm_rotor_assignshed(vlm_ctx *ctx, int ir, int edge);    /* classdef.f90:4297-4325 */
+int vlm_rotor_age_wake(vlm_ctx *ctx, int ir, double dt);      /* 4331-4354 */
+int vlm_rotor_dissipate_wake(vlm_ctx *ctx, int ir, double dt,
+                             double kinematicVisc, double apparentViscCoeff,
+                             double decayCoeff);              /* 4356-4408 */
+int vlm_rotor_strain_wake(vlm_ctx *ctx, int ir);              /* 4410-4422 */
+int vlm_rotor_rollup(vlm_ctx *ctx, int ir);                   /* 4515-4605 + shift 4481-4513 */
+int vlm_rotor_convectwake(vlm_ctx *ctx, int ir, double dt,
+                          int wakeType);                      /* 1515-1702, 4786-4830 */
+
+/* Induced velocity at m host points by one source set of rotor ir
+ * (classdef.f90:4424-4479); accumulate != 0 adds into v. */
+int vlm_rotor_induced_velocity(vlm_ctx *ctx, int ir, int set, int64_t m,
+                               const double *p, double *v, int accumulate);
+
+/* The N-body pass: velNwake/velFwake (wakeType C) or velNwakePredicted /
+ * velFwakePredicted (P) of every rotor = sum over all rotors of wing +
+ * wake induced velocity (libCommon.f90:114-211, main.f90:800-841 and
+ * 1052-1086). */
+int vlm_wake_velocity(vlm_ctx *ctx, int wakeType);
+
+/* One whole convection block, main.f90:800-1406: pass 1, the scheme
+ * algebra of fdScheme (FT1, PC2, AB2 or AM2), pass 2 where the scheme has
+ * one, and the final convection of every rotor. */
+int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt);
+
+/* State access (tests, restart files, plots). */
+int vlm_rotor_get_wake(vlm_ctx *ctx, int ir, int wakeType, double *nodes,
+                       double *le, double *gam, double *rc4, double *far_fc,
+                       double *far_gam, double *far_rc);
+int vlm_rotor_set_wake(vlm_ctx *ctx, int ir, const double *nodes,
+                       const double *le, const double *gam, const double *rc4,
+                       const double *far_fc, const double *far_gam,
+                       const double *far_rc, int rowNear, int rowFar);
+int vlm_rotor_get_velocity(vlm_ctx *ctx, int ir, int which, double *velN,
+                           double *velF);
+
+/* ---- a7: AIC and the circulation solve -------------------------------- */
+/* AIC(row,col) = vr_vind(ring col, CP row) . nCap(row)
+ * (rotor_calcAIC, classdef.f90:4151-4179); cp and ncap are 3 x N with
+ * N = nb*nc*ns in the reference's row order.  The matrix is assembled,
+ * factorised (MFMA-FP64 blocked LU, partial pivoting) and kept on the
+ * device; A (N x N column-major) is copied out when non-NULL. */
+int vlm_rotor_calc_aic(vlm_ctx *ctx, int ir, const double *cp,
+                       const double *ncap, double *A);
+/* gamVec = AIC^-1 RHS (main.f90:596, matmulAX libMath.f90:105-122) by LU
+ * triangular solves plus iterative refinement with an FP64 residual,
+ * stopping when ||r||_inf <= tol*||RHS||_inf or after max_iter steps.
+ * The solved circulation also becomes the rotor's wing circulation
+ * (rotor_map_gam, classdef.f90:4181-4196). */
+int vlm_rotor_solve(vlm_ctx *ctx, int ir, const double *rhs, double *gam,
+                    int max_iter, double tol, int *iters);
+
+/* Dense kernels on explicit matrices (the LU microbenchmark and tests):
+ * factorise an n x n column-major matrix in place (ipiv 0-based), and
+ * solve with refinement against the unfactorised A. */
+int vlm_lu_factor(vlm_ctx *ctx, int n, double *A, int *ipiv);
+int vlm_lu_solve_refine(vlm_ctx *ctx, int n, const double *A,
+                        const double *LU, const int *ipiv, const double *rhs,
+                        double *x, int max_iter, double tol, int *iters);
+
+/* ---- kernel timing ----------------------------------------------------- */
+/* When enabled, every Biot-Savart pair launch is bracketed by HIP events on
+ * the context stream.  vlm_profile_get returns the summed kernel time (ms),
+ * the launch count and the algorithmic pair count since the last reset. */
+int vlm_profile_enable(vlm_ctx *ctx, int enable);
+int vlm_profile_reset(vlm_ctx *ctx);
+int vlm_profile_get(vlm_ctx *ctx, double *bs_ms, int64_t *bs_launches,
+                    double *pairs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VLM_C_H */

@@ -1,0 +1,238 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) -- test infrastructure.
+
+Mirrors the method names of volcanor_amd.vlm.Context so that a parity test
+runs one call sequence against both.  Only tests/, __graft_entry__.smoke()
+and bench.py's CPU baseline load this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_P, _I, _L, _D = C.c_void_p, C.c_int, C.c_int64, C.c_double
+_SIGS = {
+    "or_vf_vind": ([_P, _P, _P, _D, _P], None),
+    "or_filament_sum": ([_L, _P, _P, _P, _P, _L, _P, _P], None),
+    "or_filament_sum_omp": ([_L, _P, _P, _P, _P, _L, _P, _P, _I], None),
+    "or_sys_create": ([], _P),
+    "or_sys_destroy": ([_P], None),
+    "or_rotor_create": ([_P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I], _I),
+    "or_rotor_set_wing": ([_P, _I, _P, _P, _P], None),
+    "or_rotor_set_axis": ([_P, _I, _P, _P], None),
+    "or_rotor_init_wake_cores": ([_P, _I, _D, _P], None),
+    "or_rotor_init_cursors": ([_P, _I], None),
+    "or_rotor_begin_step": ([_P, _I, _I], None),
+    "or_rotor_cursors": ([_P, _I, C.POINTER(_I), C.POINTER(_I)], None),
+    "or_rotor_assignshed": ([_P, _I, _I], None),
+    "or_rotor_age_wake": ([_P, _I, _D, _D], None),
+    "or_rotor_dissipate_wake": ([_P, _I, _D, _D, _D, _D], None),
+    "or_rotor_strain_wake": ([_P, _I], None),
+    "or_rotor_rollup": ([_P, _I], None),
+    "or_rotor_convectwake": ([_P, _I, _D, _I, _I], None),
+    "or_rotor_vind": ([_P, _I, _I, _L, _P, _P], None),
+    "or_wake_velocity": ([_P, _I], None),
+    "or_convect": ([_P, _I, _I, _D, _I], None),
+    "or_rotor_get_wake": ([_P, _I, _I, _P, _P, _P, _P, _P, _P, _P], None),
+    "or_rotor_set_wake": ([_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _I], None),
+    "or_rotor_get_velocity": ([_P, _I, _I, _P, _P], None),
+    "or_rotor_calc_aic": ([_P, _I, _P, _P, _P], None),
+    "or_lu_factor": ([_I, _P, _P], _I),
+    "or_lu_solve": ([_I, _P, _P, _P], None),
+    "or_matvec": ([_I, _P, _P, _P], None),
+    "or_mlp_forward": ([_P, _I, _P, _P], None),
+    "or_mlp_loss_grad": ([_P, _I, _P, _P, _P], _D),
+}
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built at {LIB_PATH}; run `make -C oracle`")
+        lib = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+def vf_vind(P, A, B, rc):
+    lib = load()
+    v = np.zeros(3)
+    lib.or_vf_vind(_ptr(_f64(P)), _ptr(_f64(A)), _ptr(_f64(B)), float(rc), _ptr(v))
+    return v
+
+
+def filament_sum(a, b, gam, rc, p, threads=0):
+    lib = load()
+    a, b, p = _f64(a, (-1, 3)), _f64(b, (-1, 3)), _f64(p, (-1, 3))
+    gam, rc = _f64(gam).ravel(), _f64(rc).ravel()
+    v = np.zeros_like(p)
+    if threads:
+        lib.or_filament_sum_omp(a.shape[0], _ptr(a), _ptr(b), _ptr(gam), _ptr(rc), p.shape[0],
+                                _ptr(p), _ptr(v), threads)
+    else:
+        lib.or_filament_sum(a.shape[0], _ptr(a), _ptr(b), _ptr(gam), _ptr(rc), p.shape[0],
+                            _ptr(p), _ptr(v))
+    return v
+
+
+def lu_factor(A):
+    lib = load()
+    A = np.asfortranarray(A, dtype=np.float64).copy(order="F")
+    piv = np.zeros(A.shape[0], dtype=np.int32)
+    info = lib.or_lu_factor(A.shape[0], _ptr(A), _ptr(piv))
+    return A, piv, info
+
+
+def lu_solve(LU, piv, b):
+    lib = load()
+    x = _f64(b).copy()
+    lib.or_lu_solve(LU.shape[0], _ptr(np.asfortranarray(LU)), _ptr(np.ascontiguousarray(piv, np.int32)), _ptr(x))
+    return x
+
+
+def mlp_forward(params, x):
+    lib = load()
+    x = _f64(x, (-1, 45))
+    y = np.zeros((x.shape[0], 3))
+    lib.or_mlp_forward(_ptr(_f64(params)), x.shape[0], _ptr(x), _ptr(y))
+    return y
+
+
+def mlp_loss_grad(params, x, target):
+    lib = load()
+    params = _f64(params)
+    x, target = _f64(x, (-1, 45)), _f64(target, (-1, 3))
+    g = np.zeros_like(params)
+    loss = lib.or_mlp_loss_grad(_ptr(params), x.shape[0], _ptr(x), _ptr(target), _ptr(g))
+    return loss, g
+
+
+class System:
+    """Ring-form rotor system with the same method names as vlm.Context."""
+
+    def __init__(self):
+        self.lib = load()
+        self.h = self.lib.or_sys_create()
+        self.bug_compat = 1
+
+    def __del__(self):
+        try:
+            self.lib.or_sys_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_bug_compat(self, on):
+        self.bug_compat = int(on)
+
+    def rotor_create(self, nb, nc, ns, nNwake, nFwake, nbConvect=None, rollupStart=1,
+                     rollupEnd=None, rollupMin=1, suppressFwake=0, spanwiseCore=0.0,
+                     streamwiseCoreVec=None):
+        ir = self.lib.or_rotor_create(self.h, nb, nc, ns, nNwake, nFwake,
+                                      nb if nbConvect is None else nbConvect, rollupStart,
+                                      ns if rollupEnd is None else rollupEnd, rollupMin,
+                                      suppressFwake)
+        scv = _f64(np.zeros(ns + 1) if streamwiseCoreVec is None else streamwiseCoreVec)
+        self.lib.or_rotor_init_wake_cores(self.h, ir, spanwiseCore, _ptr(scv))
+        self.lib.or_rotor_init_cursors(self.h, ir)
+        return ir
+
+    def rotor_set_wing(self, ir, corners=None, rc=None, gam=None):
+        c = None if corners is None else _f64(corners).ravel()
+        r = None if rc is None else _f64(rc).ravel()
+        g = None if gam is None else _f64(gam).ravel()
+        self.lib.or_rotor_set_wing(self.h, ir, _ptr(c), _ptr(r), _ptr(g))
+
+    def rotor_set_axis(self, ir, shaft, hub):
+        self.lib.or_rotor_set_axis(self.h, ir, _ptr(_f64(shaft)), _ptr(_f64(hub)))
+
+    def rotor_init_cursors(self, ir):
+        self.lib.or_rotor_init_cursors(self.h, ir)
+
+    def rotor_begin_step(self, ir, it):
+        self.lib.or_rotor_begin_step(self.h, ir, it)
+
+    def rotor_cursors(self, ir):
+        a, b = C.c_int(), C.c_int()
+        self.lib.or_rotor_cursors(self.h, ir, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def rotor_assignshed(self, ir, edge):
+        self.lib.or_rotor_assignshed(self.h, ir, edge)
+
+    def rotor_age_wake(self, ir, dt):
+        self.lib.or_rotor_age_wake(self.h, ir, dt, 0.0)
+
+    def rotor_dissipate_wake(self, ir, dt, nu, avc, decay=0.0):
+        self.lib.or_rotor_dissipate_wake(self.h, ir, dt, nu, avc, decay)
+
+    def rotor_strain_wake(self, ir):
+        self.lib.or_rotor_strain_wake(self.h, ir)
+
+    def rotor_rollup(self, ir):
+        self.lib.or_rotor_rollup(self.h, ir)
+
+    def rotor_convectwake(self, ir, dt, wake_type):
+        self.lib.or_rotor_convectwake(self.h, ir, dt, wake_type, self.bug_compat)
+
+    def rotor_induced_velocity(self, ir, sset, p, v=None):
+        p = _f64(p, (-1, 3))
+        out = np.zeros_like(p) if v is None else _f64(v, (-1, 3)).copy()
+        self.lib.or_rotor_vind(self.h, ir, sset, p.shape[0], _ptr(p), _ptr(out))
+        return out
+
+    def wake_velocity(self, wake_type=0):
+        self.lib.or_wake_velocity(self.h, wake_type)
+
+    def convect(self, scheme, it, dt):
+        self.lib.or_convect(self.h, scheme, it, dt, self.bug_compat)
+
+    def rotor_get_wake(self, ir, dims, wake_type=0):
+        nb, ns, nN, nF = dims
+        nodes = np.zeros((nb, nN, ns + 1, 3))
+        le = np.zeros((nb, ns + 1, 3))
+        gam = np.zeros((nb, nN, ns))
+        rc4 = np.zeros((nb, nN, ns, 4))
+        ffc = np.zeros((nb, max(nF, 0), 2, 3))
+        fg = np.zeros((nb, max(nF, 0)))
+        frc = np.zeros((nb, max(nF, 0)))
+        self.lib.or_rotor_get_wake(self.h, ir, wake_type, _ptr(nodes), _ptr(le), _ptr(gam),
+                                   _ptr(rc4), _ptr(ffc), _ptr(fg), _ptr(frc))
+        return dict(nodes=nodes, le=le, gam=gam, rc4=rc4, far_fc=ffc, far_gam=fg, far_rc=frc)
+
+    def rotor_set_wake(self, ir, w, rowNear, rowFar):
+        arrs = [_f64(w[k]) for k in ("nodes", "le", "gam", "rc4", "far_fc", "far_gam",
+                                      "far_rc")]
+        self.lib.or_rotor_set_wake(self.h, ir, *[_ptr(a) for a in arrs], rowNear, rowFar)
+
+    def rotor_get_velocity(self, ir, which, dims):
+        nb, ns, nN, nF = dims
+        vn = np.zeros((nb, nN, ns + 1, 3))
+        vf = np.zeros((nb, max(nF, 0), 3))
+        self.lib.or_rotor_get_velocity(self.h, ir, which, _ptr(vn), _ptr(vf))
+        return vn, vf
+
+    def rotor_calc_aic(self, ir, cp, ncap, n):
+        A = np.zeros((n, n), order="F")
+        self.lib.or_rotor_calc_aic(self.h, ir, _ptr(_f64(cp, (-1, 3))), _ptr(_f64(ncap, (-1, 3))),
+                                   _ptr(A))
+        return A

@@ -1,0 +1,108 @@
+"""Biot-Savart kernel parity (FP64 and mixed precision) with the oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(precision=0):
+    from volcanor_amd import vlm
+    c = vlm.Context(0)
+    c.set_precision(precision)
+    return c
+
+
+def helix_segments(n_helix, per, rng, jitter=1e-3):
+    """Filaments along random-pitch helices (radius 1), as in the
+    microbenchmark workload, scaled down."""
+    a, b = [], []
+    for _ in range(n_helix):
+        c = rng.uniform(-1, 1, size=2)
+        pitch = rng.uniform(0.1, 0.3)
+        ph = rng.uniform(0, 2 * np.pi)
+        z = np.linspace(-2, 2, per + 1)
+        th = ph + 2 * np.pi * (z + 2) / pitch
+        pts = np.stack([c[0] + np.cos(th), c[1] + np.sin(th), z], axis=1)
+        pts += jitter * rng.standard_normal(pts.shape)
+        a.append(pts[:-1])
+        b.append(pts[1:])
+    return np.concatenate(a), np.concatenate(b)
+
+
+@pytest.mark.parametrize("nseg,m", [(1, 1), (7, 300), (1000, 1), (3000, 700), (20000, 257)])
+def test_fp64_matches_oracle(nseg, m):
+    rng = np.random.default_rng(nseg + m)
+    a, b = rng.uniform(-2, 2, size=(2, nseg, 3))
+    g, rc = rng.normal(size=nseg), np.full(nseg, 0.01)
+    p = rng.uniform(-2, 2, size=(m, 3))
+    v = _ctx().filament_vind(a, b, g, rc, p)
+    w = orc.filament_sum(a, b, g, rc, p, threads=8)
+    np.testing.assert_allclose(v, w, rtol=1e-10, atol=1e-12 * np.max(np.abs(w)))
+
+
+def test_edge_cases():
+    ctx = _ctx()
+    A = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    B = np.array([[1.0, 0.0, 0.0], [1.0, 1.0, 0.0]])
+    g = np.array([1.0, -2.0])
+    for rc in (0.0, 0.05):
+        rcv = np.full(2, rc)
+        P = np.array([[0.0, 0.0, 0.0], [0.5, 0.0, 0.0], [2.0, 0.0, 0.0], [1.0, 1.0, 0.0],
+                      [0.3, 0.4, 0.1], [1e3, -1e3, 5.0]])
+        v = ctx.filament_vind(A, B, g, rcv, P)
+        w = orc.filament_sum(A, B, g, rcv, P)
+        np.testing.assert_allclose(v, w, rtol=1e-12, atol=1e-15)
+    # empty source list, accumulate semantics
+    P = np.ones((4, 3))
+    v = ctx.filament_vind(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0), np.zeros(0), P)
+    assert np.all(v == 0)
+    v0 = np.full((4, 3), 2.5)
+    v = ctx.filament_vind(A, B, g, np.full(2, 0.1), P, v=v0)
+    np.testing.assert_allclose(v, 2.5 + orc.filament_sum(A, B, g, np.full(2, 0.1), P), rtol=1e-12)
+
+
+def test_repeatable_bitwise():
+    rng = np.random.default_rng(9)
+    a, b = rng.uniform(-2, 2, size=(2, 5000, 3))
+    g, rc = rng.normal(size=5000), np.full(5000, 0.01)
+    p = rng.uniform(-2, 2, size=(999, 3))
+    ctx = _ctx()
+    v1 = ctx.filament_vind(a, b, g, rc, p)
+    v2 = ctx.filament_vind(a, b, g, rc, p)
+    assert np.array_equal(v1, v2)
+
+
+def test_mixed_precision_helix_error_bound():
+    """Mixed FP32/FP64 against the FP64 oracle on helix filaments (the
+    microbenchmark geometry at reduced size)."""
+    rng = np.random.default_rng(20261202)
+    a, b = helix_segments(8, 2000, rng)
+    g = rng.normal(size=len(a))
+    rc = np.full(len(a), 0.01)
+    p = rng.uniform(-2, 2, size=(500, 3))
+    w = orc.filament_sum(a, b, g, rc, p, threads=8)
+    v = _ctx(1).filament_vind(a, b, g, rc, p)
+    rms = np.sqrt(np.mean(np.sum(w**2, axis=1)))
+    err = np.linalg.norm(v - w, axis=1)
+    assert np.max(err) / rms < 2e-5
+    assert np.sqrt(np.mean(err**2)) / rms < 2e-6
+
+
+def test_resident_sets_and_profile():
+    rng = np.random.default_rng(4)
+    a, b = rng.uniform(-2, 2, size=(2, 4096, 3))
+    g, rc = rng.normal(size=4096), np.full(4096, 0.02)
+    p = rng.uniform(-2, 2, size=(2048, 3))
+    ctx = _ctx()
+    fs = ctx.fset_create(a, b, g, rc)
+    ps, m = ctx.pset_create(p)
+    ctx.profile_enable(True)
+    ctx.profile_reset()
+    ctx.fset_vind(fs, ps)
+    ms, n, pairs = ctx.profile_get()
+    assert n == 1 and pairs == 4096 * 2048 and ms > 0
+    v = ctx.pset_velocity(ps, m)
+    w = orc.filament_sum(a, b, g, rc, p, threads=8)
+    np.testing.assert_allclose(v, w, rtol=1e-10, atol=1e-12 * np.max(np.abs(w)))

@@ -1,0 +1,125 @@
+"""Parity of the device-resident free wake (lattice form, ring buffer,
+unique segments) with the ring-form CPU oracle, step by step."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests import wake_harness as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(precision=0):
+    from volcanor_amd import vlm
+    c = vlm.Context(0)
+    c.set_precision(precision)
+    return c
+
+
+def _cmp(a, b, scale, tol, what):
+    err = np.max(np.abs(a - b)) if a.size else 0.0
+    assert err <= tol * max(scale, 1e-300), f"{what}: max err {err:.3e} scale {scale:.3e}"
+
+
+def compare(sa, sb, case, tol=1e-11):
+    for ra, rb in zip(sa, sb):
+        assert ra["cursors"] == rb["cursors"]
+        rn, rf = ra["cursors"][0] - 1, ra["cursors"][1] - 1
+        if rn < case.nNwake:
+            sc = np.max(np.abs(ra["nodes"][:, rn:])) + 1.0
+            _cmp(ra["nodes"][:, rn:], rb["nodes"][:, rn:], sc, tol, "near nodes")
+            _cmp(ra["le"], rb["le"], sc, tol, "shed line")
+            sg = np.max(np.abs(ra["gam"][:, rn:])) + 1e-30
+            _cmp(ra["gam"][:, rn:], rb["gam"][:, rn:], sg, tol, "near gam")
+            _cmp(ra["rc4"][:, rn:], rb["rc4"][:, rn:], 1.0, tol, "near cores")
+        if rf < case.nFwake:
+            sc = np.max(np.abs(ra["far_fc"][:, rf:])) + 1.0
+            _cmp(ra["far_fc"][:, rf:], rb["far_fc"][:, rf:], sc, tol, "far ends")
+            _cmp(ra["far_gam"][:, rf:], rb["far_gam"][:, rf:], 1.0, tol, "far gam")
+            _cmp(ra["far_rc"][:, rf:], rb["far_rc"][:, rf:], 1.0, tol, "far cores")
+        for k in ("vn", "vn1", "vf", "vf1"):
+            sv = np.max(np.abs(rb[k])) + 1e-30 if rb[k].size else 1.0
+            _cmp(ra[k], rb[k], sv, tol, k)
+
+
+CASES = {
+    "wing_am2": dict(nb=1, nc=3, ns=5, nNwake=5, nFwake=4, nt=12, scheme=3, vel=(10, 0, 0)),
+    "rotor_am2": dict(nb=2, nc=1, ns=6, nNwake=6, nFwake=4, nt=13, scheme=3, omega=30.0,
+                      vel=(0, 0, 1.0)),
+    "rotor_am2_axisym_diss_strain": dict(nb=3, nc=1, ns=5, nNwake=4, nFwake=5, nt=12, scheme=3,
+                                         omega=25.0, vel=(0, 0, 2.0), axisym=True,
+                                         dissipation=True, strain=True),
+    "rotor_am2_fixed_ploop": dict(nb=2, nc=2, ns=4, nNwake=5, nFwake=3, nt=11, scheme=3,
+                                  omega=30.0, vel=(0, 0, 1.0), bug_compat=False,
+                                  dissipation=True),
+    "wing_ft1": dict(nb=1, nc=2, ns=4, nNwake=4, nFwake=3, nt=9, scheme=0),
+    "wing_pc2_strain": dict(nb=1, nc=2, ns=4, nNwake=4, nFwake=4, nt=10, scheme=1, strain=True),
+    "rotor_ab2": dict(nb=2, nc=1, ns=4, nNwake=4, nFwake=3, nt=9, scheme=2, omega=20.0,
+                      vel=(0, 0, 1.0)),
+    "two_rotors_am2": dict(nb=2, nc=1, ns=4, nNwake=4, nFwake=3, nt=9, scheme=3, omega=25.0,
+                           vel=(0, 0, 1.0), nr=2, hub_dx=2.5, dissipation=True),
+    "nonuniform_cores": dict(nb=1, nc=2, ns=4, nNwake=4, nFwake=2, nt=8, scheme=3,
+                             scv=np.array([0.01, 0.02, 0.03, 0.02, 0.01]), dissipation=True),
+    "truncated_no_far": dict(nb=1, nc=1, ns=3, nNwake=3, nFwake=0, nt=7, scheme=3, rollup=(2, 3)),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_wake_steps_match_oracle(name):
+    case = H.Case(**CASES[name])
+    dev, ref = _ctx(), orc.System()
+    ia, ib = H.setup(dev, case), H.setup(ref, case)
+    for it in range(1, case.nt + 1):
+        H.step(dev, case, ia, it)
+        H.step(ref, case, ib, it)
+        compare(H.snapshot(dev, case, ia), H.snapshot(ref, case, ib), case)
+
+
+def test_mixed_precision_wake_close_to_fp64():
+    case = H.Case(**CASES["rotor_am2"])
+    dev, ref = _ctx(precision=1), orc.System()
+    ia, ib = H.setup(dev, case), H.setup(ref, case)
+    for it in range(1, case.nt + 1):
+        H.step(dev, case, ia, it)
+        H.step(ref, case, ib, it)
+    compare(H.snapshot(dev, case, ia), H.snapshot(ref, case, ib), case, tol=2e-5)
+
+
+def test_induced_velocity_sets_match_oracle():
+    case = H.Case(**CASES["rotor_am2"])
+    dev, ref = _ctx(), orc.System()
+    ia, ib = H.setup(dev, case), H.setup(ref, case)
+    for it in range(1, 9):
+        H.step(dev, case, ia, it)
+        H.step(ref, case, ib, it)
+    p = np.random.default_rng(0).normal(size=(37, 3)) * 0.8
+    for s in (0, 1, 2):
+        a = dev.rotor_induced_velocity(ia[0], s, p)
+        b = ref.rotor_induced_velocity(ib[0], s, p)
+        np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-13 * np.max(np.abs(b)))
+    # accumulate adds into the given array
+    a2 = dev.rotor_induced_velocity(ia[0], 2, p, v=np.ones_like(p))
+    np.testing.assert_allclose(a2, 1.0 + ref.rotor_induced_velocity(ib[0], 2, p), rtol=1e-11)
+
+
+def test_set_wake_roundtrip_and_errors():
+    from volcanor_amd import vlm
+    case = H.Case(**CASES["wing_am2"])
+    dev = _ctx()
+    ia = H.setup(dev, case)
+    for it in range(1, 8):
+        H.step(dev, case, ia, it)
+    w = dev.rotor_get_wake(ia[0], case.dims)
+    rn, rf = dev.rotor_cursors(ia[0])
+    dev2 = _ctx()
+    ib = H.setup(dev2, case)
+    dev2.rotor_set_wake(ib[0], w, rn, rf)
+    w2 = dev2.rotor_get_wake(ib[0], case.dims)
+    for k in ("nodes", "le", "gam", "rc4"):
+        assert np.array_equal(w[k][:, rn - 1:], w2[k][:, rn - 1:]), k
+    with pytest.raises(vlm.VLMError):
+        dev2.rotor_set_wake(ib[0], w, 0, rf)
+    with pytest.raises(vlm.VLMError):
+        dev2.rotor_assignshed(ib[0], 7)
+    with pytest.raises(vlm.VLMError):
+        dev2.convect(9, 1, 0.1)

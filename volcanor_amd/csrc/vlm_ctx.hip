@@ -1,0 +1,408 @@
+// Context, error handling, segment compaction, explicit filament sets and
+// kernel timing for libvlm.
+#include <hipcub/hipcub.hpp>
+
+#include <cstring>
+
+#include "vlm_internal.h"
+
+namespace vlm {
+
+int fail(vlm_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_fail(vlm_ctx *ctx, hipError_t e, const char *what)
+{
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(ctx, e == hipErrorOutOfMemory ? VLM_ERR_ALLOC : VLM_ERR_HIP, m);
+}
+
+namespace {
+
+// One record per staged slot -> SoA FP64 and FP32 copies at the compacted
+// position.  The last thread publishes the count.
+__global__ void compact_kernel(const Slot *slots, const int *flags, const int *idx,
+                               int64_t n, double *d64, float *d32, int64_t cap,
+                               long long *count)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i == n - 1) *count = (long long)idx[i] + (long long)flags[i];
+    if (!flags[i]) return;
+    const int64_t k = idx[i];
+    const Slot s = slots[i];
+    const double g = s.gam * kInv4Pi;
+    const double rc2 = s.rc * s.rc;
+    d64[k] = s.ax;
+    d64[cap + k] = s.ay;
+    d64[2 * cap + k] = s.az;
+    d64[3 * cap + k] = s.bx;
+    d64[4 * cap + k] = s.by;
+    d64[5 * cap + k] = s.bz;
+    d64[6 * cap + k] = g;
+    d64[7 * cap + k] = rc2;
+    const double r0x = s.bx - s.ax, r0y = s.by - s.ay, r0z = s.bz - s.az;
+    const double n0 = r0x * r0x + r0y * r0y + r0z * r0z;
+    const double q = rc2 * n0;
+    d32[k] = (float)s.ax;
+    d32[cap + k] = (float)s.ay;
+    d32[2 * cap + k] = (float)s.az;
+    d32[3 * cap + k] = (float)r0x;
+    d32[4 * cap + k] = (float)r0y;
+    d32[5 * cap + k] = (float)r0z;
+    d32[6 * cap + k] = (float)g;
+    d32[7 * cap + k] = (float)(q * q);
+}
+
+__global__ void zero_count_kernel(long long *count) { *count = 0; }
+
+}  // namespace
+
+int ensure_slots(vlm_ctx *ctx, int64_t n)
+{
+    VLM_HIP(ctx->slots.reserve((size_t)n));
+    VLM_HIP(ctx->flags.reserve((size_t)n));
+    VLM_HIP(ctx->idx.reserve((size_t)n));
+    return VLM_OK;
+}
+
+int compact_into(vlm_ctx *ctx, const Slot *slots, const int *flags, int64_t nslots,
+                 DBuf<double> &d64, DBuf<float> &d32, DBuf<long long> &count, int64_t cap)
+{
+    VLM_HIP(count.reserve(1));
+    if (nslots <= 0) {
+        hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, ctx->stream, count.p);
+        VLM_HIP(hipGetLastError());
+        return VLM_OK;
+    }
+    if (nslots > 0x7fffffff) return fail(ctx, VLM_ERR_ARG, "segment list too large");
+    size_t bytes = 0;
+    VLM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flags, ctx->idx.p, (int)nslots,
+                                             ctx->stream));
+    if (bytes > ctx->scan_tmp.n) VLM_HIP(ctx->scan_tmp.reserve(bytes));
+    bytes = ctx->scan_tmp.n;
+    VLM_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, bytes, flags, ctx->idx.p,
+                                             (int)nslots, ctx->stream));
+    hipLaunchKernelGGL(compact_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0,
+                       ctx->stream, slots, flags, ctx->idx.p, nslots, d64.p, d32.p, cap,
+                       count.p);
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+int compact_slots(vlm_ctx *ctx, int64_t nslots)
+{
+    const int64_t cap = nslots > 0 ? nslots : 1;
+    if (ctx->seg.cap < cap) {
+        VLM_HIP(ctx->seg.d64.reserve((size_t)8 * cap));
+        VLM_HIP(ctx->seg.d32.reserve((size_t)8 * cap));
+        ctx->seg.cap = cap;
+    }
+    return compact_into(ctx, ctx->slots.p, ctx->flags.p, nslots, ctx->seg.d64, ctx->seg.d32,
+                        ctx->seg.count, ctx->seg.cap);
+}
+
+namespace {
+
+__global__ void stage_fil_kernel(int64_t n, const double *a, const double *b, const double *g,
+                                 const double *rc, Slot *slots, int *flags)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Slot s;
+    s.ax = a[3 * i];
+    s.ay = a[3 * i + 1];
+    s.az = a[3 * i + 2];
+    s.bx = b[3 * i];
+    s.by = b[3 * i + 1];
+    s.bz = b[3 * i + 2];
+    s.gam = g[i];
+    s.rc = rc[i];
+    slots[i] = s;
+    flags[i] = 1;  // explicit lists keep every filament (no |gam| skip)
+}
+
+// AoS host layout (3 x m) -> SoA device layout
+__global__ void aos_to_soa_kernel(int64_t m, const double *aos, double *soa, int64_t stride)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    soa[i] = aos[3 * i];
+    soa[stride + i] = aos[3 * i + 1];
+    soa[2 * stride + i] = aos[3 * i + 2];
+}
+
+__global__ void soa_to_aos_kernel(int64_t m, const double *soa, int64_t stride, double *aos)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    aos[3 * i] = soa[i];
+    aos[3 * i + 1] = soa[stride + i];
+    aos[3 * i + 2] = soa[2 * stride + i];
+}
+
+unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+int upload_fset(vlm_ctx *ctx, int64_t nfil, const double *a, const double *b,
+                const double *gam, const double *rc, DBuf<double> &d64, DBuf<float> &d32,
+                DBuf<long long> &count)
+{
+    DBuf<double> raw;
+    VLM_HIP(raw.reserve((size_t)8 * nfil));
+    VLM_HIP(hipMemcpyAsync(raw.p, a, sizeof(double) * 3 * nfil, hipMemcpyHostToDevice,
+                           ctx->stream));
+    VLM_HIP(hipMemcpyAsync(raw.p + 3 * nfil, b, sizeof(double) * 3 * nfil,
+                           hipMemcpyHostToDevice, ctx->stream));
+    VLM_HIP(hipMemcpyAsync(raw.p + 6 * nfil, gam, sizeof(double) * nfil, hipMemcpyHostToDevice,
+                           ctx->stream));
+    VLM_HIP(hipMemcpyAsync(raw.p + 7 * nfil, rc, sizeof(double) * nfil, hipMemcpyHostToDevice,
+                           ctx->stream));
+    VLM_TRY(ensure_slots(ctx, nfil));
+    hipLaunchKernelGGL(stage_fil_kernel, dim3(nblk(nfil)), dim3(256), 0, ctx->stream, nfil,
+                       raw.p, raw.p + 3 * nfil, raw.p + 6 * nfil, raw.p + 7 * nfil,
+                       ctx->slots.p, ctx->flags.p);
+    VLM_HIP(hipGetLastError());
+    VLM_HIP(d64.reserve((size_t)8 * nfil));
+    VLM_HIP(d32.reserve((size_t)8 * nfil));
+    VLM_TRY(compact_into(ctx, ctx->slots.p, ctx->flags.p, nfil, d64, d32, count, nfil));
+    VLM_HIP(hipStreamSynchronize(ctx->stream));  // raw is released on return
+    return VLM_OK;
+}
+
+}  // namespace
+
+int points_to_device(vlm_ctx *ctx, int64_t m, const double *p, double *soa, int64_t stride)
+{
+    VLM_HIP(ctx->hostio.reserve((size_t)3 * m));
+    VLM_HIP(hipMemcpyAsync(ctx->hostio.p, p, sizeof(double) * 3 * m, hipMemcpyHostToDevice,
+                           ctx->stream));
+    hipLaunchKernelGGL(aos_to_soa_kernel, dim3(nblk(m)), dim3(256), 0, ctx->stream, m,
+                       ctx->hostio.p, soa, stride);
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+int points_to_host(vlm_ctx *ctx, int64_t m, const double *soa, int64_t stride, double *p)
+{
+    VLM_HIP(ctx->hostio.reserve((size_t)3 * m));
+    hipLaunchKernelGGL(soa_to_aos_kernel, dim3(nblk(m)), dim3(256), 0, ctx->stream, m, soa,
+                       stride, ctx->hostio.p);
+    VLM_HIP(hipGetLastError());
+    VLM_HIP(hipMemcpyAsync(p, ctx->hostio.p, sizeof(double) * 3 * m, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    return VLM_OK;
+}
+
+}  // namespace vlm
+
+using namespace vlm;
+
+extern "C" {
+
+int vlm_create(vlm_ctx **out, int device)
+{
+    if (!out) return VLM_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return VLM_ERR_HIP;
+    if (device < 0 || device >= ndev) return VLM_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return VLM_ERR_HIP;
+    vlm_ctx *ctx = new vlm_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return VLM_ERR_HIP;
+    }
+    ctx->prof_cap = 1 << 16;
+    if (hipHostMalloc(reinterpret_cast<void **>(&ctx->prof_nseg),
+                      ctx->prof_cap * sizeof(long long), hipHostMallocDefault) != hipSuccess) {
+        ctx->prof_nseg = nullptr;
+        ctx->prof_cap = 0;
+    }
+    *out = ctx;
+    return VLM_OK;
+}
+
+int vlm_destroy(vlm_ctx *ctx)
+{
+    if (!ctx) return VLM_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto *r : ctx->rotors) delete r;
+    for (auto *f : ctx->fsets) delete f;
+    for (auto *p : ctx->psets) delete p;
+    for (auto *p : ctx->aic) (void)hipFree(p);
+    for (auto *p : ctx->lu) (void)hipFree(p);
+    for (auto *p : ctx->piv) (void)hipFree(p);
+    for (auto &t : ctx->timers) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    if (ctx->prof_nseg) (void)hipHostFree(ctx->prof_nseg);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return VLM_OK;
+}
+
+const char *vlm_last_error(const vlm_ctx *ctx)
+{
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+int vlm_set_precision(vlm_ctx *ctx, int precision)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    if (precision != VLM_FP64 && precision != VLM_MIXED)
+        return fail(ctx, VLM_ERR_ARG, "precision must be VLM_FP64 or VLM_MIXED");
+    ctx->precision = precision;
+    return VLM_OK;
+}
+
+int vlm_set_bug_compat(vlm_ctx *ctx, int bug_compat)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    ctx->bug_compat = bug_compat ? 1 : 0;
+    return VLM_OK;
+}
+
+int vlm_synchronize(vlm_ctx *ctx)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    return VLM_OK;
+}
+
+int vlm_filament_vind(vlm_ctx *ctx, int64_t nfil, const double *a, const double *b,
+                      const double *gam, const double *rc, int64_t m, const double *p,
+                      double *v, int accumulate)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    if (nfil < 0 || m < 0) return fail(ctx, VLM_ERR_ARG, "negative size");
+    if (m == 0) return VLM_OK;
+    if (!p || !v || (nfil > 0 && (!a || !b || !gam || !rc)))
+        return fail(ctx, VLM_ERR_ARG, "null buffer");
+    VLM_HIP(hipSetDevice(ctx->device));
+    FSet fs;
+    if (nfil > 0) VLM_TRY(upload_fset(ctx, nfil, a, b, gam, rc, fs.d64, fs.d32, fs.count));
+    else {
+        VLM_HIP(fs.d64.reserve(8));
+        VLM_HIP(fs.d32.reserve(8));
+        VLM_TRY(compact_into(ctx, nullptr, nullptr, 0, fs.d64, fs.d32, fs.count, 1));
+    }
+    VLM_HIP(ctx->tgt.reserve((size_t)3 * m));
+    VLM_HIP(ctx->out.reserve((size_t)3 * m));
+    VLM_TRY(points_to_device(ctx, m, p, ctx->tgt.p, m));
+    if (accumulate) VLM_TRY(points_to_device(ctx, m, v, ctx->out.p, m));
+    const int64_t stride = nfil > 0 ? nfil : 1;
+    VLM_TRY(bs_eval(ctx, fs.d64.p, fs.d32.p, stride, fs.count.p, nfil, ctx->tgt.p, m, m,
+                    ctx->out.p, accumulate));
+    VLM_TRY(points_to_host(ctx, m, ctx->out.p, m, v));
+    return VLM_OK;
+}
+
+int vlm_fset_create(vlm_ctx *ctx, int64_t nfil, const double *a, const double *b,
+                    const double *gam, const double *rc, int *fset)
+{
+    if (!ctx || !fset) return VLM_ERR_ARG;
+    if (nfil <= 0 || !a || !b || !gam || !rc) return fail(ctx, VLM_ERR_ARG, "empty filament set");
+    VLM_HIP(hipSetDevice(ctx->device));
+    FSet *fs = new FSet();
+    int rc_ = upload_fset(ctx, nfil, a, b, gam, rc, fs->d64, fs->d32, fs->count);
+    if (rc_ != VLM_OK) {
+        delete fs;
+        return rc_;
+    }
+    fs->n = nfil;
+    ctx->fsets.push_back(fs);
+    *fset = (int)ctx->fsets.size() - 1;
+    return VLM_OK;
+}
+
+int vlm_pset_create(vlm_ctx *ctx, int64_t m, const double *p, int *pset)
+{
+    if (!ctx || !pset) return VLM_ERR_ARG;
+    if (m <= 0 || !p) return fail(ctx, VLM_ERR_ARG, "empty point set");
+    VLM_HIP(hipSetDevice(ctx->device));
+    PSet *ps = new PSet();
+    if (ps->p.reserve((size_t)3 * m) != hipSuccess || ps->v.reserve((size_t)3 * m) != hipSuccess) {
+        delete ps;
+        return fail(ctx, VLM_ERR_ALLOC, "point set allocation failed");
+    }
+    ps->m = m;
+    int rc_ = points_to_device(ctx, m, p, ps->p.p, m);
+    if (rc_ != VLM_OK) {
+        delete ps;
+        return rc_;
+    }
+    ctx->psets.push_back(ps);
+    *pset = (int)ctx->psets.size() - 1;
+    return VLM_OK;
+}
+
+int vlm_fset_vind(vlm_ctx *ctx, int fset, int pset)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    if (fset < 0 || fset >= (int)ctx->fsets.size() || pset < 0 ||
+        pset >= (int)ctx->psets.size())
+        return fail(ctx, VLM_ERR_ARG, "unknown filament or point set");
+    FSet *fs = ctx->fsets[fset];
+    PSet *ps = ctx->psets[pset];
+    return bs_eval(ctx, fs->d64.p, fs->d32.p, fs->n, fs->count.p, fs->n, ps->p.p, ps->m, ps->m,
+                   ps->v.p, 0);
+}
+
+int vlm_pset_velocity(vlm_ctx *ctx, int pset, double *v)
+{
+    if (!ctx || !v) return VLM_ERR_ARG;
+    if (pset < 0 || pset >= (int)ctx->psets.size())
+        return fail(ctx, VLM_ERR_ARG, "unknown point set");
+    PSet *ps = ctx->psets[pset];
+    return points_to_host(ctx, ps->m, ps->v.p, ps->m, v);
+}
+
+int vlm_profile_enable(vlm_ctx *ctx, int enable)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    ctx->profile = enable ? 1 : 0;
+    return VLM_OK;
+}
+
+int vlm_profile_reset(vlm_ctx *ctx)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto &t : ctx->timers) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    ctx->timers.clear();
+    ctx->prof_ms = 0.0;
+    ctx->prof_pairs = 0.0;
+    ctx->prof_launches = 0;
+    return VLM_OK;
+}
+
+int vlm_profile_get(vlm_ctx *ctx, double *bs_ms, int64_t *bs_launches, double *pairs)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    double ms = 0.0, pr = 0.0;
+    for (size_t i = 0; i < ctx->timers.size(); ++i) {
+        float e = 0.f;
+        VLM_HIP(hipEventElapsedTime(&e, ctx->timers[i].a, ctx->timers[i].b));
+        ms += e;
+        if (i < ctx->prof_cap && ctx->timers[i].pairs > 0)
+            pr += ctx->timers[i].pairs * (double)ctx->prof_nseg[i];
+    }
+    if (bs_ms) *bs_ms = ms;
+    if (bs_launches) *bs_launches = (int64_t)ctx->timers.size();
+    if (pairs) *pairs = pr;
+    return VLM_OK;
+}
+
+}  // extern "C"

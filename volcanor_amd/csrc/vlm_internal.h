@@ -1,0 +1,175 @@
+// Internal declarations of libvlm (MI355X free-wake hot path).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "vlm_c.h"
+
+namespace vlm {
+
+// Device buffer owned by the library.  reserve() grows and discards.
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t reserve(size_t k)
+    {
+        if (k <= n && p) return hipSuccess;
+        release();
+        if (k == 0) k = 1;
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), k * sizeof(T));
+        if (e == hipSuccess) n = k;
+        else p = nullptr;
+        return e;
+    }
+};
+
+// Compacted straight-segment list: structure of arrays.
+//   d64: ax ay az bx by bz g rc2       (g = gam/(4 pi), rc2 = rc^2)
+//   d32: ax ay az r0x r0y r0z g q2     (r0 = b - a, q2 = (rc^2 |r0|^2)^2)
+// count: device int64 with the number of valid segments.
+struct SegList {
+    DBuf<double> d64;
+    DBuf<float> d32;
+    DBuf<long long> count;
+    int64_t cap = 0;
+};
+
+// Staging slots written by the segment builders before compaction.
+struct Slot {
+    double ax, ay, az, bx, by, bz, gam, rc;
+};
+
+struct Rotor {
+    vlm_rotor_desc d{};
+    int rowNear = 0, rowFar = 0;  // 1-based, as in the reference
+    int head = 0;                 // physical slot of logical near-wake row 0
+    double shaft[3] = {0, 0, 1}, hub[3] = {0, 0, 0};
+    int nring = 0;                // nb*nc*ns wing rings
+    // wing (uploaded by the host after every rigid motion)
+    DBuf<double> wc, wrc, wg;     // 12*nring, 4*nring, nring
+    // near wake, physical rows
+    DBuf<double> node, pnode;     // 3*nb*nN*(ns+1)
+    DBuf<double> le, ple;         // 3*nb*(ns+1)
+    DBuf<double> gam, rc4, age;   // nb*nN*ns, 4*nb*nN*ns, nb*nN*ns
+    // far wake
+    DBuf<double> ffc, pffc;       // 6*nb*nF
+    DBuf<double> fgam, frc, frc0, fl0, flc, fage;  // nb*nF
+    DBuf<double> ftmp;            // shift staging, 11*nb*nF
+    // velocities, logical rows: [VLM_VEL, PREV, PRED, STEP]
+    DBuf<double> velN[4], velF[4];
+    size_t nvn() const { return 3u * d.nb * (size_t)d.nNwake * (d.ns + 1); }
+    size_t nvf() const { return 3u * d.nb * (size_t)(d.nFwake > 0 ? d.nFwake : 0); }
+};
+
+struct Timer {
+    hipEvent_t a, b;
+    double pairs;
+};
+
+struct FSet {
+    DBuf<double> d64;
+    DBuf<float> d32;
+    DBuf<long long> count;
+    int64_t n = 0;
+};
+
+struct PSet {
+    DBuf<double> p;   // SoA 3*m
+    DBuf<double> v;   // SoA 3*m
+    int64_t m = 0;
+};
+
+}  // namespace vlm
+
+struct vlm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int precision = VLM_FP64;
+    int bug_compat = 1;
+    std::string err;
+    std::vector<vlm::Rotor *> rotors;
+    std::vector<vlm::FSet *> fsets;
+    std::vector<vlm::PSet *> psets;
+    // scratch
+    vlm::SegList seg;
+    vlm::DBuf<vlm::Slot> slots;
+    vlm::DBuf<int> flags, idx;
+    vlm::DBuf<unsigned char> scan_tmp;
+    size_t scan_tmp_bytes = 0;
+    vlm::DBuf<double> tgt, out, part;
+    vlm::DBuf<double> hostio;  // device staging for host point lists
+    // profiling
+    int profile = 0;
+    std::vector<vlm::Timer> timers;
+    double prof_ms = 0.0, prof_pairs = 0.0;
+    long long *prof_nseg = nullptr;  // pinned, one slot per timed launch
+    size_t prof_cap = 0;
+    int64_t prof_launches = 0;
+    // dense solve state per rotor (AIC, LU, pivots)
+    std::vector<double *> aic, lu;
+    std::vector<int *> piv;
+    std::vector<int> aic_n;
+    vlm::DBuf<double> solve_tmp;
+};
+
+namespace vlm {
+
+int fail(vlm_ctx *ctx, int code, const std::string &msg);
+int hip_fail(vlm_ctx *ctx, hipError_t e, const char *what);
+
+#define VLM_HIP(call)                                          \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return vlm::hip_fail(ctx, e_, #call); \
+    } while (0)
+
+#define VLM_TRY(call)                 \
+    do {                              \
+        int rc_ = (call);             \
+        if (rc_ != VLM_OK) return rc_; \
+    } while (0)
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kInv4Pi = 0.25 / kPi;
+constexpr double kEps = 2.220446049250313080847263336181640625e-16;  // epsilon(1._dp)
+
+// Biot-Savart evaluation of a compacted segment list at n targets stored
+// as SoA (x at t[0..], y at t[tstride..], z at t[2*tstride..]).  Result
+// goes to out (SoA, same stride); accumulate adds into it.  nseg_max is an
+// upper bound of the device count used to size the launch.
+int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32,
+            int64_t segstride, const long long *nseg_dev, int64_t nseg_max,
+            const double *t, int64_t m, int64_t tstride, double *out,
+            int accumulate);
+
+// Compact staged slots (flags != 0) into ctx->seg.  nslots on host.
+int compact_slots(vlm_ctx *ctx, int64_t nslots);
+int compact_into(vlm_ctx *ctx, const Slot *slots, const int *flags,
+                 int64_t nslots, DBuf<double> &d64, DBuf<float> &d32,
+                 DBuf<long long> &count, int64_t cap);
+
+// Source-set builders write slots starting at slot offset off; each
+// returns the number of slots it used through *used.
+int stage_wing(vlm_ctx *ctx, Rotor &r, int set, int64_t off, int64_t *used);
+int stage_wake(vlm_ctx *ctx, Rotor &r, int pred, int64_t off, int64_t *used);
+int64_t wing_slots(const Rotor &r, int set);
+int64_t wake_slots(const Rotor &r);
+
+int ensure_slots(vlm_ctx *ctx, int64_t n);
+
+}  // namespace vlm

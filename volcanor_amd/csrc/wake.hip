@@ -1,0 +1,1391 @@
+// Device-resident free wake: rotor objects, segment builders, the N-body
+// wake pass and the fused wake-update passes (convection, continuity, age,
+// dissipation, strain, shed, rollup, shift, scheme algebra).
+//
+// HBM layout per rotor (see vlm_c.h for the exported lattice layout):
+//   * Near wake as a node lattice: node(b, p, j) with p the PHYSICAL row of
+//     a ring buffer; logical row i lives in p = (head + i) mod nNwake.  The
+//     reference shifts every near-wake ring down one row per step
+//     (rotor_shiftwake, classdef.f90:4481-4498); here the shift is a head
+//     move plus one row copy.  Shared ring corners exist once, so the
+//     corner copies of blade_wake_continuity (classdef.f90:1609-1702) are
+//     implicit.
+//   * Velocity arrays keep LOGICAL rows, exactly like velNwake/velNwake1 in
+//     the reference, which are not shifted with the wake.
+//   * Far wake: per-filament end points (fc1 = trailing end, fc2 = leading).
+//
+// Sources are built per pass as unique segments: the two ring filaments
+// that share a lattice edge are merged into one segment with net
+// circulation when their core radii are equal (the common case), otherwise
+// both are kept.  Rings with |gam| <= eps are skipped as in
+// blade_vind_bywake (classdef.f90:1450-1456).
+#include <cmath>
+#include <cstring>
+
+#include "vlm_internal.h"
+
+namespace vlm {
+
+int points_to_device(vlm_ctx *ctx, int64_t m, const double *p, double *soa, int64_t stride);
+int points_to_host(vlm_ctx *ctx, int64_t m, const double *soa, int64_t stride, double *p);
+
+namespace {
+
+unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+constexpr double EPS = kEps;
+
+struct RV {  // raw views of a rotor's device state
+    int nb, nc, ns, nN, nF, nbc, head, rn, rf;  // rn, rf 0-based
+    double *node, *pnode, *le, *ple, *gam, *rc4, *age;
+    double *ffc, *pffc, *fgam, *frc, *frc0, *fl0, *flc, *fage;
+    double *wc, *wrc, *wg;
+    double *vel[4], *velF[4];
+};
+
+RV view(Rotor &r)
+{
+    RV v;
+    v.nb = r.d.nb;
+    v.nc = r.d.nc;
+    v.ns = r.d.ns;
+    v.nN = r.d.nNwake;
+    v.nF = r.d.nFwake;
+    v.nbc = r.d.nbConvect;
+    v.head = r.head;
+    v.rn = r.rowNear - 1;
+    v.rf = r.rowFar - 1;
+    v.node = r.node.p;
+    v.pnode = r.pnode.p;
+    v.le = r.le.p;
+    v.ple = r.ple.p;
+    v.gam = r.gam.p;
+    v.rc4 = r.rc4.p;
+    v.age = r.age.p;
+    v.ffc = r.ffc.p;
+    v.pffc = r.pffc.p;
+    v.fgam = r.fgam.p;
+    v.frc = r.frc.p;
+    v.frc0 = r.frc0.p;
+    v.fl0 = r.fl0.p;
+    v.flc = r.flc.p;
+    v.fage = r.fage.p;
+    v.wc = r.wc.p;
+    v.wrc = r.wrc.p;
+    v.wg = r.wg.p;
+    for (int k = 0; k < 4; ++k) {
+        v.vel[k] = r.velN[k].p;
+        v.velF[k] = r.velF[k].p;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int phys(const RV &v, int i)
+{
+    int p = v.head + i;
+    return p >= v.nN ? p - v.nN : p;
+}
+__device__ __forceinline__ size_t nidx(const RV &v, int b, int p, int j)
+{
+    return ((size_t)b * v.nN + p) * (v.ns + 1) + j;
+}
+__device__ __forceinline__ size_t ridx(const RV &v, int b, int p, int j)
+{
+    return ((size_t)b * v.nN + p) * v.ns + j;
+}
+__device__ __forceinline__ size_t vidx(const RV &v, int b, int i, int j)
+{
+    return ((size_t)b * v.nN + i) * (v.ns + 1) + j;
+}
+
+__device__ __forceinline__ void put(Slot *s, int *f, const double *A, const double *B, double g,
+                                    double rc)
+{
+    Slot q;
+    q.ax = A[0];
+    q.ay = A[1];
+    q.az = A[2];
+    q.bx = B[0];
+    q.by = B[1];
+    q.bz = B[2];
+    q.gam = g;
+    q.rc = rc;
+    *s = q;
+    *f = g != 0.0 ? 1 : 0;
+}
+
+__device__ __forceinline__ double skipg(double g) { return fabs(g) > EPS ? g : 0.0; }
+
+// Wing rings (ring form).  set 0: four filaments per ring; set 1: bound
+// vortices vf2 + vf4, without vf2 of the trailing row
+// (blade_vind_bywing_boundVortices, classdef.f90:1376-1395).
+__global__ void stage_wing_kernel(RV v, int set, Slot *slots, int *flags)
+{
+    const int nring = v.nb * v.nc * v.ns;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nring) return;
+    const double *c = v.wc + 12 * (size_t)k;
+    const double *rc = v.wrc + 4 * (size_t)k;
+    const double g = v.wg[k];
+    if (set == 0) {
+        for (int f = 0; f < 4; ++f)
+            put(slots + 4 * (size_t)k + f, flags + 4 * (size_t)k + f, c + 3 * f,
+                c + 3 * ((f + 1) & 3), g, rc[f]);
+    } else {
+        const int ic = k % v.nc;
+        put(slots + 2 * (size_t)k, flags + 2 * (size_t)k, c + 3 * 3, c, g, rc[3]);
+        put(slots + 2 * (size_t)k + 1, flags + 2 * (size_t)k + 1, c + 3, c + 6,
+            ic < v.nc - 1 ? g : 0.0, rc[1]);
+    }
+}
+
+// Near- and far-wake unique segments of every blade.  Edge enumeration per
+// blade (R active rows, F active far filaments):
+//   [0, R*ns)            trailing spanwise edge of row i: T(i,j) -> T(i,j+1)
+//   [.., + ns)           leading edge of the newest row:  LE(j+1) -> LE(j)
+//   [.., + R*(ns+1))     streamwise edge: U(i,j) -> T(i,j)
+//   [.., + F)            far filament fc1 -> fc2
+// Two slots per edge.
+__global__ void stage_wake_kernel(RV v, int pred, int far_exists, int64_t per_blade,
+                                  Slot *slots, int *flags)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per_blade * v.nb) return;
+    const int b = (int)(e / per_blade);
+    int64_t k = e - (int64_t)b * per_blade;
+    Slot *s = slots + 2 * e;
+    int *f = flags + 2 * e;
+    f[0] = 0;
+    f[1] = 0;
+    const double *N = pred ? v.pnode : v.node;
+    const double *L = (pred ? v.ple : v.le) + 3 * (size_t)b * (v.ns + 1);
+    const double *FC = pred ? v.pffc : v.ffc;
+    const int R = v.nN - v.rn;
+    const int ns = v.ns;
+    if (k < (int64_t)R * ns) {
+        const int i = v.rn + (int)(k / ns), j = (int)(k % ns);
+        const int p = phys(v, i);
+        const double *A = N + 3 * nidx(v, b, p, j);
+        const double *B = N + 3 * nidx(v, b, p, j + 1);
+        const double ga = skipg(v.gam[ridx(v, b, p, j)]);
+        const double rca = v.rc4[4 * ridx(v, b, p, j) + 1];
+        double gb = 0.0, rcb = rca;
+        bool hasb = false;
+        if (i + 1 < v.nN) {
+            const int q = phys(v, i + 1);
+            gb = -skipg(v.gam[ridx(v, b, q, j)]);
+            rcb = v.rc4[4 * ridx(v, b, q, j) + 3];
+            hasb = true;
+        } else if (far_exists) {
+            // open horseshoe (classdef.f90:1458-1463): not |gam|-guarded
+            gb = -v.gam[ridx(v, b, p, j)];
+            rcb = rca;
+            hasb = true;
+        }
+        if (!hasb || rca == rcb) {
+            put(s, f, A, B, ga + gb, rca);
+        } else {
+            put(s, f, A, B, ga, rca);
+            put(s + 1, f + 1, A, B, gb, rcb);
+        }
+        return;
+    }
+    k -= (int64_t)R * ns;
+    if (k < ns) {
+        const int j = (int)k;
+        const int p = phys(v, v.rn);
+        put(s, f, L + 3 * (j + 1), L + 3 * j, skipg(v.gam[ridx(v, b, p, j)]),
+            v.rc4[4 * ridx(v, b, p, j) + 3]);
+        return;
+    }
+    k -= ns;
+    if (k < (int64_t)R * (ns + 1)) {
+        const int i = v.rn + (int)(k / (ns + 1)), j = (int)(k % (ns + 1));
+        const int p = phys(v, i);
+        const double *T = N + 3 * nidx(v, b, p, j);
+        const double *U = (i == v.rn) ? L + 3 * j : N + 3 * nidx(v, b, phys(v, i - 1), j);
+        double ga = 0.0, rca = 0.0, gb = 0.0, rcb = 0.0;
+        bool hasa = j < ns, hasb = j >= 1;
+        if (hasa) {
+            ga = skipg(v.gam[ridx(v, b, p, j)]);
+            rca = v.rc4[4 * ridx(v, b, p, j) + 0];
+        }
+        if (hasb) {
+            gb = -skipg(v.gam[ridx(v, b, p, j - 1)]);
+            rcb = v.rc4[4 * ridx(v, b, p, j - 1) + 2];
+        }
+        if (hasa && hasb && rca == rcb) {
+            put(s, f, U, T, ga + gb, rca);
+        } else {
+            if (hasa) put(s, f, U, T, ga, rca);
+            if (hasb) put(s + 1, f + 1, U, T, gb, rcb);
+        }
+        return;
+    }
+    k -= (int64_t)R * (ns + 1);
+    {
+        const int i = v.rf + (int)k;
+        const size_t q = (size_t)b * v.nF + i;
+        put(s, f, FC + 6 * q, FC + 6 * q + 3, skipg(v.fgam[q]), v.frc[q]);
+    }
+}
+
+// Targets: near-wake nodes of rows rn..nN-1 then far trailing ends, for
+// blades < nbConvect, written as SoA at t[off..].
+__global__ void gather_targets_kernel(RV v, int pred, double *t, int64_t stride, int64_t off)
+{
+    const int R = v.nN - v.rn, F = v.nF - v.rf;
+    const int64_t per = (int64_t)R * (v.ns + 1) + F;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * v.nbc) return;
+    const int b = (int)(e / per);
+    const int64_t k = e - b * per;
+    const double *P;
+    if (k < (int64_t)R * (v.ns + 1)) {
+        const int i = v.rn + (int)(k / (v.ns + 1)), j = (int)(k % (v.ns + 1));
+        P = (pred ? v.pnode : v.node) + 3 * nidx(v, b, phys(v, i), j);
+    } else {
+        const int i = v.rf + (int)(k - (int64_t)R * (v.ns + 1));
+        P = (pred ? v.pffc : v.ffc) + 6 * ((size_t)b * v.nF + i);
+    }
+    t[off + e] = P[0];
+    t[stride + off + e] = P[1];
+    t[2 * stride + off + e] = P[2];
+}
+
+__global__ void scatter_vel_kernel(RV v, int which, const double *o, int64_t stride, int64_t off)
+{
+    const int R = v.nN - v.rn, F = v.nF - v.rf;
+    const int64_t per = (int64_t)R * (v.ns + 1) + F;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * v.nbc) return;
+    const int b = (int)(e / per);
+    const int64_t k = e - b * per;
+    double *dst;
+    if (k < (int64_t)R * (v.ns + 1)) {
+        const int i = v.rn + (int)(k / (v.ns + 1)), j = (int)(k % (v.ns + 1));
+        dst = v.vel[which] + 3 * vidx(v, b, i, j);
+    } else {
+        const int i = v.rf + (int)(k - (int64_t)R * (v.ns + 1));
+        dst = v.velF[which] + 3 * ((size_t)b * v.nF + i);
+    }
+    dst[0] = o[off + e];
+    dst[1] = o[stride + off + e];
+    dst[2] = o[2 * stride + off + e];
+}
+
+// blade_convectwake (classdef.f90:1515-1575): nodes of logical rows
+// [i0, i1) move by velNwake*dt; far trailing ends of rows rf.. move by
+// velFwake*dt.  Both 'C' and 'P' use velNwake, as the reference does.
+__global__ void convect_kernel(RV v, int pred, int i0, int i1, double dt)
+{
+    const int rows = i1 - i0;
+    const int64_t nnear = (int64_t)rows * (v.ns + 1);
+    const int F = v.nF - v.rf;
+    const int64_t per = nnear + F;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * v.nbc) return;
+    const int b = (int)(e / per);
+    const int64_t k = e - b * per;
+    if (k < nnear) {
+        const int i = i0 + (int)(k / (v.ns + 1)), j = (int)(k % (v.ns + 1));
+        double *P = (pred ? v.pnode : v.node) + 3 * nidx(v, b, phys(v, i), j);
+        const double *V = v.vel[VLM_VEL] + 3 * vidx(v, b, i, j);
+        P[0] += V[0] * dt;
+        P[1] += V[1] * dt;
+        P[2] += V[2] * dt;
+    } else {
+        const int i = v.rf + (int)(k - nnear);
+        double *P = (pred ? v.pffc : v.ffc) + 6 * ((size_t)b * v.nF + i);
+        const double *V = v.velF[VLM_VEL] + 3 * ((size_t)b * v.nF + i);
+        P[0] += V[0] * dt;
+        P[1] += V[1] * dt;
+        P[2] += V[2] * dt;
+    }
+}
+
+// Far-wake continuity: fc2(i) <- fc1(i-1) for i = rf+1 .. nF-1.
+__global__ void far_continuity_kernel(RV v, int pred)
+{
+    const int F = v.nF - v.rf - 1;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (F <= 0 || e >= F * v.nbc) return;
+    const int b = e / F, i = v.rf + 1 + e % F;
+    double *FC = pred ? v.pffc : v.ffc;
+    const size_t q = (size_t)b * v.nF + i;
+    FC[6 * q + 3] = FC[6 * (q - 1) + 0];
+    FC[6 * q + 4] = FC[6 * (q - 1) + 1];
+    FC[6 * q + 5] = FC[6 * (q - 1) + 2];
+}
+
+struct RotMats {
+    double T[16][9];  // column-major getTransformAxis per blade
+};
+
+__device__ __forceinline__ void rotp(const double *T, const double *o, const double *p, double *q)
+{
+    const double x = p[0] - o[0], y = p[1] - o[1], z = p[2] - o[2];
+    q[0] = T[0] * x + T[3] * y + T[6] * z + o[0];
+    q[1] = T[1] * x + T[4] * y + T[7] * z + o[1];
+    q[2] = T[2] * x + T[5] * y + T[8] * z + o[2];
+}
+
+// Axisymmetric copy of blade 1's wake to blades 2..nb rotated by
+// 2 pi (ib-1)/nb about the shaft (rotor_convectwake, classdef.f90:4801-4823).
+__global__ void axisym_kernel(RV v, int pred, RotMats M, double ox, double oy, double oz)
+{
+    const int R = v.nN - v.rn, F = v.nF - v.rf;
+    const int64_t per = (int64_t)R * (v.ns + 1) + (v.ns + 1) + F + (int64_t)R * v.ns;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * (v.nb - 1)) return;
+    const int b = 1 + (int)(e / per);
+    int64_t k = e - (int64_t)(b - 1) * per;
+    const double o[3] = {ox, oy, oz};
+    const double *T = M.T[b];
+    if (k < (int64_t)R * (v.ns + 1)) {
+        const int i = v.rn + (int)(k / (v.ns + 1)), j = (int)(k % (v.ns + 1));
+        const int p = phys(v, i);
+        double *N = pred ? v.pnode : v.node;
+        rotp(T, o, N + 3 * nidx(v, 0, p, j), N + 3 * nidx(v, b, p, j));
+        return;
+    }
+    k -= (int64_t)R * (v.ns + 1);
+    if (k < v.ns + 1) {
+        double *L = pred ? v.ple : v.le;
+        rotp(T, o, L + 3 * k, L + 3 * ((size_t)b * (v.ns + 1) + k));
+        return;
+    }
+    k -= v.ns + 1;
+    if (k < F) {
+        const int i = v.rf + (int)k;
+        double *FC = pred ? v.pffc : v.ffc;
+        const size_t q0 = (size_t)i, qb = (size_t)b * v.nF + i;
+        rotp(T, o, FC + 6 * q0, FC + 6 * qb);
+        rotp(T, o, FC + 6 * q0 + 3, FC + 6 * qb + 3);
+        if (!pred) {
+            v.fgam[qb] = v.fgam[q0];
+            v.frc[qb] = v.frc[q0];
+            v.frc0[qb] = v.frc0[q0];
+            v.fl0[qb] = v.fl0[q0];
+            v.flc[qb] = v.flc[q0];
+            v.fage[qb] = v.fage[q0];
+        }
+        return;
+    }
+    k -= F;
+    if (!pred) {
+        const int i = v.rn + (int)(k / v.ns), j = (int)(k % v.ns);
+        const int p = phys(v, i);
+        const size_t r0 = ridx(v, 0, p, j), rb = ridx(v, b, p, j);
+        v.gam[rb] = v.gam[r0];
+        v.age[rb] = v.age[r0];
+        for (int q = 0; q < 4; ++q) v.rc4[4 * rb + q] = v.rc4[4 * r0 + q];
+    }
+}
+
+// rotor_assignshed (classdef.f90:4297-4325).  LE: shed line of row rowNear
+// from the wing trailing vortex ring corners 2/3, circulation of the wing
+// trailing row.  TE: trailing nodes of row max(rowNear-1, 1).
+__global__ void assignshed_kernel(RV v, int te, int row)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= v.nb * (v.ns + 1)) return;
+    const int b = e / (v.ns + 1), j = e % (v.ns + 1);
+    const int js = j < v.ns ? j : v.ns - 1;
+    const size_t ring = ((size_t)b * v.ns + js) * v.nc + (v.nc - 1);
+    const double *W = v.wc + 12 * ring + (j < v.ns ? 3 : 6);
+    const int p = phys(v, row);
+    double *dst = te ? v.node + 3 * nidx(v, b, p, j) : v.le + 3 * ((size_t)b * (v.ns + 1) + j);
+    dst[0] = W[0];
+    dst[1] = W[1];
+    dst[2] = W[2];
+    if (!te && j < v.ns) v.gam[ridx(v, b, p, j)] = v.wg[ring];
+}
+
+// rotor_age_wake (classdef.f90:4331-4354); ageAzimuthal is output-only.
+__global__ void age_kernel(RV v, double dt)
+{
+    const int R = v.nN - v.rn, F = v.nF - v.rf;
+    const int64_t per = (int64_t)R * v.ns + F;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * v.nb) return;
+    const int b = (int)(e / per);
+    const int64_t k = e - b * per;
+    if (k < (int64_t)R * v.ns) {
+        const int i = v.rn + (int)(k / v.ns), j = (int)(k % v.ns);
+        v.age[ridx(v, b, phys(v, i), j)] += dt;
+    } else {
+        v.fage[(size_t)b * v.nF + v.rf + (k - (int64_t)R * v.ns)] += dt;
+    }
+}
+
+// rotor_dissipate_wake (classdef.f90:4356-4408).  One lane per (blade,
+// span) column walks the rows, so the vf4 <- upstream vf2 copy sees the
+// grown vf2 exactly as the reference's loop order does.
+__global__ void dissipate_kernel(RV v, double inc, double decay)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < v.nb * v.ns) {
+        const int b = e / v.ns, j = e % v.ns;
+        for (int i = v.rn; i < v.nN; ++i) {
+            const size_t r = ridx(v, b, phys(v, i), j);
+            double *rc = v.rc4 + 4 * r;
+            rc[0] = sqrt(rc[0] * rc[0] + inc);
+            rc[2] = rc[0];
+            v.gam[r] = v.gam[r] * decay;
+        }
+        for (int i = v.rn; i < v.nN; ++i) {
+            double *rc = v.rc4 + 4 * ridx(v, b, phys(v, i), j);
+            rc[1] = sqrt(rc[1] * rc[1] + inc);
+        }
+        if (v.rn != v.nN - 1)
+            for (int i = v.rn + 1; i < v.nN; ++i)
+                v.rc4[4 * ridx(v, b, phys(v, i), j) + 3] =
+                    v.rc4[4 * ridx(v, b, phys(v, i - 1), j) + 1];
+    }
+    const int F = v.nF - v.rf;
+    if (F > 0 && e < v.nb * F) {
+        const size_t q = (size_t)(e / F) * v.nF + v.rf + e % F;
+        v.frc[q] = sqrt(v.frc[q] * v.frc[q] + inc);
+        v.fgam[q] = v.fgam[q] * decay;
+    }
+}
+
+// rotor_strain_wake (classdef.f90:4410-4422): far filaments only.
+__global__ void strain_kernel(RV v)
+{
+    const int F = v.nF - v.rf;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (F <= 0 || e >= v.nb * F) return;
+    const size_t q = (size_t)(e / F) * v.nF + v.rf + e % F;
+    const double *fc = v.ffc + 6 * q;
+    const double dx = fc[0] - fc[3], dy = fc[1] - fc[4], dz = fc[2] - fc[5];
+    v.flc[q] = sqrt(dx * dx + dy * dy + dz * dz);
+    v.frc[q] = v.frc0[q] * sqrt(v.fl0[q] / v.flc[q]);
+}
+
+// rotor_shiftFwake (classdef.f90:4500-4513) in two passes via staging.
+__global__ void far_stage_kernel(RV v, double *tmp)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= v.nb * v.nF) return;
+    double *t = tmp + 12 * (size_t)e;
+    for (int q = 0; q < 6; ++q) t[q] = v.ffc[6 * (size_t)e + q];
+    t[6] = v.fgam[e];
+    t[7] = v.frc[e];
+    t[8] = v.frc0[e];
+    t[9] = v.fl0[e];
+    t[10] = v.flc[e];
+    t[11] = v.fage[e];
+}
+
+__global__ void far_shift_kernel(RV v, const double *tmp)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= v.nb * v.nF) return;
+    const int i = e % v.nF;
+    if (i == 0) {
+        v.fage[e] = 0.0;
+        return;
+    }
+    const double *t = tmp + 12 * (size_t)(e - 1);
+    for (int q = 0; q < 6; ++q) v.ffc[6 * (size_t)e + q] = t[q];
+    v.fgam[e] = t[6];
+    v.frc[e] = t[7];
+    v.frc0[e] = t[8];
+    v.fl0[e] = t[9];
+    v.flc[e] = t[10];
+    v.fage[e] = t[11];
+}
+
+// rotor_rollup (classdef.f90:4515-4597): one lane per blade forms the
+// circulation-weighted centroid of the oldest near row and writes the new
+// far filament at rowFarNext (0-based fi).
+__global__ void rollup_kernel(RV v, int rs, int re, int rollup_min, int suppress, int fi)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= v.nb) return;
+    const int last = v.nN - 1;
+    const int p = phys(v, last);
+    const double *N = v.node;
+    const double *L = v.le + 3 * (size_t)b * (v.ns + 1);
+    // corner 4 of ring (last, j) = node(last-1, j+1) or the shed line
+    auto c4 = [&](int j) -> const double * {
+        return (last == v.rn) ? L + 3 * (j + 1) : N + 3 * nidx(v, b, phys(v, last - 1), j + 1);
+    };
+    double gamRollup = v.gam[ridx(v, b, p, v.ns - 1)];
+    double cLE[3] = {0, 0, 0}, cTE[3] = {0, 0, 0};
+    double radius = 0.0, gamSum = 0.0;
+    for (int j = rs; j <= re; ++j) {
+        const size_t r = ridx(v, b, p, j);
+        const double g = v.gam[r];
+        const double *A = c4(j);
+        const double *C3 = N + 3 * nidx(v, b, p, j + 1);
+        for (int d = 0; d < 3; ++d) {
+            cLE[d] += A[d] * g;
+            cTE[d] += C3[d] * g;
+        }
+        gamSum += g;
+        if (rollup_min) {
+            if (g < gamRollup) gamRollup = g;
+        } else {
+            if (g > gamRollup) gamRollup = g;
+        }
+        radius += v.rc4[4 * r + 2] * g;
+    }
+    const double ageRollup = v.age[ridx(v, b, p, v.ns - 1)];
+    if (fabs(gamSum) > EPS) {
+        for (int d = 0; d < 3; ++d) {
+            cLE[d] /= gamSum;
+            cTE[d] /= gamSum;
+        }
+        radius /= gamSum;
+    } else {
+        const double *C2 = N + 3 * nidx(v, b, p, re);
+        const double *C3 = N + 3 * nidx(v, b, p, re + 1);
+        for (int d = 0; d < 3; ++d) {
+            cLE[d] = C2[d];
+            cTE[d] = C3[d];
+        }
+        radius = v.rc4[4 * ridx(v, b, p, re) + 2];
+    }
+    if (suppress) gamRollup = 0.0;
+    if (v.nF > 0) {
+        const size_t q = (size_t)b * v.nF + fi;
+        double *fc = v.ffc + 6 * q;
+        for (int d = 0; d < 3; ++d) {
+            fc[3 + d] = cLE[d];
+            fc[d] = cTE[d];
+        }
+        v.fgam[q] = gamRollup;
+        v.fage[q] = ageRollup;
+        v.frc0[q] = radius;
+        v.frc[q] = radius;
+        const double dx = fc[0] - fc[3], dy = fc[1] - fc[4], dz = fc[2] - fc[5];
+        v.flc[q] = v.fl0[q] = sqrt(dx * dx + dy * dy + dz * dz);
+        if (fi + 1 < v.nF)
+            for (int d = 0; d < 3; ++d) v.ffc[6 * (q + 1) + 3 + d] = cTE[d];
+    }
+}
+
+// After the head moved back one slot: new logical row 0 <- row 1 (all ring
+// attributes), with age reset (rotor_shiftwake, classdef.f90:4488-4495).
+__global__ void shift_copy_kernel(RV v)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= v.nb * (v.ns + 1)) return;
+    const int b = e / (v.ns + 1), j = e % (v.ns + 1);
+    const int p0 = phys(v, 0), p1 = phys(v, 1);
+    const double *s = v.node + 3 * nidx(v, b, p1, j);
+    double *d = v.node + 3 * nidx(v, b, p0, j);
+    d[0] = s[0];
+    d[1] = s[1];
+    d[2] = s[2];
+    if (j < v.ns) {
+        const size_t r0 = ridx(v, b, p0, j), r1 = ridx(v, b, p1, j);
+        v.gam[r0] = v.gam[r1];
+        for (int q = 0; q < 4; ++q) v.rc4[4 * r0 + q] = v.rc4[4 * r1 + q];
+        v.age[r0] = 0.0;
+    }
+}
+
+// Predicted wake := current wake on the active rows (main.f90:1028-1035).
+__global__ void copy_pred_kernel(RV v)
+{
+    const int R = v.nN - v.rn, F = v.nF - v.rf;
+    const int64_t per = (int64_t)R * (v.ns + 1) + (v.ns + 1) + F;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * v.nbc) return;
+    const int b = (int)(e / per);
+    int64_t k = e - b * per;
+    if (k < (int64_t)R * (v.ns + 1)) {
+        const int i = v.rn + (int)(k / (v.ns + 1)), j = (int)(k % (v.ns + 1));
+        const size_t n = 3 * nidx(v, b, phys(v, i), j);
+        v.pnode[n] = v.node[n];
+        v.pnode[n + 1] = v.node[n + 1];
+        v.pnode[n + 2] = v.node[n + 2];
+        return;
+    }
+    k -= (int64_t)R * (v.ns + 1);
+    if (k < v.ns + 1) {
+        const size_t n = 3 * ((size_t)b * (v.ns + 1) + k);
+        v.ple[n] = v.le[n];
+        v.ple[n + 1] = v.le[n + 1];
+        v.ple[n + 2] = v.le[n + 2];
+        return;
+    }
+    k -= v.ns + 1;
+    const size_t q = 6 * ((size_t)b * v.nF + v.rf + k);
+    for (int d = 0; d < 6; ++d) v.pffc[q + d] = v.ffc[q + d];
+}
+
+// Scheme algebra over whole velocity arrays of blades < nbConvect, as the
+// reference applies it to whole arrays (main.f90:968-1115).
+//   op 0: AB2  step = 1.5 v - 0.5 v1; v1 = step; v = step
+//   op 1: AM2 predictor  step = v; v = 0.5 (3 v - v1)
+//   op 2: AM2 corrector  v = (pred + step) * 0.5
+//   op 3: history        v1 = step
+//   op 4: first step     v1 = v
+__global__ void scheme_kernel(double *v, double *v1, double *vp, double *vs, int64_t n, int op)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    switch (op) {
+    case 0: {
+        const double s = 0.5 * (3.0 * v[k] - v1[k]);
+        vs[k] = s;
+        v1[k] = s;
+        v[k] = s;
+        break;
+    }
+    case 1:
+        vs[k] = v[k];
+        v[k] = 0.5 * (3.0 * v[k] - v1[k]);
+        break;
+    case 2:
+        v[k] = (vp[k] + vs[k]) * 0.5;
+        break;
+    case 3:
+        v1[k] = vs[k];
+        break;
+    default:
+        v1[k] = v[k];
+        break;
+    }
+}
+
+// vel_order2_Nwake / vel_order2_Fwake (libCommon.f90:213-258) on the
+// active window, written to `out` (logical rows).
+__global__ void order2_kernel(const double *vn, const double *vnp1, double *out, int nbc,
+                              int rows, int stride, int cols, int base)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nbc * rows * cols * 3) return;
+    const int d = (int)(e % 3);
+    int64_t r = e / 3;
+    const int i = (int)(r % rows);
+    r /= rows;
+    const int j = (int)(r % cols);
+    const int b = (int)(r / cols);
+    auto at = [&](int ii) { return (((size_t)b * stride + base + ii) * cols + j) * 3 + d; };
+    double val;
+    if (i == 0 || i == rows - 1) val = (vnp1[at(i)] + vn[at(i)]) * 0.5;
+    else val = (vnp1[at(i)] + vnp1[at(i - 1)] + vn[at(i + 1)] + vn[at(i)]) * 0.25;
+    out[at(i)] = val;
+}
+
+__global__ void copy_window_kernel(const double *src, double *dst, int nbc, int rows, int stride,
+                                   int cols, int base)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nbc * rows * cols * 3) return;
+    const int d = (int)(e % 3);
+    int64_t r = e / 3;
+    const int i = (int)(r % rows);
+    r /= rows;
+    const int j = (int)(r % cols);
+    const int b = (int)(r / cols);
+    const size_t k = (((size_t)b * stride + base + i) * cols + j) * 3 + d;
+    dst[k] = src[k];
+}
+
+__global__ void fill_kernel(double *p, int64_t n, double val)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = val;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// host-side helpers
+// ---------------------------------------------------------------------
+int64_t wing_slots(const Rotor &r, int set)
+{
+    return (int64_t)r.nring * (set == VLM_SET_WING ? 4 : 2);
+}
+
+static int64_t wake_edges_per_blade(const Rotor &r)
+{
+    const int64_t R = r.d.nNwake - (r.rowNear - 1);
+    const int64_t F = r.d.nFwake - (r.rowFar - 1);
+    if (r.d.nNwake <= 0 || R <= 0) return F > 0 ? F : 0;
+    return R * r.d.ns + r.d.ns + R * (r.d.ns + 1) + (F > 0 ? F : 0);
+}
+
+int64_t wake_slots(const Rotor &r) { return 2 * wake_edges_per_blade(r) * r.d.nb; }
+
+int stage_wing(vlm_ctx *ctx, Rotor &r, int set, int64_t off, int64_t *used)
+{
+    *used = wing_slots(r, set);
+    if (*used == 0) return VLM_OK;
+    RV v = view(r);
+    hipLaunchKernelGGL(stage_wing_kernel, dim3(nblk(r.nring)), dim3(256), 0, ctx->stream, v,
+                       set, ctx->slots.p + off, ctx->flags.p + off);
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+int stage_wake(vlm_ctx *ctx, Rotor &r, int pred, int64_t off, int64_t *used)
+{
+    *used = 0;
+    if (r.d.nNwake <= 0) return VLM_OK;
+    const int64_t per = wake_edges_per_blade(r);
+    if (per <= 0) return VLM_OK;
+    if (r.rowNear - 1 >= r.d.nNwake) return VLM_OK;  // no active near-wake row yet
+    *used = 2 * per * r.d.nb;
+    RV v = view(r);
+    const int far_exists = r.rowFar <= r.d.nFwake ? 1 : 0;
+    hipLaunchKernelGGL(stage_wake_kernel, dim3(nblk(per * r.d.nb)), dim3(256), 0, ctx->stream,
+                       v, pred, far_exists, per, ctx->slots.p + off, ctx->flags.p + off);
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+namespace {
+
+int64_t target_count(const Rotor &r)
+{
+    if (r.d.nNwake <= 0) return 0;
+    const int64_t R = r.d.nNwake - (r.rowNear - 1);
+    const int64_t F = r.d.nFwake - (r.rowFar - 1);
+    if (R <= 0) return 0;
+    return (R * (r.d.ns + 1) + (F > 0 ? F : 0)) * r.d.nbConvect;
+}
+
+Rotor *get_rotor(vlm_ctx *ctx, int ir)
+{
+    if (!ctx || ir < 0 || ir >= (int)ctx->rotors.size()) return nullptr;
+    return ctx->rotors[ir];
+}
+
+// Build the source list for a set of (rotor, set) pairs.
+int build_sources(vlm_ctx *ctx, const std::vector<std::pair<int, int>> &src)
+{
+    int64_t total = 0;
+    for (auto &s : src) {
+        Rotor &r = *ctx->rotors[s.first];
+        if (s.second == VLM_SET_WING || s.second == VLM_SET_WING_BOUND)
+            total += wing_slots(r, s.second);
+        else if (r.rowNear - 1 < r.d.nNwake)
+            total += wake_slots(r);
+    }
+    VLM_TRY(ensure_slots(ctx, total > 0 ? total : 1));
+    int64_t off = 0, used = 0;
+    for (auto &s : src) {
+        Rotor &r = *ctx->rotors[s.first];
+        if (s.second == VLM_SET_WING || s.second == VLM_SET_WING_BOUND)
+            VLM_TRY(stage_wing(ctx, r, s.second, off, &used));
+        else
+            VLM_TRY(stage_wake(ctx, r, s.second == VLM_SET_WAKE_PRED, off, &used));
+        off += used;
+    }
+    VLM_TRY(compact_slots(ctx, off));
+    ctx->seg.cap = ctx->seg.cap;  // stride of the SoA arrays
+    return VLM_OK;
+}
+
+int eval_sources(vlm_ctx *ctx, const double *t, int64_t m, int64_t tstride, double *out, int acc,
+                 int64_t nmax)
+{
+    return bs_eval(ctx, ctx->seg.d64.p, ctx->seg.d32.p, ctx->seg.cap, ctx->seg.count.p, nmax, t,
+                   m, tstride, out, acc);
+}
+
+int64_t staged_upper_bound(vlm_ctx *ctx, const std::vector<std::pair<int, int>> &src)
+{
+    int64_t total = 0;
+    for (auto &s : src) {
+        Rotor &r = *ctx->rotors[s.first];
+        if (s.second == VLM_SET_WING || s.second == VLM_SET_WING_BOUND)
+            total += wing_slots(r, s.second);
+        else if (r.rowNear - 1 < r.d.nNwake)
+            total += wake_slots(r);
+    }
+    return total;
+}
+
+int launch_scheme(vlm_ctx *ctx, Rotor &r, int op)
+{
+    const int64_t nn = (int64_t)3 * r.d.nbConvect * r.d.nNwake * (r.d.ns + 1);
+    const int64_t nf = (int64_t)3 * r.d.nbConvect * r.d.nFwake;
+    if (nn > 0)
+        hipLaunchKernelGGL(scheme_kernel, dim3(nblk(nn)), dim3(256), 0, ctx->stream,
+                           r.velN[VLM_VEL].p, r.velN[VLM_VEL_PREV].p, r.velN[VLM_VEL_PRED].p,
+                           r.velN[VLM_VEL_STEP].p, nn, op);
+    if (nf > 0)
+        hipLaunchKernelGGL(scheme_kernel, dim3(nblk(nf)), dim3(256), 0, ctx->stream,
+                           r.velF[VLM_VEL].p, r.velF[VLM_VEL_PREV].p, r.velF[VLM_VEL_PRED].p,
+                           r.velF[VLM_VEL_STEP].p, nf, op);
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+// getTransformAxis, libMath.f90:695-726 (column-major)
+void transform_axis(double theta, const double *ax, double *T)
+{
+    const double n = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+    const double a0 = ax[0] / n, a1 = ax[1] / n, a2 = ax[2] / n;
+    const double ct = std::cos(theta), st = std::sin(theta), om = 1.0 - ct;
+    T[0] = ct + a0 * a0 * om;
+    T[1] = a2 * st + a1 * a0 * om;
+    T[2] = -a1 * st + a2 * a0 * om;
+    T[3] = -a2 * st + a0 * a1 * om;
+    T[4] = ct + a1 * a1 * om;
+    T[5] = a0 * st + a2 * a1 * om;
+    T[6] = a1 * st + a0 * a2 * om;
+    T[7] = -a0 * st + a1 * a2 * om;
+    T[8] = ct + a2 * a2 * om;
+}
+
+int copy_predicted(vlm_ctx *ctx, Rotor &r)
+{
+    const int64_t R = r.d.nNwake - (r.rowNear - 1);
+    const int64_t F = r.d.nFwake - (r.rowFar - 1);
+    const int64_t per = R * (r.d.ns + 1) + (r.d.ns + 1) + (F > 0 ? F : 0);
+    const int64_t n = per * r.d.nbConvect;
+    if (R <= 0 || n <= 0) return VLM_OK;
+    hipLaunchKernelGGL(copy_pred_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream, view(r));
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+}  // namespace
+
+int rotor_convect(vlm_ctx *ctx, Rotor &r, double dt, int pred)
+{
+    if (r.d.nNwake <= 0) return VLM_OK;
+    RV v = view(r);
+    int i0 = r.rowNear - 1, i1 = r.d.nNwake;
+    if (pred && ctx->bug_compat) {
+        i0 = 0;  // `do i = 1, rowNear, nNwake` runs for row 1 only
+        i1 = 1;
+    }
+    if (i0 >= r.d.nNwake) i1 = i0;  // nothing shed yet
+    const int64_t F = r.d.nFwake - (r.rowFar - 1);
+    const int64_t per = (int64_t)(i1 - i0) * (r.d.ns + 1) + (F > 0 ? F : 0);
+    if (per > 0) {
+        hipLaunchKernelGGL(convect_kernel, dim3(nblk(per * r.d.nbConvect)), dim3(256), 0,
+                           ctx->stream, v, pred, i0, i1, dt);
+        VLM_HIP(hipGetLastError());
+    }
+    if (F > 1) {
+        hipLaunchKernelGGL(far_continuity_kernel, dim3(nblk((F - 1) * r.d.nbConvect)), dim3(256),
+                           0, ctx->stream, v, pred);
+        VLM_HIP(hipGetLastError());
+    }
+    if (r.d.nbConvect == 1 && r.d.nb > 1) {
+        if (r.d.nb > 16) return fail(ctx, VLM_ERR_ARG, "axisymmetric copy supports <= 16 blades");
+        RotMats M;
+        std::memset(&M, 0, sizeof(M));
+        for (int b = 1; b < r.d.nb; ++b) {
+            const double theta = 2.0 * kPi / r.d.nb * b;
+            transform_axis(theta, r.shaft, M.T[b]);
+        }
+        const int64_t R = r.d.nNwake - (r.rowNear - 1);
+        const int64_t per2 = (R > 0 ? R : 0) * (r.d.ns + 1) + (r.d.ns + 1) + (F > 0 ? F : 0) +
+                             (R > 0 ? R : 0) * r.d.ns;
+        hipLaunchKernelGGL(axisym_kernel, dim3(nblk(per2 * (r.d.nb - 1))), dim3(256), 0,
+                           ctx->stream, v, pred, M, r.hub[0], r.hub[1], r.hub[2]);
+        VLM_HIP(hipGetLastError());
+    }
+    return VLM_OK;
+}
+
+int wake_pass(vlm_ctx *ctx, int pred)
+{
+    std::vector<std::pair<int, int>> src;
+    int64_t m = 0;
+    for (int ir = 0; ir < (int)ctx->rotors.size(); ++ir) {
+        src.push_back({ir, VLM_SET_WING});
+        src.push_back({ir, pred ? VLM_SET_WAKE_PRED : VLM_SET_WAKE});
+        m += target_count(*ctx->rotors[ir]);
+    }
+    if (m == 0) return VLM_OK;
+    const int64_t nmax = staged_upper_bound(ctx, src);
+    VLM_TRY(build_sources(ctx, src));
+    VLM_HIP(ctx->tgt.reserve((size_t)3 * m));
+    VLM_HIP(ctx->out.reserve((size_t)3 * m));
+    int64_t off = 0;
+    for (auto *rp : ctx->rotors) {
+        const int64_t n = target_count(*rp);
+        if (n == 0) continue;
+        hipLaunchKernelGGL(gather_targets_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+                           view(*rp), pred, ctx->tgt.p, m, off);
+        VLM_HIP(hipGetLastError());
+        off += n;
+    }
+    VLM_TRY(eval_sources(ctx, ctx->tgt.p, m, m, ctx->out.p, 0, nmax));
+    off = 0;
+    for (auto *rp : ctx->rotors) {
+        const int64_t n = target_count(*rp);
+        if (n == 0) continue;
+        hipLaunchKernelGGL(scatter_vel_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+                           view(*rp), pred ? VLM_VEL_PRED : VLM_VEL, ctx->out.p, m, off);
+        VLM_HIP(hipGetLastError());
+        off += n;
+    }
+    return VLM_OK;
+}
+
+}  // namespace vlm
+
+using namespace vlm;
+
+extern "C" {
+
+int vlm_rotor_create(vlm_ctx *ctx, const vlm_rotor_desc *d, double spanwiseCore,
+                     const double *scv, int *ir)
+{
+    if (!ctx || !d || !ir) return VLM_ERR_ARG;
+    if (d->nb <= 0 || d->nc <= 0 || d->ns <= 0 || d->nNwake < 0 || d->nFwake < 0)
+        return fail(ctx, VLM_ERR_ARG, "rotor sizes must be positive");
+    if (d->nbConvect != 1 && d->nbConvect != d->nb)
+        return fail(ctx, VLM_ERR_ARG, "nbConvect must be 1 or nb");
+    if (d->nNwake > 0 && d->nNwake < 2)
+        return fail(ctx, VLM_ERR_ARG, "at least 2 near wake rows are mandatory");
+    if (d->nNwake > 0 && (d->rollupStart < 1 || d->rollupEnd > d->ns || d->rollupStart > d->rollupEnd + 1))
+        return fail(ctx, VLM_ERR_ARG, "rollup span range outside 1..ns");
+    if (d->nNwake > 0 && !scv) return fail(ctx, VLM_ERR_ARG, "streamwiseCoreVec missing");
+    VLM_HIP(hipSetDevice(ctx->device));
+    Rotor *r = new Rotor();
+    r->d = *d;
+    r->nring = d->nb * d->nc * d->ns;
+    const size_t nb = d->nb, nN = d->nNwake, ns = d->ns, nF = d->nFwake;
+    hipError_t e = hipSuccess;
+    auto R = [&](DBuf<double> &b, size_t n) {
+        if (e == hipSuccess) e = b.reserve(n);
+    };
+    R(r->wc, 12 * (size_t)r->nring);
+    R(r->wrc, 4 * (size_t)r->nring);
+    R(r->wg, r->nring);
+    R(r->node, 3 * nb * nN * (ns + 1));
+    R(r->pnode, 3 * nb * nN * (ns + 1));
+    R(r->le, 3 * nb * (ns + 1));
+    R(r->ple, 3 * nb * (ns + 1));
+    R(r->gam, nb * nN * ns);
+    R(r->rc4, 4 * nb * nN * ns);
+    R(r->age, nb * nN * ns);
+    R(r->ffc, 6 * nb * nF);
+    R(r->pffc, 6 * nb * nF);
+    R(r->fgam, nb * nF);
+    R(r->frc, nb * nF);
+    R(r->frc0, nb * nF);
+    R(r->fl0, nb * nF);
+    R(r->flc, nb * nF);
+    R(r->fage, nb * nF);
+    R(r->ftmp, 12 * nb * nF);
+    for (int k = 0; k < 4; ++k) {
+        R(r->velN[k], r->nvn());
+        R(r->velF[k], r->nvf());
+    }
+    if (e != hipSuccess) {
+        delete r;
+        return hip_fail(ctx, e, "rotor allocation");
+    }
+    // zero everything (the reference zero-initialises every velocity array,
+    // classdef.f90:3737-3820, and all wake circulations, 3830-3831)
+    std::vector<DBuf<double> *> all = {&r->wc,   &r->wrc,  &r->wg,   &r->node, &r->pnode,
+                                       &r->le,   &r->ple,  &r->gam,  &r->rc4,  &r->age,
+                                       &r->ffc,  &r->pffc, &r->fgam, &r->frc,  &r->frc0,
+                                       &r->fl0,  &r->flc,  &r->fage, &r->ftmp};
+    for (int k = 0; k < 4; ++k) {
+        all.push_back(&r->velN[k]);
+        all.push_back(&r->velF[k]);
+    }
+    for (auto *b : all)
+        hipLaunchKernelGGL(fill_kernel, dim3(nblk((int64_t)b->n)), dim3(256), 0, ctx->stream,
+                           b->p, (int64_t)b->n, 0.0);
+    // initial core radii, classdef.f90:3826-3850
+    if (nN > 0) {
+        std::vector<double> rc4(4 * nb * nN * ns);
+        for (size_t b = 0; b < nb; ++b)
+            for (size_t i = 0; i < nN; ++i)
+                for (size_t j = 0; j < ns; ++j) {
+                    double *q = &rc4[4 * ((b * nN + i) * ns + j)];
+                    q[0] = scv[j];
+                    q[1] = spanwiseCore;
+                    q[2] = scv[j + 1];
+                    q[3] = spanwiseCore;
+                }
+        VLM_HIP(hipMemcpyAsync(r->rc4.p, rc4.data(), rc4.size() * sizeof(double),
+                               hipMemcpyHostToDevice, ctx->stream));
+        if (nF > 0) {
+            std::vector<double> frc(nb * nF, scv[ns]);
+            VLM_HIP(hipMemcpyAsync(r->frc.p, frc.data(), frc.size() * sizeof(double),
+                                   hipMemcpyHostToDevice, ctx->stream));
+            VLM_HIP(hipMemcpyAsync(r->frc0.p, frc.data(), frc.size() * sizeof(double),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        }
+        VLM_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    r->rowNear = (int)nN + 1;
+    r->rowFar = (int)nF + 1;
+    ctx->rotors.push_back(r);
+    ctx->aic.push_back(nullptr);
+    ctx->lu.push_back(nullptr);
+    ctx->piv.push_back(nullptr);
+    ctx->aic_n.push_back(0);
+    *ir = (int)ctx->rotors.size() - 1;
+    return VLM_OK;
+}
+
+int vlm_rotor_set_wing(vlm_ctx *ctx, int ir, const double *corners, const double *rc,
+                       const double *gam)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (corners)
+        VLM_HIP(hipMemcpyAsync(r->wc.p, corners, sizeof(double) * 12 * r->nring,
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (rc)
+        VLM_HIP(hipMemcpyAsync(r->wrc.p, rc, sizeof(double) * 4 * r->nring,
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (gam)
+        VLM_HIP(hipMemcpyAsync(r->wg.p, gam, sizeof(double) * r->nring, hipMemcpyHostToDevice,
+                               ctx->stream));
+    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    return VLM_OK;
+}
+
+int vlm_rotor_set_axis(vlm_ctx *ctx, int ir, const double *shaft, const double *hub)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r || !shaft || !hub) return ctx ? fail(ctx, VLM_ERR_ARG, "bad rotor axis") : VLM_ERR_ARG;
+    std::memcpy(r->shaft, shaft, sizeof(r->shaft));
+    std::memcpy(r->hub, hub, sizeof(r->hub));
+    return VLM_OK;
+}
+
+int vlm_rotor_init_cursors(vlm_ctx *ctx, int ir)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    r->rowFar = r->d.nFwake + 1;
+    r->rowNear = r->d.nNwake + 1;
+    return VLM_OK;
+}
+
+int vlm_rotor_begin_step(vlm_ctx *ctx, int ir, int iter)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    r->rowNear = r->rowNear - 1 > 1 ? r->rowNear - 1 : 1;
+    if (iter > r->d.nNwake) r->rowFar = r->rowFar - 1 > 1 ? r->rowFar - 1 : 1;
+    return VLM_OK;
+}
+
+int vlm_rotor_cursors(vlm_ctx *ctx, int ir, int *rowNear, int *rowFar)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (rowNear) *rowNear = r->rowNear;
+    if (rowFar) *rowFar = r->rowFar;
+    return VLM_OK;
+}
+
+int vlm_rotor_assignshed(vlm_ctx *ctx, int ir, int edge)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (r->d.nNwake <= 0) return VLM_OK;
+    if (edge != VLM_EDGE_LE && edge != VLM_EDGE_TE)
+        return fail(ctx, VLM_ERR_ARG, "edge must be VLM_EDGE_LE or VLM_EDGE_TE");
+    int row;
+    if (edge == VLM_EDGE_LE) {
+        if (r->rowNear > r->d.nNwake) return fail(ctx, VLM_ERR_STATE, "assignshed LE before first step");
+        row = r->rowNear - 1;
+    } else {
+        row = (r->rowNear - 1 > 1 ? r->rowNear - 1 : 1) - 1;
+    }
+    hipLaunchKernelGGL(assignshed_kernel, dim3(nblk((int64_t)r->d.nb * (r->d.ns + 1))), dim3(256),
+                       0, ctx->stream, view(*r), edge == VLM_EDGE_TE ? 1 : 0, row);
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+int vlm_rotor_age_wake(vlm_ctx *ctx, int ir, double dt)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (r->d.nNwake <= 0 || r->rowNear > r->d.nNwake) return VLM_OK;
+    const int64_t R = r->d.nNwake - (r->rowNear - 1);
+    const int64_t F = r->d.nFwake - (r->rowFar - 1);
+    const int64_t n = (R * r->d.ns + (F > 0 ? F : 0)) * r->d.nb;
+    hipLaunchKernelGGL(age_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream, view(*r), dt);
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+int vlm_rotor_dissipate_wake(vlm_ctx *ctx, int ir, double dt, double kinematicVisc,
+                             double apparentViscCoeff, double decayCoeff)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (r->d.nNwake <= 0 || r->rowNear > r->d.nNwake) return VLM_OK;
+    const double oseen = 1.2564;
+    const double inc = 4.0 * oseen * apparentViscCoeff * kinematicVisc * dt;
+    const double decay = std::exp(-decayCoeff * dt);
+    const int64_t F = r->d.nFwake - (r->rowFar - 1);
+    const int64_t n = std::max<int64_t>((int64_t)r->d.nb * r->d.ns, (F > 0 ? F : 0) * r->d.nb);
+    hipLaunchKernelGGL(dissipate_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream, view(*r), inc,
+                       decay);
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+int vlm_rotor_strain_wake(vlm_ctx *ctx, int ir)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    const int64_t F = r->d.nFwake - (r->rowFar - 1);
+    if (r->d.nNwake <= 0 || F <= 0) return VLM_OK;
+    hipLaunchKernelGGL(strain_kernel, dim3(nblk(F * r->d.nb)), dim3(256), 0, ctx->stream,
+                       view(*r));
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+int vlm_rotor_rollup(vlm_ctx *ctx, int ir)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (r->d.nNwake <= 0) return VLM_OK;
+    int fi = r->rowFar - 2;  // rowFarNext - 1, 0-based
+    if (r->d.nFwake > 0 && fi < 0) {
+        const int64_t n = (int64_t)r->d.nb * r->d.nFwake;
+        hipLaunchKernelGGL(far_stage_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream, view(*r),
+                           r->ftmp.p);
+        hipLaunchKernelGGL(far_shift_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream, view(*r),
+                           r->ftmp.p);
+        VLM_HIP(hipGetLastError());
+        fi = 0;
+    }
+    hipLaunchKernelGGL(rollup_kernel, dim3(1), dim3(64), 0, ctx->stream, view(*r),
+                       r->d.rollupStart - 1, r->d.rollupEnd - 1, r->d.rollupMin,
+                       r->d.suppressFwake, fi);
+    VLM_HIP(hipGetLastError());
+    // shiftwake: logical row i+1 <- row i, new row 0 duplicates row 1
+    r->head = (r->head + r->d.nNwake - 1) % r->d.nNwake;
+    hipLaunchKernelGGL(shift_copy_kernel, dim3(nblk((int64_t)r->d.nb * (r->d.ns + 1))), dim3(256),
+                       0, ctx->stream, view(*r));
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+int vlm_rotor_convectwake(vlm_ctx *ctx, int ir, double dt, int wakeType)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (wakeType != VLM_WAKE_C && wakeType != VLM_WAKE_P)
+        return fail(ctx, VLM_ERR_ARG, "wakeType must be VLM_WAKE_C or VLM_WAKE_P");
+    return rotor_convect(ctx, *r, dt, wakeType == VLM_WAKE_P);
+}
+
+int vlm_rotor_induced_velocity(vlm_ctx *ctx, int ir, int set, int64_t m, const double *p,
+                               double *v, int accumulate)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (set < VLM_SET_WING || set > VLM_SET_WAKE_PRED) return fail(ctx, VLM_ERR_ARG, "bad set");
+    if (m <= 0) return VLM_OK;
+    if (!p || !v) return fail(ctx, VLM_ERR_ARG, "null buffer");
+    std::vector<std::pair<int, int>> src = {{ir, set}};
+    const int64_t nmax = staged_upper_bound(ctx, src);
+    VLM_TRY(build_sources(ctx, src));
+    VLM_HIP(ctx->tgt.reserve((size_t)3 * m));
+    VLM_HIP(ctx->out.reserve((size_t)3 * m));
+    VLM_TRY(points_to_device(ctx, m, p, ctx->tgt.p, m));
+    if (accumulate) VLM_TRY(points_to_device(ctx, m, v, ctx->out.p, m));
+    VLM_TRY(eval_sources(ctx, ctx->tgt.p, m, m, ctx->out.p, accumulate, nmax));
+    return points_to_host(ctx, m, ctx->out.p, m, v);
+}
+
+int vlm_wake_velocity(vlm_ctx *ctx, int wakeType)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    if (wakeType != VLM_WAKE_C && wakeType != VLM_WAKE_P)
+        return fail(ctx, VLM_ERR_ARG, "wakeType must be VLM_WAKE_C or VLM_WAKE_P");
+    return wake_pass(ctx, wakeType == VLM_WAKE_P);
+}
+
+int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    if (scheme < VLM_SCHEME_FT1 || scheme > VLM_SCHEME_AM2)
+        return fail(ctx, VLM_ERR_ARG, "fdScheme must be 0..3");
+    VLM_TRY(wake_pass(ctx, 0));
+    auto active = [](Rotor *r) { return r->d.nNwake > 0; };
+    if (scheme == VLM_SCHEME_FT1) {
+        for (auto *r : ctx->rotors)
+            if (active(r)) VLM_TRY(rotor_convect(ctx, *r, dt, 0));
+        return VLM_OK;
+    }
+    if (scheme == VLM_SCHEME_PC2) {
+        for (auto *r : ctx->rotors)
+            if (active(r)) {
+                VLM_TRY(copy_predicted(ctx, *r));
+                VLM_TRY(rotor_convect(ctx, *r, dt, 1));
+            }
+        VLM_TRY(wake_pass(ctx, 1));
+        for (auto *r : ctx->rotors) {
+            if (!active(r)) continue;
+            const int rows = r->d.nNwake - (r->rowNear - 1);
+            const int frows = r->d.nFwake - (r->rowFar - 1);
+            const int nbc = r->d.nbConvect;
+            if (rows > 0) {
+                const int64_t n = (int64_t)nbc * rows * (r->d.ns + 1) * 3;
+                hipLaunchKernelGGL(order2_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+                                   r->velN[VLM_VEL].p, r->velN[VLM_VEL_PRED].p,
+                                   r->velN[VLM_VEL_STEP].p, nbc, rows, r->d.nNwake, r->d.ns + 1,
+                                   r->rowNear - 1);
+                hipLaunchKernelGGL(copy_window_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+                                   r->velN[VLM_VEL_STEP].p, r->velN[VLM_VEL].p, nbc, rows,
+                                   r->d.nNwake, r->d.ns + 1, r->rowNear - 1);
+            }
+            if (frows > 0) {
+                const int64_t n = (int64_t)nbc * frows * 3;
+                hipLaunchKernelGGL(order2_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+                                   r->velF[VLM_VEL].p, r->velF[VLM_VEL_PRED].p,
+                                   r->velF[VLM_VEL_STEP].p, nbc, frows, r->d.nFwake, 1,
+                                   r->rowFar - 1);
+                hipLaunchKernelGGL(copy_window_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+                                   r->velF[VLM_VEL_STEP].p, r->velF[VLM_VEL].p, nbc, frows,
+                                   r->d.nFwake, 1, r->rowFar - 1);
+            }
+            VLM_HIP(hipGetLastError());
+            VLM_TRY(rotor_convect(ctx, *r, dt, 0));
+        }
+        return VLM_OK;
+    }
+    if (iter == 1) {
+        for (auto *r : ctx->rotors)
+            if (active(r)) {
+                VLM_TRY(rotor_convect(ctx, *r, dt, 0));
+                VLM_TRY(launch_scheme(ctx, *r, 4));
+            }
+        return VLM_OK;
+    }
+    if (scheme == VLM_SCHEME_AB2) {
+        for (auto *r : ctx->rotors)
+            if (active(r)) {
+                VLM_TRY(launch_scheme(ctx, *r, 0));
+                VLM_TRY(rotor_convect(ctx, *r, dt, 0));
+            }
+        return VLM_OK;
+    }
+    // AM2
+    for (auto *r : ctx->rotors)
+        if (active(r)) {
+            VLM_TRY(copy_predicted(ctx, *r));
+            VLM_TRY(launch_scheme(ctx, *r, 1));
+            VLM_TRY(rotor_convect(ctx, *r, dt, 1));
+        }
+    VLM_TRY(wake_pass(ctx, 1));
+    for (auto *r : ctx->rotors)
+        if (active(r)) {
+            VLM_TRY(launch_scheme(ctx, *r, 2));
+            VLM_TRY(rotor_convect(ctx, *r, dt, 0));
+            VLM_TRY(launch_scheme(ctx, *r, 3));
+        }
+    return VLM_OK;
+}
+
+int vlm_rotor_get_wake(vlm_ctx *ctx, int ir, int wakeType, double *nodes, double *le,
+                       double *gam, double *rc4, double *far_fc, double *far_gam, double *far_rc)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    const int pred = wakeType == VLM_WAKE_P;
+    const size_t nb = r->d.nb, nN = r->d.nNwake, ns = r->d.ns, nF = r->d.nFwake;
+    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<double> tn(3 * nb * nN * (ns + 1)), tg(nb * nN * ns), tr(4 * nb * nN * ns);
+    if (nN > 0) {
+        VLM_HIP(hipMemcpy(tn.data(), pred ? r->pnode.p : r->node.p, tn.size() * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        VLM_HIP(hipMemcpy(tg.data(), r->gam.p, tg.size() * sizeof(double), hipMemcpyDeviceToHost));
+        VLM_HIP(hipMemcpy(tr.data(), r->rc4.p, tr.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t b = 0; b < nb; ++b)
+            for (size_t i = 0; i < nN; ++i) {
+                const size_t p = (r->head + i) % nN;
+                if (nodes)
+                    std::memcpy(nodes + 3 * (b * nN + i) * (ns + 1), &tn[3 * (b * nN + p) * (ns + 1)],
+                                3 * (ns + 1) * sizeof(double));
+                if (gam)
+                    std::memcpy(gam + (b * nN + i) * ns, &tg[(b * nN + p) * ns], ns * sizeof(double));
+                if (rc4)
+                    std::memcpy(rc4 + 4 * (b * nN + i) * ns, &tr[4 * (b * nN + p) * ns],
+                                4 * ns * sizeof(double));
+            }
+        if (le)
+            VLM_HIP(hipMemcpy(le, pred ? r->ple.p : r->le.p, 3 * nb * (ns + 1) * sizeof(double),
+                              hipMemcpyDeviceToHost));
+    }
+    if (nF > 0) {
+        if (far_fc)
+            VLM_HIP(hipMemcpy(far_fc, pred ? r->pffc.p : r->ffc.p, 6 * nb * nF * sizeof(double),
+                              hipMemcpyDeviceToHost));
+        if (far_gam)
+            VLM_HIP(hipMemcpy(far_gam, r->fgam.p, nb * nF * sizeof(double), hipMemcpyDeviceToHost));
+        if (far_rc)
+            VLM_HIP(hipMemcpy(far_rc, r->frc.p, nb * nF * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return VLM_OK;
+}
+
+int vlm_rotor_set_wake(vlm_ctx *ctx, int ir, const double *nodes, const double *le,
+                       const double *gam, const double *rc4, const double *far_fc,
+                       const double *far_gam, const double *far_rc, int rowNear, int rowFar)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    const size_t nb = r->d.nb, nN = r->d.nNwake, ns = r->d.ns, nF = r->d.nFwake;
+    if (rowNear < 1 || rowNear > (int)nN + 1 || rowFar < 1 || rowFar > (int)nF + 1)
+        return fail(ctx, VLM_ERR_ARG, "cursor out of range");
+    if (nN > 0 && (!nodes || !le || !gam || !rc4)) return fail(ctx, VLM_ERR_ARG, "null buffer");
+    if (nF > 0 && (!far_fc || !far_gam || !far_rc)) return fail(ctx, VLM_ERR_ARG, "null buffer");
+    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    r->head = 0;
+    r->rowNear = rowNear;
+    r->rowFar = rowFar;
+    if (nN > 0) {
+        std::vector<double> age(nb * nN * ns, 0.0);
+        VLM_HIP(hipMemcpy(r->node.p, nodes, 3 * nb * nN * (ns + 1) * sizeof(double),
+                          hipMemcpyHostToDevice));
+        VLM_HIP(hipMemcpy(r->le.p, le, 3 * nb * (ns + 1) * sizeof(double), hipMemcpyHostToDevice));
+        VLM_HIP(hipMemcpy(r->gam.p, gam, nb * nN * ns * sizeof(double), hipMemcpyHostToDevice));
+        VLM_HIP(hipMemcpy(r->rc4.p, rc4, 4 * nb * nN * ns * sizeof(double), hipMemcpyHostToDevice));
+        VLM_HIP(hipMemcpy(r->age.p, age.data(), age.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (nF > 0) {
+        std::vector<double> l(nb * nF), z(nb * nF, 0.0);
+        for (size_t k = 0; k < nb * nF; ++k) {
+            const double *f = far_fc + 6 * k;
+            const double dx = f[0] - f[3], dy = f[1] - f[4], dz = f[2] - f[5];
+            l[k] = std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+        VLM_HIP(hipMemcpy(r->ffc.p, far_fc, 6 * nb * nF * sizeof(double), hipMemcpyHostToDevice));
+        VLM_HIP(hipMemcpy(r->fgam.p, far_gam, nb * nF * sizeof(double), hipMemcpyHostToDevice));
+        VLM_HIP(hipMemcpy(r->frc.p, far_rc, nb * nF * sizeof(double), hipMemcpyHostToDevice));
+        VLM_HIP(hipMemcpy(r->frc0.p, far_rc, nb * nF * sizeof(double), hipMemcpyHostToDevice));
+        VLM_HIP(hipMemcpy(r->fl0.p, l.data(), nb * nF * sizeof(double), hipMemcpyHostToDevice));
+        VLM_HIP(hipMemcpy(r->flc.p, l.data(), nb * nF * sizeof(double), hipMemcpyHostToDevice));
+        VLM_HIP(hipMemcpy(r->fage.p, z.data(), nb * nF * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return VLM_OK;
+}
+
+int vlm_rotor_get_velocity(vlm_ctx *ctx, int ir, int which, double *velN, double *velF)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (which < VLM_VEL || which > VLM_VEL_STEP) return fail(ctx, VLM_ERR_ARG, "bad which");
+    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    if (velN && r->nvn() > 0)
+        VLM_HIP(hipMemcpy(velN, r->velN[which].p, r->nvn() * sizeof(double), hipMemcpyDeviceToHost));
+    if (velF && r->nvf() > 0)
+        VLM_HIP(hipMemcpy(velF, r->velF[which].p, r->nvf() * sizeof(double), hipMemcpyDeviceToHost));
+    return VLM_OK;
+}
+
+}  // extern "C"

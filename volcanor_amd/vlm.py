@@ -1,0 +1,320 @@
+"""ctypes binding of libvlm.so (include/vlm_c.h) for Python callers.
+
+The product path is the HIP library; this module only marshals numpy
+arrays across the C ABI.  It raises when the library is missing instead of
+falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libvlm.so")
+
+VLM_OK = 0
+FP64, MIXED = 0, 1
+SET_WING, SET_WING_BOUND, SET_WAKE, SET_WAKE_PRED = 0, 1, 2, 3
+WAKE_C, WAKE_P = 0, 1
+EDGE_LE, EDGE_TE = 0, 1
+SCHEME_FT1, SCHEME_PC2, SCHEME_AB2, SCHEME_AM2 = 0, 1, 2, 3
+VEL, VEL_PREV, VEL_PRED, VEL_STEP = 0, 1, 2, 3
+
+
+class RotorDesc(C.Structure):
+    _fields_ = [
+        ("nb", C.c_int), ("nc", C.c_int), ("ns", C.c_int),
+        ("nNwake", C.c_int), ("nFwake", C.c_int), ("nbConvect", C.c_int),
+        ("rollupStart", C.c_int), ("rollupEnd", C.c_int),
+        ("rollupMin", C.c_int), ("suppressFwake", C.c_int),
+    ]
+
+
+class VLMError(RuntimeError):
+    pass
+
+
+_P = C.c_void_p
+_I = C.c_int
+_L = C.c_int64
+_D = C.c_double
+
+_SIGS = {
+    "vlm_create": [C.POINTER(_P), _I],
+    "vlm_destroy": [_P],
+    "vlm_set_precision": [_P, _I],
+    "vlm_set_bug_compat": [_P, _I],
+    "vlm_synchronize": [_P],
+    "vlm_filament_vind": [_P, _L, _P, _P, _P, _P, _L, _P, _P, _I],
+    "vlm_fset_create": [_P, _L, _P, _P, _P, _P, C.POINTER(_I)],
+    "vlm_pset_create": [_P, _L, _P, C.POINTER(_I)],
+    "vlm_fset_vind": [_P, _I, _I],
+    "vlm_pset_velocity": [_P, _I, _P],
+    "vlm_rotor_create": [_P, C.POINTER(RotorDesc), _D, _P, C.POINTER(_I)],
+    "vlm_rotor_set_wing": [_P, _I, _P, _P, _P],
+    "vlm_rotor_set_axis": [_P, _I, _P, _P],
+    "vlm_rotor_init_cursors": [_P, _I],
+    "vlm_rotor_begin_step": [_P, _I, _I],
+    "vlm_rotor_cursors": [_P, _I, C.POINTER(_I), C.POINTER(_I)],
+    "vlm_rotor_assignshed": [_P, _I, _I],
+    "vlm_rotor_age_wake": [_P, _I, _D],
+    "vlm_rotor_dissipate_wake": [_P, _I, _D, _D, _D, _D],
+    "vlm_rotor_strain_wake": [_P, _I],
+    "vlm_rotor_rollup": [_P, _I],
+    "vlm_rotor_convectwake": [_P, _I, _D, _I],
+    "vlm_rotor_induced_velocity": [_P, _I, _I, _L, _P, _P, _I],
+    "vlm_wake_velocity": [_P, _I],
+    "vlm_convect": [_P, _I, _I, _D],
+    "vlm_rotor_get_wake": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P],
+    "vlm_rotor_set_wake": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _I],
+    "vlm_rotor_get_velocity": [_P, _I, _I, _P, _P],
+    "vlm_rotor_calc_aic": [_P, _I, _P, _P, _P],
+    "vlm_rotor_solve": [_P, _I, _P, _P, _I, _D, C.POINTER(_I)],
+    "vlm_lu_factor": [_P, _I, _P, _P],
+    "vlm_lu_solve_refine": [_P, _I, _P, _P, _P, _P, _P, _I, _D, C.POINTER(_I)],
+    "vlm_profile_enable": [_P, _I],
+    "vlm_profile_reset": [_P],
+    "vlm_profile_get": [_P, C.POINTER(_D), C.POINTER(_L), C.POINTER(_D)],
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libvlm.so and declare every entry point; raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise VLMError(f"libvlm.so not built at {path}; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.vlm_last_error.argtypes = [_P]
+    lib.vlm_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Context:
+    """One HIP device context (vlm_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.vlm_create(C.byref(h), device)
+        if rc != VLM_OK:
+            raise VLMError(f"vlm_create failed with code {rc}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vlm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != VLM_OK:
+            msg = self.lib.vlm_last_error(self.h)
+            raise VLMError(f"code {rc}: {msg.decode() if msg else ''}")
+
+    # -- configuration -------------------------------------------------
+    def set_precision(self, p):
+        self._chk(self.lib.vlm_set_precision(self.h, p))
+
+    def set_bug_compat(self, on):
+        self._chk(self.lib.vlm_set_bug_compat(self.h, int(on)))
+
+    def synchronize(self):
+        self._chk(self.lib.vlm_synchronize(self.h))
+
+    # -- explicit filament lists -----------------------------------------
+    def filament_vind(self, a, b, gam, rc, p, v=None):
+        a, b = _f64(a, (-1, 3)), _f64(b, (-1, 3))
+        gam, rc = _f64(gam).ravel(), _f64(rc).ravel()
+        p = _f64(p, (-1, 3))
+        acc = v is not None
+        out = _f64(v, (-1, 3)).copy() if acc else np.zeros_like(p)
+        self._chk(self.lib.vlm_filament_vind(self.h, a.shape[0], _ptr(a), _ptr(b), _ptr(gam),
+                                             _ptr(rc), p.shape[0], _ptr(p), _ptr(out), int(acc)))
+        return out
+
+    def fset_create(self, a, b, gam, rc):
+        a, b = _f64(a, (-1, 3)), _f64(b, (-1, 3))
+        gam, rc = _f64(gam).ravel(), _f64(rc).ravel()
+        k = C.c_int()
+        self._chk(self.lib.vlm_fset_create(self.h, a.shape[0], _ptr(a), _ptr(b), _ptr(gam),
+                                           _ptr(rc), C.byref(k)))
+        return k.value
+
+    def pset_create(self, p):
+        p = _f64(p, (-1, 3))
+        k = C.c_int()
+        self._chk(self.lib.vlm_pset_create(self.h, p.shape[0], _ptr(p), C.byref(k)))
+        return k.value, p.shape[0]
+
+    def fset_vind(self, fset, pset):
+        self._chk(self.lib.vlm_fset_vind(self.h, fset, pset))
+
+    def pset_velocity(self, pset, m):
+        v = np.zeros((m, 3))
+        self._chk(self.lib.vlm_pset_velocity(self.h, pset, _ptr(v)))
+        return v
+
+    # -- rotors ----------------------------------------------------------
+    def rotor_create(self, nb, nc, ns, nNwake, nFwake, nbConvect=None, rollupStart=1,
+                     rollupEnd=None, rollupMin=1, suppressFwake=0, spanwiseCore=0.0,
+                     streamwiseCoreVec=None):
+        d = RotorDesc(nb, nc, ns, nNwake, nFwake, nb if nbConvect is None else nbConvect,
+                      rollupStart, ns if rollupEnd is None else rollupEnd, rollupMin,
+                      suppressFwake)
+        scv = _f64(np.zeros(ns + 1) if streamwiseCoreVec is None else streamwiseCoreVec)
+        k = C.c_int()
+        self._chk(self.lib.vlm_rotor_create(self.h, C.byref(d), spanwiseCore, _ptr(scv),
+                                            C.byref(k)))
+        return k.value
+
+    def rotor_set_wing(self, ir, corners=None, rc=None, gam=None):
+        c = None if corners is None else _f64(corners).ravel()
+        r = None if rc is None else _f64(rc).ravel()
+        g = None if gam is None else _f64(gam).ravel()
+        self._chk(self.lib.vlm_rotor_set_wing(self.h, ir, _ptr(c), _ptr(r), _ptr(g)))
+
+    def rotor_set_axis(self, ir, shaft, hub):
+        s, h = _f64(shaft).ravel(), _f64(hub).ravel()
+        self._chk(self.lib.vlm_rotor_set_axis(self.h, ir, _ptr(s), _ptr(h)))
+
+    def rotor_init_cursors(self, ir):
+        self._chk(self.lib.vlm_rotor_init_cursors(self.h, ir))
+
+    def rotor_begin_step(self, ir, it):
+        self._chk(self.lib.vlm_rotor_begin_step(self.h, ir, it))
+
+    def rotor_cursors(self, ir):
+        a, b = C.c_int(), C.c_int()
+        self._chk(self.lib.vlm_rotor_cursors(self.h, ir, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def rotor_assignshed(self, ir, edge):
+        self._chk(self.lib.vlm_rotor_assignshed(self.h, ir, edge))
+
+    def rotor_age_wake(self, ir, dt):
+        self._chk(self.lib.vlm_rotor_age_wake(self.h, ir, dt))
+
+    def rotor_dissipate_wake(self, ir, dt, nu, avc, decay=0.0):
+        self._chk(self.lib.vlm_rotor_dissipate_wake(self.h, ir, dt, nu, avc, decay))
+
+    def rotor_strain_wake(self, ir):
+        self._chk(self.lib.vlm_rotor_strain_wake(self.h, ir))
+
+    def rotor_rollup(self, ir):
+        self._chk(self.lib.vlm_rotor_rollup(self.h, ir))
+
+    def rotor_convectwake(self, ir, dt, wake_type):
+        self._chk(self.lib.vlm_rotor_convectwake(self.h, ir, dt, wake_type))
+
+    def rotor_induced_velocity(self, ir, sset, p, v=None):
+        p = _f64(p, (-1, 3))
+        acc = v is not None
+        out = _f64(v, (-1, 3)).copy() if acc else np.zeros_like(p)
+        self._chk(self.lib.vlm_rotor_induced_velocity(self.h, ir, sset, p.shape[0], _ptr(p),
+                                                      _ptr(out), int(acc)))
+        return out
+
+    def wake_velocity(self, wake_type=WAKE_C):
+        self._chk(self.lib.vlm_wake_velocity(self.h, wake_type))
+
+    def convect(self, scheme, it, dt):
+        self._chk(self.lib.vlm_convect(self.h, scheme, it, dt))
+
+    def rotor_get_wake(self, ir, dims, wake_type=WAKE_C):
+        nb, ns, nN, nF = dims
+        nodes = np.zeros((nb, nN, ns + 1, 3))
+        le = np.zeros((nb, ns + 1, 3))
+        gam = np.zeros((nb, nN, ns))
+        rc4 = np.zeros((nb, nN, ns, 4))
+        ffc = np.zeros((nb, max(nF, 0), 2, 3))
+        fg = np.zeros((nb, max(nF, 0)))
+        frc = np.zeros((nb, max(nF, 0)))
+        self._chk(self.lib.vlm_rotor_get_wake(self.h, ir, wake_type, _ptr(nodes), _ptr(le),
+                                              _ptr(gam), _ptr(rc4), _ptr(ffc), _ptr(fg),
+                                              _ptr(frc)))
+        return dict(nodes=nodes, le=le, gam=gam, rc4=rc4, far_fc=ffc, far_gam=fg, far_rc=frc)
+
+    def rotor_set_wake(self, ir, w, rowNear, rowFar):
+        arrs = [_f64(w[k]) for k in ("nodes", "le", "gam", "rc4", "far_fc", "far_gam",
+                                      "far_rc")]
+        self._chk(self.lib.vlm_rotor_set_wake(self.h, ir, *[_ptr(a) for a in arrs], rowNear,
+                                              rowFar))
+
+    def rotor_get_velocity(self, ir, which, dims):
+        nb, ns, nN, nF = dims
+        vn = np.zeros((nb, nN, ns + 1, 3))
+        vf = np.zeros((nb, max(nF, 0), 3))
+        self._chk(self.lib.vlm_rotor_get_velocity(self.h, ir, which, _ptr(vn), _ptr(vf)))
+        return vn, vf
+
+    # -- dense solve -----------------------------------------------------
+    def rotor_calc_aic(self, ir, cp, ncap, n):
+        cp, ncap = _f64(cp, (-1, 3)), _f64(ncap, (-1, 3))
+        A = np.zeros((n, n), order="F")
+        self._chk(self.lib.vlm_rotor_calc_aic(self.h, ir, _ptr(cp), _ptr(ncap), _ptr(A)))
+        return A
+
+    def rotor_solve(self, ir, rhs, max_iter=3, tol=1e-15):
+        rhs = _f64(rhs).ravel()
+        g = np.zeros_like(rhs)
+        it = C.c_int()
+        self._chk(self.lib.vlm_rotor_solve(self.h, ir, _ptr(rhs), _ptr(g), max_iter, tol,
+                                           C.byref(it)))
+        return g, it.value
+
+    def lu_factor(self, A):
+        A = np.asfortranarray(A, dtype=np.float64).copy(order="F")
+        n = A.shape[0]
+        piv = np.zeros(n, dtype=np.int32)
+        self._chk(self.lib.vlm_lu_factor(self.h, n, _ptr(A), _ptr(piv)))
+        return A, piv
+
+    def lu_solve_refine(self, A, LU, piv, rhs, max_iter=3, tol=1e-15):
+        A = np.asfortranarray(A, dtype=np.float64)
+        LU = np.asfortranarray(LU, dtype=np.float64)
+        piv = np.ascontiguousarray(piv, dtype=np.int32)
+        rhs = _f64(rhs).ravel()
+        x = np.zeros_like(rhs)
+        it = C.c_int()
+        self._chk(self.lib.vlm_lu_solve_refine(self.h, A.shape[0], _ptr(A), _ptr(LU), _ptr(piv),
+                                               _ptr(rhs), _ptr(x), max_iter, tol, C.byref(it)))
+        return x, it.value
+
+    # -- timing ----------------------------------------------------------
+    def profile_enable(self, on=True):
+        self._chk(self.lib.vlm_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        self._chk(self.lib.vlm_profile_reset(self.h))
+
+    def profile_get(self):
+        ms, n, pairs = C.c_double(), C.c_int64(), C.c_double()
+        self._chk(self.lib.vlm_profile_get(self.h, C.byref(ms), C.byref(n), C.byref(pairs)))
+        return ms.value, n.value, pairs.value
