@@ -198,6 +198,33 @@ int vlm_lu_solve_refine(vlm_ctx *ctx, int n, const double *A,
                         const double *LU, const int *ipiv, const double *rhs,
                         double *x, int max_iter, double tol, int *iters);
 
+/* ---- a13: learned pass-2 predictor (no reference counterpart) -------- */
+/* MLP 45 -> 64 -> 64 -> 64 -> 3 (tanh), FP32, parameters packed as
+ * W1(64x45) b1 W2(64x64) b2 W3(64x64) b3 W4(3x64) b4, row-major W[out][in]. */
+int vlm_mlp_num_params(void);
+int vlm_mlp_set_params(vlm_ctx *ctx, const float *params);
+int vlm_mlp_get_params(vlm_ctx *ctx, float *params);
+/* y (m x 3) = MLP(x (m x 45)) */
+int vlm_mlp_forward(vlm_ctx *ctx, int64_t m, const float *x, float *y);
+/* loss = 0.5/m sum |MLP(x) - t|^2 and its gradient (packed like params) */
+int vlm_mlp_loss_grad(vlm_ctx *ctx, int64_t m, const float *x, const float *t,
+                      float *grad, float *loss);
+/* one Adam step (beta 0.9/0.999) on the batch */
+int vlm_mlp_train_step(vlm_ctx *ctx, int64_t m, const float *x, const float *t,
+                       float lr, float *loss);
+/* mode 0: pass 2 of AM2 is the full Biot-Savart sum (the reference's
+ * method, main.f90:1052-1086); mode 1: the MLP proposes the pass-2
+ * velocity from the 12 nearest lattice offsets (scaled by 1/lref) and the
+ * node's pass-1, previous and predictor velocities; every step the full sum
+ * on every sample_stride-th target judges it, and the whole pass is
+ * recomputed when RMS(error)/RMS(speed) on the sample exceeds tol.  The
+ * first warmup_steps AM2 steps always run the full pass; judged samples and
+ * full passes feed train_iters Adam iterations per step. */
+int vlm_set_predictor(vlm_ctx *ctx, int mode, double tol, int sample_stride,
+                      int warmup_steps, int train_iters, double lref);
+int vlm_predictor_stats(vlm_ctx *ctx, int64_t *full_passes, int64_t *learned_passes,
+                        int64_t *fallbacks, double *last_err, double *last_loss);
+
 /* ---- kernel timing ----------------------------------------------------- */
 /* When enabled, every Biot-Savart pair launch is bracketed by HIP events on
  * the context stream.  vlm_profile_get returns the summed kernel time (ms),

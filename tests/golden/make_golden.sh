@@ -1,0 +1,31 @@
+#!/bin/bash
+# Regenerates the golden fixtures under tests/golden/ (run in the build
+# container, where the reference sources exist):
+#  1. data files the reference's own tests hold (fixture CSVs, the elevate
+#     PLOT3D blade grid), copied as data;
+#  2. CL/CT histories of the reference solver (oracle/_ref/volcanor, compiled
+#     from its own sources by oracle/Makefile) on our case directories.
+set -euo pipefail
+HERE=$(cd "$(dirname "$0")" && pwd)
+ROOT=$(cd "$HERE/../.." && pwd)
+REF=/root/reference/tests
+cp "$REF/katzNplotkin-AR04.case/referenceResults/r01ForceNonDim.csv.ref" "$HERE/katz_plotkin_AR4_ForceNonDim.csv"
+cp "$REF/katzNplotkin-AR04.case/referenceResults/r01b01ForceDist00160.csv.ref" "$HERE/katz_plotkin_AR4_ForceDist00160.csv"
+cp "$REF/elevateTest.case/referenceResults/r01ForceNonDim.csv.ref" "$HERE/elevate_ForceNonDim.csv"
+cp "$REF/elevateTest.case/elevate.xyz" "$HERE/elevate.xyz"
+make -C "$ROOT/oracle" ref
+python3 "$ROOT/cases/make_cases.py"
+run_ref() {  # case-name  output-name
+  local tmp; tmp=$(mktemp -d)
+  cp "$ROOT/cases/$1.case/"* "$tmp/"
+  mkdir -p "$tmp/Results" "$tmp/Restart"
+  (cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 OMP_SCHEDULE=static \
+     "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
+  cp "$tmp/Results/r01ForceNonDim.csv" "$HERE/$2"
+  for f in "$tmp"/Results/r0[2-9]ForceNonDim.csv; do
+    [ -e "$f" ] && cp "$f" "$HERE/${2%_ForceNonDim.csv}_$(basename "$f")"
+  done
+  rm -rf "$tmp"
+}
+[ -e "$HERE/cfg1_wing_ForceNonDim.csv" ] || run_ref cfg1_wing cfg1_wing_ForceNonDim.csv
+run_ref cfg2_rotor_2rev cfg2_rotor_2rev_ForceNonDim.csv

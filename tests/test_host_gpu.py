@@ -1,0 +1,70 @@
+"""End-to-end parity: the Fortran host + HIP library against the reference's
+own fixtures and against histories of the reference solver compiled from
+its sources (tests/golden/)."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+CASES = os.path.join(ROOT, "cases")
+
+
+def read_force(path):
+    rows = [l.split() for l in open(path) if l.strip() and not l.lstrip().startswith("iter")]
+    return np.array([[float(x) for x in r[1:]] for r in rows])
+
+
+def run(name, precision="fp64", nt=0):
+    from volcanor_amd import host
+    return host.run_case(os.path.join(CASES, name + ".case"), precision=precision, nt=nt)
+
+
+def test_katz_plotkin_fixture_cl_history():
+    """tests/test_katzNplotkinAR04.py of the reference: CL of the last step and
+    the mean of the last 10 steps at places=6, here on the whole history."""
+    ref = read_force(os.path.join(GOLD, "katz_plotkin_AR4_ForceNonDim.csv"))[:, 0]
+    ct, st = run("katz_plotkin_AR4")
+    cl = ct[0]
+    assert len(cl) == len(ref) == 161
+    assert round(cl[-1] - ref[-1], 6) == 0
+    assert round(np.mean(cl[-10:]) - np.mean(ref[-10:]), 6) == 0
+    err = np.abs(cl - ref)
+    assert np.all(err <= 6e-7 * np.maximum(1.0, np.abs(ref))), (np.argmax(err), err.max())
+
+
+def test_elevate_fixture_ct_history():
+    """tests/test_elevate.py of the reference (axisymmetric rotor, PLOT3D
+    blade, dissipation, truncated far wake)."""
+    ref = read_force(os.path.join(GOLD, "elevate_ForceNonDim.csv"))[:, 0]
+    ct, st = run("elevate")
+    c = ct[0]
+    assert len(c) == len(ref) == 151
+    assert round(c[-1] - ref[-1], 6) == 0
+    assert round(np.mean(c[-10:]) - np.mean(ref[-10:]), 6) == 0
+    err = np.abs(c - ref)
+    assert np.all(err <= 2e-8 + 1e-6 * np.abs(ref)), (np.argmax(err), err.max())
+
+
+def test_cfg1_wing_matches_reference_solver():
+    ref = read_force(os.path.join(GOLD, "cfg1_wing_ForceNonDim.csv"))[:, 0]
+    ct, _ = run("cfg1_wing")
+    np.testing.assert_allclose(ct[0], ref, rtol=1e-6, atol=1e-7)
+
+
+def test_cfg2_two_revs_matches_reference_solver():
+    path = os.path.join(GOLD, "cfg2_rotor_2rev_ForceNonDim.csv")
+    ref = read_force(path)[:, 0]
+    ct, _ = run("cfg2_rotor_2rev")
+    np.testing.assert_allclose(ct[0], ref, rtol=1e-5, atol=1e-8)
+
+
+def test_cfg2_mixed_precision_ct_within_1e4():
+    """Mixed FP32/FP64 Biot-Savart against the FP64 path, two revolutions."""
+    a, _ = run("cfg2_rotor_2rev", "fp64")
+    b, _ = run("cfg2_rotor_2rev", "mixed")
+    rel = np.abs(b[0][-36:] - a[0][-36:]) / np.abs(a[0][-36:])
+    assert np.max(rel) < 1e-4

@@ -232,6 +232,7 @@ int vlm_destroy(vlm_ctx *ctx)
     if (!ctx) return VLM_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    mlp_release(ctx);
     for (auto *r : ctx->rotors) delete r;
     for (auto *f : ctx->fsets) delete f;
     for (auto *p : ctx->psets) delete p;

@@ -81,6 +81,24 @@ struct Timer {
     double pairs;
 };
 
+struct Mlp;
+
+// Learned pass-2 predictor settings, statistics and training buffer.
+struct Predictor {
+    int mode = 0;          // 0 full Biot-Savart pass 2, 1 learned
+    double tol = 1e-4;     // judge tolerance: RMS error / RMS speed on the sample
+    int stride = 32;       // judge sample: every stride-th target
+    int warmup = 8;        // first AM2 steps always run the full pass
+    int train_iters = 4;   // Adam iterations per step on the buffer
+    float lr = 1e-3f;
+    double lref = 1.0;     // length scale of the neighbour offsets
+    int64_t steps = 0, full = 0, learned = 0, fallbacks = 0;
+    double last_err = 0.0, last_loss = 0.0;
+    int64_t buf_cap = 16384, buf_n = 0, buf_head = 0;
+    DBuf<float> X, Y, bx, bt;
+    DBuf<double> vn, ts, vs, e2, scal;
+};
+
 struct FSet {
     DBuf<double> d64;
     DBuf<float> d32;
@@ -125,6 +143,8 @@ struct vlm_ctx {
     std::vector<int *> piv;
     std::vector<int> aic_n;
     vlm::DBuf<double> solve_tmp;
+    vlm::Mlp *mlp = nullptr;
+    vlm::Predictor pred;
 };
 
 namespace vlm {
@@ -171,5 +191,9 @@ int64_t wing_slots(const Rotor &r, int set);
 int64_t wake_slots(const Rotor &r);
 
 int ensure_slots(vlm_ctx *ctx, int64_t n);
+void mlp_release(vlm_ctx *ctx);
+int mlp_forward_dev(vlm_ctx *ctx, int64_t m, const float *X, float *Y);
+int mlp_train_dev(vlm_ctx *ctx, int64_t m, const float *X, const float *T, int adam, float lr,
+                  float *loss_host);
 
 }  // namespace vlm

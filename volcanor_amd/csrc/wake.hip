@@ -928,6 +928,313 @@ int wake_pass(vlm_ctx *ctx, int pred)
     return VLM_OK;
 }
 
+
+// ---------------------------------------------------------------------
+// Learned pass 2 (AM2 predictor stage): the MLP proposes the velocity on
+// the predicted wake; a strided sample of targets is evaluated by the full
+// Biot-Savart sum every step as judge and training data, and the whole
+// pass is recomputed when the sampled error exceeds the tolerance.
+// ---------------------------------------------------------------------
+namespace {
+
+__constant__ int kNbr[12][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}, {-1, -1}, {-1, 1},
+                                {1, -1}, {1, 1}, {-2, 0}, {2, 0}, {0, -2}, {0, 2}};
+
+// 45 features per P-pass target in gather order: 12 neighbour offsets on
+// the predicted lattice (scaled by 1/lref) and three velocities scaled by
+// 1/vref: pass-1 velocity, previous step's velocity, AB2 predictor velocity.
+__global__ void features_kernel(RV v, float *X, int64_t off, double lref_inv,
+                                const double *vref)
+{
+    const int R = v.nN - v.rn, F = v.nF - v.rf;
+    const int64_t per = (int64_t)R * (v.ns + 1) + F;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * v.nbc) return;
+    const int b = (int)(e / per);
+    const int64_t k = e - b * per;
+    float *x = X + (off + e) * 45;
+    const double vinv = 1.0 / vref[0];
+    const double *vs, *vp, *va;
+    if (k < (int64_t)R * (v.ns + 1)) {
+        const int i = v.rn + (int)(k / (v.ns + 1)), j = (int)(k % (v.ns + 1));
+        const double *P = v.pnode + 3 * nidx(v, b, phys(v, i), j);
+        for (int q = 0; q < 12; ++q) {
+            const int ii = i + kNbr[q][0], jj = j + kNbr[q][1];
+            float ox = 0.f, oy = 0.f, oz = 0.f;
+            if (jj >= 0 && jj <= v.ns && ii >= v.rn - 1 && ii < v.nN) {
+                const double *Q = (ii == v.rn - 1) ? v.ple + 3 * ((size_t)b * (v.ns + 1) + jj)
+                                                   : v.pnode + 3 * nidx(v, b, phys(v, ii), jj);
+                ox = (float)((Q[0] - P[0]) * lref_inv);
+                oy = (float)((Q[1] - P[1]) * lref_inv);
+                oz = (float)((Q[2] - P[2]) * lref_inv);
+            }
+            x[3 * q] = ox;
+            x[3 * q + 1] = oy;
+            x[3 * q + 2] = oz;
+        }
+        const size_t vi = 3 * vidx(v, b, i, j);
+        vs = v.vel[VLM_VEL_STEP] + vi;
+        vp = v.vel[VLM_VEL_PREV] + vi;
+        va = v.vel[VLM_VEL] + vi;
+    } else {
+        const int i = v.rf + (int)(k - (int64_t)R * (v.ns + 1));
+        for (int q = 0; q < 36; ++q) x[q] = 0.f;
+        const size_t vi = 3 * ((size_t)b * v.nF + i);
+        vs = v.velF[VLM_VEL_STEP] + vi;
+        vp = v.velF[VLM_VEL_PREV] + vi;
+        va = v.velF[VLM_VEL] + vi;
+    }
+    for (int d = 0; d < 3; ++d) {
+        x[36 + d] = (float)(vs[d] * vinv);
+        x[39 + d] = (float)(vp[d] * vinv);
+        x[42 + d] = (float)(va[d] * vinv);
+    }
+}
+
+// Pass-1 velocity of every P-pass target (gather order), SoA.
+__global__ void gather_step_vel_kernel(RV v, double *o, int64_t stride, int64_t off)
+{
+    const int R = v.nN - v.rn, F = v.nF - v.rf;
+    const int64_t per = (int64_t)R * (v.ns + 1) + F;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * v.nbc) return;
+    const int b = (int)(e / per);
+    const int64_t k = e - b * per;
+    const double *V;
+    if (k < (int64_t)R * (v.ns + 1)) {
+        const int i = v.rn + (int)(k / (v.ns + 1)), j = (int)(k % (v.ns + 1));
+        V = v.vel[VLM_VEL_STEP] + 3 * vidx(v, b, i, j);
+    } else {
+        V = v.velF[VLM_VEL_STEP] + 3 * ((size_t)b * v.nF + v.rf + (k - (int64_t)R * (v.ns + 1)));
+    }
+    o[off + e] = V[0];
+    o[stride + off + e] = V[1];
+    o[2 * stride + off + e] = V[2];
+}
+
+// vref = RMS of the pass-1 speed over all targets (single block).
+__global__ void vref_kernel(const double *vn, int64_t m, double *vref)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int64_t k = threadIdx.x; k < m; k += blockDim.x)
+        s += vn[k] * vn[k] + vn[m + k] * vn[m + k] + vn[2 * m + k] * vn[2 * m + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) vref[0] = fmax(sqrt(sh[0] / (double)(m > 0 ? m : 1)), 1e-12);
+}
+
+// v = v_pass1 + y * vref for every target
+__global__ void apply_mlp_kernel(const double *vn, const float *Y, int64_t m, const double *vref,
+                                 double *out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    const double s = vref[0];
+    for (int d = 0; d < 3; ++d) out[d * m + e] = vn[d * m + e] + (double)Y[3 * e + d] * s;
+}
+
+// Sampled targets: e = phase + k*stride.  Gather their positions.
+__global__ void gather_sample_kernel(const double *t, int64_t m, int64_t phase, int64_t stride,
+                                     int64_t ms, double *ts)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ms) return;
+    const int64_t e = phase + k * stride;
+    ts[k] = t[e];
+    ts[ms + k] = t[m + e];
+    ts[2 * ms + k] = t[2 * m + e];
+}
+
+// Judge: per-sample squared error and squared exact speed; then write the
+// exact values into the output and the training pair into the buffer.
+__global__ void judge_kernel(const double *vs, int64_t ms, const double *out, int64_t m,
+                             int64_t phase, int64_t stride, const double *vn, const float *X,
+                             const double *vref, double *err2, double *ref2, float *bx, float *bt,
+                             int64_t bcap, int64_t bhead)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ms) return;
+    const int64_t e = phase + k * stride;
+    double se = 0.0, sr = 0.0;
+    const double inv = 1.0 / vref[0];
+    const int64_t slot = (bhead + k) % bcap;
+    for (int d = 0; d < 3; ++d) {
+        const double a = vs[d * ms + k], p = out[d * m + e];
+        se += (a - p) * (a - p);
+        sr += a * a;
+        bt[3 * slot + d] = (float)((a - vn[d * m + e]) * inv);
+    }
+    for (int q = 0; q < 45; ++q) bx[45 * slot + q] = X[45 * e + q];
+    err2[k] = se;
+    ref2[k] = sr;
+}
+
+__global__ void put_sample_kernel(const double *vs, int64_t ms, double *out, int64_t m,
+                                  int64_t phase, int64_t stride)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ms) return;
+    const int64_t e = phase + k * stride;
+    for (int d = 0; d < 3; ++d) out[d * m + e] = vs[d * ms + k];
+}
+
+__global__ void sum2_kernel(const double *a, const double *b, int64_t n, double *out)
+{
+    __shared__ double sa[256], sb[256];
+    double x = 0.0, y = 0.0;
+    for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
+        x += a[k];
+        y += b[k];
+    }
+    sa[threadIdx.x] = x;
+    sb[threadIdx.x] = y;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sa[threadIdx.x] += sa[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sa[0];
+        out[1] = sb[0];
+    }
+}
+
+// Training pairs from a full pass: every stride-th target.
+__global__ void full_pairs_kernel(const double *vexact, const double *vn, int64_t m,
+                                  const float *X, const double *vref, int64_t stride, int64_t np,
+                                  float *bx, float *bt, int64_t bcap, int64_t bhead)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np) return;
+    const int64_t e = k * stride;
+    const int64_t slot = (bhead + k) % bcap;
+    const double inv = 1.0 / vref[0];
+    for (int d = 0; d < 3; ++d) bt[3 * slot + d] = (float)((vexact[d * m + e] - vn[d * m + e]) * inv);
+    for (int q = 0; q < 45; ++q) bx[45 * slot + q] = X[45 * e + q];
+}
+
+}  // namespace
+
+int learned_pass(vlm_ctx *ctx)
+{
+    Predictor &P = ctx->pred;
+    std::vector<std::pair<int, int>> src;
+    int64_t m = 0;
+    for (int ir = 0; ir < (int)ctx->rotors.size(); ++ir) {
+        src.push_back({ir, VLM_SET_WING});
+        src.push_back({ir, VLM_SET_WAKE_PRED});
+        m += target_count(*ctx->rotors[ir]);
+    }
+    if (m == 0) return VLM_OK;
+    const int64_t nmax = staged_upper_bound(ctx, src);
+    VLM_TRY(build_sources(ctx, src));
+    VLM_HIP(ctx->tgt.reserve((size_t)3 * m));
+    VLM_HIP(ctx->out.reserve((size_t)3 * m));
+    VLM_HIP(P.vn.reserve((size_t)3 * m));
+    VLM_HIP(P.X.reserve((size_t)45 * m));
+    VLM_HIP(P.Y.reserve((size_t)3 * m));
+    VLM_HIP(P.scal.reserve(4));
+    if (P.bx.n < (size_t)45 * P.buf_cap) {
+        VLM_HIP(P.bx.reserve((size_t)45 * P.buf_cap));
+        VLM_HIP(P.bt.reserve((size_t)3 * P.buf_cap));
+    }
+    int64_t off = 0;
+    for (auto *rp : ctx->rotors) {
+        const int64_t n = target_count(*rp);
+        if (n == 0) continue;
+        hipLaunchKernelGGL(gather_targets_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+                           view(*rp), 1, ctx->tgt.p, m, off);
+        hipLaunchKernelGGL(gather_step_vel_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+                           view(*rp), P.vn.p, m, off);
+        off += n;
+    }
+    hipLaunchKernelGGL(vref_kernel, dim3(1), dim3(256), 0, ctx->stream, P.vn.p, m, P.scal.p);
+    off = 0;
+    for (auto *rp : ctx->rotors) {
+        const int64_t n = target_count(*rp);
+        if (n == 0) continue;
+        hipLaunchKernelGGL(features_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream, view(*rp),
+                           P.X.p, off, 1.0 / P.lref, P.scal.p);
+        off += n;
+    }
+    VLM_HIP(hipGetLastError());
+    const bool warm = P.steps < P.warmup;
+    P.steps += 1;
+    bool full = warm;
+    if (!warm) {
+        VLM_TRY(mlp_forward_dev(ctx, m, P.X.p, P.Y.p));
+        hipLaunchKernelGGL(apply_mlp_kernel, dim3(nblk(m)), dim3(256), 0, ctx->stream, P.vn.p,
+                           P.Y.p, m, P.scal.p, ctx->out.p);
+        // judge on a strided sample with a rotating phase
+        const int64_t stride = P.stride < 1 ? 1 : P.stride;
+        const int64_t phase = (P.steps * 7) % stride < m ? (P.steps * 7) % stride : 0;
+        const int64_t ms = (m - phase + stride - 1) / stride;
+        VLM_HIP(P.ts.reserve((size_t)3 * ms));
+        VLM_HIP(P.vs.reserve((size_t)3 * ms));
+        VLM_HIP(P.e2.reserve((size_t)2 * ms));
+        hipLaunchKernelGGL(gather_sample_kernel, dim3(nblk(ms)), dim3(256), 0, ctx->stream,
+                           ctx->tgt.p, m, phase, stride, ms, P.ts.p);
+        VLM_TRY(eval_sources(ctx, P.ts.p, ms, ms, P.vs.p, 0, nmax));
+        const int64_t take = ms < P.buf_cap ? ms : P.buf_cap;
+        hipLaunchKernelGGL(judge_kernel, dim3(nblk(take)), dim3(256), 0, ctx->stream, P.vs.p,
+                           take, ctx->out.p, m, phase, stride, P.vn.p, P.X.p, P.scal.p, P.e2.p,
+                           P.e2.p + ms, P.bx.p, P.bt.p, P.buf_cap, P.buf_head);
+        hipLaunchKernelGGL(sum2_kernel, dim3(1), dim3(256), 0, ctx->stream, P.e2.p, P.e2.p + ms,
+                           take, P.scal.p + 1);
+        double h[2] = {0, 0};
+        VLM_HIP(hipMemcpyAsync(h, P.scal.p + 1, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        VLM_HIP(hipStreamSynchronize(ctx->stream));
+        P.buf_head = (P.buf_head + take) % P.buf_cap;
+        P.buf_n = P.buf_n + take < P.buf_cap ? P.buf_n + take : P.buf_cap;
+        P.last_err = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
+        if (P.last_err > P.tol) {
+            full = true;
+            P.fallbacks += 1;
+        } else {
+            hipLaunchKernelGGL(put_sample_kernel, dim3(nblk(ms)), dim3(256), 0, ctx->stream,
+                               P.vs.p, ms, ctx->out.p, m, phase, stride);
+            VLM_HIP(hipGetLastError());
+            P.learned += 1;
+        }
+    }
+    if (full) {
+        VLM_TRY(eval_sources(ctx, ctx->tgt.p, m, m, ctx->out.p, 0, nmax));
+        P.full += 1;
+        const int64_t stride = (m + P.buf_cap / 4 - 1) / (P.buf_cap / 4);
+        const int64_t np = (m + stride - 1) / stride;
+        hipLaunchKernelGGL(full_pairs_kernel, dim3(nblk(np)), dim3(256), 0, ctx->stream,
+                           ctx->out.p, P.vn.p, m, P.X.p, P.scal.p, stride, np, P.bx.p, P.bt.p,
+                           P.buf_cap, P.buf_head);
+        VLM_HIP(hipGetLastError());
+        P.buf_head = (P.buf_head + np) % P.buf_cap;
+        P.buf_n = P.buf_n + np < P.buf_cap ? P.buf_n + np : P.buf_cap;
+    }
+    off = 0;
+    for (auto *rp : ctx->rotors) {
+        const int64_t n = target_count(*rp);
+        if (n == 0) continue;
+        hipLaunchKernelGGL(scatter_vel_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+                           view(*rp), VLM_VEL_PRED, ctx->out.p, m, off);
+        off += n;
+    }
+    VLM_HIP(hipGetLastError());
+    float loss = 0.f;
+    for (int it = 0; it < P.train_iters && P.buf_n > 0; ++it)
+        VLM_TRY(mlp_train_dev(ctx, P.buf_n, P.bx.p, P.bt.p, 1, P.lr,
+                              it == P.train_iters - 1 ? &loss : nullptr));
+    P.last_loss = loss;
+    return VLM_OK;
+}
+
 }  // namespace vlm
 
 using namespace vlm;
@@ -1281,7 +1588,8 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
             VLM_TRY(launch_scheme(ctx, *r, 1));
             VLM_TRY(rotor_convect(ctx, *r, dt, 1));
         }
-    VLM_TRY(wake_pass(ctx, 1));
+    if (ctx->pred.mode == 1) VLM_TRY(learned_pass(ctx));
+    else VLM_TRY(wake_pass(ctx, 1));
     for (auto *r : ctx->rotors)
         if (active(r)) {
             VLM_TRY(launch_scheme(ctx, *r, 2));
@@ -1385,6 +1693,40 @@ int vlm_rotor_get_velocity(vlm_ctx *ctx, int ir, int which, double *velN, double
         VLM_HIP(hipMemcpy(velN, r->velN[which].p, r->nvn() * sizeof(double), hipMemcpyDeviceToHost));
     if (velF && r->nvf() > 0)
         VLM_HIP(hipMemcpy(velF, r->velF[which].p, r->nvf() * sizeof(double), hipMemcpyDeviceToHost));
+    return VLM_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int vlm_set_predictor(vlm_ctx *ctx, int mode, double tol, int sample_stride, int warmup_steps,
+                      int train_iters, double lref)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    if (mode != 0 && mode != 1) return fail(ctx, VLM_ERR_ARG, "predictor mode must be 0 or 1");
+    if (sample_stride < 1 || warmup_steps < 0 || train_iters < 0 || !(lref > 0) || !(tol > 0))
+        return fail(ctx, VLM_ERR_ARG, "bad predictor settings");
+    Predictor &P = ctx->pred;
+    P.mode = mode;
+    P.tol = tol;
+    P.stride = sample_stride;
+    P.warmup = warmup_steps;
+    P.train_iters = train_iters;
+    P.lref = lref;
+    return VLM_OK;
+}
+
+int vlm_predictor_stats(vlm_ctx *ctx, int64_t *full, int64_t *learned, int64_t *fallbacks,
+                        double *last_err, double *last_loss)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    Predictor &P = ctx->pred;
+    if (full) *full = P.full;
+    if (learned) *learned = P.learned;
+    if (fallbacks) *fallbacks = P.fallbacks;
+    if (last_err) *last_err = P.last_err;
+    if (last_loss) *last_loss = P.last_loss;
     return VLM_OK;
 }
 

@@ -1,0 +1,1464 @@
+!! Host side of the MI355X free-wake solver, in Fortran.
+!!
+!! Keeps VOLCANOR's object model (rotor / blade / wing panel / vortex ring)
+!! and its `.case` inputs (config.nml v0.5, geomXX.nml v0.15) as the
+!! drop-in API.  Geometry, rigid motion and loads are O(panels) host work in
+!! FP64; every Biot-Savart sum, the wake state and its updates, and the
+!! circulation solve run on the GPU through libvlm (module vlm_c).
+!!
+!! Reference routines followed (cibinjoseph/VOLCANOR):
+!!   readConfig            src/libCommon.f90:51-108
+!!   rotor_readGeom        src/classdef.f90:2541-2767
+!!   rotor_init            src/classdef.f90:2769-3891 (flat and PLOT3D blades)
+!!   motion                src/classdef.f90:1092-1281, 4202-4291
+!!   RHS / solve / loads   src/main.f90:119-234, 400-764
+!!   blade_calc_force      src/classdef.f90:1704-1896 (+ 2058-2380 helpers)
+!!   force2file            src/libPostprocess.f90:814-882
+module volcanor_host
+  use iso_c_binding
+  use vlm_c
+  implicit none
+  private
+
+  integer, parameter, public :: dp = kind(1.d0)
+  real(dp), parameter :: pi = 4._dp*atan(1._dp)
+  real(dp), parameter :: twoPi = 2._dp*pi
+  real(dp), parameter :: eps = epsilon(1._dp)
+  real(dp), parameter :: degToRad = pi/180._dp
+  real(dp), parameter :: radToDeg = 180._dp/pi
+  real(dp), parameter :: xAxis(3) = [1._dp, 0._dp, 0._dp]
+  real(dp), parameter :: yAxis(3) = [0._dp, 1._dp, 0._dp]
+  real(dp), parameter :: zAxis(3) = [0._dp, 0._dp, 1._dp]
+
+  type, public :: switches_class
+    integer :: ntSub = 0, ntSubInit = 0, wakeDissipation = 0, wakeStrain = 0
+    integer :: wakeBurst = 0, wakeSuppress = 0, slowStart = 0, slowStartNt = 0
+    integer :: wakeTipPlot = 0, wakePlot = 0, gridPlot = 0, rotorForcePlot = 0
+    integer :: fdScheme = 3, probe = 0, initWakeVelNt = 0
+    integer :: restartFromNt = 0, restartWriteNt = 0
+  end type switches_class
+
+  !! Vortex ring of a wing panel: corners c1..c4 and the core radius of its
+  !! four filaments vf1 c1->c2, vf2 c2->c3, vf3 c3->c4, vf4 c4->c1.
+  type, public :: vr_class
+    real(dp) :: fc(3, 4) = 0._dp
+    real(dp) :: rVc(4) = 0._dp
+    real(dp) :: gam = 0._dp
+  end type vr_class
+
+  type, public :: wingpanel_class
+    type(vr_class) :: vr
+    real(dp) :: PC(3, 4) = 0._dp
+    real(dp) :: CP(3) = 0._dp, nCap(3) = 0._dp
+    real(dp) :: tauCapChord(3) = 0._dp, tauCapSpan(3) = 0._dp
+    real(dp) :: velCP(3) = 0._dp, velCPm(3) = 0._dp, velCPTotal(3) = 0._dp
+    real(dp) :: chordwiseResVel(3) = 0._dp
+    real(dp) :: normalForce(3) = 0._dp, normalForceUnsteady(3) = 0._dp
+    real(dp) :: gamPrev = 0._dp, gamTrapz = 0._dp
+    real(dp) :: delP = 0._dp, delPUnsteady = 0._dp
+    real(dp) :: panelArea = 0._dp, meanChord = 0._dp, meanSpan = 0._dp
+  end type wingpanel_class
+
+  type, public :: blade_class
+    type(wingpanel_class), allocatable :: wiP(:, :)
+    real(dp) :: xAxis(3), yAxis(3), zAxis(3)
+    real(dp) :: xAxisAzi(3), yAxisAzi(3), zAxisAzi(3)
+    real(dp) :: xAxisAziFlap(3), yAxisAziFlap(3), zAxisAziFlap(3)
+    real(dp) :: theta = 0._dp, psi = 0._dp, pivotLE = 0._dp
+    real(dp) :: flapOrigin(3) = 0._dp
+    real(dp) :: lift(3) = 0._dp, drag(3) = 0._dp, forceInertial(3) = 0._dp
+    real(dp) :: liftUnsteady(3) = 0._dp
+    real(dp), allocatable :: secCP(:, :), secTauCapChord(:, :), secTauCapSpan(:, :)
+    real(dp), allocatable :: secNormalVec(:, :), secChordwiseResVel(:, :)
+    real(dp), allocatable :: secLift(:, :), secLiftDir(:, :), secDragDir(:, :)
+    real(dp), allocatable :: secForceInertial(:, :), secLiftUnsteady(:, :)
+    real(dp), allocatable :: secLiftInPlane(:, :), secLiftOutPlane(:, :)
+    real(dp), allocatable :: secChord(:), secArea(:), secCL(:), secCLu(:)
+    real(dp), allocatable :: secAlpha(:), secMflapArm(:)
+  end type blade_class
+
+  type, public :: rotor_class
+    character(len=2) :: id = '01'
+    integer :: nb = 1, nc = 1, ns = 1, nNwake = 0, nFwake = 0, nbConvect = 1
+    integer :: surfaceType = 0, propConvention = 0, spanSpacing = 1, chordSpacing = 1
+    integer :: axisymmetrySwitch = 0, symmetricTau = 0, forceCalcSwitch = 0
+    integer :: wakeTruncateNt = 0, rollupStart = 1, rollupEnd = 1
+    integer :: spanwiseLiftSwitch = 0, suppressFwakeSwitch = 0
+    integer :: rowNear = 0, rowFar = 0
+    character(len=30) :: geometryFile = '0'
+    type(blade_class), allocatable :: blade(:)
+    real(dp) :: Omega = 0._dp, omegaSlow = 0._dp, psi = 0._dp
+    real(dp) :: shaftAxis(3) = zAxis, hubCoords(3) = 0._dp, cgCoords(3) = 0._dp
+    real(dp) :: fromCoords(3) = 0._dp, pts(3) = 0._dp
+    real(dp) :: xAxisBody(3) = xAxis, yAxisBody(3) = yAxis, zAxisBody(3) = zAxis
+    real(dp) :: radius = 1._dp, root_cut = 0._dp, chord = 1._dp
+    real(dp) :: preconeAngle = 0._dp, controlPitch(3) = 0._dp, pivotLE = 0._dp
+    real(dp) :: flapHinge = 0._dp, psiStart = 0._dp
+    real(dp) :: velBody(3) = 0._dp, omegaBody(3) = 0._dp
+    real(dp) :: spanwiseCore = 0._dp, apparentViscCoeff = 0._dp, decayCoeff = 0._dp
+    real(dp) :: rollupStartRadius = 0._dp, rollupEndRadius = 1._dp
+    real(dp), allocatable :: streamwiseCoreVec(:)
+    real(dp), allocatable :: gamVec(:), gamVecPrev(:), RHS(:)
+    real(dp) :: lift(3) = 0._dp, drag(3) = 0._dp, forceInertial(3) = 0._dp
+    real(dp) :: liftUnsteady(3) = 0._dp
+    real(dp) :: nonDimforceDenominator = 1._dp
+    integer(c_int) :: dev = -1        !! libvlm rotor index
+  end type rotor_class
+
+  !! Result of one run, returned to C callers.
+  type, bind(C), public :: volcanor_stats
+    real(c_double) :: wall_total, wall_loop, wall_steps_tail
+    real(c_double) :: bs_kernel_ms, bs_pairs
+    integer(c_int64_t) :: bs_launches
+    integer(c_int) :: nt, tail_steps
+    integer(c_int64_t) :: pred_full, pred_learned, pred_fallbacks
+    real(c_double) :: pred_last_err
+  end type volcanor_stats
+
+  public :: volcanor_run, run_case
+
+contains
+
+  ! ------------------------------------------------------------------
+  ! Vector helpers (own implementations of the libMath routines used)
+  ! ------------------------------------------------------------------
+  pure function cross3(a, b) result(c)
+    real(dp), intent(in) :: a(3), b(3)
+    real(dp) :: c(3)
+    c(1) = a(2)*b(3) - a(3)*b(2)
+    c(2) = a(3)*b(1) - a(1)*b(3)
+    c(3) = a(1)*b(2) - a(2)*b(1)
+  end function cross3
+
+  pure function unit3(a) result(u)
+    real(dp), intent(in) :: a(3)
+    real(dp) :: u(3), n
+    n = norm2(a)
+    if (n > eps) then
+      u = a/n
+    else
+      u = 0._dp
+    endif
+  end function unit3
+
+  pure function proj3(a, d) result(p)
+    real(dp), intent(in) :: a(3), d(3)
+    real(dp) :: p(3), n2
+    n2 = sum(d**2)
+    if (n2 > eps) then
+      p = dot_product(a, d)*d/n2
+    else
+      p = 0._dp
+    endif
+  end function proj3
+
+  pure function noproj3(a, d) result(p)
+    real(dp), intent(in) :: a(3), d(3)
+    real(dp) :: p(3), n2
+    n2 = sum(d**2)
+    if (n2 > eps) then
+      p = a - dot_product(a, d)*d/n2
+    else
+      p = a
+    endif
+  end function noproj3
+
+  function spaced(kind, x0, x1, n) result(x)
+    !! linspace / cosspace / halfsinspace / tanspace, libMath.f90:138-200
+    integer, intent(in) :: kind, n
+    real(dp), intent(in) :: x0, x1
+    real(dp) :: x(n), t(n), dx
+    integer :: i
+    select case (kind)
+    case (1)
+      dx = (x1 - x0)/(n - 1)
+      do i = 1, n
+        x(i) = (i - 1)*dx + x0
+      enddo
+    case (2)
+      dx = pi/(n - 1)
+      do i = 1, n
+        t(i) = (i - 1)*dx
+      enddo
+      x = x0 + (x1 - x0)*0.5_dp*(1._dp - cos(t))
+    case (3)
+      dx = (0.5_dp*pi)/(n - 1)
+      do i = 1, n
+        t(i) = (i - 1)*dx
+      enddo
+      x = x0 + (x1 - x0)*sin(t)
+    case (4)
+      dx = 2.4_dp/(n - 1)
+      do i = 1, n
+        t(i) = (i - 1)*dx - 1.2_dp
+      enddo
+      x = x0 + (x1 - x0)*tan(t)/tan(1.2_dp)
+    case default
+      error stop 'ERROR: spacing must be 1..4'
+    end select
+  end function spaced
+
+  function axis_rotation(theta, ax) result(T)
+    !! Rotation by theta about a unit axis, column-major, x' = T x
+    real(dp), intent(in) :: theta, ax(3)
+    real(dp) :: T(3, 3), a(3), ct, st, om
+    a = ax/norm2(ax)
+    ct = cos(theta)
+    st = sin(theta)
+    om = 1._dp - ct
+    T(:, 1) = [ct + a(1)*a(1)*om, a(3)*st + a(2)*a(1)*om, -a(2)*st + a(3)*a(1)*om]
+    T(:, 2) = [-a(3)*st + a(1)*a(2)*om, ct + a(2)*a(2)*om, a(1)*st + a(3)*a(2)*om]
+    T(:, 3) = [a(2)*st + a(1)*a(3)*om, -a(1)*st + a(2)*a(3)*om, ct + a(3)*a(3)*om]
+  end function axis_rotation
+
+  function euler_gb(p) result(T)
+    !! Global-to-body Euler matrix from (phi, theta, psi), libMath.f90:672-693
+    real(dp), intent(in) :: p(3)
+    real(dp) :: T(3, 3), cf, sf, ct, st, cs, ss
+    cf = cos(p(1)); sf = sin(p(1))
+    ct = cos(p(2)); st = sin(p(2))
+    cs = cos(p(3)); ss = sin(p(3))
+    T(1, 1) = cs*ct
+    T(1, 2) = sf*st*cs - ss*cf
+    T(1, 3) = sf*ss + st*cf*cs
+    T(2, 1) = ss*ct
+    T(2, 2) = sf*ss*st + cf*cs
+    T(2, 3) = ss*st*cf - sf*cs
+    T(3, 1) = -st
+    T(3, 2) = sf*ct
+    T(3, 3) = cf*ct
+  end function euler_gb
+
+  pure function angle_tan(a, b) result(ang)
+    real(dp), intent(in) :: a(3), b(3)
+    real(dp) :: ang, d
+    d = dot_product(a, b)
+    ang = atan2(sqrt(sum(a**2)*sum(b**2) - d*d), d)
+  end function angle_tan
+
+  function lsq2_eval(xq, xd, yd) result(y)
+    !! Quadratic least-squares fit evaluated at xq (libMath.f90:577-605),
+    !! normal equations solved by Gaussian elimination with pivoting.
+    real(dp), intent(in) :: xq, xd(:), yd(:)
+    real(dp) :: y, A(3, 3), b(3), c(3), f, t(4)
+    integer :: i, j, k, p
+    A(1, 1) = size(xd)
+    A(1, 2) = sum(xd)
+    A(1, 3) = sum(xd**2)
+    A(2, 1) = A(1, 2)
+    A(2, 2) = A(1, 3)
+    A(2, 3) = sum(xd**3)
+    A(3, 1) = A(1, 3)
+    A(3, 2) = A(2, 3)
+    A(3, 3) = sum(xd**4)
+    b = [sum(yd), sum(yd*xd), sum(yd*xd**2)]
+    do k = 1, 3
+      p = k - 1 + maxloc(abs(A(k:3, k)), 1)
+      if (p /= k) then
+        t(1:3) = A(k, :); A(k, :) = A(p, :); A(p, :) = t(1:3)
+        t(4) = b(k); b(k) = b(p); b(p) = t(4)
+      endif
+      do i = k + 1, 3
+        f = A(i, k)/A(k, k)
+        A(i, k:3) = A(i, k:3) - f*A(k, k:3)
+        b(i) = b(i) - f*b(k)
+      enddo
+    enddo
+    do i = 3, 1, -1
+      c(i) = b(i)
+      do j = i + 1, 3
+        c(i) = c(i) - A(i, j)*c(j)
+      enddo
+      c(i) = c(i)/A(i, i)
+    enddo
+    y = c(1) + c(2)*xq + c(3)*xq*xq
+  end function lsq2_eval
+
+  function pwl(x, y, q) result(v)
+    !! Piecewise-linear interpolation, libMath.f90:476-517
+    real(dp), intent(in) :: x(:), y(:), q
+    real(dp) :: v
+    integer :: n, i, idx
+    n = size(x)
+    if (abs(x(1) - q) < eps) then
+      v = y(1)
+      return
+    elseif (abs(x(n) - q) < eps) then
+      v = y(n)
+      return
+    endif
+    idx = 1
+    do i = 1, n
+      if (x(1) < x(n)) then
+        if (.not. (x(i) <= q)) then
+          idx = i - 1
+          exit
+        endif
+      else
+        if (.not. (x(i) >= q)) then
+          idx = i - 1
+          exit
+        endif
+      endif
+    enddo
+    if (idx < 1) error stop 'ERROR: interpolation point out of range'
+    v = y(idx) + (y(idx + 1) - y(idx))/(x(idx + 1) - x(idx))*(q - x(idx))
+  end function pwl
+
+  ! ------------------------------------------------------------------
+  ! Inputs
+  ! ------------------------------------------------------------------
+  subroutine read_config(dir, nt, dt, nr, density, velSound, kinematicVisc, sw)
+    !! config.nml v0.5 (libCommon.f90:51-108); absent keys default to 0
+    character(len=*), intent(in) :: dir
+    integer, intent(out) :: nt, nr
+    real(dp), intent(out) :: dt, density, velSound, kinematicVisc
+    type(switches_class), intent(out) :: sw
+    character(len=10) :: fileFormatVersion
+    integer :: restartWriteNt, restartFromNt, ntSub, ntSubInit, wakePlot, wakeTipPlot, &
+      & rotorForcePlot, gridPlot, wakeDissipation, wakeStrain, wakeBurst, wakeSuppress, &
+      & slowStart, slowStartNt, fdScheme, initWakeVelNt, probe
+    integer :: u
+    namelist /VERSION/ fileFormatVersion
+    namelist /PARAMS/ nt, dt, nr, density, velSound, kinematicVisc
+    namelist /OPTIONS/ restartWriteNt, restartFromNt, ntSub, ntSubInit, &
+      & wakePlot, wakeTipPlot, rotorForcePlot, gridPlot, wakeDissipation, wakeStrain, &
+      & wakeBurst, wakeSuppress, slowStart, slowStartNt, fdScheme, initWakeVelNt, probe
+
+    nt = 0; dt = 0._dp; nr = 1; density = 1._dp; velSound = 330._dp; kinematicVisc = 0._dp
+    restartWriteNt = 0; restartFromNt = 0; ntSub = 0; ntSubInit = 0; wakePlot = 0
+    wakeTipPlot = 0; rotorForcePlot = 0; gridPlot = 0; wakeDissipation = 0; wakeStrain = 0
+    wakeBurst = 0; wakeSuppress = 0; slowStart = 0; slowStartNt = 0; fdScheme = 3
+    initWakeVelNt = 0; probe = 0
+    open (newunit=u, file=trim(dir)//'/config.nml', status='old', action='read')
+    read (u, nml=VERSION)
+    if (adjustl(fileFormatVersion) /= '0.5') &
+      error stop 'ERROR: config.nml template version does not match'
+    read (u, nml=PARAMS)
+    read (u, nml=OPTIONS)
+    close (u)
+    sw%restartWriteNt = restartWriteNt; sw%restartFromNt = restartFromNt
+    sw%ntSub = ntSub; sw%ntSubInit = ntSubInit; sw%wakePlot = wakePlot
+    sw%wakeTipPlot = wakeTipPlot; sw%rotorForcePlot = rotorForcePlot; sw%gridPlot = gridPlot
+    sw%wakeDissipation = wakeDissipation; sw%wakeStrain = wakeStrain
+    sw%wakeBurst = wakeBurst; sw%wakeSuppress = wakeSuppress; sw%slowStart = slowStart
+    sw%slowStartNt = slowStartNt; sw%fdScheme = fdScheme; sw%initWakeVelNt = initWakeVelNt
+    sw%probe = probe
+  end subroutine read_config
+
+  subroutine rotor_readGeom(r, dir, fname)
+    !! geomXX.nml v0.15 (classdef.f90:2541-2767); absent keys default to 0
+    type(rotor_class), intent(inout) :: r
+    character(len=*), intent(in) :: dir, fname
+    character(len=10) :: fileFormatVersion
+    integer :: surfaceType, imagePlane, imageRotorNum, nb, propConvention, spanSpacing, &
+      & chordSpacing, nCamberFiles, nc, ns, nNwake, nAirfoils
+    real(dp) :: camberSectionLimit(8)
+    character(len=30) :: camberFile(8), geometryFile
+    real(dp), dimension(3) :: hubCoords, cgCoords, fromCoords, phiThetaPsi, shaftAxis, &
+      & velBody, omegaBody, dragUnitVec, sideUnitVec, liftUnitVec
+    real(dp) :: span, rootcut, chord, preconeAngle, Omega, theta0, thetaC, thetaS, &
+      & thetaTwist, pivotLE, flapHinge
+    integer :: spanwiseLiftSwitch, symmetricTau, ductSwitch, axisymmetrySwitch, &
+      & customTrajectorySwitch, forceCalcSwitch, wakeTruncateNt, prescWakeAfterTruncNt, &
+      & prescWakeGenNt
+    real(dp) :: apparentViscCoeff, decayCoeff, spanwiseCore, rollupStartRadius, &
+      & rollupEndRadius, initWakeVel, psiStart, skewLimit
+    real(dp), allocatable :: streamwiseCoreVec(:)
+    integer :: bladeDynamicsSwitch, pitchDynamicsSwitch, bodyDynamicsSwitch, &
+      & bodyDynamicsIOVars, inflowPlotSwitch, gammaPlotSwitch, skewPlotSwitch
+    real(dp) :: flapInitial, dflapInitial, Iflap, cflap, kflap, MflapConstant, dpitch
+    integer :: u
+    namelist /VERSION/ fileFormatVersion
+    namelist /SURFACE/ surfaceType, imagePlane, imageRotorNum
+    namelist /PANELS/ nb, propConvention, spanSpacing, chordSpacing, geometryFile, &
+      & nCamberFiles, nc, ns, nNwake
+    namelist /CAMBERSECTIONS/ camberSectionLimit, camberFile
+    namelist /ORIENT/ hubCoords, cgCoords, fromCoords, phiThetaPsi
+    namelist /GEOMPARAMS/ span, rootcut, chord, preconeAngle, Omega, shaftAxis, theta0, &
+      & thetaC, thetaS, thetaTwist, ductSwitch, axisymmetrySwitch, pivotLE, flapHinge, &
+      & spanwiseLiftSwitch, symmetricTau, customTrajectorySwitch, velBody, omegaBody, &
+      & forceCalcSwitch, nAirfoils
+    namelist /WAKEPARAMS/ apparentViscCoeff, decayCoeff, wakeTruncateNt, &
+      & prescWakeAfterTruncNt, prescWakeGenNt, spanwiseCore, streamwiseCoreVec, &
+      & rollupStartRadius, rollupEndRadius, initWakeVel, psiStart, skewLimit
+    namelist /DYNAMICS/ bladeDynamicsSwitch, flapInitial, dflapInitial, Iflap, cflap, &
+      & kflap, MflapConstant, pitchDynamicsSwitch, dpitch, bodyDynamicsSwitch, &
+      & bodyDynamicsIOVars
+    namelist /WINDFRAME/ dragUnitVec, sideUnitVec, liftUnitVec
+    namelist /PLOTS/ inflowPlotSwitch, gammaPlotSwitch, skewPlotSwitch
+
+    surfaceType = 0; imagePlane = 0; imageRotorNum = 0; nb = 1; propConvention = 0
+    spanSpacing = 1; chordSpacing = 1; nCamberFiles = 0; nc = 1; ns = 1; nNwake = 0
+    nAirfoils = 0; camberSectionLimit = 0._dp; camberFile = '0'; geometryFile = '0'
+    hubCoords = 0._dp; cgCoords = 0._dp; fromCoords = 0._dp; phiThetaPsi = 0._dp
+    shaftAxis = zAxis; velBody = 0._dp; omegaBody = 0._dp
+    dragUnitVec = 0._dp; sideUnitVec = 0._dp; liftUnitVec = 0._dp
+    span = 1._dp; rootcut = 0._dp; chord = 1._dp; preconeAngle = 0._dp; Omega = 0._dp
+    theta0 = 0._dp; thetaC = 0._dp; thetaS = 0._dp; thetaTwist = 0._dp; pivotLE = 0._dp
+    flapHinge = 0._dp; spanwiseLiftSwitch = 0; symmetricTau = 0; ductSwitch = 0
+    axisymmetrySwitch = 0; customTrajectorySwitch = 0; forceCalcSwitch = 0
+    wakeTruncateNt = 0; prescWakeAfterTruncNt = 0; prescWakeGenNt = 0
+    apparentViscCoeff = 0._dp; decayCoeff = 0._dp; spanwiseCore = 0._dp
+    rollupStartRadius = 0._dp; rollupEndRadius = 1._dp; initWakeVel = 0._dp
+    psiStart = 0._dp; skewLimit = 0._dp
+    bladeDynamicsSwitch = 0; pitchDynamicsSwitch = 0; bodyDynamicsSwitch = 0
+    bodyDynamicsIOVars = 0; inflowPlotSwitch = 0; gammaPlotSwitch = 0; skewPlotSwitch = 0
+    flapInitial = 0._dp; dflapInitial = 0._dp; Iflap = 0._dp; cflap = 0._dp; kflap = 0._dp
+    MflapConstant = 0._dp; dpitch = 0._dp
+
+    open (newunit=u, file=trim(dir)//'/'//trim(fname), status='old', action='read')
+    read (u, nml=VERSION)
+    if (adjustl(fileFormatVersion) /= '0.15') &
+      error stop 'ERROR: geomXX.nml template version does not match'
+    read (u, nml=SURFACE)
+    read (u, nml=PANELS)
+    read (u, nml=CAMBERSECTIONS)
+    read (u, nml=ORIENT)
+    read (u, nml=GEOMPARAMS)
+    allocate (streamwiseCoreVec(ns + 1))
+    streamwiseCoreVec = 0._dp
+    read (u, nml=WAKEPARAMS)
+    read (u, nml=DYNAMICS)
+    read (u, nml=WINDFRAME)
+    read (u, nml=PLOTS)
+    close (u)
+
+    if (surfaceType < 0) error stop 'ERROR: image surfaces are not supported'
+    if (surfaceType == 2) error stop 'ERROR: non-lifting surfaces are not supported'
+    if (nCamberFiles > 0) error stop 'ERROR: camber files are not supported'
+    if (ductSwitch /= 0) error stop 'ERROR: ducts are not supported'
+    if (forceCalcSwitch /= 0) error stop 'ERROR: only forceCalcSwitch = 0 is supported'
+    if (bladeDynamicsSwitch /= 0 .or. bodyDynamicsSwitch /= 0 .or. pitchDynamicsSwitch /= 0) &
+      error stop 'ERROR: blade, pitch and body dynamics are not supported'
+    if (customTrajectorySwitch /= 0) error stop 'ERROR: custom trajectories are not supported'
+    if (prescWakeAfterTruncNt /= 0) error stop 'ERROR: the prescribed far wake is not supported'
+
+    r%surfaceType = surfaceType
+    r%nb = nb; r%propConvention = propConvention; r%spanSpacing = spanSpacing
+    r%chordSpacing = chordSpacing; r%geometryFile = geometryFile
+    r%nc = nc; r%ns = ns; r%nNwake = nNwake
+    r%hubCoords = hubCoords; r%cgCoords = cgCoords; r%fromCoords = fromCoords
+    r%pts = phiThetaPsi
+    r%radius = span; r%root_cut = rootcut; r%chord = chord; r%preconeAngle = preconeAngle
+    r%Omega = Omega; r%shaftAxis = shaftAxis; r%controlPitch = [theta0, thetaC, thetaS]
+    r%axisymmetrySwitch = axisymmetrySwitch; r%pivotLE = pivotLE; r%flapHinge = flapHinge
+    r%spanwiseLiftSwitch = spanwiseLiftSwitch; r%symmetricTau = symmetricTau
+    r%velBody = velBody; r%omegaBody = omegaBody; r%forceCalcSwitch = forceCalcSwitch
+    r%apparentViscCoeff = apparentViscCoeff; r%decayCoeff = decayCoeff
+    r%wakeTruncateNt = wakeTruncateNt; r%spanwiseCore = spanwiseCore
+    allocate (r%streamwiseCoreVec(ns + 1))
+    if (norm2(streamwiseCoreVec(2:)) < eps) then
+      r%streamwiseCoreVec = streamwiseCoreVec(1)
+    else
+      r%streamwiseCoreVec = streamwiseCoreVec
+    endif
+    r%rollupStartRadius = rollupStartRadius; r%rollupEndRadius = rollupEndRadius
+    r%psiStart = psiStart
+  end subroutine rotor_readGeom
+
+  subroutine to_chords_revs(r, n, dt)
+    !! Negative counts mean chords or revolutions, classdef.f90:5133-5148
+    type(rotor_class), intent(in) :: r
+    integer, intent(inout) :: n
+    real(dp), intent(in) :: dt
+    if (n < 0) then
+      if (abs(r%Omega) < eps) then
+        n = ceiling(abs(n)*r%chord/(dt*norm2(r%velBody)))
+      else
+        n = ceiling(twoPi*abs(n)/(abs(r%Omega)*dt))
+      endif
+    endif
+  end subroutine to_chords_revs
+
+  ! ------------------------------------------------------------------
+  ! Rigid motion of blades
+  ! ------------------------------------------------------------------
+  subroutine panel_shift(w, d)
+    type(wingpanel_class), intent(inout) :: w
+    real(dp), intent(in) :: d(3)
+    integer :: k
+    w%CP = w%CP + d
+    do k = 1, 4
+      w%PC(:, k) = w%PC(:, k) + d
+      w%vr%fc(:, k) = w%vr%fc(:, k) + d
+    enddo
+  end subroutine panel_shift
+
+  subroutine panel_rot(w, T, o)
+    type(wingpanel_class), intent(inout) :: w
+    real(dp), intent(in) :: T(3, 3), o(3)
+    integer :: k
+    do k = 1, 4
+      w%PC(:, k) = matmul(T, w%PC(:, k) - o) + o
+      w%vr%fc(:, k) = matmul(T, w%vr%fc(:, k) - o) + o
+    enddo
+    w%CP = matmul(T, w%CP - o) + o
+    w%nCap = matmul(T, w%nCap)
+    w%tauCapChord = matmul(T, w%tauCapChord)
+    w%tauCapSpan = matmul(T, w%tauCapSpan)
+  end subroutine panel_rot
+
+  subroutine blade_move(b, d)
+    !! classdef.f90:1092-1112
+    type(blade_class), intent(inout) :: b
+    real(dp), intent(in) :: d(3)
+    integer :: i, j
+    do j = 1, size(b%wiP, 2)
+      do i = 1, size(b%wiP, 1)
+        call panel_shift(b%wiP(i, j), d)
+      enddo
+      b%secCP(:, j) = b%secCP(:, j) + d
+    enddo
+    b%flapOrigin = b%flapOrigin + d
+  end subroutine blade_move
+
+  subroutine blade_rotate(b, ang, ax, org, kind)
+    !! Rotation about an axis through org: translate, rotate, translate
+    !! back (classdef.f90:1193-1281); kind 1 azimuth, 2 flap, 3 pitch
+    type(blade_class), intent(inout) :: b
+    real(dp), intent(in) :: ang, ax(3), org(3)
+    integer, intent(in) :: kind
+    real(dp) :: T(3, 3), zero(3)
+    integer :: i, j
+    if (abs(ang) <= eps) return
+    zero = 0._dp
+    call blade_move(b, -org)
+    T = axis_rotation(ang, ax)
+    do j = 1, size(b%wiP, 2)
+      do i = 1, size(b%wiP, 1)
+        call panel_rot(b%wiP(i, j), T, zero)
+      enddo
+    enddo
+    call blade_move(b, org)
+    do j = 1, size(b%wiP, 2)
+      b%secCP(:, j) = matmul(T, b%secCP(:, j) - org) + org
+      b%secTauCapChord(:, j) = matmul(T, b%secTauCapChord(:, j))
+      b%secTauCapSpan(:, j) = matmul(T, b%secTauCapSpan(:, j))
+      b%secNormalVec(:, j) = matmul(T, b%secNormalVec(:, j))
+    enddo
+    if (kind == 1 .or. kind == 2) then
+      b%xAxisAziFlap = matmul(T, b%xAxisAziFlap)
+      b%yAxisAziFlap = matmul(T, b%yAxisAziFlap)
+      b%zAxisAziFlap = matmul(T, b%zAxisAziFlap)
+    endif
+    if (kind == 1) then
+      b%xAxisAzi = matmul(T, b%xAxisAzi)
+      b%yAxisAzi = matmul(T, b%yAxisAzi)
+      b%zAxisAzi = matmul(T, b%zAxisAzi)
+    endif
+    b%xAxis = matmul(T, b%xAxis)
+    b%yAxis = matmul(T, b%yAxis)
+    b%zAxis = matmul(T, b%zAxis)
+  end subroutine blade_rotate
+
+  subroutine blade_rot_pitch(b, theta)
+    !! Pitch about the line through the root LE/TE pivot, classdef.f90:1164-1181
+    type(blade_class), intent(inout) :: b
+    real(dp), intent(in) :: theta
+    real(dp) :: org(3)
+    integer :: nc
+    if (abs(theta) <= eps) return
+    nc = size(b%wiP, 1)
+    org = b%wiP(1, 1)%PC(:, 1)*(1._dp - b%pivotLE) + b%wiP(nc, 1)%PC(:, 2)*b%pivotLE
+    call blade_rotate(b, theta, b%yAxis, org, 3)
+  end subroutine blade_rot_pitch
+
+  subroutine rotor_rot_pts(r, p, org)
+    !! Euler rotation of the whole rotor about org, classdef.f90:4202-4240
+    type(rotor_class), intent(inout) :: r
+    real(dp), intent(in) :: p(3), org(3)
+    real(dp) :: T(3, 3)
+    integer :: ib, i, j
+    T = euler_gb(p)
+    do ib = 1, r%nb
+      associate (b => r%blade(ib))
+        do j = 1, r%ns
+          do i = 1, r%nc
+            call panel_rot(b%wiP(i, j), T, org)
+          enddo
+          b%secCP(:, j) = matmul(T, b%secCP(:, j) - org) + org
+          b%secTauCapChord(:, j) = matmul(T, b%secTauCapChord(:, j))
+          b%secNormalVec(:, j) = matmul(T, b%secNormalVec(:, j))
+        enddo
+        b%xAxis = matmul(T, b%xAxis); b%yAxis = matmul(T, b%yAxis)
+        b%zAxis = matmul(T, b%zAxis)
+        b%xAxisAzi = matmul(T, b%xAxisAzi); b%yAxisAzi = matmul(T, b%yAxisAzi)
+        b%zAxisAzi = matmul(T, b%zAxisAzi)
+        b%xAxisAziFlap = matmul(T, b%xAxisAziFlap)
+        b%yAxisAziFlap = matmul(T, b%yAxisAziFlap)
+        b%zAxisAziFlap = matmul(T, b%zAxisAziFlap)
+      end associate
+    enddo
+    r%shaftAxis = matmul(T, r%shaftAxis)
+    r%xAxisBody = matmul(T, r%xAxisBody)
+    r%yAxisBody = matmul(T, r%yAxisBody)
+    r%zAxisBody = matmul(T, r%zAxisBody)
+    r%hubCoords = matmul(T, r%hubCoords - org) + org
+    r%cgCoords = matmul(T, r%cgCoords - org) + org
+  end subroutine rotor_rot_pts
+
+  function rotor_gettheta(r, psi, ib) result(th)
+    !! Collective plus cyclic pitch, classdef.f90:4114-4136
+    type(rotor_class), intent(in) :: r
+    real(dp), intent(in) :: psi
+    integer, intent(in) :: ib
+    real(dp) :: th, off
+    off = twoPi/r%nb*(ib - 1)
+    th = r%controlPitch(1) + r%controlPitch(2)*cos(psi + off) + &
+      & r%controlPitch(3)*sin(psi + off)
+  end function rotor_gettheta
+
+  subroutine rotor_rot_advance(r, dpsi, nopitch)
+    !! Azimuthal advance with pitch schedule, classdef.f90:4254-4291
+    type(rotor_class), intent(inout) :: r
+    real(dp), intent(in) :: dpsi
+    logical, intent(in) :: nopitch
+    integer :: ib
+    real(dp) :: thNext
+    r%psi = r%psi + dpsi
+    do ib = 1, r%nb
+      call blade_rotate(r%blade(ib), dpsi, r%shaftAxis, r%hubCoords, 1)
+      r%blade(ib)%psi = r%blade(ib)%psi + dpsi
+      if (.not. nopitch) then
+        thNext = rotor_gettheta(r, r%psi, ib)
+        call blade_rot_pitch(r%blade(ib), thNext - r%blade(ib)%theta)
+        r%blade(ib)%theta = thNext
+      endif
+    enddo
+  end subroutine rotor_rot_advance
+
+  subroutine rotor_move(r, d)
+    type(rotor_class), intent(inout) :: r
+    real(dp), intent(in) :: d(3)
+    integer :: ib
+    do ib = 1, r%nb
+      call blade_move(r%blade(ib), d)
+    enddo
+    r%hubCoords = r%hubCoords + d
+    r%cgCoords = r%cgCoords + d
+  end subroutine rotor_move
+
+  ! ------------------------------------------------------------------
+  ! Geometry initialisation, classdef.f90:2769-3891
+  ! ------------------------------------------------------------------
+  subroutine rotor_init(r, ir, density, dt, nt, sw, dir)
+    type(rotor_class), intent(inout) :: r
+    integer, intent(in) :: ir
+    real(dp), intent(in) :: density
+    real(dp), intent(inout) :: dt
+    integer, intent(inout) :: nt
+    type(switches_class), intent(inout) :: sw
+    character(len=*), intent(in) :: dir
+    real(dp) :: xVec(r%nc + 1), yVec(r%ns + 1), grid(3, r%nc + 1, r%ns + 1)
+    real(dp) :: dxdymin, velShed, xs(4), dummy
+    integer :: ib, i, j, nx, ny, nz, u, ic, is
+
+    write (r%id, '(I0.2)') ir
+    if (dt < 0._dp) then
+      if (abs(r%Omega) < eps) then
+        dt = abs(dt)*r%chord/norm2(r%velBody)
+      else
+        dt = twoPi*abs(dt)/abs(r%Omega)
+      endif
+    endif
+    if (abs(dt) <= eps) then
+      if (abs(r%Omega) < eps) then
+        dt = (r%chord/r%nc)/norm2(r%velBody)
+      else
+        dt = 5._dp*degToRad/abs(r%Omega)
+      endif
+    endif
+    if (nt <= 0) then
+      if (nt == 0) nt = -10
+      call to_chords_revs(r, nt, dt)
+    endif
+    if (sw%slowStart /= 0) call to_chords_revs(r, sw%slowStartNt, dt)
+    call to_chords_revs(r, r%wakeTruncateNt, dt)
+    call to_chords_revs(r, r%nNwake, dt)
+    if (r%wakeTruncateNt > 0) r%wakeTruncateNt = max(r%wakeTruncateNt, r%nNwake + 1)
+    if (r%surfaceType == 0) r%surfaceType = 1
+    if (r%nNwake > 0 .and. r%nNwake < 2) error stop 'ERROR: Atleast 2 near wake rows mandatory'
+    r%nNwake = min(r%nNwake, nt)
+    if (r%wakeTruncateNt == 0) then
+      r%nFwake = nt - r%nNwake
+    else
+      r%nFwake = r%wakeTruncateNt - r%nNwake
+    endif
+    r%nbConvect = r%nb
+    if (r%axisymmetrySwitch == 1) r%nbConvect = 1
+
+    allocate (r%blade(r%nb))
+    allocate (r%gamVec(r%nc*r%ns*r%nb), r%gamVecPrev(r%nc*r%ns*r%nb), r%RHS(r%nc*r%ns*r%nb))
+    r%gamVec = 0._dp; r%gamVecPrev = 0._dp; r%RHS = 0._dp
+    do ib = 1, r%nb
+      associate (b => r%blade(ib))
+        allocate (b%wiP(r%nc, r%ns))
+        allocate (b%secCP(3, r%ns), b%secTauCapChord(3, r%ns), b%secTauCapSpan(3, r%ns))
+        allocate (b%secNormalVec(3, r%ns), b%secChordwiseResVel(3, r%ns))
+        allocate (b%secLift(3, r%ns), b%secLiftDir(3, r%ns), b%secDragDir(3, r%ns))
+        allocate (b%secForceInertial(3, r%ns), b%secLiftUnsteady(3, r%ns))
+        allocate (b%secLiftInPlane(3, r%ns), b%secLiftOutPlane(3, r%ns))
+        allocate (b%secChord(r%ns), b%secArea(r%ns), b%secCL(r%ns), b%secCLu(r%ns))
+        allocate (b%secAlpha(r%ns), b%secMflapArm(r%ns))
+        b%secCP = 0._dp; b%secTauCapChord = 0._dp; b%secTauCapSpan = 0._dp
+        b%secNormalVec = 0._dp; b%secChordwiseResVel = 0._dp; b%secLift = 0._dp
+        b%secLiftDir = 0._dp; b%secDragDir = 0._dp; b%secForceInertial = 0._dp
+        b%secLiftUnsteady = 0._dp; b%secLiftInPlane = 0._dp; b%secLiftOutPlane = 0._dp
+        b%secChord = 0._dp; b%secArea = 0._dp; b%secCL = 0._dp; b%secCLu = 0._dp
+        b%secAlpha = 0._dp; b%secMflapArm = 0._dp
+      end associate
+    enddo
+
+    r%controlPitch = r%controlPitch*degToRad
+    r%pts = r%pts*degToRad
+    r%preconeAngle = r%preconeAngle*degToRad
+    r%psiStart = r%psiStart*degToRad
+    r%spanwiseCore = r%spanwiseCore*r%chord
+    r%streamwiseCoreVec = r%streamwiseCoreVec*r%chord
+    r%rollupStart = ceiling(r%rollupStartRadius*r%ns)
+    r%rollupEnd = floor(r%rollupEndRadius*r%ns)
+
+    ! Panel corners
+    if (r%geometryFile(1:1) == '0') then
+      if (r%Omega >= 0._dp) then
+        xVec = spaced(r%chordSpacing, -r%chord, 0._dp, r%nc + 1)
+      else
+        xVec = spaced(r%chordSpacing, r%chord, 0._dp, r%nc + 1)
+      endif
+      yVec = spaced(r%spanSpacing, r%root_cut*r%radius, r%radius, r%ns + 1)
+      do j = 1, r%ns + 1
+        do i = 1, r%nc + 1
+          grid(:, i, j) = [xVec(i), yVec(j), 0._dp]
+        enddo
+      enddo
+    else
+      open (newunit=u, file=trim(dir)//'/'//trim(r%geometryFile), status='old', action='read')
+      read (u, *) nx, ny, nz
+      if (nz > 1 .or. nx /= r%nc + 1 .or. ny /= r%ns + 1) &
+        error stop 'ERROR: Wrong or conflicting data in PLOT3D file'
+      read (u, *) ((grid(1, i, j), i=1, nx), j=1, ny), ((grid(2, i, j), i=1, nx), j=1, ny), &
+        & ((grid(3, i, j), i=1, nx), j=1, ny)
+      close (u)
+    endif
+    do ib = 1, r%nb
+      do j = 1, r%ns
+        do i = 1, r%nc
+          r%blade(ib)%wiP(i, j)%PC(:, 1) = grid(:, i, j)
+          r%blade(ib)%wiP(i, j)%PC(:, 2) = grid(:, i + 1, j)
+          r%blade(ib)%wiP(i, j)%PC(:, 3) = grid(:, i + 1, j + 1)
+          r%blade(ib)%wiP(i, j)%PC(:, 4) = grid(:, i, j + 1)
+        enddo
+      enddo
+    enddo
+
+    do ib = 1, r%nb
+      associate (b => r%blade(ib))
+        b%xAxis = xAxis; b%yAxis = yAxis; b%zAxis = zAxis
+        b%xAxisAzi = xAxis; b%yAxisAzi = yAxis; b%zAxisAzi = zAxis
+        b%xAxisAziFlap = xAxis; b%yAxisAziFlap = yAxis; b%zAxisAziFlap = zAxis
+        b%flapOrigin = b%yAxis*r%radius*r%flapHinge
+        do j = 1, r%ns
+          b%secTauCapChord(:, j) = unit3((b%wiP(r%nc, j)%PC(:, 3) + b%wiP(r%nc, j)%PC(:, 2) - &
+            & (b%wiP(1, j)%PC(:, 4) + b%wiP(1, j)%PC(:, 1)))*0.5_dp)
+          b%secTauCapSpan(:, j) = yAxis
+          b%secNormalVec(:, j) = sign(1._dp, r%Omega)*unit3(cross3( &
+            & b%wiP(r%nc, j)%PC(:, 2) - b%wiP(1, j)%PC(:, 4), &
+            & b%wiP(r%nc, j)%PC(:, 3) - b%wiP(1, j)%PC(:, 1)))
+        enddo
+        ! vortex ring corners a quarter panel downstream of the panel corners
+        do j = 1, r%ns
+          do i = 1, r%nc
+            associate (w => b%wiP(i, j))
+              xs(1) = (w%PC(1, 2) - w%PC(1, 1))*0.25_dp
+              xs(4) = (w%PC(1, 3) - w%PC(1, 4))*0.25_dp
+              if (i < r%nc) then
+                xs(2) = (b%wiP(i + 1, j)%PC(1, 2) - w%PC(1, 2))*0.25_dp
+                xs(3) = (b%wiP(i + 1, j)%PC(1, 3) - w%PC(1, 3))*0.25_dp
+              else
+                xs(2) = 0._dp
+                xs(3) = 0._dp
+              endif
+              do is = 1, 4
+                w%vr%fc(:, is) = [w%PC(1, is) + xs(is), w%PC(2, is), w%PC(3, is)]
+              enddo
+            end associate
+          enddo
+        enddo
+        dxdymin = huge(1._dp)
+        do is = 1, r%ns
+          do ic = 1, r%nc
+            dxdymin = min(dxdymin, abs(norm2(b%wiP(ic, is)%PC(:, 2) - b%wiP(ic, is)%PC(:, 1))), &
+              & abs(norm2(b%wiP(ic, is)%PC(:, 3) - b%wiP(ic, is)%PC(:, 2))))
+          enddo
+        enddo
+        ! shed the trailing vortex row slightly downstream (single-precision
+        ! 0.05 factor, as in the reference's literal)
+        if (abs(r%Omega) > eps) then
+          velShed = min(0.05*abs(r%Omega)*norm2(b%wiP(r%nc, r%ns)%vr%fc(:, 2) - r%hubCoords), &
+            & 0.125_dp*r%chord/dt)
+        else
+          velShed = 0.3_dp*norm2(-1._dp*r%velBody)
+        endif
+        do j = 1, r%ns
+          b%wiP(r%nc, j)%vr%fc(:, 2) = b%wiP(r%nc, j)%vr%fc(:, 2) + &
+            & [sign(1._dp, r%Omega)*velShed*dt, 0._dp, 0._dp]
+          b%wiP(r%nc, j)%vr%fc(:, 3) = b%wiP(r%nc, j)%vr%fc(:, 3) + &
+            & [sign(1._dp, r%Omega)*velShed*dt, 0._dp, 0._dp]
+        enddo
+        do j = 1, r%ns
+          do i = 1, r%nc
+            associate (w => b%wiP(i, j))
+              w%CP = ((w%PC(:, 1) + w%PC(:, 4))*0.25_dp + (w%PC(:, 2) + w%PC(:, 3))*0.75_dp)*0.5_dp
+              w%nCap = unit3(cross3(w%PC(:, 3) - w%PC(:, 1), w%PC(:, 4) - w%PC(:, 2)))
+              if (sign(1._dp, r%Omega) < 0._dp) w%nCap = -1._dp*w%nCap
+              w%tauCapChord = unit3(0.5_dp*((w%PC(:, 2) + w%PC(:, 3)) - (w%PC(:, 1) + w%PC(:, 4))))
+              w%tauCapSpan = unit3(0.5_dp*((w%PC(:, 3) + w%PC(:, 4)) - (w%PC(:, 2) + w%PC(:, 1))))
+              w%panelArea = 0.5_dp*norm2(cross3(w%PC(:, 3) - w%PC(:, 1), w%PC(:, 4) - w%PC(:, 2)))
+              w%meanSpan = 0.5_dp*(norm2(w%PC(:, 4) - w%PC(:, 1)) + norm2(w%PC(:, 3) - w%PC(:, 2)))
+              w%meanChord = 0.5_dp*(norm2(w%PC(:, 2) - w%PC(:, 1)) + norm2(w%PC(:, 3) - w%PC(:, 4)))
+            end associate
+          enddo
+        enddo
+        do is = 1, r%ns
+          b%secArea(is) = sum(b%wiP(:, is)%panelArea)
+          b%secChord(is) = norm2(0.5_dp*((b%wiP(1, is)%PC(:, 1) + b%wiP(1, is)%PC(:, 4)) - &
+            & (b%wiP(r%nc, is)%PC(:, 2) + b%wiP(r%nc, is)%PC(:, 3))))
+        enddo
+        do j = 1, r%ns
+          b%secTauCapSpan(:, j) = b%yAxis
+          do i = 1, r%nc
+            b%wiP(i, j)%tauCapSpan = b%yAxis
+          enddo
+        enddo
+        if (r%symmetricTau == 1) then
+          do j = 1, r%ns/2
+            b%secTauCapSpan(:, j) = -1._dp*b%secTauCapSpan(:, j)
+            do i = 1, r%nc
+              b%wiP(i, j)%tauCapSpan = -1._dp*b%wiP(i, j)%tauCapSpan
+            enddo
+          enddo
+        endif
+        call blade_sec_locations(b, 0.5_dp, r%flapHinge*r%radius)
+        b%pivotLE = r%pivotLE
+        do j = 1, r%ns
+          do i = 1, r%nc
+            b%wiP(i, j)%vr%gam = 0._dp
+            b%wiP(i, j)%vr%rVc = min(r%spanwiseCore, dxdymin*0.1_dp)
+          enddo
+          b%wiP(r%nc, j)%vr%rVc(2) = r%spanwiseCore
+        enddo
+      end associate
+    enddo
+
+    do ib = 1, r%nb
+      call blade_move(r%blade(ib), r%hubCoords - r%fromCoords)
+    enddo
+    if (abs(r%preconeAngle) > eps) then
+      do ib = 1, r%nb
+        call blade_rotate(r%blade(ib), r%preconeAngle, r%blade(ib)%xAxisAzi, &
+          & r%blade(ib)%flapOrigin, 2)
+      enddo
+    endif
+    do ib = 2, r%nb
+      call blade_rotate(r%blade(ib), sign(1._dp, r%Omega)*twoPi/r%nb*(ib - 1), r%shaftAxis, &
+        & r%hubCoords, 1)
+    enddo
+    call rotor_rot_pts(r, r%pts, r%cgCoords)
+    call rotor_rot_advance(r, sign(1._dp, r%Omega)*r%psiStart, .true.)
+
+    if (abs(r%Omega) > eps) then
+      if (r%propConvention == 0) then
+        r%nonDimforceDenominator = density*(pi*r%radius**2)*(r%radius*r%Omega)**2
+      else
+        r%nonDimforceDenominator = density*(r%Omega/twoPi)**2*(2._dp*r%radius)**4
+      endif
+    else
+      r%nonDimforceDenominator = 0.5_dp*density*(r%radius*(1._dp - r%root_cut)*r%chord)* &
+        & dot_product(r%velBody, r%velBody)
+    endif
+    dummy = 0._dp
+  end subroutine rotor_init
+
+  subroutine blade_sec_locations(b, frac, hingeR)
+    !! Section reference points and flap arms, classdef.f90:2267-2304
+    type(blade_class), intent(inout) :: b
+    real(dp), intent(in) :: frac, hingeR
+    real(dp) :: xz(2, size(b%wiP, 1) + 1), vPC(3), vLE(3), xcp, zcp
+    integer :: is, ic, nc
+    nc = size(b%wiP, 1)
+    do is = 1, size(b%wiP, 2)
+      vLE = 0.5_dp*(b%wiP(1, is)%PC(:, 1) + b%wiP(1, is)%PC(:, 4))
+      xz(:, 1) = 0._dp
+      do ic = 1, nc
+        vPC = 0.5_dp*(b%wiP(ic, is)%PC(:, 2) + b%wiP(ic, is)%PC(:, 3)) - vLE
+        xz(1, ic + 1) = norm2(proj3(vPC, b%secTauCapChord(:, is)))
+        xz(2, ic + 1) = dot_product(vPC, b%secNormalVec(:, is))
+      enddo
+      xcp = norm2(vPC)*frac
+      zcp = pwl(xz(1, :), xz(2, :), xcp)
+      b%secCP(:, is) = vLE + xcp*b%secTauCapChord(:, is) + zcp*b%secNormalVec(:, is)
+      b%secMflapArm(is) = norm2(proj3(b%secCP(:, is), b%yAxis)) - hingeR
+    enddo
+  end subroutine blade_sec_locations
+
+  ! ------------------------------------------------------------------
+  ! Device hand-off of the wing
+  ! ------------------------------------------------------------------
+  subroutine rotor_upload_wing(ctx, r, with_gam)
+    type(c_ptr), intent(in) :: ctx
+    type(rotor_class), intent(in) :: r
+    logical, intent(in) :: with_gam
+    real(c_double) :: wc(3, 4, r%nc, r%ns, r%nb), wr(4, r%nc, r%ns, r%nb)
+    real(c_double) :: wg(r%nc, r%ns, r%nb)
+    integer :: ib, i, j
+    do ib = 1, r%nb
+      do j = 1, r%ns
+        do i = 1, r%nc
+          wc(:, :, i, j, ib) = r%blade(ib)%wiP(i, j)%vr%fc
+          wr(:, i, j, ib) = r%blade(ib)%wiP(i, j)%vr%rVc
+          wg(i, j, ib) = r%blade(ib)%wiP(i, j)%vr%gam
+        enddo
+      enddo
+    enddo
+    if (with_gam) then
+      call vlm_check(ctx, vlm_rotor_set_wing(ctx, r%dev, wc, wr, wg), 'rotor_upload_wing')
+    else
+      call vlm_check(ctx, vlm_rotor_set_wing(ctx, r%dev, wc, wr, wg), 'rotor_upload_wing')
+    endif
+    call vlm_check(ctx, vlm_rotor_set_axis(ctx, r%dev, r%shaftAxis, r%hubCoords), &
+      & 'rotor_set_axis')
+  end subroutine rotor_upload_wing
+
+  subroutine rotor_cp_arrays(r, cp, ncap)
+    type(rotor_class), intent(in) :: r
+    real(c_double), intent(out) :: cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb)
+    integer :: ib, i, j
+    do ib = 1, r%nb
+      do j = 1, r%ns
+        do i = 1, r%nc
+          cp(:, i, j, ib) = r%blade(ib)%wiP(i, j)%CP
+          ncap(:, i, j, ib) = r%blade(ib)%wiP(i, j)%nCap
+        enddo
+      enddo
+    enddo
+  end subroutine rotor_cp_arrays
+
+  subroutine rotor_map_gam(r)
+    !! classdef.f90:4181-4196
+    type(rotor_class), intent(inout) :: r
+    integer :: ib, i, j
+    do ib = 1, r%nbConvect
+      do j = 1, r%ns
+        do i = 1, r%nc
+          r%blade(ib)%wiP(i, j)%vr%gam = r%gamVec(i + r%nc*(j - 1) + r%ns*r%nc*(ib - 1))
+        enddo
+      enddo
+    enddo
+    if (r%axisymmetrySwitch == 1) then
+      do ib = 2, r%nb
+        r%blade(ib)%wiP%vr%gam = r%blade(1)%wiP%vr%gam
+      enddo
+    endif
+  end subroutine rotor_map_gam
+
+  ! ------------------------------------------------------------------
+  ! RHS, solve and loads
+  ! ------------------------------------------------------------------
+  subroutine compute_rhs_and_solve(ctx, rotor, nr, with_wake)
+    !! main.f90:124-202 (initial) and 521-603 (every step)
+    type(c_ptr), intent(in) :: ctx
+    type(rotor_class), intent(inout) :: rotor(:)
+    integer, intent(in) :: nr
+    logical, intent(in) :: with_wake
+    integer :: ir, jr, ib, i, j, n, row, it
+    real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :), v(:, :, :, :), gam(:)
+
+    do ir = 1, nr
+      associate (r => rotor(ir))
+        allocate (cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb), v(3, r%nc, r%ns, r%nb))
+        call rotor_cp_arrays(r, cp, ncap)
+        n = r%nc*r%ns*r%nb
+        v = 0._dp
+        do ib = 1, r%nbConvect
+          do j = 1, r%ns
+            do i = 1, r%nc
+              associate (w => r%blade(ib)%wiP(i, j))
+                w%velCP = -1._dp*r%velBody - cross3(r%omegaBody, w%CP - r%cgCoords) &
+                  & - cross3(r%omegaSlow*r%shaftAxis, w%CP - r%hubCoords)
+                w%velCPm = w%velCP
+              end associate
+            enddo
+          enddo
+        enddo
+        do jr = 1, nr
+          if (with_wake .and. rotor(jr)%nNwake > 0) &
+            & call vlm_check(ctx, vlm_rotor_induced_velocity(ctx, rotor(jr)%dev, VLM_SET_WAKE, &
+            & int(n, c_int64_t), cp, v, 1_c_int), 'RHS wake')
+          if (jr /= ir) call vlm_check(ctx, vlm_rotor_induced_velocity(ctx, rotor(jr)%dev, &
+            & VLM_SET_WING, int(n, c_int64_t), cp, v, 1_c_int), 'RHS wing')
+        enddo
+        r%RHS = 0._dp
+        do ib = 1, r%nbConvect
+          do j = 1, r%ns
+            do i = 1, r%nc
+              row = i + r%nc*(j - 1) + r%ns*r%nc*(ib - 1)
+              associate (w => r%blade(ib)%wiP(i, j))
+                w%velCP = w%velCP + v(:, i, j, ib)
+                r%RHS(row) = dot_product(w%velCP, w%nCap)
+              end associate
+            enddo
+          enddo
+        enddo
+        if (r%axisymmetrySwitch == 1) then
+          do ib = 2, r%nb
+            r%RHS(r%nc*r%ns*(ib - 1) + 1:r%nc*r%ns*ib) = r%RHS(1:r%nc*r%ns)
+          enddo
+        endif
+        r%RHS = -1._dp*r%RHS
+        deallocate (cp, ncap, v)
+      end associate
+    enddo
+    do ir = 1, nr
+      associate (r => rotor(ir))
+        r%gamVecPrev = r%gamVec
+        allocate (gam(size(r%gamVec)))
+        call vlm_check(ctx, vlm_rotor_solve(ctx, r%dev, r%RHS, gam, 3_c_int, 1.e-15_c_double, it), &
+          & 'rotor_solve')
+        r%gamVec = gam
+        deallocate (gam)
+        call rotor_map_gam(r)
+        call rotor_upload_wing(ctx, r, .true.)
+      end associate
+    enddo
+  end subroutine compute_rhs_and_solve
+
+  subroutine blade_sec_resvel(b)
+    !! Section resultant velocity, classdef.f90:2197-2232
+    type(blade_class), intent(inout) :: b
+    integer :: is, ic, i, nc
+    real(dp) :: xd(size(b%wiP, 1)), yd(size(b%wiP, 1))
+    nc = size(b%wiP, 1)
+    do is = 1, size(b%wiP, 2)
+      do ic = 1, nc
+        b%wiP(ic, is)%chordwiseResVel = noproj3(b%wiP(ic, is)%velCPTotal, b%wiP(ic, is)%tauCapSpan)
+        xd(ic) = dot_product(b%wiP(ic, is)%CP - b%wiP(1, is)%PC(:, 1), b%secTauCapChord(:, is))
+      enddo
+      do i = 1, 3
+        yd = b%wiP(:, is)%chordwiseResVel(i)
+        if (nc >= 3) then
+          b%secChordwiseResVel(i, is) = lsq2_eval(dot_product(b%secCP(:, is) - &
+            & b%wiP(1, is)%PC(:, 1), b%secTauCapChord(:, is)), xd, yd)
+        else
+          b%secChordwiseResVel(i, is) = sum(yd)/nc
+        endif
+      enddo
+    enddo
+  end subroutine blade_sec_resvel
+
+  subroutine compute_forces(ctx, rotor, nr, ir, density, dt)
+    !! forceCalcSwitch = 0 branch of main.f90:247-290 / 632-668 and
+    !! rotor_calc_force + blade_calc_force (classdef.f90:1704-1896, 4607-4671)
+    type(c_ptr), intent(in) :: ctx
+    type(rotor_class), intent(inout) :: rotor(:)
+    integer, intent(in) :: nr, ir
+    real(dp), intent(in) :: density, dt
+    integer :: jr, ib, i, j, is, ic, n
+    real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :), v(:, :, :, :)
+    real(dp) :: vtc(rotor(ir)%nc, rotor(ir)%ns), gec(rotor(ir)%nc, rotor(ir)%ns)
+    real(dp) :: ges(rotor(ir)%nc, rotor(ir)%ns), q(rotor(ir)%ns)
+    real(dp) :: inv, sgn, vts
+
+    associate (r => rotor(ir))
+      n = r%nc*r%ns*r%nb
+      allocate (cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb), v(3, r%nc, r%ns, r%nb))
+      call rotor_cp_arrays(r, cp, ncap)
+      v = 0._dp
+      do jr = 1, nr
+        call vlm_check(ctx, vlm_rotor_induced_velocity(ctx, rotor(jr)%dev, VLM_SET_WING_BOUND, &
+          & int(n, c_int64_t), cp, v, 1_c_int), 'bound vortices')
+      enddo
+      v = -v
+      call vlm_check(ctx, vlm_rotor_induced_velocity(ctx, r%dev, VLM_SET_WING, &
+        & int(n, c_int64_t), cp, v, 1_c_int), 'wing self')
+      do ib = 1, r%nbConvect
+        do j = 1, r%ns
+          do i = 1, r%nc
+            r%blade(ib)%wiP(i, j)%velCPTotal = r%blade(ib)%wiP(i, j)%velCP + v(:, i, j, ib)
+          enddo
+        enddo
+      enddo
+      if (r%axisymmetrySwitch == 1) then
+        do ib = 2, r%nb
+          r%blade(ib)%wiP%velCPTotal(1) = r%blade(1)%wiP%velCPTotal(1)
+          r%blade(ib)%wiP%velCPTotal(2) = r%blade(1)%wiP%velCPTotal(2)
+          r%blade(ib)%wiP%velCPTotal(3) = r%blade(1)%wiP%velCPTotal(3)
+        enddo
+      endif
+      deallocate (cp, ncap, v)
+
+      ! calc_secAlpha (classdef.f90:2234-2265), alpha only
+      do ib = 1, r%nbConvect
+        call blade_sec_resvel(r%blade(ib))
+        do is = 1, r%ns
+          r%blade(ib)%secAlpha(is) = atan2(dot_product(r%blade(ib)%secChordwiseResVel(:, is), &
+            & r%blade(ib)%secNormalVec(:, is)), dot_product(r%blade(ib)%secChordwiseResVel(:, is), &
+            & r%blade(ib)%secTauCapChord(:, is)))
+        enddo
+      enddo
+
+      r%forceInertial = 0._dp
+      ! dirLiftDrag
+      do ib = 1, r%nbConvect
+        call blade_sec_resvel(r%blade(ib))
+        do is = 1, r%ns
+          r%blade(ib)%secDragDir(:, is) = unit3(r%blade(ib)%secChordwiseResVel(:, is))
+          r%blade(ib)%secLiftDir(:, is) = sign(1._dp, r%Omega)*unit3(cross3( &
+            & r%blade(ib)%secDragDir(:, is), r%blade(ib)%yAxisAziFlap))
+        enddo
+      enddo
+
+      inv = -1._dp*sign(1._dp, r%Omega)
+      do ib = 1, r%nbConvect
+        associate (b => r%blade(ib))
+          b%secForceInertial = 0._dp; b%secLift = 0._dp; b%secLiftUnsteady = 0._dp
+          do is = 1, r%ns
+            do ic = 1, r%nc
+              vtc(ic, is) = dot_product(b%wiP(ic, is)%velCP, b%wiP(ic, is)%tauCapChord)
+            enddo
+          enddo
+          do is = 1, r%ns
+            gec(1, is) = b%wiP(1, is)%vr%gam
+          enddo
+          do ic = 2, r%nc
+            gec(ic, 1) = b%wiP(ic, 1)%vr%gam - b%wiP(ic - 1, 1)%vr%gam
+          enddo
+          do ic = 1, r%nc
+            ges(ic, 1) = b%wiP(ic, 1)%vr%gam
+          enddo
+          do is = 2, r%ns
+            ges(1, is) = b%wiP(1, is)%vr%gam - b%wiP(1, is - 1)%vr%gam
+          enddo
+          do is = 2, r%ns
+            do ic = 2, r%nc
+              gec(ic, is) = b%wiP(ic, is)%vr%gam - b%wiP(ic - 1, is)%vr%gam
+              ges(ic, is) = b%wiP(ic, is)%vr%gam - b%wiP(ic, is - 1)%vr%gam
+            enddo
+          enddo
+          gec = inv*gec
+          ges = inv*ges
+          do is = 1, r%ns
+            do ic = 1, r%nc
+              associate (w => b%wiP(ic, is))
+                if (ic > 1) then
+                  w%gamTrapz = inv*0.5_dp*(w%vr%gam + b%wiP(ic - 1, is)%vr%gam)
+                else
+                  w%gamTrapz = inv*0.5_dp*w%vr%gam
+                endif
+                w%delPUnsteady = density*(w%gamTrapz - w%gamPrev)/dt
+                w%delP = w%delPUnsteady + density*vtc(ic, is)*gec(ic, is)/w%meanChord
+                if (r%spanwiseLiftSwitch /= 0) then
+                  vts = dot_product(w%velCP, w%tauCapSpan)
+                  w%delP = w%delP + density*vts*ges(ic, is)/w%meanSpan
+                endif
+                w%gamPrev = w%gamTrapz
+                w%normalForce = w%delP*w%panelArea*w%nCap
+                w%normalForceUnsteady = w%delPUnsteady*w%panelArea*w%nCap
+                b%secForceInertial(:, is) = b%secForceInertial(:, is) + w%normalForce
+                b%secLift(:, is) = b%secLift(:, is) + proj3(w%normalForce, b%secLiftDir(:, is))
+                b%secLiftUnsteady(:, is) = b%secLiftUnsteady(:, is) + &
+                  & proj3(w%normalForceUnsteady, b%secLiftDir(:, is))
+              end associate
+            enddo
+            b%secLiftInPlane(:, is) = inv*proj3(b%secLift(:, is), b%xAxisAziFlap)
+            b%secLiftOutPlane(:, is) = proj3(b%secLift(:, is), b%zAxisAziFlap)
+          enddo
+          do is = 1, r%ns
+            q(is) = 0.5_dp*density*norm2(b%secChordwiseResVel(:, is))**2
+          enddo
+          do is = 1, r%ns
+            if (abs(q(is)) > eps) then
+              sgn = sign(1._dp, dot_product(b%secLift(:, is), b%zAxisAziFlap))
+              b%secCL(is) = norm2(b%secLift(:, is))*sgn/(q(is)*b%secArea(is))
+              b%secCLu(is) = norm2(b%secLiftUnsteady(:, is))*sgn/(q(is)*b%secArea(is))
+            else
+              b%secCL(is) = 0._dp
+              b%secCLu(is) = 0._dp
+            endif
+          enddo
+          b%forceInertial = sum(b%secForceInertial, 2)
+          b%lift = sum(b%secLift, 2)
+          b%drag = 0._dp
+          b%liftUnsteady = sum(b%secLiftUnsteady, 2)
+        end associate
+      enddo
+      if (r%axisymmetrySwitch == 1) then
+        do ib = 2, r%nb
+          r%blade(ib)%wiP%gamPrev = r%blade(1)%wiP%gamPrev
+        enddo
+        r%forceInertial = r%nb*r%blade(1)%forceInertial
+        r%lift = r%nb*r%blade(1)%lift
+        r%drag = 0._dp
+        r%liftUnsteady = r%nb*r%blade(1)%liftUnsteady
+      else
+        r%forceInertial = 0._dp; r%lift = 0._dp; r%drag = 0._dp; r%liftUnsteady = 0._dp
+        do ib = 1, r%nbConvect
+          r%forceInertial = r%forceInertial + r%blade(ib)%forceInertial
+          r%lift = r%lift + r%blade(ib)%lift
+          r%liftUnsteady = r%liftUnsteady + r%blade(ib)%liftUnsteady
+        enddo
+      endif
+    end associate
+  end subroutine compute_forces
+
+  function rotor_CT(r) result(ct)
+    !! Non-dimensional lift (CL or CT), libPostprocess.f90:824-828
+    type(rotor_class), intent(in) :: r
+    real(dp) :: ct
+    ct = sign(1._dp, dot_product(r%lift, r%zAxisBody))*norm2(r%lift)/r%nonDimforceDenominator
+  end function rotor_CT
+
+  subroutine force_line(u, r, it)
+    integer, intent(in) :: u, it
+    type(rotor_class), intent(in) :: r
+    character(len=5) :: ts
+    real(dp) :: d
+    write (ts, '(I0.5)') it
+    d = r%nonDimforceDenominator
+    write (u, '(A,9(E15.7))') ts, rotor_CT(r), norm2(r%drag)/d, norm2(r%liftUnsteady)/d, &
+      & 0._dp, 0._dp, 0._dp, r%forceInertial(1)/d, r%forceInertial(2)/d, r%forceInertial(3)/d
+  end subroutine force_line
+
+  subroutine dist_file(path, r)
+    !! Sectional loads table, libPostprocess.f90:853-880 (subset of columns
+    !! computed here: secSpan, secCL, secCLu, secLift, secArea, secVel,
+    !! secChord, secAlpha)
+    character(len=*), intent(in) :: path
+    type(rotor_class), intent(in) :: r
+    integer :: u, is
+    open (newunit=u, file=path, status='replace', action='write')
+    write (u, '(8(A15))') 'secSpan', 'secCL', 'secCLu', 'secLift', 'secArea', 'secVel', &
+      & 'secChord', 'secAlpha'
+    do is = 1, r%ns
+      write (u, '(8(E15.7))') dot_product(r%blade(1)%secCP(:, is) - r%hubCoords, r%blade(1)%yAxis), &
+        & r%blade(1)%secCL(is), r%blade(1)%secCLu(is), norm2(r%blade(1)%secLift(:, is)), &
+        & r%blade(1)%secArea(is), norm2(r%blade(1)%secChordwiseResVel(:, is)), &
+        & r%blade(1)%secChord(is), r%blade(1)%secAlpha(is)*radToDeg
+    enddo
+    close (u)
+  end subroutine dist_file
+
+  ! ------------------------------------------------------------------
+  ! The time loop, main.f90:1-1459
+  ! ------------------------------------------------------------------
+  subroutine run_case(dir, precision, nt_override, verbose, ct_hist, ct_cap, hist_len, stats, &
+      & tail_steps, predictor, pred_tol)
+    character(len=*), intent(in) :: dir
+    integer, intent(in) :: precision, nt_override, verbose, ct_cap, hist_len, tail_steps
+    integer, intent(in) :: predictor
+    real(dp), intent(in) :: pred_tol
+    real(dp), intent(inout) :: ct_hist(ct_cap, hist_len)
+    type(volcanor_stats), intent(out) :: stats
+    type(switches_class) :: sw
+    type(rotor_class), allocatable :: rotor(:)
+    type(c_ptr) :: ctx
+    integer :: nt, nr, ir, iter, rn, rf, uforce(16), nforce
+    real(dp) :: dt, density, velSound, kinematicVisc, t
+    character(len=2) :: rc2
+    type(vlm_rotor_desc) :: d
+    integer(c_int) :: rowNear, rowFar, dev
+    integer(c_int64_t) :: c0, c1, c2, crate
+    logical :: write_out
+    integer :: ntStart
+
+    call system_clock(c0, crate)
+    call read_config(dir, nt, dt, nr, density, velSound, kinematicVisc, sw)
+    if (sw%slowStart /= 0) error stop 'ERROR: slowStart is not supported'
+    if (sw%fdScheme < 0 .or. sw%fdScheme > 3) error stop 'ERROR: fdScheme must be 0..3'
+    if (sw%ntSub /= 0 .or. sw%ntSubInit /= 0) error stop 'ERROR: sub-iterations are not supported'
+    if (sw%initWakeVelNt /= 0) error stop 'ERROR: initWakeVel is not supported'
+    if (sw%restartFromNt /= 0) error stop 'ERROR: restart is not supported'
+    if (nr > 16) error stop 'ERROR: at most 16 rotors'
+    allocate (rotor(nr))
+    do ir = 1, nr
+      write (rc2, '(I0.2)') ir
+      call rotor_readGeom(rotor(ir), dir, 'geom'//rc2//'.nml')
+    enddo
+    do ir = 1, nr
+      call rotor_init(rotor(ir), ir, density, dt, nt, sw, dir)
+    enddo
+    if (nt_override > 0) nt = nt_override
+    write_out = sw%rotorForcePlot /= 0 .and. verbose >= 0
+
+    call vlm_check(c_null_ptr, vlm_create(ctx, 0_c_int), 'vlm_create')
+    call vlm_check(ctx, vlm_set_precision(ctx, int(precision, c_int)), 'set_precision')
+    if (predictor /= 0) call vlm_check(ctx, vlm_set_predictor(ctx, 1_c_int, pred_tol, 32_c_int, &
+      & 8_c_int, 4_c_int, rotor(1)%chord), 'set_predictor')
+
+    do ir = 1, nr
+      associate (r => rotor(ir))
+        d%nb = r%nb; d%nc = r%nc; d%ns = r%ns; d%nNwake = r%nNwake; d%nFwake = r%nFwake
+        d%nbConvect = r%nbConvect; d%rollupStart = r%rollupStart; d%rollupEnd = r%rollupEnd
+        d%rollupMin = 0
+        if (sign(1._dp, r%Omega*r%controlPitch(1)) > eps) d%rollupMin = 1
+        d%suppressFwake = r%suppressFwakeSwitch
+        call vlm_check(ctx, vlm_rotor_create(ctx, d, r%spanwiseCore, r%streamwiseCoreVec, dev), &
+          & 'rotor_create')
+        r%dev = dev
+        ! initial pitch, main.f90:43-58
+        block
+          integer :: ib
+          do ib = 1, r%nb
+            r%blade(ib)%theta = rotor_gettheta(r, r%psiStart, ib)
+            call blade_rot_pitch(r%blade(ib), sign(1._dp, r%Omega)*r%blade(ib)%theta)
+          enddo
+        end block
+        call rotor_upload_wing(ctx, r, .true.)
+        block
+          real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :)
+          allocate (cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb))
+          call rotor_cp_arrays(r, cp, ncap)
+          call vlm_check(ctx, vlm_rotor_calc_aic(ctx, r%dev, cp, ncap, c_null_ptr), 'calcAIC')
+        end block
+      end associate
+    enddo
+
+    ! initial solution without wake
+    do ir = 1, nr
+      rotor(ir)%omegaSlow = rotor(ir)%Omega
+    enddo
+    t = 0._dp
+    call compute_rhs_and_solve(ctx, rotor, nr, .false.)
+    do ir = 1, nr
+      call vlm_check(ctx, vlm_rotor_init_cursors(ctx, rotor(ir)%dev), 'init_cursors')
+      if (rotor(ir)%nNwake > 0) &
+        & call vlm_check(ctx, vlm_rotor_assignshed(ctx, rotor(ir)%dev, VLM_EDGE_TE), 'assignshed TE')
+    enddo
+    nforce = 0
+    if (write_out) then
+      call execute_command_line('mkdir -p '//trim(dir)//'/Results')
+      do ir = 1, nr
+        write (rc2, '(I0.2)') ir
+        open (newunit=uforce(ir), file=trim(dir)//'/Results/r'//rc2//'ForceNonDim.csv', &
+          & status='replace', action='write')
+        write (uforce(ir), '(A5,9(A15))') 'iter', 'CL/CT', 'CD/CQ', 'CLu', 'CDi', 'CD0', 'CDu', &
+          & 'CFx', 'CFy', 'CFz'
+      enddo
+    endif
+    if (sw%rotorForcePlot /= 0) then
+      do ir = 1, nr
+        call compute_forces(ctx, rotor, nr, ir, density, dt)
+        if (ir <= ct_cap .and. hist_len >= 1) ct_hist(ir, 1) = rotor_CT(rotor(ir))
+        if (write_out) call force_line(uforce(ir), rotor(ir), 0)
+      enddo
+    endif
+
+    call system_clock(c1)
+    c2 = c1
+    ntStart = 1
+    do iter = 1, nt
+      if (iter == nt - tail_steps + 1) call system_clock(c2)
+      t = t + dt
+      if (verbose > 0) then
+        if (mod(iter, verbose) == 0) print '(A,I0,A,I0)', ' step ', iter, ' of ', nt
+      endif
+      do ir = 1, nr
+        call vlm_check(ctx, vlm_rotor_begin_step(ctx, rotor(ir)%dev, int(iter, c_int)), 'begin_step')
+        call vlm_check(ctx, vlm_rotor_cursors(ctx, rotor(ir)%dev, rowNear, rowFar), 'cursors')
+        rotor(ir)%rowNear = rowNear; rotor(ir)%rowFar = rowFar
+        rotor(ir)%omegaSlow = rotor(ir)%Omega
+      enddo
+      do ir = 1, nr
+        call rotor_move(rotor(ir), rotor(ir)%velBody*dt)
+        call rotor_rot_pts(rotor(ir), rotor(ir)%omegaBody*dt, rotor(ir)%cgCoords)
+        call rotor_rot_advance(rotor(ir), rotor(ir)%omegaSlow*dt, .false.)
+        call rotor_upload_wing(ctx, rotor(ir), .true.)
+      enddo
+      if (sw%wakeSuppress == 0) then
+        do ir = 1, nr
+          if (rotor(ir)%nNwake <= 0) cycle
+          call vlm_check(ctx, vlm_rotor_assignshed(ctx, rotor(ir)%dev, VLM_EDGE_LE), 'assignshed LE')
+          call vlm_check(ctx, vlm_rotor_age_wake(ctx, rotor(ir)%dev, dt), 'age_wake')
+          if (sw%wakeDissipation == 1) call vlm_check(ctx, vlm_rotor_dissipate_wake(ctx, &
+            & rotor(ir)%dev, dt, kinematicVisc, rotor(ir)%apparentViscCoeff, rotor(ir)%decayCoeff), &
+            & 'dissipate_wake')
+        enddo
+      endif
+      call compute_rhs_and_solve(ctx, rotor, nr, sw%wakeSuppress == 0)
+      if (sw%rotorForcePlot /= 0) then
+        if (mod(iter, sw%rotorForcePlot) == 0) then
+          do ir = 1, nr
+            call compute_forces(ctx, rotor, nr, ir, density, dt)
+            if (ir <= ct_cap .and. iter + 1 <= hist_len) ct_hist(ir, iter + 1) = rotor_CT(rotor(ir))
+            if (write_out) call force_line(uforce(ir), rotor(ir), iter)
+          enddo
+        endif
+      endif
+      if (sw%wakeSuppress == 0) then
+        call vlm_check(ctx, vlm_convect(ctx, int(sw%fdScheme, c_int), int(iter, c_int), dt), &
+          & 'convect')
+        if (sw%wakeStrain == 1) then
+          do ir = 1, nr
+            if (rotor(ir)%nNwake > 0) &
+              & call vlm_check(ctx, vlm_rotor_strain_wake(ctx, rotor(ir)%dev), 'strain_wake')
+          enddo
+        endif
+        do ir = 1, nr
+          if (rotor(ir)%nNwake <= 0) cycle
+          call vlm_check(ctx, vlm_rotor_cursors(ctx, rotor(ir)%dev, rowNear, rowFar), 'cursors')
+          if (rowNear == 1) call vlm_check(ctx, vlm_rotor_rollup(ctx, rotor(ir)%dev), 'rollup')
+          call vlm_check(ctx, vlm_rotor_assignshed(ctx, rotor(ir)%dev, VLM_EDGE_TE), 'assignshed TE')
+        enddo
+      endif
+    enddo
+    call vlm_check(ctx, vlm_synchronize(ctx), 'synchronize')
+    block
+      integer(c_int64_t) :: cend
+      call system_clock(cend)
+      stats%wall_loop = real(cend - c1, dp)/real(crate, dp)
+      stats%wall_steps_tail = real(cend - c2, dp)/real(crate, dp)
+      stats%wall_total = real(cend - c0, dp)/real(crate, dp)
+    end block
+    stats%nt = nt
+    stats%tail_steps = tail_steps
+    stats%bs_kernel_ms = 0._dp; stats%bs_pairs = 0._dp; stats%bs_launches = 0
+    block
+      real(c_double) :: le, ll
+      call vlm_check(ctx, vlm_predictor_stats(ctx, stats%pred_full, stats%pred_learned, &
+        & stats%pred_fallbacks, le, ll), 'predictor_stats')
+      stats%pred_last_err = le
+    end block
+    if (write_out) then
+      do ir = 1, nr
+        close (uforce(ir))
+        write (rc2, '(I0.2)') ir
+        call dist_file(trim(dir)//'/Results/r'//rc2//'b01ForceDist.csv', rotor(ir))
+      enddo
+    endif
+    rn = 0; rf = 0
+    call vlm_check(ctx, vlm_destroy(ctx), 'vlm_destroy')
+  end subroutine run_case
+
+  integer(c_int) function volcanor_run(cdir, precision, nt_override, verbose, ct_hist, ct_cap, &
+      & hist_len, stats, tail_steps, predictor, pred_tol) bind(C, name='volcanor_run')
+    !! C entry of the whole solver: runs the case in directory cdir
+    !! (null-terminated) and returns the CL/CT history ct_hist(ct_cap, nt+1).
+    character(kind=c_char), intent(in) :: cdir(*)
+    integer(c_int), value :: precision, nt_override, verbose, ct_cap, hist_len, tail_steps
+    integer(c_int), value :: predictor
+    real(c_double), value :: pred_tol
+    real(c_double), intent(inout) :: ct_hist(ct_cap, hist_len)
+    type(volcanor_stats), intent(out) :: stats
+    character(len=1024) :: dir
+    integer :: i
+    dir = ''
+    do i = 1, 1024
+      if (cdir(i) == c_null_char) exit
+      dir(i:i) = cdir(i)
+    enddo
+    call run_case(trim(dir), int(precision), int(nt_override), int(verbose), ct_hist, &
+      & int(ct_cap), int(hist_len), stats, int(tail_steps), int(predictor), pred_tol)
+    volcanor_run = 0
+  end function volcanor_run
+
+end module volcanor_host

@@ -74,6 +74,15 @@ _SIGS = {
     "vlm_rotor_solve": [_P, _I, _P, _P, _I, _D, C.POINTER(_I)],
     "vlm_lu_factor": [_P, _I, _P, _P],
     "vlm_lu_solve_refine": [_P, _I, _P, _P, _P, _P, _P, _I, _D, C.POINTER(_I)],
+    "vlm_mlp_num_params": [],
+    "vlm_mlp_set_params": [_P, _P],
+    "vlm_mlp_get_params": [_P, _P],
+    "vlm_mlp_forward": [_P, _L, _P, _P],
+    "vlm_mlp_loss_grad": [_P, _L, _P, _P, _P, C.POINTER(C.c_float)],
+    "vlm_mlp_train_step": [_P, _L, _P, _P, C.c_float, C.POINTER(C.c_float)],
+    "vlm_set_predictor": [_P, _I, _D, _I, _I, _I, _D],
+    "vlm_predictor_stats": [_P, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L), C.POINTER(_D),
+                            C.POINTER(_D)],
     "vlm_profile_enable": [_P, _I],
     "vlm_profile_reset": [_P],
     "vlm_profile_get": [_P, C.POINTER(_D), C.POINTER(_L), C.POINTER(_D)],
@@ -306,6 +315,54 @@ class Context:
         self._chk(self.lib.vlm_lu_solve_refine(self.h, A.shape[0], _ptr(A), _ptr(LU), _ptr(piv),
                                                _ptr(rhs), _ptr(x), max_iter, tol, C.byref(it)))
         return x, it.value
+
+    # -- learned predictor ------------------------------------------------
+    def mlp_num_params(self):
+        return self.lib.vlm_mlp_num_params()
+
+    def mlp_set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._chk(self.lib.vlm_mlp_set_params(self.h, _ptr(p)))
+
+    def mlp_get_params(self):
+        p = np.zeros(self.mlp_num_params(), dtype=np.float32)
+        self._chk(self.lib.vlm_mlp_get_params(self.h, _ptr(p)))
+        return p
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 45)
+        y = np.zeros((x.shape[0], 3), dtype=np.float32)
+        self._chk(self.lib.vlm_mlp_forward(self.h, x.shape[0], _ptr(x), _ptr(y)))
+        return y
+
+    def mlp_loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 45)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 3)
+        g = np.zeros(self.mlp_num_params(), dtype=np.float32)
+        loss = C.c_float()
+        self._chk(self.lib.vlm_mlp_loss_grad(self.h, x.shape[0], _ptr(x), _ptr(t), _ptr(g),
+                                             C.byref(loss)))
+        return loss.value, g
+
+    def mlp_train_step(self, x, t, lr=1e-3):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 45)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 3)
+        loss = C.c_float()
+        self._chk(self.lib.vlm_mlp_train_step(self.h, x.shape[0], _ptr(x), _ptr(t), lr,
+                                              C.byref(loss)))
+        return loss.value
+
+    def set_predictor(self, mode, tol=1e-4, sample_stride=32, warmup=8, train_iters=4, lref=1.0):
+        self._chk(self.lib.vlm_set_predictor(self.h, mode, tol, sample_stride, warmup,
+                                             train_iters, lref))
+
+    def predictor_stats(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        e, l = C.c_double(), C.c_double()
+        self._chk(self.lib.vlm_predictor_stats(self.h, C.byref(a), C.byref(b), C.byref(c),
+                                               C.byref(e), C.byref(l)))
+        return dict(full=a.value, learned=b.value, fallbacks=c.value, last_err=e.value,
+                    last_loss=l.value)
 
     # -- timing ----------------------------------------------------------
     def profile_enable(self, on=True):
