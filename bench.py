@@ -1,17 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X free-wake hot path.
+"""Benchmark of the MI355X free-wake hot path.  Prints one JSON line.
 
-Default workload: the Biot-Savart microbenchmark (1e5 targets, 1e6 helix
-filaments, core radius 0.01), one step = one full N-body evaluation of all
-targets against all filaments with inputs resident in HBM.  Prints one JSON
-line.  See DESIGN.md for the counting conventions.
+Workloads
+  rotor2 (default)  two-bladed hover rotor, 1 x 20 panels per blade, 5 deg
+                    azimuth steps, 10 revolutions (720 free-wake steps,
+                    cases/cfg2_rotor.case) run by the Fortran host on the GPU.
+                    A step is one free-wake timestep (RHS + circulation solve
+                    + loads + AM2 predictor-corrector convection with both
+                    Biot-Savart passes).  W warm-up steps run untimed, the
+                    last K steps are timed (default: the 10th revolution).
+  microbench        1e5 targets x 1e6 helix filaments, one N-body evaluation
+                    per step, inputs resident in HBM.
 
-  python bench.py [--steps K] [--warmup W] [--precision fp64|mixed]
+  python bench.py [--workload rotor2|microbench] [--steps K] [--warmup W]
 """
 import argparse
 import json
 import os
+import resource
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -22,7 +32,7 @@ sys.path.insert(0, ROOT)
 F_PAIR = 66.0                 # algorithmic flop per pair (FMA = 2, rsqrt = 1)
 PEAK_FP64_VALU = 78.6e12      # MI355X vector FP64, FLOP/s
 PEAK_FP32_VALU = 157.3e12     # MI355X vector FP32 (packed), FLOP/s
-PEAK_HBM = 8.0e12
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "volcanor")
 
 
 def helix_workload(n_targets=100_000, n_helix=8, per_helix=125_000, seed=20261202):
@@ -50,29 +60,17 @@ def helix_workload(n_targets=100_000, n_helix=8, per_helix=125_000, seed=2026120
     return a, b, gam, rc, p, sample
 
 
-def cpu_baseline(a, b, gam, rc, p, sample, threads):
-    """FP64 oracle (plain C, OpenMP over targets) on a bounded sample."""
-    from oracle import oracle as orc
-    t0 = time.perf_counter()
-    v = orc.filament_sum(a, b, gam, rc, p[sample], threads=threads)
-    dt = time.perf_counter() - t0
-    return len(sample) * len(a) / dt, v, dt
+def roofline(kernel_ms, launches, pairs, peak):
+    avg_ms = kernel_ms / max(launches, 1)
+    flops_launch = F_PAIR * pairs / max(launches, 1)
+    achieved = flops_launch / (avg_ms * 1e-3) if avg_ms > 0 else 0.0
+    return {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
+            "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+            "kernel_avg_ms": avg_ms, "launches": int(launches),
+            "flop_per_pair": F_PAIR}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
-              "sharding is a design in DESIGN.md and is not built.", file=sys.stderr)
-        sys.exit(2)
-
+def bench_microbench(args):
     from volcanor_amd import vlm
     a, b, gam, rc, p, sample = helix_workload()
     ctx = vlm.Context(0)
@@ -90,45 +88,157 @@ def main():
         ctx.fset_vind(fs, ps)
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    kms, nlaunch, pairs = ctx.profile_get()
-    ctx.profile_enable(False)
+    kms, nl, pairs = ctx.profile_get()
     v = ctx.pset_velocity(ps, m)
-    pairs_per_step = float(len(a)) * float(m)
-    value = pairs_per_step * args.steps / wall
-    avg_ms = kms / max(nlaunch, 1)
-    flops_launch = F_PAIR * pairs / max(nlaunch, 1)
-    achieved = flops_launch / (avg_ms * 1e-3)
-    peak = PEAK_FP64_VALU if prec == vlm.FP64 else PEAK_FP32_VALU
     out = {
         "metric": "Biot-Savart pair-interactions/s",
-        "value": value,
-        "unit": "pairs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if prec == vlm.FP64 else "f32+f64",
+        "value": float(len(a)) * m * args.steps / wall, "unit": "pairs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if prec == vlm.FP64 else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "Biot-Savart microbenchmark: 1e5 targets x 1e6 helix filaments, "
-                               "core radius 0.01, inputs resident in HBM",
-                   "targets": int(m), "filaments": int(len(a)),
+        "config": {"workload": "Biot-Savart microbenchmark: 1e5 targets x 1e6 helix "
+                               "filaments, core radius 0.01, inputs resident in HBM",
                    "precision": args.precision},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "kernel_ms": avg_ms, "flop_per_pair": F_PAIR},
+        "roofline": roofline(kms, nl, pairs, PEAK_FP64_VALU if prec == vlm.FP64
+                             else PEAK_FP32_VALU),
     }
     if not args.no_cpu:
+        from oracle import oracle as orc
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        rate, vc, dtc = cpu_baseline(a, b, gam, rc, p, sample, threads)
-        rel = np.max(np.linalg.norm(v[sample] - vc, axis=1)) / np.sqrt(np.mean(np.sum(vc**2, 1)))
-        out["cpu_baseline"] = {"value": rate, "unit": "pairs/s", "cores": threads, "kind": "port",
-                               "sample": f"{len(sample)} targets x {len(a)} filaments FP64 "
-                                         f"({dtc:.1f} s)"}
-        out["check"] = {"max_err_over_rms_vs_fp64_oracle": float(rel)}
+        t0 = time.perf_counter()
+        vc = orc.filament_sum(a, b, gam, rc, p[sample], threads=threads)
+        dtc = time.perf_counter() - t0
+        rms = np.sqrt(np.mean(np.sum(vc**2, 1)))
+        out["cpu_baseline"] = {"value": len(sample) * len(a) / dtc, "unit": "pairs/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{len(sample)} targets x {len(a)} filaments, "
+                                         f"FP64 oracle ({dtc:.1f} s)"}
+        out["check"] = {"max_err_over_rms_vs_fp64_oracle":
+                        float(np.max(np.linalg.norm(v[sample] - vc, axis=1)) / rms)}
     print(json.dumps(out), flush=True)
+
+
+def run_reference(case_dir, nsteps, threads):
+    """Reference solver (compiled from its own sources) on the first nsteps
+    steps of the case, in a scratch directory; returns wall seconds."""
+    tmp = tempfile.mkdtemp(prefix="vlmref_")
+    try:
+        for f in os.listdir(case_dir):
+            shutil.copy(os.path.join(case_dir, f), tmp)
+        os.makedirs(os.path.join(tmp, "Results"))
+        os.makedirs(os.path.join(tmp, "Restart"))
+        cfg = open(os.path.join(tmp, "config.nml")).read()
+        lines = [f"nt = {nsteps}" if l.strip().startswith("nt =") else l for l in cfg.splitlines()]
+        open(os.path.join(tmp, "config.nml"), "w").write("\n".join(lines) + "\n")
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_SCHEDULE="static",
+                   OMP_PROC_BIND="close", OMP_PLACES="cores")
+
+        def unlimited_stack():
+            resource.setrlimit(resource.RLIMIT_STACK,
+                               (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+        t0 = time.perf_counter()
+        subprocess.run([REF_BIN], cwd=tmp, env=env, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL, check=True, preexec_fn=unlimited_stack,
+                       timeout=600)
+        return time.perf_counter() - t0
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def bench_rotor(args):
+    from volcanor_amd import host
+    case = os.path.join(ROOT, "cases", "cfg2_rotor.case")
+    nt_case = 720
+    K = args.steps if args.steps else 72
+    W = args.warmup if args.warmup is not None else nt_case - K
+    nt = W + K
+
+    runs = {}
+    for name, prec, learned in (("fp64", "fp64", False), ("mixed", "mixed", False)) + \
+            ((("learned_mixed", "mixed", True),) if args.learned else ()):
+        t0 = time.perf_counter()
+        ct, st = host.run_case(case, precision=prec, nt=nt, tail_steps=K, learned=learned,
+                               tol=args.tol)
+        runs[name] = dict(ct=ct[0], st=st, wall=time.perf_counter() - t0)
+        print(f"# {name}: run {runs[name]['wall']:.2f} s, timed {st.wall_steps_tail:.3f} s for "
+              f"{K} steps, BS kernel {st.bs_kernel_ms:.1f} ms over {st.bs_launches} launches, "
+              f"pairs {st.bs_pairs:.3e}", file=sys.stderr, flush=True)
+
+    ref = runs["fp64"]["ct"]
+    for r in runs.values():
+        r["ct_rel"] = float(np.max(np.abs(r["ct"] - ref) / np.maximum(np.abs(ref), 1e-30)))
+    # primary path: mixed precision when its CT history holds 1e-4 of FP64
+    prim = "mixed" if runs["mixed"]["ct_rel"] < 1e-4 else "fp64"
+    P = runs[prim]["st"]
+    value = P.bs_pairs / P.wall_steps_tail
+    out = {
+        "metric": "Biot-Savart pair-interactions/s",
+        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": P.wall_steps_tail * 1e3 / K, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if prim == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "two-bladed hover rotor (cfg2): 1 x 20 panels per blade, "
+                               "-8 deg twist, 8 deg collective, tip Mach 0.4, 5 deg steps, "
+                               "AM2 free wake with strain and dissipation, 10 revolutions; "
+                               "timed = last K steps",
+                   "case": "cases/cfg2_rotor.case", "timed_steps": K, "total_steps": nt,
+                   "precision": prim},
+        "roofline": roofline(P.bs_kernel_ms, P.bs_launches, P.bs_pairs,
+                             PEAK_FP32_VALU if prim == "mixed" else PEAK_FP64_VALU),
+        "variants": {k: {"s_per_step": r["st"].wall_steps_tail / K,
+                         "pairs_per_s": r["st"].bs_pairs / r["st"].wall_steps_tail,
+                         "run_wall_s": r["wall"], "ct_final": float(r["ct"][-1]),
+                         "ct_max_rel_dev_vs_fp64": r["ct_rel"],
+                         "bs_kernel_avg_ms": r["st"].bs_kernel_ms / max(r["st"].bs_launches, 1),
+                         "predictor": {"full": r["st"].pred_full,
+                                       "learned": r["st"].pred_learned,
+                                       "fallbacks": r["st"].pred_fallbacks}}
+                     for k, r in runs.items()},
+    }
+    out["speedup_s_per_step_vs_fp64"] = {
+        k: runs["fp64"]["st"].wall_steps_tail / r["st"].wall_steps_tail for k, r in runs.items()}
+    if not args.no_cpu and os.path.exists(REF_BIN):
+        S = args.cpu_steps
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        _, sg = host.run_case(case, precision="fp64", nt=S, tail_steps=S)
+        wall_cpu = run_reference(case, S, threads)
+        out["cpu_baseline"] = {
+            "value": sg.bs_pairs / wall_cpu, "unit": "pairs/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"reference solver built from its sources, first {S} steps of the same "
+                      f"case ({wall_cpu:.1f} s); pairs counted as on the GPU over those steps "
+                      f"({sg.bs_pairs:.3e}); GPU FP64 on the same steps took "
+                      f"{sg.wall_steps_tail:.3f} s"}
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", choices=["rotor2", "microbench"], default="rotor2")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--learned", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=60)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
+              "sharding is a design in DESIGN.md and is not built.", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "microbench":
+        if not args.steps:
+            args.steps = 5
+        if args.warmup is None:
+            args.warmup = 1
+        bench_microbench(args)
+    else:
+        bench_rotor(args)
 
 
 if __name__ == "__main__":

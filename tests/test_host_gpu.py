@@ -59,12 +59,14 @@ def test_cfg2_two_revs_matches_reference_solver():
     path = os.path.join(GOLD, "cfg2_rotor_2rev_ForceNonDim.csv")
     ref = read_force(path)[:, 0]
     ct, _ = run("cfg2_rotor_2rev")
-    np.testing.assert_allclose(ct[0], ref, rtol=1e-5, atol=1e-8)
+    assert len(ct[0]) == len(ref) == 145
+    # printed digits of the reference output: 7 significant
+    np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
 
 
 def test_cfg2_mixed_precision_ct_within_1e4():
     """Mixed FP32/FP64 Biot-Savart against the FP64 path, two revolutions."""
     a, _ = run("cfg2_rotor_2rev", "fp64")
     b, _ = run("cfg2_rotor_2rev", "mixed")
-    rel = np.abs(b[0][-36:] - a[0][-36:]) / np.abs(a[0][-36:])
+    rel = np.abs(b[0] - a[0]) / np.abs(a[0])
     assert np.max(rel) < 1e-4

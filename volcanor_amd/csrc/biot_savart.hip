@@ -29,8 +29,8 @@
 namespace vlm {
 namespace {
 
-constexpr int BS_BLOCK = 256;
-constexpr int BS_TILE = 256;
+constexpr int BS_BLOCK = 256;      // wake passes: many targets
+constexpr int BS_BLOCK_SMALL = 64; // collocation points: tens to hundreds of targets
 constexpr int XCDS = 8;
 constexpr double EPS2 = kEps * kEps;
 constexpr float EPS2F = 4.93038065763132e-32f;
@@ -122,7 +122,8 @@ struct BsArgs {
     int nlogical;
 };
 
-__global__ __launch_bounds__(BS_BLOCK) void bs_fp64_kernel(BsArgs a)
+template <int BS_TILE>
+__global__ __launch_bounds__(BS_TILE) void bs_fp64_kernel(BsArgs a)
 {
     const int per = gridDim.x / XCDS;
     const int lid = (blockIdx.x % XCDS) * per + blockIdx.x / XCDS;
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(BS_BLOCK) void bs_fp64_kernel(BsArgs a)
     const int64_t s0 = (int64_t)chunk * a.chunk_len;
     const int64_t s1 = min(nseg, s0 + a.chunk_len);
     const int tid = threadIdx.x;
-    const int64_t t = (int64_t)tile * BS_BLOCK + tid;
+    const int64_t t = (int64_t)tile * BS_TILE + tid;
     const bool live = t < a.m;
     double px = 0.0, py = 0.0, pz = 0.0;
     if (live) {
@@ -178,7 +179,8 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x)
     s = tt;
 }
 
-__global__ __launch_bounds__(BS_BLOCK) void bs_mixed_kernel(BsArgs a)
+template <int BS_TILE>
+__global__ __launch_bounds__(BS_TILE) void bs_mixed_kernel(BsArgs a)
 {
     const int per = gridDim.x / XCDS;
     const int lid = (blockIdx.x % XCDS) * per + blockIdx.x / XCDS;
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(BS_BLOCK) void bs_mixed_kernel(BsArgs a)
     const int64_t s0 = (int64_t)chunk * a.chunk_len;
     const int64_t s1 = min(nseg, s0 + a.chunk_len);
     const int tid = threadIdx.x;
-    const int64_t t = (int64_t)tile * BS_BLOCK + tid;
+    const int64_t t = (int64_t)tile * BS_TILE + tid;
     const bool live = t < a.m;
     double px = 0.0, py = 0.0, pz = 0.0;
     if (live) {
@@ -256,13 +258,15 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32,
             int accumulate)
 {
     if (m <= 0) return VLM_OK;
-    const int ntiles = (int)((m + BS_BLOCK - 1) / BS_BLOCK);
+    const bool small = m < 1024;
+    const int blk = small ? BS_BLOCK_SMALL : BS_BLOCK;
+    const int ntiles = (int)((m + blk - 1) / blk);
     int nchunks = 0;
     int64_t chunk_len = 1;
     if (nseg_max > 0) {
         const int64_t want = 4096;
         int64_t nc = (want + ntiles - 1) / ntiles;
-        const int64_t maxc = (nseg_max + BS_TILE - 1) / BS_TILE;
+        const int64_t maxc = (nseg_max + blk - 1) / blk;
         if (nc > maxc) nc = maxc;
         if (nc < 1) nc = 1;
         chunk_len = (nseg_max + nc - 1) / nc;
@@ -284,17 +288,29 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32,
         a.nlogical = ntiles * nchunks;
         const int grid = ((a.nlogical + XCDS - 1) / XCDS) * XCDS;
         Timer tm{};
-        if (ctx->profile) {
+        const bool timed = ctx->profile && !small;
+        if (timed) {
             VLM_HIP(hipEventCreate(&tm.a));
             VLM_HIP(hipEventCreate(&tm.b));
             VLM_HIP(hipEventRecord(tm.a, ctx->stream));
         }
-        if (ctx->precision == VLM_MIXED)
-            hipLaunchKernelGGL(bs_mixed_kernel, dim3(grid), dim3(BS_BLOCK), 0, ctx->stream, a);
-        else
-            hipLaunchKernelGGL(bs_fp64_kernel, dim3(grid), dim3(BS_BLOCK), 0, ctx->stream, a);
+        if (ctx->precision == VLM_MIXED) {
+            if (small)
+                hipLaunchKernelGGL(bs_mixed_kernel<BS_BLOCK_SMALL>, dim3(grid), dim3(blk), 0,
+                                   ctx->stream, a);
+            else
+                hipLaunchKernelGGL(bs_mixed_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0,
+                                   ctx->stream, a);
+        } else {
+            if (small)
+                hipLaunchKernelGGL(bs_fp64_kernel<BS_BLOCK_SMALL>, dim3(grid), dim3(blk), 0,
+                                   ctx->stream, a);
+            else
+                hipLaunchKernelGGL(bs_fp64_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0,
+                                   ctx->stream, a);
+        }
         VLM_HIP(hipGetLastError());
-        if (ctx->profile) {
+        if (timed) {
             VLM_HIP(hipEventRecord(tm.b, ctx->stream));
             tm.pairs = -1.0;  // resolved from the device count at read-out
             ctx->timers.push_back(tm);
@@ -304,7 +320,7 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32,
     hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
                        ctx->stream, ctx->part.p, nchunks, m, out, tstride, accumulate);
     VLM_HIP(hipGetLastError());
-    if (ctx->profile && nchunks > 0) {
+    if (ctx->profile && nchunks > 0 && !small) {
         // pair count = m * device segment count, copied to pinned memory
         // without a stream synchronisation and resolved at read-out
         Timer &tb = ctx->timers.back();

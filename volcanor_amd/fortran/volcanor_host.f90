@@ -1357,7 +1357,14 @@ contains
     c2 = c1
     ntStart = 1
     do iter = 1, nt
-      if (iter == nt - tail_steps + 1) call system_clock(c2)
+      if (iter == nt - tail_steps + 1) then
+        call vlm_check(ctx, vlm_synchronize(ctx), 'synchronize')
+        if (tail_steps > 0) then
+          call vlm_check(ctx, vlm_profile_enable(ctx, 1_c_int), 'profile_enable')
+          call vlm_check(ctx, vlm_profile_reset(ctx), 'profile_reset')
+        endif
+        call system_clock(c2)
+      endif
       t = t + dt
       if (verbose > 0) then
         if (mod(iter, verbose) == 0) print '(A,I0,A,I0)', ' step ', iter, ' of ', nt
@@ -1422,6 +1429,11 @@ contains
     stats%nt = nt
     stats%tail_steps = tail_steps
     stats%bs_kernel_ms = 0._dp; stats%bs_pairs = 0._dp; stats%bs_launches = 0
+    if (tail_steps > 0) then
+      call vlm_check(ctx, vlm_profile_get(ctx, stats%bs_kernel_ms, stats%bs_launches, &
+        & stats%bs_pairs), 'profile_get')
+      call vlm_check(ctx, vlm_profile_enable(ctx, 0_c_int), 'profile_enable')
+    endif
     block
       real(c_double) :: le, ll
       call vlm_check(ctx, vlm_predictor_stats(ctx, stats%pred_full, stats%pred_learned, &
