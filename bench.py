@@ -125,7 +125,8 @@ def run_reference(case_dir, nsteps, threads):
     tmp = tempfile.mkdtemp(prefix="vlmref_")
     try:
         for f in os.listdir(case_dir):
-            shutil.copy(os.path.join(case_dir, f), tmp)
+            if os.path.isfile(os.path.join(case_dir, f)):
+                shutil.copy(os.path.join(case_dir, f), tmp)
         os.makedirs(os.path.join(tmp, "Results"))
         os.makedirs(os.path.join(tmp, "Restart"))
         cfg = open(os.path.join(tmp, "config.nml")).read()

@@ -5,36 +5,38 @@
 //   v = gam/(4 pi) * (r1 x r2) * [r0 . (r1/|r1| - r2/|r2|)]
 //       / sqrt((rc |r0|)^4 + |r1 x r2|^4),
 // zero when |r1 x r2|^2 <= eps^2, r/|r| = 0 when |r| <= eps
-// (unitVec, libMath.f90:249-262).
+// (unitVec, libMath.f90:249-262).  Per segment the library precomputes
+// r0 = b - a, g = gam/(4 pi) and q = rc^2 |r0|^2; per pair
+// r1 = p - a, r2 = r1 - r0 and r1 x r2 = r0 x r1.
 //
-// Work decomposition: one target point per lane; a workgroup of 256 lanes
-// (four wave64s) walks one chunk of the segment list, staging 256 segments
-// at a time in LDS and reading them back as wave-uniform broadcasts.  The
-// segment list is split into chunks so that the grid holds several
-// workgroups per CU even for a few thousand targets; per-chunk partial sums
-// are reduced in a fixed order afterwards, so results are bitwise
-// repeatable.  Block ids are remapped so that all tiles of one chunk run on
-// one XCD and share its L2.  No MFMA: the pair is not a contraction.
+// Work decomposition: one target point per lane; a workgroup walks one
+// (target tile, segment chunk) work item at a time, staging 256 segments in
+// LDS and reading them back as wave-uniform broadcasts.  Work items are
+// handed out by an atomic counter to a grid sized to the resident capacity
+// of the device, so every CU stays busy to the end of the launch whatever
+// the target count.  Each item writes its own partial sums, which are
+// reduced in a fixed order afterwards: results are bitwise repeatable.  No
+// MFMA: the pair is not a contraction.
 //
 // Precision modes
-//   FP64  : all arithmetic in FP64; 1/sqrt via v_rsq_f64 plus two Newton
-//           steps.
-//   MIXED : FP32 pair arithmetic (v_rsq_f32), r1 x r2 formed as r0 x r1 to
-//           avoid cancellation, per-tile FP32 sums folded into a Kahan-
-//           compensated FP32 accumulator; pairs whose target lies within
-//           kappa segment lengths of an end point are recomputed in FP64
-//           from the FP64 copy of the segment (near-singular terms).
+//   FP64  : FP64 throughout; 1/sqrt via v_rsq_f64 plus two Newton steps.
+//   MIXED : FP32 pair arithmetic on hi/lo-split coordinates
+//           (r1 = (p_hi - a_hi) + (p_lo - a_lo), accurate to FP32 relative
+//           precision of r1 itself), v_rsq_f32, per-tile FP32 sums folded
+//           into a Kahan-compensated accumulator.  Pairs with the target
+//           within two core radii of the segment are recomputed in FP64
+//           (the near-singular core and target-on-filament terms).
 #include "vlm_internal.h"
 
 namespace vlm {
 namespace {
 
-constexpr int BS_BLOCK = 256;      // wake passes: many targets
-constexpr int BS_BLOCK_SMALL = 64; // collocation points: tens to hundreds of targets
-constexpr int XCDS = 8;
+constexpr int BS_BLOCK = 256;       // wake passes: many targets
+constexpr int BS_BLOCK_SMALL = 64;  // collocation points: tens to hundreds
 constexpr double EPS2 = kEps * kEps;
 constexpr float EPS2F = 4.93038065763132e-32f;
-constexpr float KAPPA2 = 9.0f;  // near field: |r1| or |r2| < 3 |r0|
+constexpr float NEAR_CORE2 = 4.0f;  // h < 2 rc ...
+constexpr float NEAR_LEN2 = 4.0f;   // ... and |r1|^2 + |r2|^2 < 4 |r0|^2
 
 __device__ __forceinline__ double rsq64(double x)
 {
@@ -48,43 +50,44 @@ __device__ __forceinline__ double rsq64(double x)
     return y;
 }
 
-__device__ __forceinline__ void pair64(double px, double py, double pz,
-                                       double ax, double ay, double az,
-                                       double bx, double by, double bz,
-                                       double g, double rc2, double &vx,
-                                       double &vy, double &vz)
+// FP64 pair; segment record (ax ay az g | r0x r0y r0z q).
+__device__ __forceinline__ void pair64(double px, double py, double pz, double4 sa, double4 sb,
+                                       double &vx, double &vy, double &vz)
 {
-    const double r1x = px - ax, r1y = py - ay, r1z = pz - az;
-    const double r2x = px - bx, r2y = py - by, r2z = pz - bz;
-    const double r0x = r1x - r2x, r0y = r1y - r2y, r0z = r1z - r2z;
-    const double cx = r1y * r2z - r1z * r2y;
-    const double cy = r1z * r2x - r1x * r2z;
-    const double cz = r1x * r2y - r1y * r2x;
+    const double r1x = px - sa.x, r1y = py - sa.y, r1z = pz - sa.z;
+    const double r0x = sb.x, r0y = sb.y, r0z = sb.z;
+    const double r2x = r1x - r0x, r2y = r1y - r0y, r2z = r1z - r0z;
+    const double cx = r0y * r1z - r0z * r1y;
+    const double cy = r0z * r1x - r0x * r1z;
+    const double cz = r0x * r1y - r0y * r1x;
     const double c2 = cx * cx + cy * cy + cz * cz;
     const double n1 = r1x * r1x + r1y * r1y + r1z * r1z;
     const double n2 = r2x * r2x + r2y * r2y + r2z * r2z;
     const double n0 = r0x * r0x + r0y * r0y + r0z * r0z;
-    const double i1 = n1 > EPS2 ? rsq64(fmax(n1, EPS2)) : 0.0;
-    const double i2 = n2 > EPS2 ? rsq64(fmax(n2, EPS2)) : 0.0;
-    const double dot = (r0x * r1x + r0y * r1y + r0z * r1z) * i1 -
-                       (r0x * r2x + r0y * r2y + r0z * r2z) * i2;
-    const double q = rc2 * n0;
-    const double den = q * q + c2 * c2;
-    double s = g * dot * rsq64(den > 0.0 ? den : 1.0);
+    const double d1 = r0x * r1x + r0y * r1y + r0z * r1z;
+    const double d2 = d1 - n0;
+    const double i1 = n1 > EPS2 ? rsq64(n1) : 0.0;
+    const double i2 = n2 > EPS2 ? rsq64(n2) : 0.0;
+    const double dot = d1 * i1 - d2 * i2;
+    const double den = sb.w * sb.w + c2 * c2;
+    double s = sa.w * dot * rsq64(den);
     s = c2 > EPS2 ? s : 0.0;
     vx += cx * s;
     vy += cy * s;
     vz += cz * s;
 }
 
-// FP32 pair.  Returns true when the pair belongs to the FP64 near field
-// (its FP32 contribution is then zero).
-__device__ __forceinline__ bool pair32(float px, float py, float pz,
-                                       float4 s0, float4 s1, float &vx,
+// FP32 pair on split coordinates; record (ahx ahy ahz g | alx aly alz q |
+// r0x r0y r0z -).  Returns true for near-singular pairs (contribution left
+// to the FP64 branch).
+__device__ __forceinline__ bool pair32(float phx, float phy, float phz, float plx, float ply,
+                                       float plz, float4 s0, float4 s1, float4 s2, float &vx,
                                        float &vy, float &vz)
 {
-    const float r1x = px - s0.x, r1y = py - s0.y, r1z = pz - s0.z;
-    const float r0x = s0.w, r0y = s1.x, r0z = s1.y;
+    const float r1x = (phx - s0.x) + (plx - s1.x);
+    const float r1y = (phy - s0.y) + (ply - s1.y);
+    const float r1z = (phz - s0.z) + (plz - s1.z);
+    const float r0x = s2.x, r0y = s2.y, r0z = s2.z;
     const float r2x = r1x - r0x, r2y = r1y - r0y, r2z = r1z - r0z;
     const float cx = r0y * r1z - r0z * r1y;
     const float cy = r0z * r1x - r0x * r1z;
@@ -93,14 +96,14 @@ __device__ __forceinline__ bool pair32(float px, float py, float pz,
     const float n1 = r1x * r1x + r1y * r1y + r1z * r1z;
     const float n2 = r2x * r2x + r2y * r2y + r2z * r2z;
     const float n0 = r0x * r0x + r0y * r0y + r0z * r0z;
-    const bool nearp = (n1 < KAPPA2 * n0) || (n2 < KAPPA2 * n0);
+    const bool nearp = (c2 < NEAR_CORE2 * s1.w) && (n1 + n2 < NEAR_LEN2 * n0);
     const float d1 = r0x * r1x + r0y * r1y + r0z * r1z;
     const float d2 = d1 - n0;
     const float i1 = n1 > EPS2F ? __builtin_amdgcn_rsqf(n1) : 0.0f;
     const float i2 = n2 > EPS2F ? __builtin_amdgcn_rsqf(n2) : 0.0f;
     const float dot = d1 * i1 - d2 * i2;
-    const float den = s1.w + c2 * c2;
-    float s = s1.z * dot * __builtin_amdgcn_rsqf(den > 1e-37f ? den : 1.0f);
+    const float den = s1.w * s1.w + c2 * c2;
+    float s = s0.w * dot * __builtin_amdgcn_rsqf(den);
     s = (c2 > EPS2F && !nearp) ? s : 0.0f;
     vx += cx * s;
     vy += cy * s;
@@ -109,8 +112,8 @@ __device__ __forceinline__ bool pair32(float px, float py, float pz,
 }
 
 struct BsArgs {
-    const double *s64;
-    const float *s32;
+    const double *s64;   // 8 x stride SoA
+    const float *s32;    // 12 x stride SoA
     int64_t sstride;
     const long long *nseg;
     int64_t chunk_len;
@@ -119,55 +122,63 @@ struct BsArgs {
     int64_t tstride;
     double *part;
     int ntiles;
-    int nlogical;
+    int nitems;
+    int *counter;
 };
 
-template <int BS_TILE>
-__global__ __launch_bounds__(BS_TILE) void bs_fp64_kernel(BsArgs a)
+__device__ __forceinline__ int next_item(int *counter, int *sh)
 {
-    const int per = gridDim.x / XCDS;
-    const int lid = (blockIdx.x % XCDS) * per + blockIdx.x / XCDS;
-    if (lid >= a.nlogical) return;
-    const int chunk = lid / a.ntiles;
-    const int tile = lid - chunk * a.ntiles;
-    const int64_t nseg = *a.nseg;
-    const int64_t s0 = (int64_t)chunk * a.chunk_len;
-    const int64_t s1 = min(nseg, s0 + a.chunk_len);
+    if (threadIdx.x == 0) *sh = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = *sh;
+    __syncthreads();
+    return it;
+}
+
+template <int TILE>
+__global__ __launch_bounds__(TILE) void bs_fp64_kernel(BsArgs a)
+{
+    __shared__ double4 sa[TILE];
+    __shared__ double4 sb[TILE];
+    __shared__ int item_sh;
     const int tid = threadIdx.x;
-    const int64_t t = (int64_t)tile * BS_TILE + tid;
-    const bool live = t < a.m;
-    double px = 0.0, py = 0.0, pz = 0.0;
-    if (live) {
-        px = a.t[t];
-        py = a.t[a.tstride + t];
-        pz = a.t[2 * a.tstride + t];
-    }
-    __shared__ double4 sa[BS_TILE];
-    __shared__ double4 sb[BS_TILE];
-    double vx = 0.0, vy = 0.0, vz = 0.0;
+    const int64_t nseg = *a.nseg;
     const int64_t S = a.sstride;
-    for (int64_t base = s0; base < s1; base += BS_TILE) {
-        const int cnt = (int)min<int64_t>(BS_TILE, s1 - base);
-        __syncthreads();
-        if (tid < cnt) {
-            const int64_t s = base + tid;
-            sa[tid] = make_double4(a.s64[s], a.s64[S + s], a.s64[2 * S + s], a.s64[3 * S + s]);
-            sb[tid] = make_double4(a.s64[4 * S + s], a.s64[5 * S + s], a.s64[6 * S + s],
-                                   a.s64[7 * S + s]);
+    for (;;) {
+        const int item = next_item(a.counter, &item_sh);
+        if (item >= a.nitems) break;
+        const int chunk = item / a.ntiles;
+        const int tile = item - chunk * a.ntiles;
+        const int64_t s0 = (int64_t)chunk * a.chunk_len;
+        const int64_t s1 = min(nseg, s0 + a.chunk_len);
+        const int64_t t = (int64_t)tile * TILE + tid;
+        const bool live = t < a.m;
+        double px = 0.0, py = 0.0, pz = 0.0;
+        if (live) {
+            px = a.t[t];
+            py = a.t[a.tstride + t];
+            pz = a.t[2 * a.tstride + t];
         }
-        __syncthreads();
+        double vx = 0.0, vy = 0.0, vz = 0.0;
+        for (int64_t base = s0; base < s1; base += TILE) {
+            const int cnt = (int)min<int64_t>(TILE, s1 - base);
+            __syncthreads();
+            if (tid < cnt) {
+                const int64_t s = base + tid;
+                sa[tid] = make_double4(a.s64[s], a.s64[S + s], a.s64[2 * S + s], a.s64[3 * S + s]);
+                sb[tid] = make_double4(a.s64[4 * S + s], a.s64[5 * S + s], a.s64[6 * S + s],
+                                       a.s64[7 * S + s]);
+            }
+            __syncthreads();
 #pragma unroll 2
-        for (int k = 0; k < cnt; ++k) {
-            const double4 A = sa[k];
-            const double4 B = sb[k];
-            pair64(px, py, pz, A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w, vx, vy, vz);
+            for (int k = 0; k < cnt; ++k) pair64(px, py, pz, sa[k], sb[k], vx, vy, vz);
         }
-    }
-    if (live) {
-        double *o = a.part + (size_t)chunk * 3 * a.m;
-        o[t] = vx;
-        o[a.m + t] = vy;
-        o[2 * a.m + t] = vz;
+        if (live) {
+            double *o = a.part + (size_t)chunk * 3 * a.m;
+            o[t] = vx;
+            o[a.m + t] = vy;
+            o[2 * a.m + t] = vz;
+        }
     }
 }
 
@@ -179,67 +190,76 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x)
     s = tt;
 }
 
-template <int BS_TILE>
-__global__ __launch_bounds__(BS_TILE) void bs_mixed_kernel(BsArgs a)
+template <int TILE>
+__global__ __launch_bounds__(TILE) void bs_mixed_kernel(BsArgs a)
 {
-    const int per = gridDim.x / XCDS;
-    const int lid = (blockIdx.x % XCDS) * per + blockIdx.x / XCDS;
-    if (lid >= a.nlogical) return;
-    const int chunk = lid / a.ntiles;
-    const int tile = lid - chunk * a.ntiles;
-    const int64_t nseg = *a.nseg;
-    const int64_t s0 = (int64_t)chunk * a.chunk_len;
-    const int64_t s1 = min(nseg, s0 + a.chunk_len);
+    __shared__ float4 s0v[TILE];
+    __shared__ float4 s1v[TILE];
+    __shared__ float4 s2v[TILE];
+    __shared__ int item_sh;
     const int tid = threadIdx.x;
-    const int64_t t = (int64_t)tile * BS_TILE + tid;
-    const bool live = t < a.m;
-    double px = 0.0, py = 0.0, pz = 0.0;
-    if (live) {
-        px = a.t[t];
-        py = a.t[a.tstride + t];
-        pz = a.t[2 * a.tstride + t];
-    }
-    const float fx = (float)px, fy = (float)py, fz = (float)pz;
-    __shared__ float4 sa[BS_TILE];
-    __shared__ float4 sb[BS_TILE];
-    float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
-    double nx = 0.0, ny = 0.0, nz = 0.0;
+    const int64_t nseg = *a.nseg;
     const int64_t S = a.sstride;
-    for (int64_t base = s0; base < s1; base += BS_TILE) {
-        const int cnt = (int)min<int64_t>(BS_TILE, s1 - base);
-        __syncthreads();
-        if (tid < cnt) {
-            const int64_t s = base + tid;
-            sa[tid] = make_float4(a.s32[s], a.s32[S + s], a.s32[2 * S + s], a.s32[3 * S + s]);
-            sb[tid] = make_float4(a.s32[4 * S + s], a.s32[5 * S + s], a.s32[6 * S + s],
-                                  a.s32[7 * S + s]);
+    for (;;) {
+        const int item = next_item(a.counter, &item_sh);
+        if (item >= a.nitems) break;
+        const int chunk = item / a.ntiles;
+        const int tile = item - chunk * a.ntiles;
+        const int64_t sb0 = (int64_t)chunk * a.chunk_len;
+        const int64_t sb1 = min(nseg, sb0 + a.chunk_len);
+        const int64_t t = (int64_t)tile * TILE + tid;
+        const bool live = t < a.m;
+        double px = 0.0, py = 0.0, pz = 0.0;
+        if (live) {
+            px = a.t[t];
+            py = a.t[a.tstride + t];
+            pz = a.t[2 * a.tstride + t];
         }
-        __syncthreads();
-        float tx = 0.f, ty = 0.f, tz = 0.f;
-#pragma unroll 4
-        for (int k = 0; k < cnt; ++k) {
-            const bool nearp = pair32(fx, fy, fz, sa[k], sb[k], tx, ty, tz);
-            if (nearp && live) {
-                const int64_t s = base + k;
-                pair64(px, py, pz, a.s64[s], a.s64[S + s], a.s64[2 * S + s], a.s64[3 * S + s],
-                       a.s64[4 * S + s], a.s64[5 * S + s], a.s64[6 * S + s], a.s64[7 * S + s],
-                       nx, ny, nz);
+        const float phx = (float)px, phy = (float)py, phz = (float)pz;
+        const float plx = (float)(px - (double)phx), ply = (float)(py - (double)phy),
+                    plz = (float)(pz - (double)phz);
+        float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
+        double nx = 0.0, ny = 0.0, nz = 0.0;
+        for (int64_t base = sb0; base < sb1; base += TILE) {
+            const int cnt = (int)min<int64_t>(TILE, sb1 - base);
+            __syncthreads();
+            if (tid < cnt) {
+                const int64_t s = base + tid;
+                s0v[tid] = make_float4(a.s32[s], a.s32[S + s], a.s32[2 * S + s], a.s32[3 * S + s]);
+                s1v[tid] = make_float4(a.s32[4 * S + s], a.s32[5 * S + s], a.s32[6 * S + s],
+                                       a.s32[7 * S + s]);
+                s2v[tid] = make_float4(a.s32[8 * S + s], a.s32[9 * S + s], a.s32[10 * S + s], 0.f);
             }
+            __syncthreads();
+            float tx = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll 4
+            for (int k = 0; k < cnt; ++k) {
+                const bool nearp =
+                    pair32(phx, phy, phz, plx, ply, plz, s0v[k], s1v[k], s2v[k], tx, ty, tz);
+                if (nearp && live) {
+                    const int64_t s = base + k;
+                    pair64(px, py, pz,
+                           make_double4(a.s64[s], a.s64[S + s], a.s64[2 * S + s], a.s64[3 * S + s]),
+                           make_double4(a.s64[4 * S + s], a.s64[5 * S + s], a.s64[6 * S + s],
+                                        a.s64[7 * S + s]),
+                           nx, ny, nz);
+                }
+            }
+            kahan_add(Sx, Cx, tx);
+            kahan_add(Sy, Cy, ty);
+            kahan_add(Sz, Cz, tz);
         }
-        kahan_add(Sx, Cx, tx);
-        kahan_add(Sy, Cy, ty);
-        kahan_add(Sz, Cz, tz);
-    }
-    if (live) {
-        double *o = a.part + (size_t)chunk * 3 * a.m;
-        o[t] = ((double)Sx - (double)Cx) + nx;
-        o[a.m + t] = ((double)Sy - (double)Cy) + ny;
-        o[2 * a.m + t] = ((double)Sz - (double)Cz) + nz;
+        if (live) {
+            double *o = a.part + (size_t)chunk * 3 * a.m;
+            o[t] = ((double)Sx - (double)Cx) + nx;
+            o[a.m + t] = ((double)Sy - (double)Cy) + ny;
+            o[2 * a.m + t] = ((double)Sz - (double)Cz) + nz;
+        }
     }
 }
 
-__global__ void bs_reduce_kernel(const double *part, int nchunks, int64_t m,
-                                 double *out, int64_t ostride, int acc)
+__global__ void bs_reduce_kernel(const double *part, int nchunks, int64_t m, double *out,
+                                 int64_t ostride, int acc)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
@@ -250,30 +270,58 @@ __global__ void bs_reduce_kernel(const double *part, int nchunks, int64_t m,
     }
 }
 
+__global__ void zero_int_kernel(int *p) { *p = 0; }
+
+template <typename K>
+int resident_blocks(vlm_ctx *ctx, K kernel, int block, int *cache)
+{
+    if (*cache > 0) return *cache;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+            hipSuccess)
+        return -1;
+    *cache = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+    return *cache;
+}
+
 }  // namespace
 
-int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32,
-            int64_t segstride, const long long *nseg_dev, int64_t nseg_max,
-            const double *t, int64_t m, int64_t tstride, double *out,
-            int accumulate)
+int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segstride,
+            const long long *nseg_dev, int64_t nseg_max, const double *t, int64_t m,
+            int64_t tstride, double *out, int accumulate)
 {
     if (m <= 0) return VLM_OK;
     const bool small = m < 1024;
     const int blk = small ? BS_BLOCK_SMALL : BS_BLOCK;
     const int ntiles = (int)((m + blk - 1) / blk);
+    const bool mixed = ctx->precision == VLM_MIXED;
+    int cap = 0;
+    if (mixed)
+        cap = small ? resident_blocks(ctx, bs_mixed_kernel<BS_BLOCK_SMALL>, blk, &ctx->occ[0])
+                    : resident_blocks(ctx, bs_mixed_kernel<BS_BLOCK>, blk, &ctx->occ[1]);
+    else
+        cap = small ? resident_blocks(ctx, bs_fp64_kernel<BS_BLOCK_SMALL>, blk, &ctx->occ[2])
+                    : resident_blocks(ctx, bs_fp64_kernel<BS_BLOCK>, blk, &ctx->occ[3]);
+    if (cap <= 0) return fail(ctx, VLM_ERR_HIP, "occupancy query failed");
     int nchunks = 0;
     int64_t chunk_len = 1;
     if (nseg_max > 0) {
-        const int64_t want = 4096;
-        int64_t nc = (want + ntiles - 1) / ntiles;
-        const int64_t maxc = (nseg_max + blk - 1) / blk;
+        // about 16 work items per resident workgroup, chunks of at least
+        // four tiles; small target sets use at most 128 chunks so the
+        // fixed-order reduction stays short
+        const int64_t min_len = small ? 512 : 4 * BS_BLOCK;
+        int64_t nc = ((int64_t)16 * cap + ntiles - 1) / ntiles;
+        const int64_t maxc = (nseg_max + min_len - 1) / min_len;
         if (nc > maxc) nc = maxc;
+        if (small && nc > 128) nc = 128;
         if (nc < 1) nc = 1;
         chunk_len = (nseg_max + nc - 1) / nc;
         nchunks = (int)((nseg_max + chunk_len - 1) / chunk_len);
     }
     if (nchunks > 0) {
         VLM_HIP(ctx->part.reserve((size_t)nchunks * 3 * (size_t)m));
+        VLM_HIP(ctx->counter.reserve(1));
         BsArgs a;
         a.s64 = seg64;
         a.s32 = seg32;
@@ -285,8 +333,10 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32,
         a.tstride = tstride;
         a.part = ctx->part.p;
         a.ntiles = ntiles;
-        a.nlogical = ntiles * nchunks;
-        const int grid = ((a.nlogical + XCDS - 1) / XCDS) * XCDS;
+        a.nitems = ntiles * nchunks;
+        a.counter = ctx->counter.p;
+        const int grid = a.nitems < cap ? a.nitems : cap;
+        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->counter.p);
         Timer tm{};
         const bool timed = ctx->profile && !small;
         if (timed) {
@@ -294,7 +344,7 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32,
             VLM_HIP(hipEventCreate(&tm.b));
             VLM_HIP(hipEventRecord(tm.a, ctx->stream));
         }
-        if (ctx->precision == VLM_MIXED) {
+        if (mixed) {
             if (small)
                 hipLaunchKernelGGL(bs_mixed_kernel<BS_BLOCK_SMALL>, dim3(grid), dim3(blk), 0,
                                    ctx->stream, a);
@@ -312,25 +362,22 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32,
         VLM_HIP(hipGetLastError());
         if (timed) {
             VLM_HIP(hipEventRecord(tm.b, ctx->stream));
-            tm.pairs = -1.0;  // resolved from the device count at read-out
+            tm.pairs = -1.0;
             ctx->timers.push_back(tm);
             ctx->prof_launches += 1;
+            // pair count = m * device segment count, copied to pinned memory
+            // without a stream synchronisation and resolved at read-out
+            const size_t slot = ctx->timers.size() - 1;
+            if (slot < ctx->prof_cap) {
+                VLM_HIP(hipMemcpyAsync(ctx->prof_nseg + slot, nseg_dev, sizeof(long long),
+                                       hipMemcpyDeviceToHost, ctx->stream));
+                ctx->timers.back().pairs = (double)m;
+            }
         }
     }
     hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
                        ctx->stream, ctx->part.p, nchunks, m, out, tstride, accumulate);
     VLM_HIP(hipGetLastError());
-    if (ctx->profile && nchunks > 0 && !small) {
-        // pair count = m * device segment count, copied to pinned memory
-        // without a stream synchronisation and resolved at read-out
-        Timer &tb = ctx->timers.back();
-        const size_t slot = ctx->timers.size() - 1;
-        if (slot < ctx->prof_cap) {
-            VLM_HIP(hipMemcpyAsync(ctx->prof_nseg + slot, nseg_dev, sizeof(long long),
-                                   hipMemcpyDeviceToHost, ctx->stream));
-            tb.pairs = (double)m;
-        }
-    }
     return VLM_OK;
 }
 

@@ -35,26 +35,30 @@ __global__ void compact_kernel(const Slot *slots, const int *flags, const int *i
     const int64_t k = idx[i];
     const Slot s = slots[i];
     const double g = s.gam * kInv4Pi;
-    const double rc2 = s.rc * s.rc;
+    const double r0x = s.bx - s.ax, r0y = s.by - s.ay, r0z = s.bz - s.az;
+    const double n0 = r0x * r0x + r0y * r0y + r0z * r0z;
+    const double q = s.rc * s.rc * n0;
     d64[k] = s.ax;
     d64[cap + k] = s.ay;
     d64[2 * cap + k] = s.az;
-    d64[3 * cap + k] = s.bx;
-    d64[4 * cap + k] = s.by;
-    d64[5 * cap + k] = s.bz;
-    d64[6 * cap + k] = g;
-    d64[7 * cap + k] = rc2;
-    const double r0x = s.bx - s.ax, r0y = s.by - s.ay, r0z = s.bz - s.az;
-    const double n0 = r0x * r0x + r0y * r0y + r0z * r0z;
-    const double q = rc2 * n0;
-    d32[k] = (float)s.ax;
-    d32[cap + k] = (float)s.ay;
-    d32[2 * cap + k] = (float)s.az;
-    d32[3 * cap + k] = (float)r0x;
-    d32[4 * cap + k] = (float)r0y;
-    d32[5 * cap + k] = (float)r0z;
-    d32[6 * cap + k] = (float)g;
-    d32[7 * cap + k] = (float)(q * q);
+    d64[3 * cap + k] = g;
+    d64[4 * cap + k] = r0x;
+    d64[5 * cap + k] = r0y;
+    d64[6 * cap + k] = r0z;
+    d64[7 * cap + k] = q;
+    const float hx = (float)s.ax, hy = (float)s.ay, hz = (float)s.az;
+    d32[k] = hx;
+    d32[cap + k] = hy;
+    d32[2 * cap + k] = hz;
+    d32[3 * cap + k] = (float)g;
+    d32[4 * cap + k] = (float)(s.ax - (double)hx);
+    d32[5 * cap + k] = (float)(s.ay - (double)hy);
+    d32[6 * cap + k] = (float)(s.az - (double)hz);
+    d32[7 * cap + k] = (float)q;
+    d32[8 * cap + k] = (float)r0x;
+    d32[9 * cap + k] = (float)r0y;
+    d32[10 * cap + k] = (float)r0z;
+    d32[11 * cap + k] = 0.0f;
 }
 
 __global__ void zero_count_kernel(long long *count) { *count = 0; }
@@ -98,7 +102,7 @@ int compact_slots(vlm_ctx *ctx, int64_t nslots)
     const int64_t cap = nslots > 0 ? nslots : 1;
     if (ctx->seg.cap < cap) {
         VLM_HIP(ctx->seg.d64.reserve((size_t)8 * cap));
-        VLM_HIP(ctx->seg.d32.reserve((size_t)8 * cap));
+        VLM_HIP(ctx->seg.d32.reserve((size_t)12 * cap));
         ctx->seg.cap = cap;
     }
     return compact_into(ctx, ctx->slots.p, ctx->flags.p, nslots, ctx->seg.d64, ctx->seg.d32,
@@ -166,7 +170,7 @@ int upload_fset(vlm_ctx *ctx, int64_t nfil, const double *a, const double *b,
                        ctx->slots.p, ctx->flags.p);
     VLM_HIP(hipGetLastError());
     VLM_HIP(d64.reserve((size_t)8 * nfil));
-    VLM_HIP(d32.reserve((size_t)8 * nfil));
+    VLM_HIP(d32.reserve((size_t)12 * nfil));
     VLM_TRY(compact_into(ctx, ctx->slots.p, ctx->flags.p, nfil, d64, d32, count, nfil));
     VLM_HIP(hipStreamSynchronize(ctx->stream));  // raw is released on return
     return VLM_OK;
@@ -292,7 +296,7 @@ int vlm_filament_vind(vlm_ctx *ctx, int64_t nfil, const double *a, const double 
     if (nfil > 0) VLM_TRY(upload_fset(ctx, nfil, a, b, gam, rc, fs.d64, fs.d32, fs.count));
     else {
         VLM_HIP(fs.d64.reserve(8));
-        VLM_HIP(fs.d32.reserve(8));
+        VLM_HIP(fs.d32.reserve(12));
         VLM_TRY(compact_into(ctx, nullptr, nullptr, 0, fs.d64, fs.d32, fs.count, 1));
     }
     VLM_HIP(ctx->tgt.reserve((size_t)3 * m));

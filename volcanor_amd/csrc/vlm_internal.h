@@ -38,9 +38,11 @@ struct DBuf {
     }
 };
 
-// Compacted straight-segment list: structure of arrays.
-//   d64: ax ay az bx by bz g rc2       (g = gam/(4 pi), rc2 = rc^2)
-//   d32: ax ay az r0x r0y r0z g q2     (r0 = b - a, q2 = (rc^2 |r0|^2)^2)
+// Compacted straight-segment list: structure of arrays (stride cap).
+//   d64 (8):  ax ay az g | r0x r0y r0z q          g = gam/(4 pi), r0 = b - a,
+//                                                 q = rc^2 |r0|^2
+//   d32 (12): ahx ahy ahz g | alx aly alz q | r0x r0y r0z 0
+//             (a = a_hi + a_lo split into two floats)
 // count: device int64 with the number of valid segments.
 struct SegList {
     DBuf<double> d64;
@@ -130,6 +132,8 @@ struct vlm_ctx {
     vlm::DBuf<unsigned char> scan_tmp;
     size_t scan_tmp_bytes = 0;
     vlm::DBuf<double> tgt, out, part;
+    vlm::DBuf<int> counter;  // work-item counter of the Biot-Savart launches
+    int occ[4] = {0, 0, 0, 0};  // resident workgroups per kernel variant
     vlm::DBuf<double> hostio;  // device staging for host point lists
     // profiling
     int profile = 0;

@@ -419,36 +419,42 @@ __global__ void age_kernel(RV v, double dt)
     }
 }
 
-// rotor_dissipate_wake (classdef.f90:4356-4408).  One lane per (blade,
-// span) column walks the rows, so the vf4 <- upstream vf2 copy sees the
-// grown vf2 exactly as the reference's loop order does.
+// rotor_dissipate_wake (classdef.f90:4356-4408), in two fully parallel
+// passes: (1) every ring grows vf1 (copied to vf3) and vf2 and decays gam,
+// far filaments grow and decay; (2) vf4 of every row but the newest takes
+// the grown vf2 of the row upstream.
 __global__ void dissipate_kernel(RV v, double inc, double decay)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < v.nb * v.ns) {
-        const int b = e / v.ns, j = e % v.ns;
-        for (int i = v.rn; i < v.nN; ++i) {
-            const size_t r = ridx(v, b, phys(v, i), j);
-            double *rc = v.rc4 + 4 * r;
-            rc[0] = sqrt(rc[0] * rc[0] + inc);
-            rc[2] = rc[0];
-            v.gam[r] = v.gam[r] * decay;
-        }
-        for (int i = v.rn; i < v.nN; ++i) {
-            double *rc = v.rc4 + 4 * ridx(v, b, phys(v, i), j);
-            rc[1] = sqrt(rc[1] * rc[1] + inc);
-        }
-        if (v.rn != v.nN - 1)
-            for (int i = v.rn + 1; i < v.nN; ++i)
-                v.rc4[4 * ridx(v, b, phys(v, i), j) + 3] =
-                    v.rc4[4 * ridx(v, b, phys(v, i - 1), j) + 1];
-    }
-    const int F = v.nF - v.rf;
-    if (F > 0 && e < v.nb * F) {
-        const size_t q = (size_t)(e / F) * v.nF + v.rf + e % F;
+    const int R = v.nN - v.rn, F = v.nF - v.rf;
+    const int64_t per = (int64_t)R * v.ns + F;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * v.nb) return;
+    const int b = (int)(e / per);
+    const int64_t k = e - b * per;
+    if (k < (int64_t)R * v.ns) {
+        const int i = v.rn + (int)(k / v.ns), j = (int)(k % v.ns);
+        const size_t r = ridx(v, b, phys(v, i), j);
+        double *rc = v.rc4 + 4 * r;
+        rc[0] = sqrt(rc[0] * rc[0] + inc);
+        rc[2] = rc[0];
+        v.gam[r] = v.gam[r] * decay;
+        rc[1] = sqrt(rc[1] * rc[1] + inc);
+    } else {
+        const size_t q = (size_t)b * v.nF + v.rf + (k - (int64_t)R * v.ns);
         v.frc[q] = sqrt(v.frc[q] * v.frc[q] + inc);
         v.fgam[q] = v.fgam[q] * decay;
     }
+}
+
+__global__ void dissipate_vf4_kernel(RV v)
+{
+    const int R = v.nN - v.rn - 1;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (R <= 0 || e >= (int64_t)R * v.ns * v.nb) return;
+    const int b = (int)(e / ((int64_t)R * v.ns));
+    const int64_t k = e - (int64_t)b * R * v.ns;
+    const int i = v.rn + 1 + (int)(k / v.ns), j = (int)(k % v.ns);
+    v.rc4[4 * ridx(v, b, phys(v, i), j) + 3] = v.rc4[4 * ridx(v, b, phys(v, i - 1), j) + 1];
 }
 
 // rotor_strain_wake (classdef.f90:4410-4422): far filaments only.
@@ -1433,10 +1439,14 @@ int vlm_rotor_dissipate_wake(vlm_ctx *ctx, int ir, double dt, double kinematicVi
     const double oseen = 1.2564;
     const double inc = 4.0 * oseen * apparentViscCoeff * kinematicVisc * dt;
     const double decay = std::exp(-decayCoeff * dt);
+    const int64_t R = r->d.nNwake - (r->rowNear - 1);
     const int64_t F = r->d.nFwake - (r->rowFar - 1);
-    const int64_t n = std::max<int64_t>((int64_t)r->d.nb * r->d.ns, (F > 0 ? F : 0) * r->d.nb);
+    const int64_t n = (R * r->d.ns + (F > 0 ? F : 0)) * r->d.nb;
     hipLaunchKernelGGL(dissipate_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream, view(*r), inc,
                        decay);
+    if (R > 1)
+        hipLaunchKernelGGL(dissipate_vf4_kernel, dim3(nblk((R - 1) * r->d.ns * r->d.nb)), dim3(256),
+                           0, ctx->stream, view(*r));
     VLM_HIP(hipGetLastError());
     return VLM_OK;
 }

@@ -117,6 +117,15 @@ module volcanor_host
 
   public :: volcanor_run, run_case
 
+  interface
+    integer(c_int) function c_mkdir(path, mode) bind(C, name='mkdir')
+      !! POSIX mkdir(2); an existing directory is not an error here
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: mode
+    end function c_mkdir
+  end interface
+
 contains
 
   ! ------------------------------------------------------------------
@@ -1336,7 +1345,7 @@ contains
     enddo
     nforce = 0
     if (write_out) then
-      call execute_command_line('mkdir -p '//trim(dir)//'/Results')
+      ir = int(c_mkdir(trim(dir)//'/Results'//c_null_char, int(o'755', c_int)))
       do ir = 1, nr
         write (rc2, '(I0.2)') ir
         open (newunit=uforce(ir), file=trim(dir)//'/Results/r'//rc2//'ForceNonDim.csv', &
