@@ -4,9 +4,8 @@
 // libMath_noBLAS.f90 in no-BLAS builds).  Column-major, 0-based pivots,
 // info = first zero pivot (1-based) or 0.
 //
-// Small matrices (the AIC of one rotor, N <= a few hundred) are factorised
-// by a single workgroup.  Larger ones use a right-looking blocked LU whose
-// trailing update A22 -= L21 U12 runs on MFMA (v_mfma_f64_16x16x4f64).
+// The matrix is factorised by a single workgroup (unblocked, right-looking),
+// which covers the AIC of one rotor (N <= a few hundred).
 #include "vlm_internal.h"
 
 namespace vlm {

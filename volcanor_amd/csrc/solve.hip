@@ -8,9 +8,7 @@
 // device) and every step solves by two triangular sweeps plus iterative
 // refinement on an FP64 residual RHS - AIC . gam.
 //
-// Factorisation: right-looking blocked LU.  The panel (NB columns) is
-// factorised by one workgroup; the trailing update A22 -= L21 * U12 runs on
-// MFMA FP64 (v_mfma_f64_16x16x4f64) in lu_mfma.hip.
+// Factorisation: lu_factor_device (lu_mfma.hip).
 #include <cmath>
 #include <cstring>
 #include <vector>
