@@ -270,6 +270,42 @@ def bench_rotor(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_lu(args):
+    """Dense LU of an N = 2176 FP64 matrix (the size of a rotor-wing AIC
+    rebuilt every step): device time of the factorisation only."""
+    from volcanor_amd import vlm
+    n = 2176
+    A = np.asfortranarray(np.random.default_rng(20261202).standard_normal((n, n)))
+    ctx = vlm.Context(0)
+    for _ in range(args.warmup):
+        ctx.lu_factor(A)
+    ctx.profile_enable(True)
+    ctx.profile_reset()
+    for _ in range(args.steps):
+        LU, piv = ctx.lu_factor(A)
+    ms, cnt, flops = ctx.profile_get_lu()
+    achieved = flops / (ms * 1e-3)
+    out = {"metric": "dense LU factorisation, FP64", "value": achieved / 1e12, "unit": "TFLOP/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms / max(cnt, 1), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"LU with partial pivoting, N={n}, normal random matrix"},
+           "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": 78.6,
+                        "unit": "TFLOP/s", "frac": achieved / 78.6e12, "traffic": None}}
+    if not args.no_cpu:
+        from scipy.linalg import lu_factor
+        t0 = time.perf_counter()
+        lu_factor(A)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 2 * n**3 / 3 / dt / 1e12, "unit": "TFLOP/s",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                               "kind": "port", "sample": "scipy/LAPACK dgetrf, same matrix"}
+        LUl, pivl = lu_factor(A)
+        out["check"] = {"pivots_equal_lapack": bool(np.array_equal(piv, pivl)),
+                        "max_rel_diff_vs_lapack": float(np.max(np.abs(LU - LUl)) / np.max(np.abs(LUl)))}
+    print(json.dumps(out), flush=True)
+
+
 def bench_hbmcopy(args):
     """HBM calibration: a plain device-to-device copy of 1 GiB (torch's copy
     kernel).  Its rocprof FETCH_SIZE / WRITE_SIZE per dispatch, against the
@@ -296,7 +332,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", choices=["rotor2", "microbench", "hbmcopy"], default="rotor2")
+    ap.add_argument("--workload", choices=["rotor2", "microbench", "lu", "hbmcopy"], default="rotor2")
     ap.add_argument("--variants", default="mixed",
                     help="comma list of rotor2 variants run beside fp64 (mixed, learned_mixed)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
@@ -312,7 +348,11 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); a multi-GPU "
               "sharding of the wake is not built.", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "hbmcopy":
+    if args.workload == "lu":
+        args.steps = args.steps or 10
+        args.warmup = 2 if args.warmup is None else args.warmup
+        bench_lu(args)
+    elif args.workload == "hbmcopy":
         args.steps = args.steps or 20
         args.warmup = 3 if args.warmup is None else args.warmup
         bench_hbmcopy(args)

@@ -233,6 +233,11 @@ int vlm_profile_enable(vlm_ctx *ctx, int enable);
 int vlm_profile_reset(vlm_ctx *ctx);
 int vlm_profile_get(vlm_ctx *ctx, double *bs_ms, int64_t *bs_launches,
                     double *pairs);
+/* Device time of the LU factorisations (lu_factor on the device, events
+ * around the whole factorisation) since the last reset, their count and
+ * their nominal flops (2 n^3 / 3 each). */
+int vlm_profile_get_lu(vlm_ctx *ctx, double *ms, int64_t *factorisations,
+                       double *flops);
 
 #ifdef __cplusplus
 }

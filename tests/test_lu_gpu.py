@@ -1,0 +1,85 @@
+"""AIC assembly, LU factorisation (single-workgroup and MFMA blocked paths)
+and the refined circulation solve against the oracle and LAPACK."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests import wake_harness as H
+from tests.geometry import rings_from_grids
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx():
+    from volcanor_amd import vlm
+    return vlm.Context(0)
+
+
+def _matrix(n, seed):
+    rng = np.random.default_rng(seed)
+    return np.asfortranarray(rng.standard_normal((n, n)))
+
+
+@pytest.mark.parametrize("n", [40, 256, 257, 777])
+def test_lu_factor_matches_oracle(n):
+    """n <= 256 runs the one-workgroup path, n > 256 the blocked path whose
+    trailing update runs on MFMA; 257 and 777 leave ragged last panels."""
+    A = _matrix(n, n)
+    LU, piv = _ctx().lu_factor(A)
+    LUo, pivo, info = orc.lu_factor(A)
+    assert info == 0
+    np.testing.assert_array_equal(piv, pivo)
+    err = np.max(np.abs(LU - LUo)) / np.max(np.abs(LUo))
+    assert err < 1e-11, err
+
+
+def test_lu_factor_2176_matches_lapack():
+    """The LU microbenchmark size (2176 unknowns)."""
+    from scipy.linalg import lu_factor
+    A = _matrix(2176, 5)
+    LU, piv = _ctx().lu_factor(A)
+    LUl, pivl = lu_factor(A)
+    np.testing.assert_array_equal(piv, pivl)
+    err = np.max(np.abs(LU - LUl)) / np.max(np.abs(LUl))
+    assert err < 1e-10, err
+
+
+@pytest.mark.parametrize("n", [100, 600])
+def test_lu_solve_refine_reaches_fp64_residual(n):
+    A = _matrix(n, 11 + n)
+    b = np.random.default_rng(3).standard_normal(n)
+    ctx = _ctx()
+    LU, piv = ctx.lu_factor(A)
+    x, iters = ctx.lu_solve_refine(A, LU, piv, b, max_iter=3, tol=1e-15)
+    r = b - A @ x
+    assert np.max(np.abs(r)) <= 1e-13 * np.max(np.abs(b)) * n
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [50, 300])
+def test_lu_zero_pivot_is_an_error(n):
+    from volcanor_amd import vlm
+    A = _matrix(n, 2)
+    A[:, n // 3] = 0.0
+    with pytest.raises(vlm.VLMError):
+        _ctx().lu_factor(A)
+
+
+def test_rotor_aic_and_solve_match_oracle():
+    case = H.Case(nb=2, nc=3, ns=8, nNwake=4, nFwake=0)
+    gpu, cpu = _ctx(), orc.System()
+    ig, ic = H.setup(gpu, case)[0], H.setup(cpu, case)[0]
+    grids, _ = H.blade_grids(case, 0.0)
+    corners = rings_from_grids(grids).reshape(-1, 4, 3)
+    cp = corners.mean(axis=1)
+    d1 = corners[:, 2] - corners[:, 0]
+    d2 = corners[:, 3] - corners[:, 1]
+    ncap = np.cross(d1, d2)
+    ncap /= np.linalg.norm(ncap, axis=1)[:, None]
+    n = len(cp)
+    A = gpu.rotor_calc_aic(ig, cp, ncap, n)
+    Ao = cpu.rotor_calc_aic(ic, cp, ncap, n)
+    assert np.max(np.abs(A - Ao)) <= 1e-14 * np.max(np.abs(Ao))
+    rhs = np.random.default_rng(9).standard_normal(n)
+    g, _ = gpu.rotor_solve(ig, rhs)
+    np.testing.assert_allclose(g, np.linalg.solve(Ao, rhs), rtol=1e-10, atol=1e-12)

@@ -19,7 +19,8 @@
 // MFMA: the pair is not a contraction.
 //
 // Precision modes
-//   FP64  : FP64 throughout; 1/sqrt via v_rsq_f64 plus two Newton steps.
+//   FP64  : FP64 throughout; 1/sqrt via v_rsq_f64 plus one third-order
+//           correction.
 //   MIXED : FP32 pair arithmetic on hi/lo-split coordinates
 //           (r1 = (p_hi - a_hi) + (p_lo - a_lo), accurate to FP32 relative
 //           precision of r1 itself), v_rsq_f32, per-tile FP32 sums folded
@@ -40,14 +41,14 @@ constexpr float NEAR_LEN2 = 4.0f;   // ... and |r1|^2 + |r2|^2 < 4 |r0|^2
 
 __device__ __forceinline__ double rsq64(double x)
 {
-    double y = __builtin_amdgcn_rsq(x);
-    double h = x * y;
-    double r = __builtin_fma(-h, y, 1.0);
-    y = __builtin_fma(0.5 * y, r, y);
-    h = x * y;
-    r = __builtin_fma(-h, y, 1.0);
-    y = __builtin_fma(0.5 * y, r, y);
-    return y;
+    // v_rsq_f64 is accurate to about 2^-24; one third-order correction
+    // y (1 + e/2 + 3 e^2/8) with e = 1 - x y^2 brings it to FP64 rounding
+    // (the dropped 5 e^3/16 term is below 2^-69).
+    const double y = __builtin_amdgcn_rsq(x);
+    const double h = x * y;
+    const double e = __builtin_fma(-h, y, 1.0);
+    const double c = __builtin_fma(e, 0.375, 0.5);
+    return __builtin_fma(y, e * c, y);
 }
 
 // FP64 pair; segment record (ax ay az g | r0x r0y r0z q).

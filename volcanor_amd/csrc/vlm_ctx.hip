@@ -243,10 +243,11 @@ int vlm_destroy(vlm_ctx *ctx)
     for (auto *p : ctx->aic) (void)hipFree(p);
     for (auto *p : ctx->lu) (void)hipFree(p);
     for (auto *p : ctx->piv) (void)hipFree(p);
-    for (auto &t : ctx->timers) {
-        (void)hipEventDestroy(t.a);
-        (void)hipEventDestroy(t.b);
-    }
+    for (auto *v : {&ctx->timers, &ctx->lu_timers})
+        for (auto &t : *v) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
     if (ctx->prof_nseg) (void)hipHostFree(ctx->prof_nseg);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -386,6 +387,11 @@ int vlm_profile_reset(vlm_ctx *ctx)
         (void)hipEventDestroy(t.b);
     }
     ctx->timers.clear();
+    for (auto &t : ctx->lu_timers) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    ctx->lu_timers.clear();
     ctx->prof_ms = 0.0;
     ctx->prof_pairs = 0.0;
     ctx->prof_launches = 0;
@@ -407,6 +413,23 @@ int vlm_profile_get(vlm_ctx *ctx, double *bs_ms, int64_t *bs_launches, double *p
     if (bs_ms) *bs_ms = ms;
     if (bs_launches) *bs_launches = (int64_t)ctx->timers.size();
     if (pairs) *pairs = pr;
+    return VLM_OK;
+}
+
+int vlm_profile_get_lu(vlm_ctx *ctx, double *ms, int64_t *factorisations, double *flops)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    double t = 0.0, f = 0.0;
+    for (auto &tm : ctx->lu_timers) {
+        float e = 0.f;
+        VLM_HIP(hipEventElapsedTime(&e, tm.a, tm.b));
+        t += e;
+        f += tm.pairs;
+    }
+    if (ms) *ms = t;
+    if (factorisations) *factorisations = (int64_t)ctx->lu_timers.size();
+    if (flops) *flops = f;
     return VLM_OK;
 }
 

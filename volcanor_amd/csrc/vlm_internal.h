@@ -138,6 +138,7 @@ struct vlm_ctx {
     // profiling
     int profile = 0;
     std::vector<vlm::Timer> timers;
+    std::vector<vlm::Timer> lu_timers;  // LU factorisations (pairs = flops)
     double prof_ms = 0.0, prof_pairs = 0.0;
     long long *prof_nseg = nullptr;  // pinned, one slot per timed launch
     size_t prof_cap = 0;

@@ -86,6 +86,7 @@ _SIGS = {
     "vlm_profile_enable": [_P, _I],
     "vlm_profile_reset": [_P],
     "vlm_profile_get": [_P, C.POINTER(_D), C.POINTER(_L), C.POINTER(_D)],
+    "vlm_profile_get_lu": [_P, C.POINTER(_D), C.POINTER(_L), C.POINTER(_D)],
 }
 
 _lib = None
@@ -375,3 +376,9 @@ class Context:
         ms, n, pairs = C.c_double(), C.c_int64(), C.c_double()
         self._chk(self.lib.vlm_profile_get(self.h, C.byref(ms), C.byref(n), C.byref(pairs)))
         return ms.value, n.value, pairs.value
+
+    def profile_get_lu(self):
+        """(device ms, factorisations, nominal flops) of LU factorisations."""
+        ms, n, fl = C.c_double(), C.c_int64(), C.c_double()
+        self._chk(self.lib.vlm_profile_get_lu(self.h, C.byref(ms), C.byref(n), C.byref(fl)))
+        return ms.value, n.value, fl.value
