@@ -178,7 +178,8 @@ int vlm_rotor_get_velocity(vlm_ctx *ctx, int ir, int which, double *velN,
 /* AIC(row,col) = vr_vind(ring col, CP row) . nCap(row)
  * (rotor_calcAIC, classdef.f90:4151-4179); cp and ncap are 3 x N with
  * N = nb*nc*ns in the reference's row order.  The matrix is assembled,
- * factorised (LU, partial pivoting) and kept on the
+ * factorised (LU, partial pivoting; blocked with an MFMA-FP64 trailing
+ * update above order 256) and kept on the
  * device; A (N x N column-major) is copied out when non-NULL. */
 int vlm_rotor_calc_aic(vlm_ctx *ctx, int ir, const double *cp,
                        const double *ncap, double *A);

@@ -33,10 +33,13 @@ def test_lu_factor_matches_oracle(n):
     assert err < 1e-11, err
 
 
-def test_lu_factor_2176_matches_lapack():
-    """The LU microbenchmark size (2176 unknowns)."""
+@pytest.mark.parametrize("n", [2176, 2600])
+def test_lu_factor_large_matches_lapack(n):
+    """2176: the LU microbenchmark size (panels held in registers and LDS);
+    2600: the first panels exceed on-chip capacity and take the tall-panel
+    kernel."""
     from scipy.linalg import lu_factor
-    A = _matrix(2176, 5)
+    A = _matrix(n, 5)
     LU, piv = _ctx().lu_factor(A)
     LUl, pivl = lu_factor(A)
     np.testing.assert_array_equal(piv, pivl)

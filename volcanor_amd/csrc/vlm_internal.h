@@ -148,6 +148,7 @@ struct vlm_ctx {
     std::vector<int *> piv;
     std::vector<int> aic_n;
     vlm::DBuf<double> solve_tmp;
+    vlm::DBuf<int> lu_work;   // row-move lists of the blocked LU
     vlm::Mlp *mlp = nullptr;
     vlm::Predictor pred;
 };
