@@ -247,7 +247,10 @@ def bench_rotor(args):
                          "bs_kernel_avg_ms": r["st"].bs_kernel_ms / max(r["st"].bs_launches, 1),
                          "predictor": {"full": r["st"].pred_full,
                                        "learned": r["st"].pred_learned,
-                                       "fallbacks": r["st"].pred_fallbacks}}
+                                       "fallbacks": r["st"].pred_fallbacks,
+                                       "judged": r["st"].pred_judged,
+                                       "mlp_rel_rms": r["st"].pred_mlp_rms,
+                                       "extrapolation_rel_rms": r["st"].pred_base_rms}}
                      for k, r in runs.items()},
     }
     out["ct_note"] = ("ct_dev_per_rev_vs_fp64: max |CT - CT_fp64| / max|CT_fp64| per revolution. "

@@ -225,6 +225,13 @@ int vlm_set_predictor(vlm_ctx *ctx, int mode, double tol, int sample_stride,
                       int warmup_steps, int train_iters, double lref);
 int vlm_predictor_stats(vlm_ctx *ctx, int64_t *full_passes, int64_t *learned_passes,
                         int64_t *fallbacks, double *last_err, double *last_loss);
+/* Accuracy of the learned pass 2 over all judged steps so far: RMS error
+ * relative to the RMS exact speed on the judge samples, for the network and
+ * for the network-free substitute of the same cost (the linear
+ * extrapolation 2 v_n - v_(n-1) of the pass-1 velocity, which turns the AM2
+ * corrector into AB2).  base_rel_rms / mlp_rel_rms is the network's gain. */
+int vlm_predictor_errors(vlm_ctx *ctx, double *mlp_rel_rms, double *base_rel_rms,
+                         int64_t *judged_steps);
 
 /* ---- kernel timing ----------------------------------------------------- */
 /* When enabled, every Biot-Savart pair launch is bracketed by HIP events on

@@ -63,15 +63,19 @@ def test_edge_cases():
     np.testing.assert_allclose(v, 2.5 + orc.filament_sum(A, B, g, np.full(2, 0.1), P), rtol=1e-12)
 
 
-def test_repeatable_bitwise():
+@pytest.mark.parametrize("m", [999, 3000])
+def test_repeatable_bitwise(m):
+    """Fixed-order reductions: the small-target kernel (m < 1024) and the
+    work-queue kernel give bitwise identical results on repeat."""
     rng = np.random.default_rng(9)
     a, b = rng.uniform(-2, 2, size=(2, 5000, 3))
     g, rc = rng.normal(size=5000), np.full(5000, 0.01)
-    p = rng.uniform(-2, 2, size=(999, 3))
-    ctx = _ctx()
-    v1 = ctx.filament_vind(a, b, g, rc, p)
-    v2 = ctx.filament_vind(a, b, g, rc, p)
-    assert np.array_equal(v1, v2)
+    p = rng.uniform(-2, 2, size=(m, 3))
+    for prec in (0, 1):
+        ctx = _ctx(prec)
+        v1 = ctx.filament_vind(a, b, g, rc, p)
+        v2 = ctx.filament_vind(a, b, g, rc, p)
+        assert np.array_equal(v1, v2)
 
 
 def test_mixed_precision_helix_error_bound():
@@ -81,13 +85,14 @@ def test_mixed_precision_helix_error_bound():
     a, b = helix_segments(8, 2000, rng)
     g = rng.normal(size=len(a))
     rc = np.full(len(a), 0.01)
-    p = rng.uniform(-2, 2, size=(500, 3))
+    p = rng.uniform(-2, 2, size=(3000, 3))   # >= 1024 targets: the mixed kernel
     w = orc.filament_sum(a, b, g, rc, p, threads=8)
     v = _ctx(1).filament_vind(a, b, g, rc, p)
     rms = np.sqrt(np.mean(np.sum(w**2, axis=1)))
     err = np.linalg.norm(v - w, axis=1)
     assert np.max(err) / rms < 2e-5
     assert np.sqrt(np.mean(err**2)) / rms < 2e-6
+    assert np.max(err) / rms > 1e-10   # FP32 pair arithmetic did run
 
 
 def test_resident_sets_and_profile():

@@ -27,13 +27,14 @@
 //           into a Kahan-compensated accumulator.  Pairs with the target
 //           within two core radii of the segment are recomputed in FP64
 //           (the near-singular core and target-on-filament terms).
+//           Launches with fewer than 1024 targets (collocation points, judge
+//           samples) run the FP64 small-target kernel in both modes.
 #include "vlm_internal.h"
 
 namespace vlm {
 namespace {
 
 constexpr int BS_BLOCK = 256;       // wake passes: many targets
-constexpr int BS_BLOCK_SMALL = 64;  // collocation points: tens to hundreds
 constexpr double EPS2 = kEps * kEps;
 constexpr float EPS2F = 4.93038065763132e-32f;
 constexpr float NEAR_CORE2 = 4.0f;  // h < 2 rc ...
@@ -259,6 +260,69 @@ __global__ __launch_bounds__(TILE) void bs_mixed_kernel(BsArgs a)
     }
 }
 
+// Few targets (collocation points, judge samples: m < 1024): each block
+// takes SM_T targets and one segment chunk; its 256 lanes stride over the
+// chunk (coalesced SoA loads, each record shared by the SM_T targets) and
+// the partial sums are reduced in a fixed order (butterfly within the wave,
+// waves in order).  Always FP64: these launches are a small fraction of the
+// pairs.
+constexpr int SM_T = 4;
+constexpr int SM_BLOCK = 256;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__global__ __launch_bounds__(SM_BLOCK) void bs_small_kernel(const double *s64, int64_t S,
+                                                            const long long *nseg_p, int nchunks,
+                                                            const double *t, int64_t m,
+                                                            int64_t tstride, double *part)
+{
+    __shared__ double red[3 * SM_T][SM_BLOCK / 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t nseg = *nseg_p;
+    const int64_t len = (nseg + nchunks - 1) / nchunks;
+    const int64_t s0 = (int64_t)blockIdx.x * len;
+    const int64_t s1 = min(nseg, s0 + len);
+    const int64_t g0 = (int64_t)blockIdx.y * SM_T;
+    double px[SM_T], py[SM_T], pz[SM_T], vx[SM_T], vy[SM_T], vz[SM_T];
+#pragma unroll
+    for (int q = 0; q < SM_T; ++q) {
+        const int64_t tq = g0 + q < m ? g0 + q : m - 1;
+        px[q] = t[tq];
+        py[q] = t[tstride + tq];
+        pz[q] = t[2 * tstride + tq];
+        vx[q] = vy[q] = vz[q] = 0.0;
+    }
+    for (int64_t sg = s0 + tid; sg < s1; sg += SM_BLOCK) {
+        const double4 sa = make_double4(s64[sg], s64[S + sg], s64[2 * S + sg], s64[3 * S + sg]);
+        const double4 sb =
+            make_double4(s64[4 * S + sg], s64[5 * S + sg], s64[6 * S + sg], s64[7 * S + sg]);
+#pragma unroll
+        for (int q = 0; q < SM_T; ++q) pair64(px[q], py[q], pz[q], sa, sb, vx[q], vy[q], vz[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < SM_T; ++q) {
+        const double a = wave_sum(vx[q]), b = wave_sum(vy[q]), c = wave_sum(vz[q]);
+        if (lane == 0) {
+            red[3 * q][w] = a;
+            red[3 * q + 1][w] = b;
+            red[3 * q + 2][w] = c;
+        }
+    }
+    __syncthreads();
+    if (tid < 3 * SM_T) {
+        const int q = tid / 3, d = tid % 3;
+        if (g0 + q < m) {
+            double v = 0.0;
+            for (int k = 0; k < SM_BLOCK / 64; ++k) v += red[tid][k];
+            part[((size_t)blockIdx.x * 3 + d) * m + g0 + q] = v;
+        }
+    }
+}
+
 __global__ void bs_reduce_kernel(const double *part, int nchunks, int64_t m, double *out,
                                  int64_t ostride, int acc)
 {
@@ -293,29 +357,41 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
             int64_t tstride, double *out, int accumulate)
 {
     if (m <= 0) return VLM_OK;
-    const bool small = m < 1024;
-    const int blk = small ? BS_BLOCK_SMALL : BS_BLOCK;
+    if (m < 1024) {
+        // about 2048 blocks, at least 8 segments per lane
+        const int64_t groups = (m + SM_T - 1) / SM_T;
+        int64_t nc = (2048 + groups - 1) / groups;
+        const int64_t maxc = nseg_max / (8 * SM_BLOCK);
+        if (nc > maxc) nc = maxc;
+        if (nc < 1) nc = 1;
+        const int nchunks = nseg_max > 0 ? (int)nc : 0;
+        if (nchunks > 0) {
+            VLM_HIP(ctx->part.reserve((size_t)nchunks * 3 * (size_t)m));
+            hipLaunchKernelGGL(bs_small_kernel, dim3(nchunks, (unsigned)groups), dim3(SM_BLOCK),
+                               0, ctx->stream, seg64, segstride, nseg_dev, nchunks, t, m, tstride,
+                               ctx->part.p);
+            VLM_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
+                           ctx->stream, ctx->part.p, nchunks, m, out, tstride, accumulate);
+        VLM_HIP(hipGetLastError());
+        return VLM_OK;
+    }
+    const int blk = BS_BLOCK;
     const int ntiles = (int)((m + blk - 1) / blk);
     const bool mixed = ctx->precision == VLM_MIXED;
-    int cap = 0;
-    if (mixed)
-        cap = small ? resident_blocks(ctx, bs_mixed_kernel<BS_BLOCK_SMALL>, blk, &ctx->occ[0])
-                    : resident_blocks(ctx, bs_mixed_kernel<BS_BLOCK>, blk, &ctx->occ[1]);
-    else
-        cap = small ? resident_blocks(ctx, bs_fp64_kernel<BS_BLOCK_SMALL>, blk, &ctx->occ[2])
+    int cap = mixed ? resident_blocks(ctx, bs_mixed_kernel<BS_BLOCK>, blk, &ctx->occ[1])
                     : resident_blocks(ctx, bs_fp64_kernel<BS_BLOCK>, blk, &ctx->occ[3]);
     if (cap <= 0) return fail(ctx, VLM_ERR_HIP, "occupancy query failed");
     int nchunks = 0;
     int64_t chunk_len = 1;
     if (nseg_max > 0) {
         // about 16 work items per resident workgroup, chunks of at least
-        // four tiles; small target sets use at most 128 chunks so the
-        // fixed-order reduction stays short
-        const int64_t min_len = small ? 512 : 4 * BS_BLOCK;
+        // four tiles
+        const int64_t min_len = 4 * BS_BLOCK;
         int64_t nc = ((int64_t)16 * cap + ntiles - 1) / ntiles;
         const int64_t maxc = (nseg_max + min_len - 1) / min_len;
         if (nc > maxc) nc = maxc;
-        if (small && nc > 128) nc = 128;
         if (nc < 1) nc = 1;
         chunk_len = (nseg_max + nc - 1) / nc;
         nchunks = (int)((nseg_max + chunk_len - 1) / chunk_len);
@@ -339,27 +415,18 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
         const int grid = a.nitems < cap ? a.nitems : cap;
         hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->counter.p);
         Timer tm{};
-        const bool timed = ctx->profile && !small;
+        const bool timed = ctx->profile;
         if (timed) {
             VLM_HIP(hipEventCreate(&tm.a));
             VLM_HIP(hipEventCreate(&tm.b));
             VLM_HIP(hipEventRecord(tm.a, ctx->stream));
         }
-        if (mixed) {
-            if (small)
-                hipLaunchKernelGGL(bs_mixed_kernel<BS_BLOCK_SMALL>, dim3(grid), dim3(blk), 0,
-                                   ctx->stream, a);
-            else
-                hipLaunchKernelGGL(bs_mixed_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0,
-                                   ctx->stream, a);
-        } else {
-            if (small)
-                hipLaunchKernelGGL(bs_fp64_kernel<BS_BLOCK_SMALL>, dim3(grid), dim3(blk), 0,
-                                   ctx->stream, a);
-            else
-                hipLaunchKernelGGL(bs_fp64_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0,
-                                   ctx->stream, a);
-        }
+        if (mixed)
+            hipLaunchKernelGGL(bs_mixed_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0, ctx->stream,
+                               a);
+        else
+            hipLaunchKernelGGL(bs_fp64_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0, ctx->stream,
+                               a);
         VLM_HIP(hipGetLastError());
         if (timed) {
             VLM_HIP(hipEventRecord(tm.b, ctx->stream));

@@ -8,11 +8,12 @@
 // Parameter packing (row-major W[out][in]):
 //   W1 64x45 | b1 64 | W2 64x64 | b2 64 | W3 64x64 | b3 64 | W4 3x64 | b4 3
 //
-// Inference: one sample per lane, all weights staged in LDS (45.8 KB) and
-// read as wave-uniform broadcasts.  Training: kernel A runs the forward and
-// backward sweep per sample and stores activations and deltas; kernel B
-// forms each parameter's gradient as a fixed-order sum over the batch
-// (deterministic, no atomics) and applies the Adam update.
+// Every layer is a small matrix product on the matrix cores (FP32 MFMA):
+// a workgroup takes a tile of 64 samples, each of its four waves 16 of
+// them, with the parameters and activations in LDS.  Training runs the
+// forward pass, the loss, the backward deltas and the tile's weight
+// gradients in one kernel per tile; a second kernel adds the tile
+// gradients in a fixed order (deterministic, no atomics) and applies Adam.
 #include <cmath>
 #include <vector>
 
@@ -30,131 +31,255 @@ constexpr int OFF_B3 = OFF_W3 + MLP_H * MLP_H;
 constexpr int OFF_W4 = OFF_B3 + MLP_H;
 constexpr int OFF_B4 = OFF_W4 + MLP_OUT * MLP_H;
 constexpr int MLP_NP = OFF_B4 + MLP_OUT;  // 11459
-constexpr int ACT = MLP_IN + 3 * MLP_H;    // stored activations per sample
-constexpr int DEL = 3 * MLP_H + MLP_OUT;   // stored deltas per sample
 
 namespace {
 
-__device__ __forceinline__ void dense_tanh(const float *W, const float *b, const float *x, int nin,
-                                           float *y)
+// Matrix-core layout of v_mfma_f32_16x16x4f32 (checked on gfx950): lane l
+// supplies A[l%16][l/16] and B[l/16][l%16] of the 16x16x4 step; result
+// register r of lane l is D[4*(l/16) + r][l%16].
+typedef float fx4 __attribute__((ext_vector_type(4)));
+
+constexpr int TS = 64;            // samples per tile (one workgroup, four waves)
+constexpr int K1 = 48;            // MLP_IN padded to a multiple of 4
+constexpr int LW1 = K1 + 1;       // LDS row strides
+constexpr int LH = MLP_H + 1;
+constexpr int LDY = 16 + 1;
+// LDS image of the parameters: W1 64 x LW1 | W2 64 x LH | W3 64 x LH |
+// W4 16 x LH (rows 3..15 zero) | b1 b2 b3 (64 each) | b4 (16, padded)
+constexpr int S_W1 = 0;
+constexpr int S_W2 = S_W1 + MLP_H * LW1;
+constexpr int S_W3 = S_W2 + MLP_H * LH;
+constexpr int S_W4 = S_W3 + MLP_H * LH;
+constexpr int S_B1 = S_W4 + 16 * LH;
+constexpr int S_B2 = S_B1 + MLP_H;
+constexpr int S_B3 = S_B2 + MLP_H;
+constexpr int S_B4 = S_B3 + MLP_H;
+constexpr int S_NP = S_B4 + 16;
+
+__device__ void stage_params(const float *params, float *sp)
 {
-    for (int o = 0; o < MLP_H; ++o) {
-        float a = b[o];
-        for (int i = 0; i < nin; ++i) a = __builtin_fmaf(W[o * nin + i], x[i], a);
-        y[o] = tanhf(a);
+    for (int k = threadIdx.x; k < S_NP; k += blockDim.x) {
+        float v = 0.f;
+        if (k < S_W2) {
+            const int o = k / LW1, i = k % LW1;
+            if (i < MLP_IN) v = params[OFF_W1 + o * MLP_IN + i];
+        } else if (k < S_W3) {
+            const int o = (k - S_W2) / LH, i = (k - S_W2) % LH;
+            if (i < MLP_H) v = params[OFF_W2 + o * MLP_H + i];
+        } else if (k < S_W4) {
+            const int o = (k - S_W3) / LH, i = (k - S_W3) % LH;
+            if (i < MLP_H) v = params[OFF_W3 + o * MLP_H + i];
+        } else if (k < S_B1) {
+            const int o = (k - S_W4) / LH, i = (k - S_W4) % LH;
+            if (o < MLP_OUT && i < MLP_H) v = params[OFF_W4 + o * MLP_H + i];
+        } else if (k < S_B2) {
+            v = params[OFF_B1 + (k - S_B1)];
+        } else if (k < S_B3) {
+            v = params[OFF_B2 + (k - S_B2)];
+        } else if (k < S_B4) {
+            v = params[OFF_B3 + (k - S_B3)];
+        } else if (k - S_B4 < MLP_OUT) {
+            v = params[OFF_B4 + (k - S_B4)];
+        }
+        sp[k] = v;
+    }
+}
+
+// X tile (TS x MLP_IN, row-major in global) into LDS, zero padded.
+__device__ void stage_x(const float *X, int64_t s0, int64_t m, float *Xs)
+{
+    for (int e = threadIdx.x; e < TS * LW1; e += blockDim.x) {
+        const int s = e / LW1, k = e % LW1;
+        Xs[e] = (k < MLP_IN && s0 + s < m) ? X[(s0 + s) * MLP_IN + k] : 0.f;
+    }
+}
+
+// Rows [16w, 16w+16) of O = act(A . W^T + b): A is TS x K (stride LA), W is
+// NOB*16 x K (stride LWt), O stride LO.
+template <int K, int LA, int LWt, int NOB, bool TANH>
+__device__ __forceinline__ void dense_fwd(const float *A, const float *W, const float *b, float *O,
+                                          int LO, int w, int l)
+{
+    const int li = l & 15, lk = l >> 4;
+#pragma unroll
+    for (int ob = 0; ob < NOB; ++ob) {
+        fx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k0 = 0; k0 < K; k0 += 4)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[(16 * w + li) * LA + k0 + lk],
+                                                       W[(16 * ob + li) * LWt + k0 + lk], acc, 0,
+                                                       0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int srow = 16 * w + 4 * lk + r, j = 16 * ob + li;
+            const float v = acc[r] + b[j];
+            O[srow * LO + j] = TANH ? tanhf(v) : v;
+        }
+    }
+}
+
+// Rows [16w, 16w+16) of the backward delta: H <- (D . W) * (1 - H^2), where
+// D is TS x K (stride LD) and W is K x 64 (stride LWt); H holds tanh
+// outputs on entry and the deltas on exit.
+template <int K, int LD, int LWt>
+__device__ __forceinline__ void dense_bwd(const float *D, const float *W, float *H, int w, int l)
+{
+    const int li = l & 15, lk = l >> 4;
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib) {
+        fx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k0 = 0; k0 < K; k0 += 4)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(D[(16 * w + li) * LD + k0 + lk],
+                                                       W[(k0 + lk) * LWt + 16 * ib + li], acc, 0,
+                                                       0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int srow = 16 * w + 4 * lk + r, j = 16 * ib + li;
+            const float h = H[srow * LH + j];
+            H[srow * LH + j] = acc[r] * (1.f - h * h);
+        }
+    }
+}
+
+// Tile gradient of one layer: dW[o][i] = sum_s D[s][o] Act[s][i] for the
+// 16 outputs of block ob and NIB input blocks, written to the tile's
+// partial-gradient row; bias gradients by lanes of wave 0.
+template <int NIB, int LD, int LA>
+__device__ __forceinline__ void dense_grad(const float *D, const float *Act, int ob, int nout,
+                                           int nin, float *gw, int l)
+{
+    const int li = l & 15, lk = l >> 4;
+#pragma unroll
+    for (int ib = 0; ib < NIB; ++ib) {
+        fx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k0 = 0; k0 < TS; k0 += 4)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(D[(k0 + lk) * LD + 16 * ob + li],
+                                                       Act[(k0 + lk) * LA + 16 * ib + li], acc, 0,
+                                                       0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int o = 16 * ob + 4 * lk + r, i = 16 * ib + li;
+            if (o < nout && i < nin) gw[o * nin + i] = acc[r];
+        }
+    }
+}
+
+__device__ __forceinline__ void bias_grad(const float *D, int LD, int nout, float *gb)
+{
+    const int o = threadIdx.x;
+    if (o < nout) {
+        float g = 0.f;
+        for (int s = 0; s < TS; ++s) g += D[s * LD + o];
+        gb[o] = g;
     }
 }
 
 __global__ __launch_bounds__(256) void mlp_forward_kernel(const float *params, int64_t m,
                                                           const float *X, float *Y)
 {
-    extern __shared__ float sw[];
-    for (int k = threadIdx.x; k < MLP_NP; k += blockDim.x) sw[k] = params[k];
-    __syncthreads();
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m) return;
-    float x[MLP_IN], h1[MLP_H], h2[MLP_H];
-    for (int i = 0; i < MLP_IN; ++i) x[i] = X[s * MLP_IN + i];
-    dense_tanh(sw + OFF_W1, sw + OFF_B1, x, MLP_IN, h1);
-    dense_tanh(sw + OFF_W2, sw + OFF_B2, h1, MLP_H, h2);
-    dense_tanh(sw + OFF_W3, sw + OFF_B3, h2, MLP_H, h1);
-    for (int o = 0; o < MLP_OUT; ++o) {
-        float a = sw[OFF_B4 + o];
-        for (int i = 0; i < MLP_H; ++i) a = __builtin_fmaf(sw[OFF_W4 + o * MLP_H + i], h1[i], a);
-        Y[s * MLP_OUT + o] = a;
+    __shared__ float sp[S_NP];
+    __shared__ float Xs[TS * LW1];
+    __shared__ float Ha[TS * LH], Hb[TS * LH];
+    __shared__ float Ys[TS * LDY];
+    stage_params(params, sp);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int64_t ntiles = (m + TS - 1) / TS;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int64_t s0 = t * TS;
+        __syncthreads();
+        stage_x(X, s0, m, Xs);
+        __syncthreads();
+        dense_fwd<K1, LW1, LW1, 4, true>(Xs, sp + S_W1, sp + S_B1, Ha, LH, w, l);
+        __syncthreads();
+        dense_fwd<MLP_H, LH, LH, 4, true>(Ha, sp + S_W2, sp + S_B2, Hb, LH, w, l);
+        __syncthreads();
+        dense_fwd<MLP_H, LH, LH, 4, true>(Hb, sp + S_W3, sp + S_B3, Ha, LH, w, l);
+        __syncthreads();
+        dense_fwd<MLP_H, LH, LH, 1, false>(Ha, sp + S_W4, sp + S_B4, Ys, LDY, w, l);
+        __syncthreads();
+        for (int e = threadIdx.x; e < TS * MLP_OUT; e += blockDim.x) {
+            const int sl = e / MLP_OUT, o = e % MLP_OUT;
+            if (s0 + sl < m) Y[(s0 + sl) * MLP_OUT + o] = Ys[sl * LDY + o];
+        }
     }
 }
 
-// Forward + backward of one sample per lane; stores activations
-// [x | h1 | h2 | h3] and deltas [d1 | d2 | d3 | dy], and the sample loss.
-__global__ __launch_bounds__(256) void mlp_backprop_kernel(const float *params, int64_t m,
-                                                           const float *X, const float *T,
-                                                           float *acts, float *dels,
-                                                           float *loss)
+// One tile of a training batch: forward, loss 0.5 |y - t|^2 / m, backward
+// deltas and the tile's partial gradient (row `tile` of part).
+__global__ __launch_bounds__(256) void mlp_train_tile_kernel(const float *params, int64_t m,
+                                                             const float *X, const float *T,
+                                                             float *part, float *loss)
 {
-    extern __shared__ float sw[];
-    for (int k = threadIdx.x; k < MLP_NP; k += blockDim.x) sw[k] = params[k];
+    __shared__ float sp[S_NP];
+    __shared__ float Xs[TS * LW1];
+    __shared__ float H1[TS * LH], H2[TS * LH], H3[TS * LH];
+    __shared__ float DY[TS * LDY];
+    __shared__ float red[256];
+    stage_params(params, sp);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int64_t tile = blockIdx.x, s0 = tile * TS;
+    float *g = part + (size_t)tile * MLP_NP;
+    stage_x(X, s0, m, Xs);
     __syncthreads();
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m) return;
-    float *a = acts + s * ACT;
-    float *d = dels + s * DEL;
-    float *x = a, *h1 = a + MLP_IN, *h2 = h1 + MLP_H, *h3 = h2 + MLP_H;
-    for (int i = 0; i < MLP_IN; ++i) x[i] = X[s * MLP_IN + i];
-    float r1[MLP_H], r2[MLP_H], r3[MLP_H];
-    dense_tanh(sw + OFF_W1, sw + OFF_B1, x, MLP_IN, r1);
-    dense_tanh(sw + OFF_W2, sw + OFF_B2, r1, MLP_H, r2);
-    dense_tanh(sw + OFF_W3, sw + OFF_B3, r2, MLP_H, r3);
-    float dy[MLP_OUT], l = 0.f;
+    dense_fwd<K1, LW1, LW1, 4, true>(Xs, sp + S_W1, sp + S_B1, H1, LH, w, l);
+    __syncthreads();
+    dense_fwd<MLP_H, LH, LH, 4, true>(H1, sp + S_W2, sp + S_B2, H2, LH, w, l);
+    __syncthreads();
+    dense_fwd<MLP_H, LH, LH, 4, true>(H2, sp + S_W3, sp + S_B3, H3, LH, w, l);
+    __syncthreads();
+    dense_fwd<MLP_H, LH, LH, 1, false>(H3, sp + S_W4, sp + S_B4, DY, LDY, w, l);
+    __syncthreads();
+    // dy = (y - t) / m on real samples and outputs, zero elsewhere
     const float inv_m = 1.0f / (float)m;
-    for (int o = 0; o < MLP_OUT; ++o) {
-        float y = sw[OFF_B4 + o];
-        for (int i = 0; i < MLP_H; ++i) y = __builtin_fmaf(sw[OFF_W4 + o * MLP_H + i], r3[i], y);
-        const float e = y - T[s * MLP_OUT + o];
-        l += 0.5f * e * e * inv_m;
-        dy[o] = e * inv_m;
+    float lsum = 0.f;
+    for (int e = threadIdx.x; e < TS * 16; e += blockDim.x) {
+        const int sl = e / 16, o = e % 16;
+        float d = 0.f;
+        if (o < MLP_OUT && s0 + sl < m) {
+            const float err = DY[sl * LDY + o] - T[(s0 + sl) * MLP_OUT + o];
+            lsum += 0.5f * err * err * inv_m;
+            d = err * inv_m;
+        }
+        DY[sl * LDY + o] = d;
     }
-    float d3[MLP_H], d2[MLP_H];
-    for (int i = 0; i < MLP_H; ++i) {
-        float acc = 0.f;
-        for (int o = 0; o < MLP_OUT; ++o) acc = __builtin_fmaf(sw[OFF_W4 + o * MLP_H + i], dy[o], acc);
-        d3[i] = acc * (1.f - r3[i] * r3[i]);
+    red[threadIdx.x] = lsum;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
     }
-    for (int i = 0; i < MLP_H; ++i) {
-        float acc = 0.f;
-        for (int o = 0; o < MLP_H; ++o) acc = __builtin_fmaf(sw[OFF_W3 + o * MLP_H + i], d3[o], acc);
-        d2[i] = acc * (1.f - r2[i] * r2[i]);
-    }
-    for (int i = 0; i < MLP_H; ++i) {
-        float acc = 0.f;
-        for (int o = 0; o < MLP_H; ++o) acc = __builtin_fmaf(sw[OFF_W2 + o * MLP_H + i], d2[o], acc);
-        d[i] = acc * (1.f - r1[i] * r1[i]);  // d1
-    }
-    for (int i = 0; i < MLP_H; ++i) {
-        h1[i] = r1[i];
-        h2[i] = r2[i];
-        h3[i] = r3[i];
-        d[MLP_H + i] = d2[i];
-        d[2 * MLP_H + i] = d3[i];
-    }
-    for (int o = 0; o < MLP_OUT; ++o) d[3 * MLP_H + o] = dy[o];
-    loss[s] = l;
+    if (threadIdx.x == 0) loss[tile] = red[0];
+    // layer 4
+    if (w == 0) dense_grad<4, LDY, LH>(DY, H3, 0, MLP_OUT, MLP_H, g + OFF_W4, l);
+    bias_grad(DY, LDY, MLP_OUT, g + OFF_B4);
+    __syncthreads();
+    dense_bwd<16, LDY, LH>(DY, sp + S_W4, H3, w, l);       // H3 <- delta3
+    __syncthreads();
+    dense_grad<4, LH, LH>(H3, H2, w, MLP_H, MLP_H, g + OFF_W3, l);
+    bias_grad(H3, LH, MLP_H, g + OFF_B3);
+    __syncthreads();
+    dense_bwd<MLP_H, LH, LH>(H3, sp + S_W3, H2, w, l);     // H2 <- delta2
+    __syncthreads();
+    dense_grad<4, LH, LH>(H2, H1, w, MLP_H, MLP_H, g + OFF_W2, l);
+    bias_grad(H2, LH, MLP_H, g + OFF_B2);
+    __syncthreads();
+    dense_bwd<MLP_H, LH, LH>(H2, sp + S_W2, H1, w, l);     // H1 <- delta1
+    __syncthreads();
+    dense_grad<3, LH, LW1>(H1, Xs, w, MLP_H, MLP_IN, g + OFF_W1, l);
+    bias_grad(H1, LH, MLP_H, g + OFF_B1);
 }
 
-// One lane per parameter: gradient as a fixed-order sum over the batch,
-// then (optionally) an Adam step.
-__global__ void mlp_grad_kernel(float *params, int64_t m, const float *acts, const float *dels,
-                                float *grad, float *mom, float *var, int adam, float lr,
-                                float b1, float b2, float bc1, float bc2, float epsa)
+// Sum of the tile gradients in tile order, then (optionally) Adam.
+__global__ void mlp_grad_kernel(float *params, const float *part, int ntiles, float *grad,
+                                float *mom, float *var, int adam, float lr, float b1, float b2,
+                                float bc1, float bc2, float epsa)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= MLP_NP) return;
-    // locate (layer, out, in)
-    int dsel, asel, nin, isbias;
-    int o, i;
-    if (k < OFF_B1) {
-        o = (k - OFF_W1) / MLP_IN; i = (k - OFF_W1) % MLP_IN; dsel = 0; asel = 0; nin = MLP_IN; isbias = 0;
-    } else if (k < OFF_W2) {
-        o = k - OFF_B1; i = 0; dsel = 0; asel = 0; nin = 0; isbias = 1;
-    } else if (k < OFF_B2) {
-        o = (k - OFF_W2) / MLP_H; i = (k - OFF_W2) % MLP_H; dsel = MLP_H; asel = MLP_IN; nin = MLP_H; isbias = 0;
-    } else if (k < OFF_W3) {
-        o = k - OFF_B2; i = 0; dsel = MLP_H; asel = 0; nin = 0; isbias = 1;
-    } else if (k < OFF_B3) {
-        o = (k - OFF_W3) / MLP_H; i = (k - OFF_W3) % MLP_H; dsel = 2 * MLP_H; asel = MLP_IN + MLP_H; nin = MLP_H; isbias = 0;
-    } else if (k < OFF_W4) {
-        o = k - OFF_B3; i = 0; dsel = 2 * MLP_H; asel = 0; nin = 0; isbias = 1;
-    } else if (k < OFF_B4) {
-        o = (k - OFF_W4) / MLP_H; i = (k - OFF_W4) % MLP_H; dsel = 3 * MLP_H; asel = MLP_IN + 2 * MLP_H; nin = MLP_H; isbias = 0;
-    } else {
-        o = k - OFF_B4; i = 0; dsel = 3 * MLP_H; asel = 0; nin = 0; isbias = 1;
-    }
-    (void)nin;
     float g = 0.f;
-    for (int64_t s = 0; s < m; ++s) {
-        const float dv = dels[s * DEL + dsel + o];
-        g = isbias ? g + dv : __builtin_fmaf(dv, acts[s * ACT + asel + i], g);
-    }
+    for (int c = 0; c < ntiles; ++c) g += part[(size_t)c * MLP_NP + k];
     grad[k] = g;
     if (adam) {
         const float mk = b1 * mom[k] + (1.f - b1) * g;
@@ -182,8 +307,8 @@ __global__ void sum_kernel(const float *x, int64_t n, float *out)
 }  // namespace
 
 struct Mlp {
-    DBuf<float> params, mom, var, grad;
-    DBuf<float> acts, dels, loss, lsum;
+    DBuf<float> params, mom, var, grad, part;
+    DBuf<float> loss, lsum;
     DBuf<float> xin, yout;
     int64_t steps = 0;
     bool ready = false;
@@ -196,8 +321,10 @@ int mlp_forward_dev(vlm_ctx *ctx, int64_t m, const float *X, float *Y)
     Mlp *M = get_mlp(ctx);
     if (!M) return fail(ctx, VLM_ERR_ALLOC, "mlp allocation failed");
     if (m <= 0) return VLM_OK;
-    hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256),
-                       MLP_NP * sizeof(float), ctx->stream, M->params.p, m, X, Y);
+    const int64_t ntiles = (m + TS - 1) / TS;
+    const unsigned grid = (unsigned)(ntiles < 1024 ? ntiles : 1024);
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3(grid), dim3(256), 0, ctx->stream, M->params.p,
+                       m, X, Y);
     VLM_HIP(hipGetLastError());
     return VLM_OK;
 }
@@ -209,12 +336,11 @@ int mlp_train_dev(vlm_ctx *ctx, int64_t m, const float *X, const float *T, int a
     Mlp *M = get_mlp(ctx);
     if (!M) return fail(ctx, VLM_ERR_ALLOC, "mlp allocation failed");
     if (m <= 0) return VLM_OK;
-    VLM_HIP(M->acts.reserve((size_t)m * ACT));
-    VLM_HIP(M->dels.reserve((size_t)m * DEL));
-    VLM_HIP(M->loss.reserve((size_t)m));
-    hipLaunchKernelGGL(mlp_backprop_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256),
-                       MLP_NP * sizeof(float), ctx->stream, M->params.p, m, X, T, M->acts.p,
-                       M->dels.p, M->loss.p);
+    const int64_t ntiles = (m + TS - 1) / TS;
+    VLM_HIP(M->part.reserve((size_t)ntiles * MLP_NP));
+    VLM_HIP(M->loss.reserve((size_t)ntiles));
+    hipLaunchKernelGGL(mlp_train_tile_kernel, dim3((unsigned)ntiles), dim3(256), 0, ctx->stream,
+                       M->params.p, m, X, T, M->part.p, M->loss.p);
     VLM_HIP(hipGetLastError());
     float bc1 = 1.f, bc2 = 1.f;
     const float b1 = 0.9f, b2 = 0.999f;
@@ -224,11 +350,11 @@ int mlp_train_dev(vlm_ctx *ctx, int64_t m, const float *X, const float *T, int a
         bc2 = 1.f - std::pow(b2, (float)M->steps);
     }
     hipLaunchKernelGGL(mlp_grad_kernel, dim3((MLP_NP + 255) / 256), dim3(256), 0, ctx->stream,
-                       M->params.p, m, M->acts.p, M->dels.p, M->grad.p, M->mom.p, M->var.p, adam,
+                       M->params.p, M->part.p, (int)ntiles, M->grad.p, M->mom.p, M->var.p, adam,
                        lr, b1, b2, bc1, bc2, 1e-8f);
     VLM_HIP(hipGetLastError());
     if (loss_host) {
-        hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, ctx->stream, M->loss.p, m,
+        hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, ctx->stream, M->loss.p, ntiles,
                            M->lsum.p);
         VLM_HIP(hipMemcpyAsync(loss_host, M->lsum.p, sizeof(float), hipMemcpyDeviceToHost,
                                ctx->stream));

@@ -94,8 +94,12 @@ struct Predictor {
     int train_iters = 4;   // Adam iterations per step on the buffer
     float lr = 1e-3f;
     double lref = 1.0;     // length scale of the neighbour offsets
-    int64_t steps = 0, full = 0, learned = 0, fallbacks = 0;
+    int64_t batch = 2048;  // minibatch of each Adam iteration
+    int64_t steps = 0, full = 0, learned = 0, fallbacks = 0, judged = 0, train_calls = 0;
     double last_err = 0.0, last_loss = 0.0;
+    // accumulated over judged steps: squared error of the network, of the
+    // network-free substitute, and squared exact speed
+    double sum_err2 = 0.0, sum_base2 = 0.0, sum_ref2 = 0.0;
     int64_t buf_cap = 16384, buf_n = 0, buf_head = 0;
     DBuf<float> X, Y, bx, bt;
     DBuf<double> vn, ts, vs, e2, scal;

@@ -1058,26 +1058,33 @@ __global__ void gather_sample_kernel(const double *t, int64_t m, int64_t phase, 
 
 // Judge: per-sample squared error and squared exact speed; then write the
 // exact values into the output and the training pair into the buffer.
+// The same sample also scores the network-free substitute for pass 2, the
+// linear extrapolation 2 v_n - v_(n-1) of the pass-1 velocity (with it the
+// AM2 corrector reduces to AB2), so that the network's gain is measured on
+// identical targets.
 __global__ void judge_kernel(const double *vs, int64_t ms, const double *out, int64_t m,
                              int64_t phase, int64_t stride, const double *vn, const float *X,
-                             const double *vref, double *err2, double *ref2, float *bx, float *bt,
-                             int64_t bcap, int64_t bhead)
+                             const double *vref, double *err2, double *ref2, double *base2,
+                             float *bx, float *bt, int64_t bcap, int64_t bhead)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ms) return;
     const int64_t e = phase + k * stride;
-    double se = 0.0, sr = 0.0;
+    double se = 0.0, sr = 0.0, sb = 0.0;
     const double inv = 1.0 / vref[0];
     const int64_t slot = (bhead + k) % bcap;
     for (int d = 0; d < 3; ++d) {
         const double a = vs[d * ms + k], p = out[d * m + e];
+        const double ex = 2.0 * vn[d * m + e] - (double)X[45 * e + 39 + d] * vref[0];
         se += (a - p) * (a - p);
         sr += a * a;
+        sb += (a - ex) * (a - ex);
         bt[3 * slot + d] = (float)((a - vn[d * m + e]) * inv);
     }
     for (int q = 0; q < 45; ++q) bx[45 * slot + q] = X[45 * e + q];
     err2[k] = se;
     ref2[k] = sr;
+    base2[k] = sb;
 }
 
 __global__ void put_sample_kernel(const double *vs, int64_t ms, double *out, int64_t m,
@@ -1089,28 +1096,22 @@ __global__ void put_sample_kernel(const double *vs, int64_t ms, double *out, int
     for (int d = 0; d < 3; ++d) out[d * m + e] = vs[d * ms + k];
 }
 
-__global__ void sum2_kernel(const double *a, const double *b, int64_t n, double *out)
+// Sums of three length-n arrays a, a+n, a+2n (one block, fixed order).
+__global__ void sum3_kernel(const double *a, int64_t n, double *out)
 {
-    __shared__ double sa[256], sb[256];
-    double x = 0.0, y = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
-        x += a[k];
-        y += b[k];
-    }
-    sa[threadIdx.x] = x;
-    sb[threadIdx.x] = y;
+    __shared__ double sh[3][256];
+    double x[3] = {0.0, 0.0, 0.0};
+    for (int64_t k = threadIdx.x; k < n; k += blockDim.x)
+        for (int c = 0; c < 3; ++c) x[c] += a[c * n + k];
+    for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] = x[c];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            sa[threadIdx.x] += sa[threadIdx.x + w];
-            sb[threadIdx.x] += sb[threadIdx.x + w];
-        }
+        if (threadIdx.x < w)
+            for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        out[0] = sa[0];
-        out[1] = sb[0];
-    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) out[c] = sh[c][0];
 }
 
 // Training pairs from a full pass: every stride-th target.
@@ -1185,23 +1186,28 @@ int learned_pass(vlm_ctx *ctx)
         const int64_t ms = (m - phase + stride - 1) / stride;
         VLM_HIP(P.ts.reserve((size_t)3 * ms));
         VLM_HIP(P.vs.reserve((size_t)3 * ms));
-        VLM_HIP(P.e2.reserve((size_t)2 * ms));
+        VLM_HIP(P.e2.reserve((size_t)3 * ms));
         hipLaunchKernelGGL(gather_sample_kernel, dim3(nblk(ms)), dim3(256), 0, ctx->stream,
                            ctx->tgt.p, m, phase, stride, ms, P.ts.p);
         VLM_TRY(eval_sources(ctx, P.ts.p, ms, ms, P.vs.p, 0, nmax));
         const int64_t take = ms < P.buf_cap ? ms : P.buf_cap;
         hipLaunchKernelGGL(judge_kernel, dim3(nblk(take)), dim3(256), 0, ctx->stream, P.vs.p,
                            take, ctx->out.p, m, phase, stride, P.vn.p, P.X.p, P.scal.p, P.e2.p,
-                           P.e2.p + ms, P.bx.p, P.bt.p, P.buf_cap, P.buf_head);
-        hipLaunchKernelGGL(sum2_kernel, dim3(1), dim3(256), 0, ctx->stream, P.e2.p, P.e2.p + ms,
-                           take, P.scal.p + 1);
-        double h[2] = {0, 0};
-        VLM_HIP(hipMemcpyAsync(h, P.scal.p + 1, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                           P.e2.p + take, P.e2.p + 2 * take, P.bx.p, P.bt.p, P.buf_cap,
+                           P.buf_head);
+        hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(256), 0, ctx->stream, P.e2.p, take,
+                           P.scal.p + 1);
+        double h[3] = {0, 0, 0};
+        VLM_HIP(hipMemcpyAsync(h, P.scal.p + 1, 3 * sizeof(double), hipMemcpyDeviceToHost,
                                ctx->stream));
         VLM_HIP(hipStreamSynchronize(ctx->stream));
         P.buf_head = (P.buf_head + take) % P.buf_cap;
         P.buf_n = P.buf_n + take < P.buf_cap ? P.buf_n + take : P.buf_cap;
         P.last_err = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
+        P.sum_err2 += h[0];
+        P.sum_ref2 += h[1];
+        P.sum_base2 += h[2];
+        P.judged += 1;
         if (P.last_err > P.tol) {
             full = true;
             P.fallbacks += 1;
@@ -1233,10 +1239,16 @@ int learned_pass(vlm_ctx *ctx)
         off += n;
     }
     VLM_HIP(hipGetLastError());
+    // Adam on minibatches: contiguous windows of the replay buffer at a
+    // rotating offset
     float loss = 0.f;
-    for (int it = 0; it < P.train_iters && P.buf_n > 0; ++it)
-        VLM_TRY(mlp_train_dev(ctx, P.buf_n, P.bx.p, P.bt.p, 1, P.lr,
+    for (int it = 0; it < P.train_iters && P.buf_n > 0; ++it) {
+        const int64_t B = P.buf_n < P.batch ? P.buf_n : P.batch;
+        const int64_t start = (P.train_calls * 7919) % (P.buf_n - B + 1);
+        P.train_calls += 1;
+        VLM_TRY(mlp_train_dev(ctx, B, P.bx.p + 45 * start, P.bt.p + 3 * start, 1, P.lr,
                               it == P.train_iters - 1 ? &loss : nullptr));
+    }
     P.last_loss = loss;
     return VLM_OK;
 }
@@ -1710,6 +1722,18 @@ int vlm_rotor_get_velocity(vlm_ctx *ctx, int ir, int which, double *velN, double
 
 extern "C" {
 
+int vlm_predictor_errors(vlm_ctx *ctx, double *mlp_rel_rms, double *base_rel_rms,
+                         int64_t *judged_steps)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    const Predictor &P = ctx->pred;
+    const double r = P.sum_ref2 > 0 ? P.sum_ref2 : 1.0;
+    if (mlp_rel_rms) *mlp_rel_rms = std::sqrt(P.sum_err2 / r);
+    if (base_rel_rms) *base_rel_rms = std::sqrt(P.sum_base2 / r);
+    if (judged_steps) *judged_steps = P.judged;
+    return VLM_OK;
+}
+
 int vlm_set_predictor(vlm_ctx *ctx, int mode, double tol, int sample_stride, int warmup_steps,
                       int train_iters, double lref)
 {
@@ -1724,6 +1748,8 @@ int vlm_set_predictor(vlm_ctx *ctx, int mode, double tol, int sample_stride, int
     P.warmup = warmup_steps;
     P.train_iters = train_iters;
     P.lref = lref;
+    P.sum_err2 = P.sum_base2 = P.sum_ref2 = 0.0;
+    P.judged = 0;
     return VLM_OK;
 }
 

@@ -203,6 +203,22 @@ module vlm_c
       real(c_double), intent(out) :: last_err, last_loss
     end function
 
+    integer(c_int) function vlm_predictor_errors(ctx, mlp_rel_rms, base_rel_rms, judged) &
+        bind(C, name='vlm_predictor_errors')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: mlp_rel_rms, base_rel_rms
+      integer(c_int64_t), intent(out) :: judged
+    end function
+
+    integer(c_int) function vlm_profile_get_lu(ctx, ms, factorisations, flops) &
+        bind(C, name='vlm_profile_get_lu')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ms, flops
+      integer(c_int64_t), intent(out) :: factorisations
+    end function
+
     integer(c_int) function vlm_profile_enable(ctx, on) bind(C, name='vlm_profile_enable')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -113,6 +113,8 @@ module volcanor_host
     integer(c_int) :: nt, tail_steps
     integer(c_int64_t) :: pred_full, pred_learned, pred_fallbacks
     real(c_double) :: pred_last_err
+    real(c_double) :: pred_mlp_rms, pred_base_rms
+    integer(c_int64_t) :: pred_judged
   end type volcanor_stats
 
   public :: volcanor_run, run_case
@@ -1448,6 +1450,8 @@ contains
       call vlm_check(ctx, vlm_predictor_stats(ctx, stats%pred_full, stats%pred_learned, &
         & stats%pred_fallbacks, le, ll), 'predictor_stats')
       stats%pred_last_err = le
+      call vlm_check(ctx, vlm_predictor_errors(ctx, stats%pred_mlp_rms, stats%pred_base_rms, &
+        & stats%pred_judged), 'predictor_errors')
     end block
     if (write_out) then
       do ir = 1, nr

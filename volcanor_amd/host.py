@@ -22,7 +22,9 @@ class Stats(C.Structure):
                 ("bs_pairs", C.c_double), ("bs_launches", C.c_int64),
                 ("nt", C.c_int), ("tail_steps", C.c_int),
                 ("pred_full", C.c_int64), ("pred_learned", C.c_int64),
-                ("pred_fallbacks", C.c_int64), ("pred_last_err", C.c_double)]
+                ("pred_fallbacks", C.c_int64), ("pred_last_err", C.c_double),
+                ("pred_mlp_rms", C.c_double), ("pred_base_rms", C.c_double),
+                ("pred_judged", C.c_int64)]
 
 
 _host = None

@@ -83,6 +83,7 @@ _SIGS = {
     "vlm_set_predictor": [_P, _I, _D, _I, _I, _I, _D],
     "vlm_predictor_stats": [_P, C.POINTER(_L), C.POINTER(_L), C.POINTER(_L), C.POINTER(_D),
                             C.POINTER(_D)],
+    "vlm_predictor_errors": [_P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_L)],
     "vlm_profile_enable": [_P, _I],
     "vlm_profile_reset": [_P],
     "vlm_profile_get": [_P, C.POINTER(_D), C.POINTER(_L), C.POINTER(_D)],
@@ -366,6 +367,13 @@ class Context:
                     last_loss=l.value)
 
     # -- timing ----------------------------------------------------------
+    def predictor_errors(self):
+        """Relative RMS error of the learned pass 2 and of the network-free
+        substitute on the judge samples, and the number of judged steps."""
+        a, b, n = C.c_double(), C.c_double(), C.c_int64()
+        self._chk(self.lib.vlm_predictor_errors(self.h, C.byref(a), C.byref(b), C.byref(n)))
+        return {"mlp": a.value, "base": b.value, "judged": n.value}
+
     def profile_enable(self, on=True):
         self._chk(self.lib.vlm_profile_enable(self.h, int(on)))
 
