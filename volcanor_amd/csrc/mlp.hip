@@ -358,7 +358,7 @@ int mlp_train_dev(vlm_ctx *ctx, int64_t m, const float *X, const float *T, int a
                            M->lsum.p);
         VLM_HIP(hipMemcpyAsync(loss_host, M->lsum.p, sizeof(float), hipMemcpyDeviceToHost,
                                ctx->stream));
-        VLM_HIP(hipStreamSynchronize(ctx->stream));
+        VLM_TRY(sync_stream(ctx));
     }
     return VLM_OK;
 }
@@ -439,7 +439,7 @@ int vlm_mlp_get_params(vlm_ctx *ctx, float *params)
     if (!ctx || !params) return VLM_ERR_ARG;
     Mlp *M = get_mlp(ctx);
     if (!M) return fail(ctx, VLM_ERR_ALLOC, "mlp allocation failed");
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    VLM_TRY(sync_stream(ctx));
     VLM_HIP(hipMemcpy(params, M->params.p, MLP_NP * sizeof(float), hipMemcpyDeviceToHost));
     return VLM_OK;
 }
@@ -457,7 +457,7 @@ int vlm_mlp_forward(vlm_ctx *ctx, int64_t m, const float *x, float *y)
     VLM_TRY(mlp_forward_dev(ctx, m, M->xin.p, M->yout.p));
     VLM_HIP(hipMemcpyAsync(y, M->yout.p, sizeof(float) * m * MLP_OUT, hipMemcpyDeviceToHost,
                            ctx->stream));
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    VLM_TRY(sync_stream(ctx));
     return VLM_OK;
 }
 

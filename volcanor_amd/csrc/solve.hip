@@ -75,8 +75,9 @@ __global__ void aic_kernel(int n, const double *wc, const double *wrc, const dou
 
 // Forward/back substitution with the row pivots, one workgroup.
 __global__ __launch_bounds__(1024) void lu_solve_kernel(int n, const double *LU, const int *ipiv,
-                                                        double *x)
+                                                        double *x, const int *done)
 {
+    if (done && *done) return;
     const int tid = threadIdx.x;
     if (tid == 0)
         for (int k = 0; k < n; ++k)
@@ -111,42 +112,98 @@ __global__ void residual_kernel(int n, const double *A, const double *x, const d
     r[i] = s;
 }
 
-__global__ void axpy_kernel(int n, const double *d, double *x)
+__global__ void axpy_kernel(int n, const double *d, double *x, const int *done)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] += d[i];
+    if (i < n && !*done) x[i] += d[i];
 }
 
-int solve_refine(vlm_ctx *ctx, int n, const double *A, const double *LU, const int *ipiv,
-                 const double *rhs_h, double *x_h, int max_iter, double tol, int *iters)
+// rotor_map_gam (classdef.f90:4181-4196): blades ib <= nbConvect take
+// their part of gamVec; with axisymmetry (nbConvect = 1) every blade takes
+// blade 1's.
+__global__ void map_gam_kernel(const double *x, double *wg, int per_blade, int nb, int nbc)
 {
-    VLM_HIP(ctx->solve_tmp.reserve((size_t)3 * n));
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per_blade * nb) return;
+    const int b = e / per_blade, k = e - b * per_blade;
+    if (b < nbc) wg[e] = x[e];
+    else if (nbc == 1) wg[e] = x[k];
+}
+
+__global__ void solve_state_init_kernel(int *st)
+{
+    st[0] = 0;  // converged
+    st[1] = 0;  // corrections applied
+}
+
+// Convergence test of the refinement on the device: ||r||_inf against
+// tol ||b||_inf; counts the correction that follows when not converged.
+__global__ void refine_check_kernel(int n, const double *r, const double *b, double tol, int *st)
+{
+    __shared__ double sr[256], sb[256];
+    double mr = 0.0, mb = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        mr = fmax(mr, fabs(r[i]));
+        mb = fmax(mb, fabs(b[i]));
+    }
+    sr[threadIdx.x] = mr;
+    sb[threadIdx.x] = mb;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + w]);
+            sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && st[0] == 0) {
+        if (sr[0] <= tol * sb[0]) st[0] = 1;
+        else st[1] += 1;
+    }
+}
+
+// x = A^-1 rhs by the LU factors plus up to max_iter refinement steps on
+// the FP64 residual.  Everything is enqueued: the convergence test runs on
+// the device (later corrections are skipped once it passes), so the only
+// synchronisation is the final download.  With a rotor given, the solution
+// also becomes its device wing circulation (rotor_map_gam).
+int solve_refine(vlm_ctx *ctx, int n, const double *A, const double *LU, const int *ipiv,
+                 const double *rhs_h, double *x_h, int max_iter, double tol, int *iters,
+                 Rotor *rotor = nullptr)
+{
+    VLM_HIP(ctx->solve_tmp.reserve((size_t)3 * n + 1));
     double *b = ctx->solve_tmp.p, *x = b + n, *r = x + n;
-    VLM_HIP(hipMemcpyAsync(b, rhs_h, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    int *st = reinterpret_cast<int *>(r + n);
+    VLM_TRY(h2d(ctx, b, rhs_h, sizeof(double) * n));
     VLM_HIP(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
-    hipLaunchKernelGGL(lu_solve_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, LU, ipiv, x);
-    VLM_HIP(hipGetLastError());
-    double bn = 0.0;
-    for (int i = 0; i < n; ++i) bn = std::fmax(bn, std::fabs(rhs_h[i]));
-    std::vector<double> rh(n);
-    int it = 0;
-    for (; it < max_iter; ++it) {
+    hipLaunchKernelGGL(solve_state_init_kernel, dim3(1), dim3(1), 0, ctx->stream, st);
+    hipLaunchKernelGGL(lu_solve_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, LU, ipiv, x,
+                       nullptr);
+    for (int it = 0; it < max_iter; ++it) {
         hipLaunchKernelGGL(residual_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
                            A, x, b, r);
-        VLM_HIP(hipGetLastError());
-        VLM_HIP(hipMemcpyAsync(rh.data(), r, sizeof(double) * n, hipMemcpyDeviceToHost,
-                               ctx->stream));
-        VLM_HIP(hipStreamSynchronize(ctx->stream));
-        double rn = 0.0;
-        for (int i = 0; i < n; ++i) rn = std::fmax(rn, std::fabs(rh[i]));
-        if (rn <= tol * bn) break;
-        hipLaunchKernelGGL(lu_solve_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, LU, ipiv, r);
-        hipLaunchKernelGGL(axpy_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, r, x);
+        hipLaunchKernelGGL(refine_check_kernel, dim3(1), dim3(256), 0, ctx->stream, n, r, b, tol,
+                           st);
+        hipLaunchKernelGGL(lu_solve_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, LU, ipiv, r,
+                           st);
+        hipLaunchKernelGGL(axpy_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, r, x,
+                           st);
+    }
+    VLM_HIP(hipGetLastError());
+    if (rotor) {
+        const int per = rotor->d.nc * rotor->d.ns;
+        hipLaunchKernelGGL(map_gam_kernel, dim3((per * rotor->d.nb + 255) / 256), dim3(256), 0,
+                           ctx->stream, x, rotor->wg.p, per, rotor->d.nb, rotor->d.nbConvect);
         VLM_HIP(hipGetLastError());
     }
-    if (iters) *iters = it;
-    VLM_HIP(hipMemcpyAsync(x_h, x, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    char *px = nullptr, *ps = nullptr;
+    VLM_TRY(pin_take(ctx, sizeof(double) * n, &px));
+    VLM_TRY(pin_take(ctx, 2 * sizeof(int), &ps));
+    VLM_HIP(hipMemcpyAsync(px, x, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    VLM_HIP(hipMemcpyAsync(ps, st, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    VLM_TRY(sync_stream(ctx));
+    std::memcpy(x_h, px, sizeof(double) * n);
+    if (iters) *iters = reinterpret_cast<const int *>(ps)[1];
     return VLM_OK;
 }
 
@@ -176,9 +233,8 @@ int vlm_rotor_calc_aic(vlm_ctx *ctx, int ir, const double *cp, const double *nca
     }
     DBuf<double> geo;
     VLM_HIP(geo.reserve((size_t)6 * n));
-    VLM_HIP(hipMemcpyAsync(geo.p, cp, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
-    VLM_HIP(hipMemcpyAsync(geo.p + 3 * n, ncap, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
-                           ctx->stream));
+    VLM_TRY(h2d(ctx, geo.p, cp, sizeof(double) * 3 * n));
+    VLM_TRY(h2d(ctx, geo.p + 3 * n, ncap, sizeof(double) * 3 * n));
     const int64_t nn = (int64_t)n * n;
     hipLaunchKernelGGL(aic_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, ctx->stream,
                        n, r.wc.p, r.wrc.p, geo.p, geo.p + 3 * n, ctx->aic[ir]);
@@ -192,7 +248,7 @@ int vlm_rotor_calc_aic(vlm_ctx *ctx, int ir, const double *cp, const double *nca
     if (Aout)
         VLM_HIP(hipMemcpyAsync(Aout, ctx->aic[ir], sizeof(double) * nn, hipMemcpyDeviceToHost,
                                ctx->stream));
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    VLM_TRY(sync_stream(ctx));
     if (info != 0) return fail(ctx, VLM_ERR_SINGULAR, "AIC is singular (zero pivot)");
     return VLM_OK;
 }
@@ -204,13 +260,8 @@ int vlm_rotor_solve(vlm_ctx *ctx, int ir, const double *rhs, double *gam, int ma
     if (!rhs || !gam) return fail(ctx, VLM_ERR_ARG, "null buffer");
     if (!ctx->lu[ir]) return fail(ctx, VLM_ERR_STATE, "vlm_rotor_calc_aic not called");
     const int n = ctx->aic_n[ir];
-    VLM_TRY(solve_refine(ctx, n, ctx->aic[ir], ctx->lu[ir], ctx->piv[ir], rhs, gam, max_iter, tol,
-                         iters));
-    Rotor &r = *ctx->rotors[ir];
-    // rotor_map_gam: gamVec order (ic, is, ib) is the wing ring order
-    VLM_HIP(hipMemcpyAsync(r.wg.p, gam, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
-    return VLM_OK;
+    return solve_refine(ctx, n, ctx->aic[ir], ctx->lu[ir], ctx->piv[ir], rhs, gam, max_iter, tol,
+                        iters, ctx->rotors[ir]);
 }
 
 int vlm_lu_factor(vlm_ctx *ctx, int n, double *A, int *ipiv)
@@ -226,7 +277,7 @@ int vlm_lu_factor(vlm_ctx *ctx, int n, double *A, int *ipiv)
     VLM_HIP(hipMemcpyAsync(A, dA.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
     VLM_HIP(hipMemcpyAsync(ipiv, dp.p, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
     VLM_HIP(hipMemcpyAsync(&info, dp.p + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    VLM_TRY(sync_stream(ctx));
     if (info != 0) return fail(ctx, VLM_ERR_SINGULAR, "matrix is singular (zero pivot)");
     return VLM_OK;
 }

@@ -172,17 +172,66 @@ int upload_fset(vlm_ctx *ctx, int64_t nfil, const double *a, const double *b,
     VLM_HIP(d64.reserve((size_t)8 * nfil));
     VLM_HIP(d32.reserve((size_t)12 * nfil));
     VLM_TRY(compact_into(ctx, ctx->slots.p, ctx->flags.p, nfil, d64, d32, count, nfil));
-    VLM_HIP(hipStreamSynchronize(ctx->stream));  // raw is released on return
+    VLM_TRY(sync_stream(ctx));  // raw is released on return
     return VLM_OK;
 }
 
 }  // namespace
 
+int sync_stream(vlm_ctx *ctx)
+{
+    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->pin_off = 0;
+    return VLM_OK;
+}
+
+// Reserve bytes of the pinned arena (64-byte aligned), synchronising and
+// recycling it when full, growing it when a request exceeds its size.
+int pin_take(vlm_ctx *ctx, size_t bytes, char **out)
+{
+    const size_t need = (bytes + 63) & ~(size_t)63;
+    if (ctx->pin_off + need > ctx->pin_cap) {
+        VLM_TRY(sync_stream(ctx));
+        if (need > ctx->pin_cap) {
+            if (ctx->pin) VLM_HIP(hipHostFree(ctx->pin));
+            ctx->pin = nullptr;
+            ctx->pin_cap = 0;
+            size_t cap = need > ((size_t)8 << 20) ? need : ((size_t)8 << 20);
+            VLM_HIP(hipHostMalloc(reinterpret_cast<void **>(&ctx->pin), cap,
+                                  hipHostMallocDefault));
+            ctx->pin_cap = cap;
+        }
+    }
+    *out = ctx->pin + ctx->pin_off;
+    ctx->pin_off += need;
+    return VLM_OK;
+}
+
+int h2d(vlm_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return VLM_OK;
+    char *p = nullptr;
+    VLM_TRY(pin_take(ctx, bytes, &p));
+    std::memcpy(p, src, bytes);
+    VLM_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return VLM_OK;
+}
+
+int d2h(vlm_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return sync_stream(ctx);
+    char *p = nullptr;
+    VLM_TRY(pin_take(ctx, bytes, &p));
+    VLM_HIP(hipMemcpyAsync(p, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    VLM_TRY(sync_stream(ctx));
+    std::memcpy(dst, p, bytes);
+    return VLM_OK;
+}
+
 int points_to_device(vlm_ctx *ctx, int64_t m, const double *p, double *soa, int64_t stride)
 {
     VLM_HIP(ctx->hostio.reserve((size_t)3 * m));
-    VLM_HIP(hipMemcpyAsync(ctx->hostio.p, p, sizeof(double) * 3 * m, hipMemcpyHostToDevice,
-                           ctx->stream));
+    VLM_TRY(h2d(ctx, ctx->hostio.p, p, sizeof(double) * 3 * m));
     hipLaunchKernelGGL(aos_to_soa_kernel, dim3(nblk(m)), dim3(256), 0, ctx->stream, m,
                        ctx->hostio.p, soa, stride);
     VLM_HIP(hipGetLastError());
@@ -195,10 +244,7 @@ int points_to_host(vlm_ctx *ctx, int64_t m, const double *soa, int64_t stride, d
     hipLaunchKernelGGL(soa_to_aos_kernel, dim3(nblk(m)), dim3(256), 0, ctx->stream, m, soa,
                        stride, ctx->hostio.p);
     VLM_HIP(hipGetLastError());
-    VLM_HIP(hipMemcpyAsync(p, ctx->hostio.p, sizeof(double) * 3 * m, hipMemcpyDeviceToHost,
-                           ctx->stream));
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
-    return VLM_OK;
+    return d2h(ctx, p, ctx->hostio.p, sizeof(double) * 3 * m);
 }
 
 }  // namespace vlm
@@ -249,6 +295,7 @@ int vlm_destroy(vlm_ctx *ctx)
             (void)hipEventDestroy(t.b);
         }
     if (ctx->prof_nseg) (void)hipHostFree(ctx->prof_nseg);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return VLM_OK;
@@ -279,7 +326,7 @@ int vlm_set_bug_compat(vlm_ctx *ctx, int bug_compat)
 int vlm_synchronize(vlm_ctx *ctx)
 {
     if (!ctx) return VLM_ERR_ARG;
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    VLM_TRY(sync_stream(ctx));
     return VLM_OK;
 }
 
@@ -381,7 +428,7 @@ int vlm_profile_enable(vlm_ctx *ctx, int enable)
 int vlm_profile_reset(vlm_ctx *ctx)
 {
     if (!ctx) return VLM_ERR_ARG;
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    VLM_TRY(sync_stream(ctx));
     for (auto &t : ctx->timers) {
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
@@ -401,7 +448,7 @@ int vlm_profile_reset(vlm_ctx *ctx)
 int vlm_profile_get(vlm_ctx *ctx, double *bs_ms, int64_t *bs_launches, double *pairs)
 {
     if (!ctx) return VLM_ERR_ARG;
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    VLM_TRY(sync_stream(ctx));
     double ms = 0.0, pr = 0.0;
     for (size_t i = 0; i < ctx->timers.size(); ++i) {
         float e = 0.f;
@@ -419,7 +466,7 @@ int vlm_profile_get(vlm_ctx *ctx, double *bs_ms, int64_t *bs_launches, double *p
 int vlm_profile_get_lu(vlm_ctx *ctx, double *ms, int64_t *factorisations, double *flops)
 {
     if (!ctx) return VLM_ERR_ARG;
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    VLM_TRY(sync_stream(ctx));
     double t = 0.0, f = 0.0;
     for (auto &tm : ctx->lu_timers) {
         float e = 0.f;

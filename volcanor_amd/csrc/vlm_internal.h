@@ -11,7 +11,9 @@
 
 namespace vlm {
 
-// Device buffer owned by the library.  reserve() grows and discards.
+// Device buffer owned by the library.  reserve() grows and discards the
+// contents; growth is geometric (at least doubling) because buffers sized
+// by the wake grow every step and hipFree synchronises the device.
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -29,10 +31,11 @@ struct DBuf {
     hipError_t reserve(size_t k)
     {
         if (k <= n && p) return hipSuccess;
+        size_t want = k == 0 ? 1 : k;
+        if (p && want < 2 * n) want = 2 * n;
         release();
-        if (k == 0) k = 1;
-        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), k * sizeof(T));
-        if (e == hipSuccess) n = k;
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), want * sizeof(T));
+        if (e == hipSuccess) n = want;
         else p = nullptr;
         return e;
     }
@@ -139,6 +142,10 @@ struct vlm_ctx {
     vlm::DBuf<int> counter;  // work-item counter of the Biot-Savart launches
     int occ[4] = {0, 0, 0, 0};  // resident workgroups per kernel variant
     vlm::DBuf<double> hostio;  // device staging for host point lists
+    // pinned host arena for host<->device copies: truly asynchronous copies,
+    // reset whenever the stream has been synchronised
+    char *pin = nullptr;
+    size_t pin_cap = 0, pin_off = 0;
     // profiling
     int profile = 0;
     std::vector<vlm::Timer> timers;
@@ -202,6 +209,16 @@ int64_t wake_slots(const Rotor &r);
 
 int ensure_slots(vlm_ctx *ctx, int64_t n);
 void mlp_release(vlm_ctx *ctx);
+
+// Host<->device copies through the pinned arena (ctx->pin): h2d copies the
+// host bytes into the arena and enqueues an asynchronous copy, so the
+// caller's buffer is free on return without a stream synchronisation;
+// sync_stream waits for the stream and recycles the arena; d2h enqueues the
+// copy, synchronises and copies out of the arena.
+int pin_take(vlm_ctx *ctx, size_t bytes, char **out);
+int h2d(vlm_ctx *ctx, void *dst, const void *src, size_t bytes);
+int d2h(vlm_ctx *ctx, void *dst, const void *src, size_t bytes);
+int sync_stream(vlm_ctx *ctx);
 int mlp_forward_dev(vlm_ctx *ctx, int64_t m, const float *X, float *Y);
 int mlp_train_dev(vlm_ctx *ctx, int64_t m, const float *X, const float *T, int adam, float lr,
                   float *loss_host);

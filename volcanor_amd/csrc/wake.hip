@@ -1198,9 +1198,7 @@ int learned_pass(vlm_ctx *ctx)
         hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(256), 0, ctx->stream, P.e2.p, take,
                            P.scal.p + 1);
         double h[3] = {0, 0, 0};
-        VLM_HIP(hipMemcpyAsync(h, P.scal.p + 1, 3 * sizeof(double), hipMemcpyDeviceToHost,
-                               ctx->stream));
-        VLM_HIP(hipStreamSynchronize(ctx->stream));
+        VLM_TRY(d2h(ctx, h, P.scal.p + 1, 3 * sizeof(double)));
         P.buf_head = (P.buf_head + take) % P.buf_cap;
         P.buf_n = P.buf_n + take < P.buf_cap ? P.buf_n + take : P.buf_cap;
         P.last_err = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
@@ -1342,7 +1340,7 @@ int vlm_rotor_create(vlm_ctx *ctx, const vlm_rotor_desc *d, double spanwiseCore,
             VLM_HIP(hipMemcpyAsync(r->frc0.p, frc.data(), frc.size() * sizeof(double),
                                    hipMemcpyHostToDevice, ctx->stream));
         }
-        VLM_HIP(hipStreamSynchronize(ctx->stream));
+        VLM_TRY(sync_stream(ctx));
     }
     r->rowNear = (int)nN + 1;
     r->rowFar = (int)nF + 1;
@@ -1360,16 +1358,9 @@ int vlm_rotor_set_wing(vlm_ctx *ctx, int ir, const double *corners, const double
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
-    if (corners)
-        VLM_HIP(hipMemcpyAsync(r->wc.p, corners, sizeof(double) * 12 * r->nring,
-                               hipMemcpyHostToDevice, ctx->stream));
-    if (rc)
-        VLM_HIP(hipMemcpyAsync(r->wrc.p, rc, sizeof(double) * 4 * r->nring,
-                               hipMemcpyHostToDevice, ctx->stream));
-    if (gam)
-        VLM_HIP(hipMemcpyAsync(r->wg.p, gam, sizeof(double) * r->nring, hipMemcpyHostToDevice,
-                               ctx->stream));
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    if (corners) VLM_TRY(h2d(ctx, r->wc.p, corners, sizeof(double) * 12 * r->nring));
+    if (rc) VLM_TRY(h2d(ctx, r->wrc.p, rc, sizeof(double) * 4 * r->nring));
+    if (gam) VLM_TRY(h2d(ctx, r->wg.p, gam, sizeof(double) * r->nring));
     return VLM_OK;
 }
 
@@ -1628,7 +1619,7 @@ int vlm_rotor_get_wake(vlm_ctx *ctx, int ir, int wakeType, double *nodes, double
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
     const int pred = wakeType == VLM_WAKE_P;
     const size_t nb = r->d.nb, nN = r->d.nNwake, ns = r->d.ns, nF = r->d.nFwake;
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    VLM_TRY(sync_stream(ctx));
     std::vector<double> tn(3 * nb * nN * (ns + 1)), tg(nb * nN * ns), tr(4 * nb * nN * ns);
     if (nN > 0) {
         VLM_HIP(hipMemcpy(tn.data(), pred ? r->pnode.p : r->node.p, tn.size() * sizeof(double),
@@ -1674,7 +1665,7 @@ int vlm_rotor_set_wake(vlm_ctx *ctx, int ir, const double *nodes, const double *
         return fail(ctx, VLM_ERR_ARG, "cursor out of range");
     if (nN > 0 && (!nodes || !le || !gam || !rc4)) return fail(ctx, VLM_ERR_ARG, "null buffer");
     if (nF > 0 && (!far_fc || !far_gam || !far_rc)) return fail(ctx, VLM_ERR_ARG, "null buffer");
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    VLM_TRY(sync_stream(ctx));
     r->head = 0;
     r->rowNear = rowNear;
     r->rowFar = rowFar;
@@ -1710,7 +1701,7 @@ int vlm_rotor_get_velocity(vlm_ctx *ctx, int ir, int which, double *velN, double
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
     if (which < VLM_VEL || which > VLM_VEL_STEP) return fail(ctx, VLM_ERR_ARG, "bad which");
-    VLM_HIP(hipStreamSynchronize(ctx->stream));
+    VLM_TRY(sync_stream(ctx));
     if (velN && r->nvn() > 0)
         VLM_HIP(hipMemcpy(velN, r->velN[which].p, r->nvn() * sizeof(double), hipMemcpyDeviceToHost));
     if (velF && r->nvf() > 0)

@@ -1038,8 +1038,9 @@ contains
           & 'rotor_solve')
         r%gamVec = gam
         deallocate (gam)
+        ! the library has already set the device wing circulation from
+        ! gamVec (rotor_map_gam on the device, axisymmetric copy included)
         call rotor_map_gam(r)
-        call rotor_upload_wing(ctx, r, .true.)
       end associate
     enddo
   end subroutine compute_rhs_and_solve
