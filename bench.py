@@ -9,10 +9,18 @@ Workloads
                     + loads + AM2 predictor-corrector convection with both
                     Biot-Savart passes).  W warm-up steps run untimed, the
                     last K steps are timed (default: the 10th revolution).
+                    Variants: plain FP64 (the value), mixed FP32/FP64, mixed
+                    with the learned pass 2, and an FP64 twin with a 1e-9 deg
+                    cyclic pitch (the chaos floor of CT).
+  quad              quad-rotor (cases/cfg4_quad.case), 4 three-bladed rotors
+                    x 30 panels, 10 deg steps; steps 351-370 (10 revolutions)
+                    timed; same variants.
   microbench        1e5 targets x 1e6 helix filaments, one N-body evaluation
                     per step, inputs resident in HBM.
+  lu                dense FP64 LU with partial pivoting, N = 2176.
+  hbmcopy           1 GiB device copy (HBM counter calibration).
 
-  python bench.py [--workload rotor2|microbench] [--steps K] [--warmup W]
+  python bench.py [--workload rotor2|quad|microbench|lu|hbmcopy] [--steps K] [--warmup W]
 """
 import argparse
 import json
@@ -177,71 +185,80 @@ def run_reference(case_dir, nsteps, threads):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+WORKLOADS = {
+    # name: case, total steps (10 revolutions), default timed tail, steps per
+    # revolution, rotors, description
+    "rotor2": ("cfg2_rotor.case", 720, 72, 72, 1,
+               "two-bladed hover rotor (cfg2): 1 x 20 panels per blade, -8 deg twist, 8 deg "
+               "collective, tip Mach 0.4, 5 deg steps, AM2 free wake with strain and "
+               "dissipation, 10 revolutions; timed = last K steps"),
+    "quad": ("cfg4_quad.case", 370, 20, 36, 4,
+             "quad-rotor (cfg4): 4 three-bladed rotors, 1 x 30 panels per blade, 8 deg "
+             "collective, alternating rotation, 10 deg steps, AM2 free wake, 15-revolution "
+             "truncated lattice; steps 351-370 (10 revolutions) timed"),
+}
+
+
 def bench_rotor(args):
     from volcanor_amd import host
-    case = os.path.join(ROOT, "cases", "cfg2_rotor.case")
-    nt_case = 720
-    K = args.steps if args.steps else 72
+    case_name, nt_case, K_def, steps_rev, nrot, desc = WORKLOADS[args.workload]
+    case = os.path.join(ROOT, "cases", case_name)
+    K = args.steps if args.steps else K_def
     W = args.warmup if args.warmup is not None else nt_case - K
     nt = W + K
 
     runs = {}
-    plan = (("fp64", "fp64", False), ("mixed", "mixed", False)) + \
-        ((("learned_mixed", "mixed", True),) if args.learned or "learned_mixed" in args.variants
-           else ())
+    plan = (("fp64", "fp64", False), ("mixed", "mixed", False), ("learned_mixed", "mixed", True))
     want = set(args.variants.split(",")) | ({"learned_mixed"} if args.learned else set())
     plan = tuple(v for v in plan if v[0] == "fp64" or v[0] in want)
     for name, prec, learned in plan:
         t0 = time.perf_counter()
         ct, st = host.run_case(case, precision=prec, nt=nt, tail_steps=K, learned=learned,
-                               tol=args.tol)
-        runs[name] = dict(ct=ct[0], st=st, wall=time.perf_counter() - t0)
+                               tol=args.tol, nrotors=nrot)
+        runs[name] = dict(ct=ct, st=st, wall=time.perf_counter() - t0)
         print(f"# {name}: run {runs[name]['wall']:.2f} s, timed {st.wall_steps_tail:.3f} s for "
               f"{K} steps, BS kernel {st.bs_kernel_ms:.1f} ms over {st.bs_launches} launches, "
               f"pairs {st.bs_pairs:.3e}", file=sys.stderr, flush=True)
 
     if not args.no_twin:
-        # FP64 twin with a 1e-9 deg cyclic pitch: a blade-asymmetric
-        # perturbation of relative size ~1e-10 that measures how far the
-        # hover wake's own dynamics amplify round-off-sized differences.
+        # FP64 twin with a 1e-9 deg cyclic pitch on rotor 1: a blade-
+        # asymmetric perturbation of relative size ~1e-10 that measures how
+        # far the wake's own dynamics amplify round-off-sized differences.
         tdir = case_variant(case, "thetaC = 0.0", "thetaC = 1.0e-9")
         try:
             t0 = time.perf_counter()
-            ct, st = host.run_case(tdir, precision="fp64", nt=nt, tail_steps=K)
-            runs["fp64_twin_thetaC_1e-9"] = dict(ct=ct[0], st=st, wall=time.perf_counter() - t0)
+            ct, st = host.run_case(tdir, precision="fp64", nt=nt, tail_steps=K, nrotors=nrot)
+            runs["fp64_twin_thetaC_1e-9"] = dict(ct=ct, st=st, wall=time.perf_counter() - t0)
         finally:
             shutil.rmtree(tdir, ignore_errors=True)
     ref = runs["fp64"]["ct"]
-    steps_rev = 72
     for r in runs.values():
         r["ct_rel"] = float(np.max(np.abs(r["ct"] - ref) / np.maximum(np.abs(ref), 1e-30)))
-        dev = np.abs(r["ct"] - ref) / np.abs(ref).max()
-        r["ct_dev_per_rev"] = [float(dev[1 + steps_rev * k: 1 + steps_rev * (k + 1)].max())
-                               for k in range((len(dev) - 1) // steps_rev)]
+        dev = np.abs(r["ct"] - ref) / np.abs(ref).max(axis=1, keepdims=True)
+        r["ct_dev_per_rev"] = [float(dev[:, 1 + steps_rev * k: 1 + steps_rev * (k + 1)].max())
+                               for k in range((dev.shape[1] - 1) // steps_rev)]
     # primary path: mixed precision when its CT history holds 1e-4 of FP64
     prim = "mixed" if "mixed" in runs and runs["mixed"]["ct_rel"] < 1e-4 else "fp64"
     P = runs[prim]["st"]
     value = P.bs_pairs / P.wall_steps_tail
+    traffic_key = None
+    if prim == "fp64" and args.workload == "rotor2" and K == 72 and nt == nt_case:
+        traffic_key = "rotor_tail144:bs_fp64_kernel<256>"
     out = {
         "metric": "Biot-Savart pair-interactions/s",
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": P.wall_steps_tail * 1e3 / K, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64" if prim == "mixed" else "f64",
         "data": "synthetic",
-        "config": {"workload": "two-bladed hover rotor (cfg2): 1 x 20 panels per blade, "
-                               "-8 deg twist, 8 deg collective, tip Mach 0.4, 5 deg steps, "
-                               "AM2 free wake with strain and dissipation, 10 revolutions; "
-                               "timed = last K steps",
-                   "case": "cases/cfg2_rotor.case", "timed_steps": K, "total_steps": nt,
-                   "precision": prim},
+        "config": {"workload": desc, "case": f"cases/{case_name}", "timed_steps": K,
+                   "total_steps": nt, "precision": prim},
         "roofline": roofline(P.bs_kernel_ms, P.bs_launches, P.bs_pairs,
                              PEAK_FP32_VALU if prim == "mixed" else PEAK_FP64_VALU,
                              "bs_mixed_kernel<256>" if prim == "mixed" else "bs_fp64_kernel<256>",
-                             None if prim == "mixed" or K != 72 or nt != nt_case
-                             else "rotor_tail144:bs_fp64_kernel<256>"),
+                             traffic_key),
         "variants": {k: {"s_per_step": r["st"].wall_steps_tail / K,
                          "pairs_per_s": r["st"].bs_pairs / r["st"].wall_steps_tail,
-                         "run_wall_s": r["wall"], "ct_final": float(r["ct"][-1]),
+                         "run_wall_s": r["wall"], "ct_final": [float(x) for x in r["ct"][:, -1]],
                          "ct_max_rel_dev_vs_fp64": r["ct_rel"],
                          "ct_dev_per_rev_vs_fp64": r["ct_dev_per_rev"],
                          "bs_kernel_avg_ms": r["st"].bs_kernel_ms / max(r["st"].bs_launches, 1),
@@ -253,15 +270,28 @@ def bench_rotor(args):
                                        "extrapolation_rel_rms": r["st"].pred_base_rms}}
                      for k, r in runs.items()},
     }
-    out["ct_note"] = ("ct_dev_per_rev_vs_fp64: max |CT - CT_fp64| / max|CT_fp64| per revolution. "
-                      "The fp64 twin differs from fp64 only by a 1e-9 deg cyclic pitch; its "
-                      "growth is the wake's own amplification of round-off-sized differences.")
+    out["ct_note"] = ("ct_dev_per_rev_vs_fp64: max |CT - CT_fp64| / max|CT_fp64| per revolution "
+                      "(over rotors). The fp64 twin differs from fp64 only by a 1e-9 deg cyclic "
+                      "pitch; its growth is the wake's own amplification of round-off-sized "
+                      "differences.")
     out["speedup_s_per_step_vs_fp64"] = {
         k: runs["fp64"]["st"].wall_steps_tail / r["st"].wall_steps_tail for k, r in runs.items()}
+    if "learned_mixed" in runs:
+        L = runs["learned_mixed"]
+        out["learned_vs_plain"] = {
+            "s_per_step_plain_fp64": runs["fp64"]["st"].wall_steps_tail / K,
+            "s_per_step_learned_mixed": L["st"].wall_steps_tail / K,
+            "speedup": runs["fp64"]["st"].wall_steps_tail / L["st"].wall_steps_tail,
+            "tol": args.tol,
+            "ct_dev_per_rev": L["ct_dev_per_rev"],
+            "fp64_twin_ct_dev_per_rev": runs.get("fp64_twin_thetaC_1e-9", {}).get("ct_dev_per_rev"),
+            "revs_within_1e-4": sum(1 for x in L["ct_dev_per_rev"] if x <= 1e-4),
+            "network_gain_over_extrapolation":
+                L["st"].pred_base_rms / L["st"].pred_mlp_rms if L["st"].pred_mlp_rms > 0 else None}
     if not args.no_cpu and os.path.exists(REF_BIN):
         S = args.cpu_steps
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        _, sg = host.run_case(case, precision="fp64", nt=S, tail_steps=S)
+        _, sg = host.run_case(case, precision="fp64", nt=S, tail_steps=S, nrotors=nrot)
         wall_cpu = run_reference(case, S, threads)
         out["cpu_baseline"] = {
             "value": sg.bs_pairs / wall_cpu, "unit": "pairs/s", "cores": threads,
@@ -335,8 +365,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", choices=["rotor2", "microbench", "lu", "hbmcopy"], default="rotor2")
-    ap.add_argument("--variants", default="mixed",
+    ap.add_argument("--workload", choices=["rotor2", "quad", "microbench", "lu", "hbmcopy"], default="rotor2")
+    ap.add_argument("--variants", default="mixed,learned_mixed",
                     help="comma list of rotor2 variants run beside fp64 (mixed, learned_mixed)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--learned", action="store_true")
