@@ -8,6 +8,8 @@ PLOT3D blade file) used by the tests and the benchmark.
   cfg4_quad          four three-bladed rotors, 1 x 30, 10 deg steps, 20 revs,
                      15-rev truncated lattice
   cfg5_rotor_wing    three-bladed rotor in forward flight above an AR-8 wing
+  refprobe_rotor     2 x (2 x 3) rotor in forward flight, 14 steps: the case on
+                     which oracle/refprobe.f90 dumps the reference's wake state
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
   elevate            the reference's 5-bladed axisymmetric test rotor
   (the last two reproduce tests/katzNplotkin-AR04.case and
@@ -169,6 +171,18 @@ def make_all(out):
                    chord=c, Omega=om, theta0=8.0, pivotLE=0.25, geometryFile="blade.xyz",
                    velBody=[0.0, 0.0, 0.0], apparentViscCoeff=ROTOR_AVC,
                    spanwiseCore=ROTOR_CORE, streamwiseCoreVec=ROTOR_CORE)
+
+    # small forward-flight rotor for the per-node wake comparison with the
+    # reference (oracle/refprobe.f90): far wake, rollup, truncation shift,
+    # dissipation and strain within 14 steps
+    d = case("refprobe_rotor")
+    write_config(os.path.join(d, "config.nml"), nt=14, dt=0.004, nr=1, wakeDissipation=1,
+                 wakeStrain=1, rotorForcePlot=0)
+    write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=2, ns=3, nNwake=5, span=1.0,
+               rootcut=0.2, chord=0.2, Omega=60.0, theta0=8.0, pivotLE=0.25,
+               velBody=[-2.0, 0.0, 0.0], wakeTruncateNt=9, apparentViscCoeff=50.0,
+               spanwiseCore=0.2, streamwiseCoreVec=0.2, rollupStartRadius=0.6,
+               rollupEndRadius=1.0)
 
     # cfg4: quad-rotor, 20 revolutions, 15-rev truncated near wake
     dt4 = math.radians(10.0) / om

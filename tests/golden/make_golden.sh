@@ -29,3 +29,13 @@ run_ref() {  # case-name  output-name
 }
 [ -e "$HERE/cfg1_wing_ForceNonDim.csv" ] || run_ref cfg1_wing cfg1_wing_ForceNonDim.csv
 run_ref cfg2_rotor_2rev cfg2_rotor_2rev_ForceNonDim.csv
+# 3. per-step wing and wake state of the reference (oracle/_ref/refprobe,
+#    our driver linked against the reference objects) on the small
+#    forward-flight rotor: ring corners, core radii, circulation, velocity
+#    histories after the initial solution and after each of 14 steps.
+tmp=$(mktemp -d)
+cp "$ROOT/cases/refprobe_rotor.case/"* "$tmp/"
+mkdir -p "$tmp/Results"
+(cd "$tmp" && OMP_NUM_THREADS=1 "$ROOT/oracle/_ref/refprobe" > run.log 2>&1)
+cp "$tmp/Results/refprobe.txt" "$HERE/refprobe_rotor_wake.txt"
+rm -rf "$tmp"
