@@ -184,32 +184,41 @@ def make_all(out):
                spanwiseCore=0.2, streamwiseCoreVec=0.2, rollupStartRadius=0.6,
                rollupEndRadius=1.0)
 
-    # cfg4: quad-rotor, 20 revolutions, 15-rev truncated near wake
+    # cfg4: quad-rotor, 20 revolutions, 15-rev truncated near wake; the
+    # _short variant (one revolution, 12-step truncation) is compared with
+    # the reference solver step by step
     dt4 = math.radians(10.0) / om
-    d = case("cfg4_quad")
-    write_config(os.path.join(d, "config.nml"), nt=720, dt=dt4, nr=4, velSound=340.0)
     hubs = [(1.25, 1.25), (-1.25, 1.25), (-1.25, -1.25), (1.25, -1.25)]
-    for k, (hx, hy) in enumerate(hubs):
-        sgn = 1.0 if k % 2 == 0 else -1.0
-        write_geom(os.path.join(d, f"geom{k + 1:02d}.nml"), nb=3, nc=1, ns=30, nNwake=540,
-                   span=R, chord=c, Omega=sgn * om, theta0=8.0, pivotLE=0.25,
-                   hubCoords=[hx, hy, 0.0], velBody=[0.0, 0.0, 0.0], wakeTruncateNt=540,
+    for name, nt4, nN4, tr4 in (("cfg4_quad", 720, 540, 540), ("cfg4_quad_short", 36, 12, 24)):
+        d = case(name)
+        write_config(os.path.join(d, "config.nml"), nt=nt4, dt=dt4, nr=4, velSound=340.0)
+        for k, (hx, hy) in enumerate(hubs):
+            sgn = 1.0 if k % 2 == 0 else -1.0
+            write_geom(os.path.join(d, f"geom{k + 1:02d}.nml"), nb=3, nc=1, ns=30, nNwake=nN4,
+                       span=R, chord=c, Omega=sgn * om, theta0=8.0, pivotLE=0.25,
+                       hubCoords=[hx, hy, 0.0], velBody=[0.0, 0.0, 0.0], wakeTruncateNt=tr4,
+                       apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+                       streamwiseCoreVec=ROTOR_CORE)
+
+    # cfg5: rotor in forward flight above a wing (15 revolutions); the
+    # _short variant (half a revolution) is compared with the reference.  The
+    # reference solver turns NaN from the first step whenever two coupled
+    # surfaces have different spanwise panel counts (rotor 30, wing 40 here;
+    # each alone, or both with 30, run normally), so the short parity case
+    # gives the wing 30 spanwise panels.
+    dt5 = math.radians(5.0) / om
+    for name, nt5, ns_w in (("cfg5_rotor_wing", 1080, 40), ("cfg5_rotor_wing_short", 36, 30)):
+        d = case(name)
+        write_config(os.path.join(d, "config.nml"), nt=nt5, dt=dt5, nr=2, velSound=340.0)
+        write_geom(os.path.join(d, "geom01.nml"), nb=3, nc=1, ns=30, nNwake=nt5, span=R,
+                   chord=c, Omega=om, theta0=10.0, pivotLE=0.25, velBody=[-13.6, 0.0, 0.0],
                    apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
                    streamwiseCoreVec=ROTOR_CORE)
-
-    # cfg5: rotor in forward flight above a wing
-    dt5 = math.radians(5.0) / om
-    d = case("cfg5_rotor_wing")
-    write_config(os.path.join(d, "config.nml"), nt=1080, dt=dt5, nr=2, velSound=340.0)
-    write_geom(os.path.join(d, "geom01.nml"), nb=3, nc=1, ns=30, nNwake=1080, span=R, chord=c,
-               Omega=om, theta0=10.0, pivotLE=0.25, velBody=[-13.6, 0.0, 0.0],
-               apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
-               streamwiseCoreVec=ROTOR_CORE)
-    write_geom(os.path.join(d, "geom02.nml"), nb=1, nc=4, ns=40, nNwake=1080, span=2.0,
-               chord=0.25, Omega=0.0, theta0=3.0, hubCoords=[0.0, 0.0, -0.5],
-               fromCoords=[0.0, 1.0, 0.0], velBody=[-13.6, 0.0, 0.0],
-               apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
-               streamwiseCoreVec=ROTOR_CORE)
+        write_geom(os.path.join(d, "geom02.nml"), nb=1, nc=4, ns=ns_w, nNwake=nt5, span=2.0,
+                   chord=0.25, Omega=0.0, theta0=3.0, hubCoords=[0.0, 0.0, -0.5],
+                   fromCoords=[0.0, 1.0, 0.0], velBody=[-13.6, 0.0, 0.0],
+                   apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+                   streamwiseCoreVec=ROTOR_CORE)
 
 
 if __name__ == "__main__":

@@ -39,3 +39,7 @@ mkdir -p "$tmp/Results"
 (cd "$tmp" && OMP_NUM_THREADS=1 "$ROOT/oracle/_ref/refprobe" > run.log 2>&1)
 cp "$tmp/Results/refprobe.txt" "$HERE/refprobe_rotor_wake.txt"
 rm -rf "$tmp"
+# 4. multi-rotor coupling: quad-rotor (one revolution, far wake and
+#    truncation within it) and rotor above a wing (half a revolution)
+run_ref cfg4_quad_short cfg4_quad_short_ForceNonDim.csv
+run_ref cfg5_rotor_wing_short cfg5_rotor_wing_short_ForceNonDim.csv
