@@ -234,8 +234,8 @@ int vlm_predictor_errors(vlm_ctx *ctx, double *mlp_rel_rms, double *base_rel_rms
                          int64_t *judged_steps);
 
 /* ---- kernel timing ----------------------------------------------------- */
-/* When enabled, every Biot-Savart pair launch is bracketed by HIP events on
- * the context stream.  vlm_profile_get returns the summed kernel time (ms),
+/* When enabled, every work-queue Biot-Savart launch (1024 targets or more)
+ * is bracketed by HIP events on the context stream.  vlm_profile_get returns the summed kernel time (ms),
  * the launch count and the algorithmic pair count since the last reset. */
 int vlm_profile_enable(vlm_ctx *ctx, int enable);
 int vlm_profile_reset(vlm_ctx *ctx);

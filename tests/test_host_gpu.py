@@ -82,6 +82,25 @@ def test_cfg2_mixed_precision_ct_within_1e4():
     assert np.max(rel) < 1e-4
 
 
+def test_learned_predictor_gain_over_extrapolation():
+    """Learned pass 2 on the two-revolution cfg2 rotor (mixed precision,
+    judge tolerance 1e-3).  The judge scores the network and the
+    network-free substitute of the same cost (linear extrapolation of the
+    pass-1 velocity, i.e. AM2 degraded to AB2) on the same exact samples:
+    the network must be at least 10x more accurate, some passes must be
+    accepted, and CT must stay within 1e-4 of the reference solver."""
+    from volcanor_amd import host
+    ref = read_force(os.path.join(GOLD, "cfg2_rotor_2rev_ForceNonDim.csv"))[:, 0]
+    ct, st = host.run_case(os.path.join(CASES, "cfg2_rotor_2rev.case"), precision="mixed",
+                           learned=True, tol=1e-3)
+    print(f"judged {st.pred_judged} learned {st.pred_learned} fallbacks {st.pred_fallbacks} "
+          f"mlp {st.pred_mlp_rms:.3e} extrapolation {st.pred_base_rms:.3e}", flush=True)
+    assert st.pred_judged > 0 and st.pred_learned > 0
+    assert 0.0 < st.pred_mlp_rms < 0.1 * st.pred_base_rms
+    assert len(ct[0]) == len(ref)
+    assert np.max(np.abs(ct[0] - ref)) < 1e-4 * np.max(np.abs(ref))
+
+
 @pytest.mark.parametrize("name,nr", [("cfg4_quad_short", 4), ("cfg5_rotor_wing_short", 2)])
 def test_multi_rotor_matches_reference_solver(name, nr):
     """Several lifting surfaces coupled through the right-hand side (every
