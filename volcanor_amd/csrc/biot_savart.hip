@@ -6,8 +6,8 @@
 //       / sqrt((rc |r0|)^4 + |r1 x r2|^4),
 // zero when |r1 x r2|^2 <= eps^2, r/|r| = 0 when |r| <= eps
 // (unitVec, libMath.f90:249-262).  Per segment the library precomputes
-// r0 = b - a, g = gam/(4 pi) and q = rc^2 |r0|^2; per pair
-// r1 = p - a, r2 = r1 - r0 and r1 x r2 = r0 x r1.
+// r0 = b - a, g = gam/(4 pi) and q = rc^2 |r0|^2 (stored squared in the
+// FP64 record); per pair r1 = p - a, r2 = r1 - r0 and r1 x r2 = r0 x r1.
 //
 // Work decomposition: one target point per lane; a workgroup walks one
 // (target tile, segment chunk) work item at a time, staging 256 segments in
@@ -37,8 +37,7 @@ namespace {
 constexpr int BS_BLOCK = 256;       // wake passes: many targets
 constexpr double EPS2 = kEps * kEps;
 constexpr float EPS2F = 4.93038065763132e-32f;
-constexpr float NEAR_CORE2 = 4.0f;  // h < 2 rc ...
-constexpr float NEAR_LEN2 = 4.0f;   // ... and |r1|^2 + |r2|^2 < 4 |r0|^2
+constexpr float NEAR_CORE2 = 4.0f;  // h < 2 rc, and |r1|^2 + |r2|^2 < 4 |r0|^2 (stored)
 
 __device__ __forceinline__ double rsq64(double x)
 {
@@ -52,7 +51,22 @@ __device__ __forceinline__ double rsq64(double x)
     return __builtin_fma(y, e * c, y);
 }
 
-// FP64 pair; segment record (ax ay az g | r0x r0y r0z q).
+// x where c holds, else a value that acts as zero here: only the high dword
+// (sign, exponent, top of the mantissa) is cleared, one select instead of
+// two.  What is left is a positive number below 2^-1042.  It only enters
+// products with a factor below 2^-50 (d1 when |r1| <= eps, c when
+// |r1 x r2| <= eps), which underflow to exactly 0, so the sums are bitwise
+// those of a true zero.
+__device__ __forceinline__ double zero_unless(bool c, double x)
+{
+    const int hi = __double2hiint(x);
+    return __hiloint2double(c ? hi : 0, __double2loint(x));
+}
+
+// FP64 pair; segment record (ax ay az g | r0x r0y r0z qq), qq = (rc |r0|)^4.
+// The reference's r0 . (r1/|r1| - r2/|r2|) / sqrt(qq + c2^2) is evaluated as
+// d1 / sqrt(n1 den) - d2 / sqrt(n2 den): two reciprocal square roots per
+// pair instead of three; the unitVec guards (|r| <= eps -> 0) stay on n1, n2.
 __device__ __forceinline__ void pair64(double px, double py, double pz, double4 sa, double4 sb,
                                        double &vx, double &vy, double &vz)
 {
@@ -65,23 +79,21 @@ __device__ __forceinline__ void pair64(double px, double py, double pz, double4 
     const double c2 = cx * cx + cy * cy + cz * cz;
     const double n1 = r1x * r1x + r1y * r1y + r1z * r1z;
     const double n2 = r2x * r2x + r2y * r2y + r2z * r2z;
-    const double n0 = r0x * r0x + r0y * r0y + r0z * r0z;
     const double d1 = r0x * r1x + r0y * r1y + r0z * r1z;
-    const double d2 = d1 - n0;
-    const double i1 = n1 > EPS2 ? rsq64(n1) : 0.0;
-    const double i2 = n2 > EPS2 ? rsq64(n2) : 0.0;
-    const double dot = d1 * i1 - d2 * i2;
-    const double den = sb.w * sb.w + c2 * c2;
-    double s = sa.w * dot * rsq64(den);
-    s = c2 > EPS2 ? s : 0.0;
+    const double d2 = r0x * r2x + r0y * r2y + r0z * r2z;
+    const double den = __builtin_fma(c2, c2, sb.w);
+    const double j1 = zero_unless(n1 > EPS2, rsq64(n1 * den));
+    const double j2 = zero_unless(n2 > EPS2, rsq64(n2 * den));
+    const double s = zero_unless(c2 > EPS2, sa.w * (d1 * j1 - d2 * j2));
     vx += cx * s;
     vy += cy * s;
     vz += cz * s;
 }
 
 // FP32 pair on split coordinates; record (ahx ahy ahz g | alx aly alz q |
-// r0x r0y r0z -).  Returns true for near-singular pairs (contribution left
-// to the FP64 branch).
+// r0x r0y r0z 4|r0|^2).  Returns true for near-singular pairs, whose
+// contribution is left to the FP64 recompute (zero here).  Same evaluation
+// order as pair64: d1 / sqrt(n1 den) - d2 / sqrt(n2 den).
 __device__ __forceinline__ bool pair32(float phx, float phy, float phz, float plx, float ply,
                                        float plz, float4 s0, float4 s1, float4 s2, float &vx,
                                        float &vy, float &vz)
@@ -97,15 +109,13 @@ __device__ __forceinline__ bool pair32(float phx, float phy, float phz, float pl
     const float c2 = cx * cx + cy * cy + cz * cz;
     const float n1 = r1x * r1x + r1y * r1y + r1z * r1z;
     const float n2 = r2x * r2x + r2y * r2y + r2z * r2z;
-    const float n0 = r0x * r0x + r0y * r0y + r0z * r0z;
-    const bool nearp = (c2 < NEAR_CORE2 * s1.w) && (n1 + n2 < NEAR_LEN2 * n0);
+    const bool nearp = (c2 < NEAR_CORE2 * s1.w) && (n1 + n2 < s2.w);
     const float d1 = r0x * r1x + r0y * r1y + r0z * r1z;
-    const float d2 = d1 - n0;
-    const float i1 = n1 > EPS2F ? __builtin_amdgcn_rsqf(n1) : 0.0f;
-    const float i2 = n2 > EPS2F ? __builtin_amdgcn_rsqf(n2) : 0.0f;
-    const float dot = d1 * i1 - d2 * i2;
-    const float den = s1.w * s1.w + c2 * c2;
-    float s = s0.w * dot * __builtin_amdgcn_rsqf(den);
+    const float d2 = r0x * r2x + r0y * r2y + r0z * r2z;
+    const float den = __builtin_fmaf(c2, c2, s1.w * s1.w);
+    const float j1 = n1 > EPS2F ? __builtin_amdgcn_rsqf(n1 * den) : 0.0f;
+    const float j2 = n2 > EPS2F ? __builtin_amdgcn_rsqf(n2 * den) : 0.0f;
+    float s = s0.w * (d1 * j1 - d2 * j2);
     s = (c2 > EPS2F && !nearp) ? s : 0.0f;
     vx += cx * s;
     vy += cy * s;
@@ -192,9 +202,16 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x)
     s = tt;
 }
 
+// Near pairs are flagged in a 32-bit mask per group of 32 staged segments
+// and recomputed in FP64 after the group, one wave-wide iteration per set
+// bit of the busiest lane (records read from the FP64 list in global
+// memory), so the FP32 loop stays free of the FP64 branch.  Staged groups
+// are padded with zero records, which contribute exactly zero and are
+// never near (r0 = 0).
 template <int TILE>
 __global__ __launch_bounds__(TILE) void bs_mixed_kernel(BsArgs a)
 {
+    static_assert(TILE % 32 == 0, "groups of 32 segments");
     __shared__ float4 s0v[TILE];
     __shared__ float4 s1v[TILE];
     __shared__ float4 s2v[TILE];
@@ -224,22 +241,36 @@ __global__ __launch_bounds__(TILE) void bs_mixed_kernel(BsArgs a)
         double nx = 0.0, ny = 0.0, nz = 0.0;
         for (int64_t base = sb0; base < sb1; base += TILE) {
             const int cnt = (int)min<int64_t>(TILE, sb1 - base);
+            const int cnt32 = (cnt + 31) & ~31;
             __syncthreads();
             if (tid < cnt) {
                 const int64_t s = base + tid;
                 s0v[tid] = make_float4(a.s32[s], a.s32[S + s], a.s32[2 * S + s], a.s32[3 * S + s]);
                 s1v[tid] = make_float4(a.s32[4 * S + s], a.s32[5 * S + s], a.s32[6 * S + s],
                                        a.s32[7 * S + s]);
-                s2v[tid] = make_float4(a.s32[8 * S + s], a.s32[9 * S + s], a.s32[10 * S + s], 0.f);
+                s2v[tid] = make_float4(a.s32[8 * S + s], a.s32[9 * S + s], a.s32[10 * S + s],
+                                       a.s32[11 * S + s]);
+            } else if (tid < cnt32) {
+                const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+                s0v[tid] = z;
+                s1v[tid] = z;
+                s2v[tid] = z;
             }
             __syncthreads();
             float tx = 0.f, ty = 0.f, tz = 0.f;
+            for (int g = 0; g < cnt32; g += 32) {
+                unsigned mask = 0u;
 #pragma unroll 4
-            for (int k = 0; k < cnt; ++k) {
-                const bool nearp =
-                    pair32(phx, phy, phz, plx, ply, plz, s0v[k], s1v[k], s2v[k], tx, ty, tz);
-                if (nearp && live) {
-                    const int64_t s = base + k;
+                for (int j = 0; j < 32; ++j) {
+                    const bool nearp = pair32(phx, phy, phz, plx, ply, plz, s0v[g + j],
+                                              s1v[g + j], s2v[g + j], tx, ty, tz);
+                    mask |= (nearp ? 1u : 0u) << j;
+                }
+                if (!live) mask = 0u;
+                while (mask) {
+                    const int j = __builtin_ctz(mask);
+                    mask &= mask - 1u;
+                    const int64_t s = base + g + j;
                     pair64(px, py, pz,
                            make_double4(a.s64[s], a.s64[S + s], a.s64[2 * S + s], a.s64[3 * S + s]),
                            make_double4(a.s64[4 * S + s], a.s64[5 * S + s], a.s64[6 * S + s],

@@ -45,7 +45,7 @@ __global__ void compact_kernel(const Slot *slots, const int *flags, const int *i
     d64[4 * cap + k] = r0x;
     d64[5 * cap + k] = r0y;
     d64[6 * cap + k] = r0z;
-    d64[7 * cap + k] = q;
+    d64[7 * cap + k] = q * q;
     const float hx = (float)s.ax, hy = (float)s.ay, hz = (float)s.az;
     d32[k] = hx;
     d32[cap + k] = hy;
@@ -58,7 +58,7 @@ __global__ void compact_kernel(const Slot *slots, const int *flags, const int *i
     d32[8 * cap + k] = (float)r0x;
     d32[9 * cap + k] = (float)r0y;
     d32[10 * cap + k] = (float)r0z;
-    d32[11 * cap + k] = 0.0f;
+    d32[11 * cap + k] = (float)(4.0 * n0);
 }
 
 __global__ void zero_count_kernel(long long *count) { *count = 0; }

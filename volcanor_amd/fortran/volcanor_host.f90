@@ -75,6 +75,7 @@ module volcanor_host
     real(dp), allocatable :: secLiftInPlane(:, :), secLiftOutPlane(:, :)
     real(dp), allocatable :: secChord(:), secArea(:), secCL(:), secCLu(:)
     real(dp), allocatable :: secAlpha(:), secMflapArm(:)
+    real(dp), allocatable :: secPhi(:), secTheta(:), secViz(:), secVix(:)
   end type blade_class
 
   type, public :: rotor_class
@@ -241,6 +242,7 @@ contains
   end function euler_gb
 
   pure function angle_tan(a, b) result(ang)
+    !! Angle between two vectors by the tangent formula, libMath.f90:224-236
     real(dp), intent(in) :: a(3), b(3)
     real(dp) :: ang, d
     d = dot_product(a, b)
@@ -712,12 +714,14 @@ contains
         allocate (b%secLiftInPlane(3, r%ns), b%secLiftOutPlane(3, r%ns))
         allocate (b%secChord(r%ns), b%secArea(r%ns), b%secCL(r%ns), b%secCLu(r%ns))
         allocate (b%secAlpha(r%ns), b%secMflapArm(r%ns))
+        allocate (b%secPhi(r%ns), b%secTheta(r%ns), b%secViz(r%ns), b%secVix(r%ns))
         b%secCP = 0._dp; b%secTauCapChord = 0._dp; b%secTauCapSpan = 0._dp
         b%secNormalVec = 0._dp; b%secChordwiseResVel = 0._dp; b%secLift = 0._dp
         b%secLiftDir = 0._dp; b%secDragDir = 0._dp; b%secForceInertial = 0._dp
         b%secLiftUnsteady = 0._dp; b%secLiftInPlane = 0._dp; b%secLiftOutPlane = 0._dp
         b%secChord = 0._dp; b%secArea = 0._dp; b%secCL = 0._dp; b%secCLu = 0._dp
         b%secAlpha = 0._dp; b%secMflapArm = 0._dp
+        b%secPhi = 0._dp; b%secTheta = 0._dp; b%secViz = 0._dp; b%secVix = 0._dp
       end associate
     enddo
 
@@ -1079,7 +1083,7 @@ contains
     real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :), v(:, :, :, :)
     real(dp) :: vtc(rotor(ir)%nc, rotor(ir)%ns), gec(rotor(ir)%nc, rotor(ir)%ns)
     real(dp) :: ges(rotor(ir)%nc, rotor(ir)%ns), q(rotor(ir)%ns)
-    real(dp) :: inv, sgn, vts
+    real(dp) :: inv, sgn, vts, vi(3)
 
     associate (r => rotor(ir))
       n = r%nc*r%ns*r%nb
@@ -1109,14 +1113,23 @@ contains
       endif
       deallocate (cp, ncap, v)
 
-      ! calc_secAlpha (classdef.f90:2234-2265), alpha only
+      ! rotor_calc_secAlpha / blade_calc_secAlpha (classdef.f90:4766-4784,
+      ! 2234-2265); the vertical axis is the blade's zAxisAziFlap
       do ib = 1, r%nbConvect
-        call blade_sec_resvel(r%blade(ib))
-        do is = 1, r%ns
-          r%blade(ib)%secAlpha(is) = atan2(dot_product(r%blade(ib)%secChordwiseResVel(:, is), &
-            & r%blade(ib)%secNormalVec(:, is)), dot_product(r%blade(ib)%secChordwiseResVel(:, is), &
-            & r%blade(ib)%secTauCapChord(:, is)))
-        enddo
+        associate (b => r%blade(ib))
+          call blade_sec_resvel(b)
+          do is = 1, r%ns
+            b%secAlpha(is) = atan2(dot_product(b%secChordwiseResVel(:, is), &
+              & b%secNormalVec(:, is)), dot_product(b%secChordwiseResVel(:, is), &
+              & b%secTauCapChord(:, is)))
+            b%secPhi(is) = angle_tan(b%secChordwiseResVel(:, is), &
+              & noproj3(b%wiP(1, is)%velCPm, b%secTauCapSpan(:, is)))
+            vi = b%secChordwiseResVel(:, is) - b%wiP(1, is)%velCPm
+            b%secViz(is) = vi(3)
+            b%secVix(is) = norm2(noproj3(vi, b%secTauCapSpan(:, is)) - [0._dp, 0._dp, vi(3)])
+            b%secTheta(is) = pi*0.5_dp - angle_tan(-1._dp*b%secTauCapChord(:, is), b%zAxisAziFlap)
+          enddo
+        end associate
       enddo
 
       r%forceInertial = 0._dp
@@ -1241,21 +1254,26 @@ contains
       & 0._dp, 0._dp, 0._dp, r%forceInertial(1)/d, r%forceInertial(2)/d, r%forceInertial(3)/d
   end subroutine force_line
 
-  subroutine dist_file(path, r)
-    !! Sectional loads table, libPostprocess.f90:853-880 (subset of columns
-    !! computed here: secSpan, secCL, secCLu, secLift, secArea, secVel,
-    !! secChord, secAlpha)
+  subroutine dist_file(path, b, hub, ns)
+    !! Sectional loads of one blade in the format of force2file's ForceDist
+    !! table (libPostprocess.f90:848-881).  Drag is zero in this load model
+    !! (forceCalcSwitch = 0: no induced or profile drag, classdef.f90:1856-1863),
+    !! so secCD and secDrag are written as 0.
     character(len=*), intent(in) :: path
-    type(rotor_class), intent(in) :: r
+    type(blade_class), intent(in) :: b
+    real(dp), intent(in) :: hub(3)
+    integer, intent(in) :: ns
     integer :: u, is
     open (newunit=u, file=path, status='replace', action='write')
-    write (u, '(8(A15))') 'secSpan', 'secCL', 'secCLu', 'secLift', 'secArea', 'secVel', &
-      & 'secChord', 'secAlpha'
-    do is = 1, r%ns
-      write (u, '(8(E15.7))') dot_product(r%blade(1)%secCP(:, is) - r%hubCoords, r%blade(1)%yAxis), &
-        & r%blade(1)%secCL(is), r%blade(1)%secCLu(is), norm2(r%blade(1)%secLift(:, is)), &
-        & r%blade(1)%secArea(is), norm2(r%blade(1)%secChordwiseResVel(:, is)), &
-        & r%blade(1)%secChord(is), r%blade(1)%secAlpha(is)*radToDeg
+    write (u, '(16(A15))') 'secSpan', 'secCL', 'secCD', 'secCLu', 'secLift', 'secDrag', &
+      & 'secLiftInPl', 'secLiftOutPl', 'secArea', 'secVel', 'secChord', 'secTheta', &
+      & 'secAlpha', 'secPhi', 'secViz', 'secVix'
+    do is = 1, ns
+      write (u, '(16(E15.7))') dot_product(b%secCP(:, is) - hub, b%yAxis), b%secCL(is), &
+        & 0._dp, b%secCLu(is), norm2(b%secLift(:, is)), 0._dp, &
+        & norm2(b%secLiftInPlane(:, is)), norm2(b%secLiftOutPlane(:, is)), b%secArea(is), &
+        & norm2(b%secChordwiseResVel(:, is)), b%secChord(is), b%secTheta(is)*radToDeg, &
+        & b%secAlpha(is)*radToDeg, b%secPhi(is)*radToDeg, b%secViz(is), b%secVix(is)
     enddo
     close (u)
   end subroutine dist_file
@@ -1274,9 +1292,10 @@ contains
     type(switches_class) :: sw
     type(rotor_class), allocatable :: rotor(:)
     type(c_ptr) :: ctx
-    integer :: nt, nr, ir, iter, rn, rf, uforce(16), nforce
+    integer :: nt, nr, ir, ib, iter, rn, rf, uforce(16), nforce
     real(dp) :: dt, density, velSound, kinematicVisc, t
-    character(len=2) :: rc2
+    character(len=2) :: rc2, bc2
+    character(len=5) :: ts5
     type(vlm_rotor_desc) :: d
     integer(c_int) :: rowNear, rowFar, dev
     integer(c_int64_t) :: c0, c1, c2, crate
@@ -1455,10 +1474,17 @@ contains
         & stats%pred_judged), 'predictor_errors')
     end block
     if (write_out) then
+      ! sectional loads of the last step, named as force2file names them
+      ! (rXXbYYForceDistNNNNN.csv, libPostprocess.f90:848-852)
+      write (ts5, '(I0.5)') nt
       do ir = 1, nr
         close (uforce(ir))
         write (rc2, '(I0.2)') ir
-        call dist_file(trim(dir)//'/Results/r'//rc2//'b01ForceDist.csv', rotor(ir))
+        do ib = 1, rotor(ir)%nbConvect
+          write (bc2, '(I0.2)') ib
+          call dist_file(trim(dir)//'/Results/r'//rc2//'b'//bc2//'ForceDist'//ts5//'.csv', &
+            & rotor(ir)%blade(ib), rotor(ir)%hubCoords, rotor(ir)%ns)
+        enddo
       enddo
     endif
     rn = 0; rf = 0
