@@ -47,8 +47,9 @@ def test_lu_factor_large_matches_lapack(n):
     assert err < 1e-10, err
 
 
-@pytest.mark.parametrize("n", [100, 600])
+@pytest.mark.parametrize("n", [1, 7, 40, 64, 65, 100, 600])
 def test_lu_solve_refine_reaches_fp64_residual(n):
+    """n <= 64 runs the one-wave solve, larger n the multi-launch path."""
     A = _matrix(n, 11 + n)
     b = np.random.default_rng(3).standard_normal(n)
     ctx = _ctx()
@@ -57,6 +58,10 @@ def test_lu_solve_refine_reaches_fp64_residual(n):
     r = b - A @ x
     assert np.max(np.abs(r)) <= 1e-13 * np.max(np.abs(b)) * n
     np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-9, atol=1e-12)
+    assert 0 <= iters <= 3
+    x0, it0 = ctx.lu_solve_refine(A, LU, piv, b, max_iter=0, tol=1e-15)
+    assert it0 == 0
+    np.testing.assert_allclose(x0, x, rtol=1e-9, atol=1e-12)
 
 
 @pytest.mark.parametrize("n", [50, 300])

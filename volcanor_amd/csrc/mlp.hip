@@ -312,6 +312,7 @@ struct Mlp {
     DBuf<float> xin, yout;
     int64_t steps = 0;
     bool ready = false;
+    bool have_loss = false;
 };
 
 static Mlp *get_mlp(vlm_ctx *ctx);
@@ -331,7 +332,7 @@ int mlp_forward_dev(vlm_ctx *ctx, int64_t m, const float *X, float *Y)
 
 // Gradient (and optionally one Adam step) on a device batch.
 int mlp_train_dev(vlm_ctx *ctx, int64_t m, const float *X, const float *T, int adam, float lr,
-                  float *loss_host)
+                  float *loss_host, int keep_loss)
 {
     Mlp *M = get_mlp(ctx);
     if (!M) return fail(ctx, VLM_ERR_ALLOC, "mlp allocation failed");
@@ -353,14 +354,24 @@ int mlp_train_dev(vlm_ctx *ctx, int64_t m, const float *X, const float *T, int a
                        M->params.p, M->part.p, (int)ntiles, M->grad.p, M->mom.p, M->var.p, adam,
                        lr, b1, b2, bc1, bc2, 1e-8f);
     VLM_HIP(hipGetLastError());
-    if (loss_host) {
+    if (loss_host || keep_loss) {
+        VLM_HIP(M->lsum.reserve(1));
         hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, ctx->stream, M->loss.p, ntiles,
                            M->lsum.p);
-        VLM_HIP(hipMemcpyAsync(loss_host, M->lsum.p, sizeof(float), hipMemcpyDeviceToHost,
-                               ctx->stream));
-        VLM_TRY(sync_stream(ctx));
+        VLM_HIP(hipGetLastError());
+        M->have_loss = true;
     }
+    if (loss_host) VLM_TRY(d2h(ctx, loss_host, M->lsum.p, sizeof(float)));
     return VLM_OK;
+}
+
+// Loss of the last training call that kept it on the device (0 if none).
+int mlp_read_loss(vlm_ctx *ctx, float *loss)
+{
+    *loss = 0.f;
+    Mlp *M = ctx->mlp;
+    if (!M || !M->have_loss) return VLM_OK;
+    return d2h(ctx, loss, M->lsum.p, sizeof(float));
 }
 
 // Deterministic small-scale initialisation (uniform +-sqrt(6/(in+out)),

@@ -162,6 +162,80 @@ __global__ void refine_check_kernel(int n, const double *r, const double *b, dou
     }
 }
 
+// The whole solve with refinement for n <= 64 in one wave: lane l holds
+// row l of x, b and r; LU and A are staged in LDS; values move between
+// lanes by shuffles.  Same operations in the same order as lu_solve_kernel,
+// residual_kernel, refine_check_kernel and axpy_kernel, so the result is
+// bitwise that of the multi-launch path.
+constexpr int SOLVE_WAVE_N = 64;
+
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+
+__device__ double wave_lu_solve(int n, const double *sLU, const int *spiv, double v)
+{
+    const int l = threadIdx.x;
+    for (int k = 0; k < n; ++k) {
+        const int p = spiv[k];
+        if (p != k) {
+            const double vk = __shfl(v, k), vp = __shfl(v, p);
+            if (l == k) v = vp;
+            else if (l == p) v = vk;
+        }
+    }
+    for (int j = 0; j < n; ++j) {
+        const double xj = __shfl(v, j);
+        if (l > j && l < n) v -= sLU[l + n * j] * xj;
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        if (l == j) v /= sLU[j + n * j];
+        const double xj = __shfl(v, j);
+        if (l < j) v -= sLU[l + n * j] * xj;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(64) void solve_wave_kernel(int n, const double *A, const double *LU,
+                                                        const int *ipiv, const double *b,
+                                                        double *x, int max_iter, double tol,
+                                                        int *st)
+{
+    __shared__ double sLU[SOLVE_WAVE_N * SOLVE_WAVE_N], sA[SOLVE_WAVE_N * SOLVE_WAVE_N];
+    __shared__ int spiv[SOLVE_WAVE_N];
+    const int l = threadIdx.x;
+    for (int k = l; k < n * n; k += 64) {
+        sLU[k] = LU[k];
+        sA[k] = A[k];
+    }
+    if (l < n) spiv[l] = ipiv[l];
+    __syncthreads();
+    const double bl = l < n ? b[l] : 0.0;
+    const double bmax = wave_max(fabs(bl));
+    double xv = wave_lu_solve(n, sLU, spiv, bl);
+    int conv = 0, corr = 0;
+    for (int it = 0; it < max_iter; ++it) {
+        double rl = 0.0;
+        if (l < n) {
+            rl = bl;
+            for (int j = 0; j < n; ++j) rl -= sA[l + n * j] * __shfl(xv, j);
+        }
+        if (wave_max(fabs(rl)) <= tol * bmax) {
+            conv = 1;
+            break;
+        }
+        ++corr;
+        xv += wave_lu_solve(n, sLU, spiv, rl);
+    }
+    if (l < n) x[l] = xv;
+    if (l == 0) {
+        st[0] = conv;
+        st[1] = corr;
+    }
+}
+
 // x = A^-1 rhs by the LU factors plus up to max_iter refinement steps on
 // the FP64 residual.  Everything is enqueued: the convergence test runs on
 // the device (later corrections are skipped once it passes), so the only
@@ -175,10 +249,16 @@ int solve_refine(vlm_ctx *ctx, int n, const double *A, const double *LU, const i
     double *b = ctx->solve_tmp.p, *x = b + n, *r = x + n;
     int *st = reinterpret_cast<int *>(r + n);
     VLM_TRY(h2d(ctx, b, rhs_h, sizeof(double) * n));
-    VLM_HIP(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
-    hipLaunchKernelGGL(solve_state_init_kernel, dim3(1), dim3(1), 0, ctx->stream, st);
-    hipLaunchKernelGGL(lu_solve_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, LU, ipiv, x,
-                       nullptr);
+    if (n <= SOLVE_WAVE_N) {
+        hipLaunchKernelGGL(solve_wave_kernel, dim3(1), dim3(64), 0, ctx->stream, n, A, LU, ipiv,
+                           b, x, max_iter, tol, st);
+        max_iter = 0;
+    } else {
+        VLM_HIP(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        hipLaunchKernelGGL(solve_state_init_kernel, dim3(1), dim3(1), 0, ctx->stream, st);
+        hipLaunchKernelGGL(lu_solve_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, LU, ipiv, x,
+                           nullptr);
+    }
     for (int it = 0; it < max_iter; ++it) {
         hipLaunchKernelGGL(residual_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
                            A, x, b, r);

@@ -296,6 +296,8 @@ int vlm_destroy(vlm_ctx *ctx)
         }
     if (ctx->prof_nseg) (void)hipHostFree(ctx->prof_nseg);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
+    if (ctx->judge_ev) (void)hipEventDestroy(ctx->judge_ev);
+    if (ctx->judge_pin) (void)hipHostFree(ctx->judge_pin);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return VLM_OK;

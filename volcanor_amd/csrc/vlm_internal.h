@@ -99,7 +99,7 @@ struct Predictor {
     double lref = 1.0;     // length scale of the neighbour offsets
     int64_t batch = 2048;  // minibatch of each Adam iteration
     int64_t steps = 0, full = 0, learned = 0, fallbacks = 0, judged = 0, train_calls = 0;
-    double last_err = 0.0, last_loss = 0.0;
+    double last_err = 0.0;
     // accumulated over judged steps: squared error of the network, of the
     // network-free substitute, and squared exact speed
     double sum_err2 = 0.0, sum_base2 = 0.0, sum_ref2 = 0.0;
@@ -162,6 +162,8 @@ struct vlm_ctx {
     vlm::DBuf<int> lu_work;   // row-move lists of the blocked LU
     vlm::Mlp *mlp = nullptr;
     vlm::Predictor pred;
+    hipEvent_t judge_ev = nullptr;  // learned pass: the judge sums reached judge_pin
+    double *judge_pin = nullptr;    // pinned, 4 doubles
 };
 
 namespace vlm {
@@ -220,7 +222,10 @@ int h2d(vlm_ctx *ctx, void *dst, const void *src, size_t bytes);
 int d2h(vlm_ctx *ctx, void *dst, const void *src, size_t bytes);
 int sync_stream(vlm_ctx *ctx);
 int mlp_forward_dev(vlm_ctx *ctx, int64_t m, const float *X, float *Y);
+// Training step on a device batch; loss_host downloads the loss (and
+// synchronises), keep_loss only sums it on the device for mlp_read_loss.
 int mlp_train_dev(vlm_ctx *ctx, int64_t m, const float *X, const float *T, int adam, float lr,
-                  float *loss_host);
+                  float *loss_host, int keep_loss = 0);
+int mlp_read_loss(vlm_ctx *ctx, float *loss);
 
 }  // namespace vlm

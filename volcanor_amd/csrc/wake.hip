@@ -1019,15 +1019,15 @@ __global__ void gather_step_vel_kernel(RV v, double *o, int64_t stride, int64_t 
 }
 
 // vref = RMS of the pass-1 speed over all targets (single block).
-__global__ void vref_kernel(const double *vn, int64_t m, double *vref)
+__global__ __launch_bounds__(1024) void vref_kernel(const double *vn, int64_t m, double *vref)
 {
-    __shared__ double sh[256];
+    __shared__ double sh[1024];
     double s = 0.0;
     for (int64_t k = threadIdx.x; k < m; k += blockDim.x)
         s += vn[k] * vn[k] + vn[m + k] * vn[m + k] + vn[2 * m + k] * vn[2 * m + k];
     sh[threadIdx.x] = s;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = 512; w > 0; w >>= 1) {
         if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
@@ -1130,6 +1130,24 @@ __global__ void full_pairs_kernel(const double *vexact, const double *vn, int64_
 
 }  // namespace
 
+// Full Biot-Savart pass 2 of the learned mode, plus training pairs from it
+// (every stride-th target, a quarter of the replay buffer).
+static int full_pass(vlm_ctx *ctx, int64_t m, int64_t nmax)
+{
+    Predictor &P = ctx->pred;
+    VLM_TRY(eval_sources(ctx, ctx->tgt.p, m, m, ctx->out.p, 0, nmax));
+    P.full += 1;
+    const int64_t stride = (m + P.buf_cap / 4 - 1) / (P.buf_cap / 4);
+    const int64_t np = (m + stride - 1) / stride;
+    hipLaunchKernelGGL(full_pairs_kernel, dim3(nblk(np)), dim3(256), 0, ctx->stream, ctx->out.p,
+                       P.vn.p, m, P.X.p, P.scal.p, stride, np, P.bx.p, P.bt.p, P.buf_cap,
+                       P.buf_head);
+    VLM_HIP(hipGetLastError());
+    P.buf_head = (P.buf_head + np) % P.buf_cap;
+    P.buf_n = P.buf_n + np < P.buf_cap ? P.buf_n + np : P.buf_cap;
+    return VLM_OK;
+}
+
 int learned_pass(vlm_ctx *ctx)
 {
     Predictor &P = ctx->pred;
@@ -1163,7 +1181,7 @@ int learned_pass(vlm_ctx *ctx)
                            view(*rp), P.vn.p, m, off);
         off += n;
     }
-    hipLaunchKernelGGL(vref_kernel, dim3(1), dim3(256), 0, ctx->stream, P.vn.p, m, P.scal.p);
+    hipLaunchKernelGGL(vref_kernel, dim3(1), dim3(1024), 0, ctx->stream, P.vn.p, m, P.scal.p);
     off = 0;
     for (auto *rp : ctx->rotors) {
         const int64_t n = target_count(*rp);
@@ -1176,14 +1194,16 @@ int learned_pass(vlm_ctx *ctx)
     const bool warm = P.steps < P.warmup;
     P.steps += 1;
     bool full = warm;
+    int64_t ms = 0, phase = 0, stride = 1;
+    const double *judged_sums = nullptr;
     if (!warm) {
         VLM_TRY(mlp_forward_dev(ctx, m, P.X.p, P.Y.p));
         hipLaunchKernelGGL(apply_mlp_kernel, dim3(nblk(m)), dim3(256), 0, ctx->stream, P.vn.p,
                            P.Y.p, m, P.scal.p, ctx->out.p);
         // judge on a strided sample with a rotating phase
-        const int64_t stride = P.stride < 1 ? 1 : P.stride;
-        const int64_t phase = (P.steps * 7) % stride < m ? (P.steps * 7) % stride : 0;
-        const int64_t ms = (m - phase + stride - 1) / stride;
+        stride = P.stride < 1 ? 1 : P.stride;
+        phase = (P.steps * 7) % stride < m ? (P.steps * 7) % stride : 0;
+        ms = (m - phase + stride - 1) / stride;
         VLM_HIP(P.ts.reserve((size_t)3 * ms));
         VLM_HIP(P.vs.reserve((size_t)3 * ms));
         VLM_HIP(P.e2.reserve((size_t)3 * ms));
@@ -1197,36 +1217,51 @@ int learned_pass(vlm_ctx *ctx)
                            P.buf_head);
         hipLaunchKernelGGL(sum3_kernel, dim3(1), dim3(256), 0, ctx->stream, P.e2.p, take,
                            P.scal.p + 1);
-        double h[3] = {0, 0, 0};
-        VLM_TRY(d2h(ctx, h, P.scal.p + 1, 3 * sizeof(double)));
+        VLM_HIP(hipGetLastError());
+        // the sums travel to pinned memory behind an event; the host waits
+        // for them only after the training below is enqueued, so the GPU
+        // trains while the host decides
+        if (!ctx->judge_ev) {
+            VLM_HIP(hipEventCreateWithFlags(&ctx->judge_ev, hipEventDisableTiming));
+            VLM_HIP(hipHostMalloc(reinterpret_cast<void **>(&ctx->judge_pin), 4 * sizeof(double)));
+        }
+        VLM_HIP(hipMemcpyAsync(ctx->judge_pin, P.scal.p + 1, 3 * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        VLM_HIP(hipEventRecord(ctx->judge_ev, ctx->stream));
+        judged_sums = ctx->judge_pin;
         P.buf_head = (P.buf_head + take) % P.buf_cap;
         P.buf_n = P.buf_n + take < P.buf_cap ? P.buf_n + take : P.buf_cap;
-        P.last_err = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
-        P.sum_err2 += h[0];
-        P.sum_ref2 += h[1];
-        P.sum_base2 += h[2];
+    } else {
+        VLM_TRY(full_pass(ctx, m, nmax));
+    }
+    // Adam on minibatches: contiguous windows of the replay buffer at a
+    // rotating offset (on a judged step before the decision, so the pairs
+    // of a fallback pass are trained on from the next step)
+    for (int it = 0; it < P.train_iters && P.buf_n > 0; ++it) {
+        const int64_t B = P.buf_n < P.batch ? P.buf_n : P.batch;
+        const int64_t start = (P.train_calls * 7919) % (P.buf_n - B + 1);
+        P.train_calls += 1;
+        VLM_TRY(mlp_train_dev(ctx, B, P.bx.p + 45 * start, P.bt.p + 3 * start, 1, P.lr, nullptr,
+                              it == P.train_iters - 1));
+    }
+    if (judged_sums) {
+        VLM_HIP(hipEventSynchronize(ctx->judge_ev));
+        const double h0 = judged_sums[0], h1 = judged_sums[1], h2 = judged_sums[2];
+        P.last_err = h1 > 0 ? std::sqrt(h0 / h1) : 0.0;
+        P.sum_err2 += h0;
+        P.sum_ref2 += h1;
+        P.sum_base2 += h2;
         P.judged += 1;
         if (P.last_err > P.tol) {
             full = true;
             P.fallbacks += 1;
+            VLM_TRY(full_pass(ctx, m, nmax));
         } else {
             hipLaunchKernelGGL(put_sample_kernel, dim3(nblk(ms)), dim3(256), 0, ctx->stream,
                                P.vs.p, ms, ctx->out.p, m, phase, stride);
             VLM_HIP(hipGetLastError());
             P.learned += 1;
         }
-    }
-    if (full) {
-        VLM_TRY(eval_sources(ctx, ctx->tgt.p, m, m, ctx->out.p, 0, nmax));
-        P.full += 1;
-        const int64_t stride = (m + P.buf_cap / 4 - 1) / (P.buf_cap / 4);
-        const int64_t np = (m + stride - 1) / stride;
-        hipLaunchKernelGGL(full_pairs_kernel, dim3(nblk(np)), dim3(256), 0, ctx->stream,
-                           ctx->out.p, P.vn.p, m, P.X.p, P.scal.p, stride, np, P.bx.p, P.bt.p,
-                           P.buf_cap, P.buf_head);
-        VLM_HIP(hipGetLastError());
-        P.buf_head = (P.buf_head + np) % P.buf_cap;
-        P.buf_n = P.buf_n + np < P.buf_cap ? P.buf_n + np : P.buf_cap;
     }
     off = 0;
     for (auto *rp : ctx->rotors) {
@@ -1237,17 +1272,6 @@ int learned_pass(vlm_ctx *ctx)
         off += n;
     }
     VLM_HIP(hipGetLastError());
-    // Adam on minibatches: contiguous windows of the replay buffer at a
-    // rotating offset
-    float loss = 0.f;
-    for (int it = 0; it < P.train_iters && P.buf_n > 0; ++it) {
-        const int64_t B = P.buf_n < P.batch ? P.buf_n : P.batch;
-        const int64_t start = (P.train_calls * 7919) % (P.buf_n - B + 1);
-        P.train_calls += 1;
-        VLM_TRY(mlp_train_dev(ctx, B, P.bx.p + 45 * start, P.bt.p + 3 * start, 1, P.lr,
-                              it == P.train_iters - 1 ? &loss : nullptr));
-    }
-    P.last_loss = loss;
     return VLM_OK;
 }
 
@@ -1753,7 +1777,11 @@ int vlm_predictor_stats(vlm_ctx *ctx, int64_t *full, int64_t *learned, int64_t *
     if (learned) *learned = P.learned;
     if (fallbacks) *fallbacks = P.fallbacks;
     if (last_err) *last_err = P.last_err;
-    if (last_loss) *last_loss = P.last_loss;
+    if (last_loss) {
+        float l = 0.f;
+        VLM_TRY(mlp_read_loss(ctx, &l));
+        *last_loss = l;
+    }
     return VLM_OK;
 }
 
