@@ -152,6 +152,23 @@ int vlm_rotor_convectwake(vlm_ctx *ctx, int ir, double dt,
 int vlm_rotor_induced_velocity(vlm_ctx *ctx, int ir, int set, int64_t m,
                                const double *p, double *v, int accumulate);
 
+/* A sum of source sets at the same m host points, applied in order on the
+ * device with one upload and one download:
+ *   u = accumulate ? v : 0;
+ *   for k < nterms: if (flags[k] & VLM_TERM_NEGATE) u = -u;
+ *                   u += velocity of set[k] of rotor ir[k];
+ *   v = u.
+ * Same arithmetic, in the same order, as one vlm_rotor_induced_velocity
+ * call per term with accumulate set (negating v on the host where flagged),
+ * without a synchronisation per term.  Replaces the sums over rotors of
+ * rotor%vind_bywake / vind_bywing at the collocation points (RHS,
+ * main.f90:549-561) and the bound-vortex and wing sums of the loads
+ * (classdef.f90:1376-1418, 1734). */
+#define VLM_TERM_NEGATE 1
+int vlm_induced_velocity_terms(vlm_ctx *ctx, int nterms, const int *ir, const int *set,
+                               const int *flags, int64_t m, const double *p, double *v,
+                               int accumulate);
+
 /* The N-body pass: velNwake/velFwake (wakeType C) or velNwakePredicted /
  * velFwakePredicted (P) of every rotor = sum over all rotors of wing +
  * wake induced velocity (libCommon.f90:114-211, main.f90:800-841 and

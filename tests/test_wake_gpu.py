@@ -102,6 +102,36 @@ def test_induced_velocity_sets_match_oracle():
     np.testing.assert_allclose(a2, 1.0 + ref.rotor_induced_velocity(ib[0], 2, p), rtol=1e-11)
 
 
+def test_induced_velocity_terms_equal_sequential_calls():
+    """vlm_induced_velocity_terms against one vlm_rotor_induced_velocity call
+    per term with the negation done on the host: bitwise equal, for a
+    target count on the small-target path (37) and on the work-queue path
+    (1500), with and without accumulate."""
+    case = H.Case(**CASES["rotor_am2"])
+    dev = _ctx()
+    ia = H.setup(dev, case)
+    for it in range(1, 9):
+        H.step(dev, case, ia, it)
+    r = ia[0]
+    terms = [(r, 2, False), (r, 1, False), (r, 0, True), (r, 2, True)]
+    rng = np.random.default_rng(5)
+    for m in (37, 1500):
+        p = rng.normal(size=(m, 3)) * 0.8
+        for v0 in (None, rng.normal(size=(m, 3))):
+            seq = None if v0 is None else v0.copy()
+            for ir, st, neg in terms:
+                if neg:
+                    seq = -seq
+                seq = dev.rotor_induced_velocity(ir, st, p, v=seq if seq is not None else np.zeros_like(p))
+            got = dev.induced_velocity_terms(terms, p, v=v0)
+            assert np.array_equal(got, seq), (m, v0 is None, np.max(np.abs(got - seq)))
+    assert np.array_equal(dev.induced_velocity_terms([], p), np.zeros_like(p))
+    with pytest.raises(Exception):
+        dev.induced_velocity_terms([(r + 7, 2, False)], p)
+    with pytest.raises(Exception):
+        dev.induced_velocity_terms([(r, 9, False)], p)
+
+
 def test_set_wake_roundtrip_and_errors():
     from volcanor_amd import vlm
     case = H.Case(**CASES["wing_am2"])

@@ -354,16 +354,31 @@ __global__ __launch_bounds__(SM_BLOCK) void bs_small_kernel(const double *s64, i
     }
 }
 
-__global__ void bs_reduce_kernel(const double *part, int nchunks, int64_t m, double *out,
-                                 int64_t ostride, int acc)
+// Fixed-order sum of the per-chunk partials: one lane per (target,
+// component), chunks added in increasing order; loads are issued eight
+// chunks ahead of the adds.
+constexpr int RED_BLOCK = 64;
+
+__global__ __launch_bounds__(RED_BLOCK) void bs_reduce_kernel(const double *part, int nchunks,
+                                                              int64_t m, double *out,
+                                                              int64_t ostride, int acc)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x;
+    const int d = blockIdx.y;
     if (t >= m) return;
-    for (int d = 0; d < 3; ++d) {
-        double s = acc ? out[d * ostride + t] : 0.0;
-        for (int c = 0; c < nchunks; ++c) s += part[((size_t)c * 3 + d) * m + t];
-        out[d * ostride + t] = s;
+    const double *p = part + (size_t)d * m + t;
+    const size_t cs = 3 * (size_t)m;
+    double s = acc ? out[d * ostride + t] : 0.0;
+    int c = 0;
+    for (; c + 8 <= nchunks; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = p[(size_t)(c + k) * cs];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += v[k];
     }
+    for (; c < nchunks; ++c) s += p[(size_t)c * cs];
+    out[d * ostride + t] = s;
 }
 
 __global__ void zero_int_kernel(int *p) { *p = 0; }
@@ -403,7 +418,8 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
                                ctx->part.p);
             VLM_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + RED_BLOCK - 1) / RED_BLOCK), 3),
+                           dim3(RED_BLOCK), 0,
                            ctx->stream, ctx->part.p, nchunks, m, out, tstride, accumulate);
         VLM_HIP(hipGetLastError());
         return VLM_OK;
@@ -474,7 +490,8 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
             }
         }
     }
-    hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + RED_BLOCK - 1) / RED_BLOCK), 3),
+                       dim3(RED_BLOCK), 0,
                        ctx->stream, ctx->part.p, nchunks, m, out, tstride, accumulate);
     VLM_HIP(hipGetLastError());
     return VLM_OK;

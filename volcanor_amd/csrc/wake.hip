@@ -695,6 +695,12 @@ __global__ void copy_window_kernel(const double *src, double *dst, int nbc, int 
     dst[k] = src[k];
 }
 
+__global__ void negate_kernel(double *p, int64_t n)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] = -p[e];
+}
+
 __global__ void fill_kernel(double *p, int64_t n, double val)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1542,6 +1548,44 @@ int vlm_rotor_induced_velocity(vlm_ctx *ctx, int ir, int set, int64_t m, const d
     VLM_TRY(points_to_device(ctx, m, p, ctx->tgt.p, m));
     if (accumulate) VLM_TRY(points_to_device(ctx, m, v, ctx->out.p, m));
     VLM_TRY(eval_sources(ctx, ctx->tgt.p, m, m, ctx->out.p, accumulate, nmax));
+    return points_to_host(ctx, m, ctx->out.p, m, v);
+}
+
+int vlm_induced_velocity_terms(vlm_ctx *ctx, int nterms, const int *ir, const int *set,
+                               const int *flags, int64_t m, const double *p, double *v,
+                               int accumulate)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    if (nterms < 0 || (nterms > 0 && (!ir || !set || !flags)))
+        return fail(ctx, VLM_ERR_ARG, "bad term list");
+    for (int k = 0; k < nterms; ++k) {
+        if (!get_rotor(ctx, ir[k])) return fail(ctx, VLM_ERR_ARG, "unknown rotor");
+        if (set[k] < VLM_SET_WING || set[k] > VLM_SET_WAKE_PRED)
+            return fail(ctx, VLM_ERR_ARG, "bad set");
+    }
+    if (m <= 0) return VLM_OK;
+    if (!p || !v) return fail(ctx, VLM_ERR_ARG, "null buffer");
+    VLM_HIP(ctx->tgt.reserve((size_t)3 * m));
+    VLM_HIP(ctx->out.reserve((size_t)3 * m));
+    VLM_TRY(points_to_device(ctx, m, p, ctx->tgt.p, m));
+    if (accumulate) {
+        VLM_TRY(points_to_device(ctx, m, v, ctx->out.p, m));
+    } else {
+        hipLaunchKernelGGL(fill_kernel, dim3(nblk(3 * m)), dim3(256), 0, ctx->stream,
+                           ctx->out.p, 3 * m, 0.0);
+        VLM_HIP(hipGetLastError());
+    }
+    for (int k = 0; k < nterms; ++k) {
+        if (flags[k] & VLM_TERM_NEGATE) {
+            hipLaunchKernelGGL(negate_kernel, dim3(nblk(3 * m)), dim3(256), 0, ctx->stream,
+                               ctx->out.p, 3 * m);
+            VLM_HIP(hipGetLastError());
+        }
+        std::vector<std::pair<int, int>> src = {{ir[k], set[k]}};
+        const int64_t nmax = staged_upper_bound(ctx, src);
+        VLM_TRY(build_sources(ctx, src));
+        VLM_TRY(eval_sources(ctx, ctx->tgt.p, m, m, ctx->out.p, 1, nmax));
+    }
     return points_to_host(ctx, m, ctx->out.p, m, v);
 }
 

@@ -14,6 +14,7 @@ module vlm_c
   integer(c_int), parameter :: VLM_SET_WAKE = 2, VLM_SET_WAKE_PRED = 3
   integer(c_int), parameter :: VLM_WAKE_C = 0, VLM_WAKE_P = 1
   integer(c_int), parameter :: VLM_EDGE_LE = 0, VLM_EDGE_TE = 1
+  integer(c_int), parameter :: VLM_TERM_NEGATE = 1
   integer(c_int), parameter :: VLM_VEL = 0, VLM_VEL_PREV = 1
   integer(c_int), parameter :: VLM_VEL_PRED = 2, VLM_VEL_STEP = 3
 
@@ -144,6 +145,18 @@ module vlm_c
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: ctx
       integer(c_int), value :: ir, set
+      integer(c_int64_t), value :: m
+      real(c_double), intent(in) :: p(*)
+      real(c_double), intent(inout) :: v(*)
+      integer(c_int), value :: acc
+    end function
+
+    integer(c_int) function vlm_induced_velocity_terms(ctx, nterms, ir, set, flags, m, p, v, &
+        & acc) bind(C, name='vlm_induced_velocity_terms')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nterms
+      integer(c_int), intent(in) :: ir(*), set(*), flags(*)
       integer(c_int64_t), value :: m
       real(c_double), intent(in) :: p(*)
       real(c_double), intent(inout) :: v(*)

@@ -986,7 +986,8 @@ contains
     type(rotor_class), intent(inout) :: rotor(:)
     integer, intent(in) :: nr
     logical, intent(in) :: with_wake
-    integer :: ir, jr, ib, i, j, n, row, it
+    integer :: ir, jr, ib, i, j, n, row, it, nt
+    integer(c_int) :: tir(2*nr), tset(2*nr), tfl(2*nr)
     real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :), v(:, :, :, :), gam(:)
 
     do ir = 1, nr
@@ -1006,13 +1007,19 @@ contains
             enddo
           enddo
         enddo
+        ! sum over rotors of wake and (other) wing velocities, main.f90:549-561,
+        ! in the reference's order, as one device-side sequence
+        nt = 0
         do jr = 1, nr
-          if (with_wake .and. rotor(jr)%nNwake > 0) &
-            & call vlm_check(ctx, vlm_rotor_induced_velocity(ctx, rotor(jr)%dev, VLM_SET_WAKE, &
-            & int(n, c_int64_t), cp, v, 1_c_int), 'RHS wake')
-          if (jr /= ir) call vlm_check(ctx, vlm_rotor_induced_velocity(ctx, rotor(jr)%dev, &
-            & VLM_SET_WING, int(n, c_int64_t), cp, v, 1_c_int), 'RHS wing')
+          if (with_wake .and. rotor(jr)%nNwake > 0) then
+            nt = nt + 1; tir(nt) = rotor(jr)%dev; tset(nt) = VLM_SET_WAKE; tfl(nt) = 0
+          endif
+          if (jr /= ir) then
+            nt = nt + 1; tir(nt) = rotor(jr)%dev; tset(nt) = VLM_SET_WING; tfl(nt) = 0
+          endif
         enddo
+        call vlm_check(ctx, vlm_induced_velocity_terms(ctx, int(nt, c_int), tir, tset, tfl, &
+          & int(n, c_int64_t), cp, v, 0_c_int), 'RHS velocity')
         r%RHS = 0._dp
         do ib = 1, r%nbConvect
           do j = 1, r%ns
@@ -1080,6 +1087,7 @@ contains
     integer, intent(in) :: nr, ir
     real(dp), intent(in) :: density, dt
     integer :: jr, ib, i, j, is, ic, n
+    integer(c_int) :: tir(nr + 1), tset(nr + 1), tfl(nr + 1)
     real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :), v(:, :, :, :)
     real(dp) :: vtc(rotor(ir)%nc, rotor(ir)%ns), gec(rotor(ir)%nc, rotor(ir)%ns)
     real(dp) :: ges(rotor(ir)%nc, rotor(ir)%ns), q(rotor(ir)%ns)
@@ -1089,14 +1097,14 @@ contains
       n = r%nc*r%ns*r%nb
       allocate (cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb), v(3, r%nc, r%ns, r%nb))
       call rotor_cp_arrays(r, cp, ncap)
-      v = 0._dp
+      ! velocity of the wing's own rings less every rotor's bound vortices
+      ! (classdef.f90:1376-1418, 1734), as one device-side sequence
       do jr = 1, nr
-        call vlm_check(ctx, vlm_rotor_induced_velocity(ctx, rotor(jr)%dev, VLM_SET_WING_BOUND, &
-          & int(n, c_int64_t), cp, v, 1_c_int), 'bound vortices')
+        tir(jr) = rotor(jr)%dev; tset(jr) = VLM_SET_WING_BOUND; tfl(jr) = 0
       enddo
-      v = -v
-      call vlm_check(ctx, vlm_rotor_induced_velocity(ctx, r%dev, VLM_SET_WING, &
-        & int(n, c_int64_t), cp, v, 1_c_int), 'wing self')
+      tir(nr + 1) = r%dev; tset(nr + 1) = VLM_SET_WING; tfl(nr + 1) = VLM_TERM_NEGATE
+      call vlm_check(ctx, vlm_induced_velocity_terms(ctx, int(nr + 1, c_int), tir, tset, tfl, &
+        & int(n, c_int64_t), cp, v, 0_c_int), 'load velocity')
       do ib = 1, r%nbConvect
         do j = 1, r%ns
           do i = 1, r%nc

@@ -19,6 +19,7 @@ FP64, MIXED = 0, 1
 SET_WING, SET_WING_BOUND, SET_WAKE, SET_WAKE_PRED = 0, 1, 2, 3
 WAKE_C, WAKE_P = 0, 1
 EDGE_LE, EDGE_TE = 0, 1
+TERM_NEGATE = 1
 SCHEME_FT1, SCHEME_PC2, SCHEME_AB2, SCHEME_AM2 = 0, 1, 2, 3
 VEL, VEL_PREV, VEL_PRED, VEL_STEP = 0, 1, 2, 3
 
@@ -65,6 +66,7 @@ _SIGS = {
     "vlm_rotor_rollup": [_P, _I],
     "vlm_rotor_convectwake": [_P, _I, _D, _I],
     "vlm_rotor_induced_velocity": [_P, _I, _I, _L, _P, _P, _I],
+    "vlm_induced_velocity_terms": [_P, _I, _P, _P, _P, _L, _P, _P, _I],
     "vlm_wake_velocity": [_P, _I],
     "vlm_convect": [_P, _I, _I, _D],
     "vlm_rotor_get_wake": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P],
@@ -250,6 +252,19 @@ class Context:
         out = _f64(v, (-1, 3)).copy() if acc else np.zeros_like(p)
         self._chk(self.lib.vlm_rotor_induced_velocity(self.h, ir, sset, p.shape[0], _ptr(p),
                                                       _ptr(out), int(acc)))
+        return out
+
+    def induced_velocity_terms(self, terms, p, v=None):
+        """terms: list of (ir, set, negate_before); see vlm_induced_velocity_terms."""
+        p = _f64(p, (-1, 3))
+        acc = v is not None
+        out = _f64(v, (-1, 3)).copy() if acc else np.zeros_like(p)
+        ir = np.array([t[0] for t in terms], dtype=np.int32)
+        st = np.array([t[1] for t in terms], dtype=np.int32)
+        fl = np.array([TERM_NEGATE if t[2] else 0 for t in terms], dtype=np.int32)
+        self._chk(self.lib.vlm_induced_velocity_terms(self.h, len(terms), _ptr(ir), _ptr(st),
+                                                      _ptr(fl), p.shape[0], _ptr(p), _ptr(out),
+                                                      int(acc)))
         return out
 
     def wake_velocity(self, wake_type=WAKE_C):
