@@ -208,6 +208,23 @@ int vlm_rotor_calc_aic(vlm_ctx *ctx, int ir, const double *cp,
 int vlm_rotor_solve(vlm_ctx *ctx, int ir, const double *rhs, double *gam,
                     int max_iter, double tol, int *iters);
 
+/* ---- deferred results ---------------------------------------------------
+ * The _deferred variants enqueue the same work as their plain counterparts
+ * and return at once; their host outputs (gam and *iters, v) are written
+ * by vlm_wait, which waits only for the device work those results need,
+ * or by any call that synchronises the stream (including vlm_destroy).
+ * The output buffers must stay valid and unread until then; host inputs
+ * are copied at the call.  A caller enqueues the wake passes in between and
+ * computes loads while they run (main.f90 computes loads between the solve
+ * and the convection; they depend only on the wing and its circulation,
+ * which the convection does not change).  No reference counterpart. */
+int vlm_rotor_solve_deferred(vlm_ctx *ctx, int ir, const double *rhs, double *gam,
+                             int max_iter, double tol, int *iters);
+int vlm_induced_velocity_terms_deferred(vlm_ctx *ctx, int nterms, const int *ir,
+                                        const int *set, const int *flags, int64_t m,
+                                        const double *p, double *v, int accumulate);
+int vlm_wait(vlm_ctx *ctx);
+
 /* Dense kernels on explicit matrices (the LU microbenchmark and tests):
  * factorise an n x n column-major matrix in place (ipiv 0-based), and
  * solve with refinement against the unfactorised A. */

@@ -132,6 +132,36 @@ def test_induced_velocity_terms_equal_sequential_calls():
         dev.induced_velocity_terms([(r, 9, False)], p)
 
 
+def test_deferred_results_equal_plain_calls():
+    """vlm_rotor_solve_deferred and vlm_induced_velocity_terms_deferred give
+    the plain calls' results once vlm_wait returns, with device work
+    enqueued in between; a synchronising call also completes them."""
+    case = H.Case(**CASES["rotor_am2"])
+    dev = _ctx()
+    ia = H.setup(dev, case)
+    for it in range(1, 6):
+        H.step(dev, case, ia, it)
+    r = ia[0]
+    terms = [(r, 1, False), (r, 0, True)]
+    p = np.random.default_rng(9).normal(size=(1200, 3)) * 0.8
+    n = case.nb * case.nc * case.ns
+    cp = np.random.default_rng(11).normal(size=(n, 3))
+    ncap = np.random.default_rng(12).normal(size=(n, 3))
+    ncap /= np.linalg.norm(ncap, axis=1)[:, None]
+    dev.rotor_calc_aic(r, cp, ncap, n)
+    rhs = np.random.default_rng(10).normal(size=n)
+    g0, it0 = dev.rotor_solve(r, rhs)    # also sets the wing circulation
+    plain = dev.induced_velocity_terms(terms, p)
+    v = dev.induced_velocity_terms_deferred(terms, p)
+    g, it = dev.rotor_solve_deferred(r, rhs)
+    dev.convect(3, 6, case.dt)           # device work after the deferred copies
+    dev.wait()
+    assert np.array_equal(v, plain) and np.array_equal(g, g0) and int(it[0]) == it0
+    v2 = dev.induced_velocity_terms_deferred(terms, p)
+    dev.synchronize()
+    assert np.array_equal(v2, dev.induced_velocity_terms(terms, p))
+
+
 def test_set_wake_roundtrip_and_errors():
     from volcanor_amd import vlm
     case = H.Case(**CASES["wing_am2"])

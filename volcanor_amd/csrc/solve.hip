@@ -243,7 +243,7 @@ __global__ __launch_bounds__(64) void solve_wave_kernel(int n, const double *A, 
 // also becomes its device wing circulation (rotor_map_gam).
 int solve_refine(vlm_ctx *ctx, int n, const double *A, const double *LU, const int *ipiv,
                  const double *rhs_h, double *x_h, int max_iter, double tol, int *iters,
-                 Rotor *rotor = nullptr)
+                 Rotor *rotor = nullptr, bool deferred = false)
 {
     VLM_HIP(ctx->solve_tmp.reserve((size_t)3 * n + 1));
     double *b = ctx->solve_tmp.p, *x = b + n, *r = x + n;
@@ -275,6 +275,11 @@ int solve_refine(vlm_ctx *ctx, int n, const double *A, const double *LU, const i
         hipLaunchKernelGGL(map_gam_kernel, dim3((per * rotor->d.nb + 255) / 256), dim3(256), 0,
                            ctx->stream, x, rotor->wg.p, per, rotor->d.nb, rotor->d.nbConvect);
         VLM_HIP(hipGetLastError());
+    }
+    if (deferred) {
+        VLM_TRY(d2h_deferred(ctx, x_h, x, sizeof(double) * n));
+        if (iters) VLM_TRY(d2h_deferred(ctx, iters, st + 1, sizeof(int)));
+        return VLM_OK;
     }
     char *px = nullptr, *ps = nullptr;
     VLM_TRY(pin_take(ctx, sizeof(double) * n, &px));
@@ -342,6 +347,17 @@ int vlm_rotor_solve(vlm_ctx *ctx, int ir, const double *rhs, double *gam, int ma
     const int n = ctx->aic_n[ir];
     return solve_refine(ctx, n, ctx->aic[ir], ctx->lu[ir], ctx->piv[ir], rhs, gam, max_iter, tol,
                         iters, ctx->rotors[ir]);
+}
+
+int vlm_rotor_solve_deferred(vlm_ctx *ctx, int ir, const double *rhs, double *gam, int max_iter,
+                             double tol, int *iters)
+{
+    if (!ctx || ir < 0 || ir >= (int)ctx->rotors.size()) return VLM_ERR_ARG;
+    if (!rhs || !gam) return fail(ctx, VLM_ERR_ARG, "null buffer");
+    if (!ctx->lu[ir]) return fail(ctx, VLM_ERR_STATE, "vlm_rotor_calc_aic not called");
+    const int n = ctx->aic_n[ir];
+    return solve_refine(ctx, n, ctx->aic[ir], ctx->lu[ir], ctx->piv[ir], rhs, gam, max_iter, tol,
+                        iters, ctx->rotors[ir], true);
 }
 
 int vlm_lu_factor(vlm_ctx *ctx, int n, double *A, int *ipiv)

@@ -178,10 +178,39 @@ int upload_fset(vlm_ctx *ctx, int64_t nfil, const double *a, const double *b,
 
 }  // namespace
 
+static void flush_deferred(vlm_ctx *ctx)
+{
+    for (const auto &d : ctx->deferred) std::memcpy(d.dst, d.src, d.bytes);
+    ctx->deferred.clear();
+}
+
 int sync_stream(vlm_ctx *ctx)
 {
     VLM_HIP(hipStreamSynchronize(ctx->stream));
+    flush_deferred(ctx);
     ctx->pin_off = 0;
+    return VLM_OK;
+}
+
+int d2h_deferred(vlm_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return VLM_OK;
+    char *p = nullptr;
+    VLM_TRY(pin_take(ctx, bytes, &p));
+    VLM_HIP(hipMemcpyAsync(p, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (!ctx->defer_ev) VLM_HIP(hipEventCreateWithFlags(&ctx->defer_ev, hipEventDisableTiming));
+    VLM_HIP(hipEventRecord(ctx->defer_ev, ctx->stream));
+    ctx->deferred.push_back({dst, p, bytes});
+    return VLM_OK;
+}
+
+// Waits only for the work enqueued up to the last deferred copy; the arena
+// is not recycled here because later work on the stream may still use it.
+int wait_deferred(vlm_ctx *ctx)
+{
+    if (ctx->deferred.empty()) return VLM_OK;
+    VLM_HIP(hipEventSynchronize(ctx->defer_ev));
+    flush_deferred(ctx);
     return VLM_OK;
 }
 
@@ -238,12 +267,14 @@ int points_to_device(vlm_ctx *ctx, int64_t m, const double *p, double *soa, int6
     return VLM_OK;
 }
 
-int points_to_host(vlm_ctx *ctx, int64_t m, const double *soa, int64_t stride, double *p)
+int points_to_host(vlm_ctx *ctx, int64_t m, const double *soa, int64_t stride, double *p,
+                   bool deferred = false)
 {
     VLM_HIP(ctx->hostio.reserve((size_t)3 * m));
     hipLaunchKernelGGL(soa_to_aos_kernel, dim3(nblk(m)), dim3(256), 0, ctx->stream, m, soa,
                        stride, ctx->hostio.p);
     VLM_HIP(hipGetLastError());
+    if (deferred) return d2h_deferred(ctx, p, ctx->hostio.p, sizeof(double) * 3 * m);
     return d2h(ctx, p, ctx->hostio.p, sizeof(double) * 3 * m);
 }
 
@@ -282,6 +313,7 @@ int vlm_destroy(vlm_ctx *ctx)
     if (!ctx) return VLM_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    flush_deferred(ctx);
     mlp_release(ctx);
     for (auto *r : ctx->rotors) delete r;
     for (auto *f : ctx->fsets) delete f;
@@ -297,6 +329,7 @@ int vlm_destroy(vlm_ctx *ctx)
     if (ctx->prof_nseg) (void)hipHostFree(ctx->prof_nseg);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->judge_ev) (void)hipEventDestroy(ctx->judge_ev);
+    if (ctx->defer_ev) (void)hipEventDestroy(ctx->defer_ev);
     if (ctx->judge_pin) (void)hipHostFree(ctx->judge_pin);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -330,6 +363,12 @@ int vlm_synchronize(vlm_ctx *ctx)
     if (!ctx) return VLM_ERR_ARG;
     VLM_TRY(sync_stream(ctx));
     return VLM_OK;
+}
+
+int vlm_wait(vlm_ctx *ctx)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    return wait_deferred(ctx);
 }
 
 int vlm_filament_vind(vlm_ctx *ctx, int64_t nfil, const double *a, const double *b,

@@ -164,6 +164,15 @@ struct vlm_ctx {
     vlm::Predictor pred;
     hipEvent_t judge_ev = nullptr;  // learned pass: the judge sums reached judge_pin
     double *judge_pin = nullptr;    // pinned, 4 doubles
+    // deferred downloads: device -> pinned arena copies already enqueued,
+    // copied to the caller's buffers at vlm_wait or at any stream sync
+    struct Deferred {
+        void *dst;
+        const char *src;
+        size_t bytes;
+    };
+    std::vector<Deferred> deferred;
+    hipEvent_t defer_ev = nullptr;  // recorded after the last deferred copy
 };
 
 namespace vlm {
@@ -221,6 +230,10 @@ int pin_take(vlm_ctx *ctx, size_t bytes, char **out);
 int h2d(vlm_ctx *ctx, void *dst, const void *src, size_t bytes);
 int d2h(vlm_ctx *ctx, void *dst, const void *src, size_t bytes);
 int sync_stream(vlm_ctx *ctx);
+// Deferred download: the copy is enqueued now, dst is written at
+// wait_deferred or at the next sync_stream (whichever comes first).
+int d2h_deferred(vlm_ctx *ctx, void *dst, const void *src, size_t bytes);
+int wait_deferred(vlm_ctx *ctx);
 int mlp_forward_dev(vlm_ctx *ctx, int64_t m, const float *X, float *Y);
 // Training step on a device batch; loss_host downloads the loss (and
 // synchronises), keep_loss only sums it on the device for mlp_read_loss.

@@ -27,7 +27,8 @@
 namespace vlm {
 
 int points_to_device(vlm_ctx *ctx, int64_t m, const double *p, double *soa, int64_t stride);
-int points_to_host(vlm_ctx *ctx, int64_t m, const double *soa, int64_t stride, double *p);
+int points_to_host(vlm_ctx *ctx, int64_t m, const double *soa, int64_t stride, double *p,
+                   bool deferred = false);
 
 namespace {
 
@@ -1551,9 +1552,9 @@ int vlm_rotor_induced_velocity(vlm_ctx *ctx, int ir, int set, int64_t m, const d
     return points_to_host(ctx, m, ctx->out.p, m, v);
 }
 
-int vlm_induced_velocity_terms(vlm_ctx *ctx, int nterms, const int *ir, const int *set,
-                               const int *flags, int64_t m, const double *p, double *v,
-                               int accumulate)
+static int induced_terms(vlm_ctx *ctx, int nterms, const int *ir, const int *set,
+                         const int *flags, int64_t m, const double *p, double *v,
+                         int accumulate, bool deferred)
 {
     if (!ctx) return VLM_ERR_ARG;
     if (nterms < 0 || (nterms > 0 && (!ir || !set || !flags)))
@@ -1586,7 +1587,21 @@ int vlm_induced_velocity_terms(vlm_ctx *ctx, int nterms, const int *ir, const in
         VLM_TRY(build_sources(ctx, src));
         VLM_TRY(eval_sources(ctx, ctx->tgt.p, m, m, ctx->out.p, 1, nmax));
     }
-    return points_to_host(ctx, m, ctx->out.p, m, v);
+    return points_to_host(ctx, m, ctx->out.p, m, v, deferred);
+}
+
+int vlm_induced_velocity_terms(vlm_ctx *ctx, int nterms, const int *ir, const int *set,
+                               const int *flags, int64_t m, const double *p, double *v,
+                               int accumulate)
+{
+    return induced_terms(ctx, nterms, ir, set, flags, m, p, v, accumulate, false);
+}
+
+int vlm_induced_velocity_terms_deferred(vlm_ctx *ctx, int nterms, const int *ir,
+                                        const int *set, const int *flags, int64_t m,
+                                        const double *p, double *v, int accumulate)
+{
+    return induced_terms(ctx, nterms, ir, set, flags, m, p, v, accumulate, true);
 }
 
 int vlm_wake_velocity(vlm_ctx *ctx, int wakeType)

@@ -178,6 +178,37 @@ module vlm_c
       type(c_ptr), value :: A
     end function
 
+    ! deferred variants: the outputs (passed as C addresses of TARGET
+    ! arrays) are written by vlm_wait or by the next synchronising call
+    integer(c_int) function vlm_rotor_solve_deferred(ctx, ir, rhs, gam, max_iter, tol, iters) &
+        bind(C, name='vlm_rotor_solve_deferred')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir
+      real(c_double), intent(in) :: rhs(*)
+      type(c_ptr), value :: gam
+      integer(c_int), value :: max_iter
+      real(c_double), value :: tol
+      type(c_ptr), value :: iters
+    end function
+
+    integer(c_int) function vlm_induced_velocity_terms_deferred(ctx, nterms, ir, set, flags, &
+        & m, p, v, acc) bind(C, name='vlm_induced_velocity_terms_deferred')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nterms
+      integer(c_int), intent(in) :: ir(*), set(*), flags(*)
+      integer(c_int64_t), value :: m
+      real(c_double), intent(in) :: p(*)
+      type(c_ptr), value :: v
+      integer(c_int), value :: acc
+    end function
+
+    integer(c_int) function vlm_wait(ctx) bind(C, name='vlm_wait')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
     integer(c_int) function vlm_rotor_solve(ctx, ir, rhs, gam, max_iter, tol, iters) &
         bind(C, name='vlm_rotor_solve')
       import :: c_int, c_ptr, c_double

@@ -100,6 +100,8 @@ module volcanor_host
     real(dp) :: rollupStartRadius = 0._dp, rollupEndRadius = 1._dp
     real(dp), allocatable :: streamwiseCoreVec(:)
     real(dp), allocatable :: gamVec(:), gamVecPrev(:), RHS(:)
+    real(dp), allocatable :: vload(:, :, :, :)  !! load velocities at the CPs (deferred download)
+    integer(c_int) :: solve_iters = 0
     real(dp) :: lift(3) = 0._dp, drag(3) = 0._dp, forceInertial(3) = 0._dp
     real(dp) :: liftUnsteady(3) = 0._dp
     real(dp) :: nonDimforceDenominator = 1._dp
@@ -981,14 +983,16 @@ contains
   ! RHS, solve and loads
   ! ------------------------------------------------------------------
   subroutine compute_rhs_and_solve(ctx, rotor, nr, with_wake)
-    !! main.f90:124-202 (initial) and 521-603 (every step)
+    !! main.f90:124-202 (initial) and 521-603 (every step).  The solve is
+    !! deferred: gamVec arrives at vlm_wait, after which finish_solve maps it
+    !! onto the panels (the library has set the device wing circulation).
     type(c_ptr), intent(in) :: ctx
-    type(rotor_class), intent(inout) :: rotor(:)
+    type(rotor_class), intent(inout), target :: rotor(:)
     integer, intent(in) :: nr
     logical, intent(in) :: with_wake
-    integer :: ir, jr, ib, i, j, n, row, it, nt
+    integer :: ir, jr, ib, i, j, n, row, nt
     integer(c_int) :: tir(2*nr), tset(2*nr), tfl(2*nr)
-    real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :), v(:, :, :, :), gam(:)
+    real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :), v(:, :, :, :)
 
     do ir = 1, nr
       associate (r => rotor(ir))
@@ -1042,19 +1046,22 @@ contains
       end associate
     enddo
     do ir = 1, nr
-      associate (r => rotor(ir))
-        r%gamVecPrev = r%gamVec
-        allocate (gam(size(r%gamVec)))
-        call vlm_check(ctx, vlm_rotor_solve(ctx, r%dev, r%RHS, gam, 3_c_int, 1.e-15_c_double, it), &
-          & 'rotor_solve')
-        r%gamVec = gam
-        deallocate (gam)
-        ! the library has already set the device wing circulation from
-        ! gamVec (rotor_map_gam on the device, axisymmetric copy included)
-        call rotor_map_gam(r)
-      end associate
+      rotor(ir)%gamVecPrev = rotor(ir)%gamVec
+      call vlm_check(ctx, vlm_rotor_solve_deferred(ctx, rotor(ir)%dev, rotor(ir)%RHS, &
+        & c_loc(rotor(ir)%gamVec), 3_c_int, 1.e-15_c_double, c_loc(rotor(ir)%solve_iters)), &
+        & 'rotor_solve')
     enddo
   end subroutine compute_rhs_and_solve
+
+  subroutine finish_solve(rotor, nr)
+    !! After vlm_wait: rotor_map_gam on the host panels
+    type(rotor_class), intent(inout) :: rotor(:)
+    integer, intent(in) :: nr
+    integer :: ir
+    do ir = 1, nr
+      call rotor_map_gam(rotor(ir))
+    enddo
+  end subroutine finish_solve
 
   subroutine blade_sec_resvel(b)
     !! Section resultant velocity, classdef.f90:2197-2232
@@ -1079,36 +1086,51 @@ contains
     enddo
   end subroutine blade_sec_resvel
 
+  subroutine forces_launch(ctx, rotor, nr, ir)
+    !! Enqueues the load velocities at the collocation points of rotor ir:
+    !! the wing's own rings less every rotor's bound vortices
+    !! (classdef.f90:1376-1418, 1734), as one device-side sequence whose
+    !! result reaches rotor(ir)%vload at vlm_wait.
+    type(c_ptr), intent(in) :: ctx
+    type(rotor_class), intent(inout), target :: rotor(:)
+    integer, intent(in) :: nr, ir
+    integer :: jr, n
+    integer(c_int) :: tir(nr + 1), tset(nr + 1), tfl(nr + 1)
+    real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :)
+
+    associate (r => rotor(ir))
+      n = r%nc*r%ns*r%nb
+      allocate (cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb))
+      if (.not. allocated(r%vload)) allocate (r%vload(3, r%nc, r%ns, r%nb))
+      call rotor_cp_arrays(r, cp, ncap)
+      do jr = 1, nr
+        tir(jr) = rotor(jr)%dev; tset(jr) = VLM_SET_WING_BOUND; tfl(jr) = 0
+      enddo
+      tir(nr + 1) = r%dev; tset(nr + 1) = VLM_SET_WING; tfl(nr + 1) = VLM_TERM_NEGATE
+      call vlm_check(ctx, vlm_induced_velocity_terms_deferred(ctx, int(nr + 1, c_int), tir, &
+        & tset, tfl, int(n, c_int64_t), cp, c_loc(r%vload), 0_c_int), 'load velocity')
+      deallocate (cp, ncap)
+    end associate
+  end subroutine forces_launch
+
   subroutine compute_forces(ctx, rotor, nr, ir, density, dt)
     !! forceCalcSwitch = 0 branch of main.f90:247-290 / 632-668 and
-    !! rotor_calc_force + blade_calc_force (classdef.f90:1704-1896, 4607-4671)
+    !! rotor_calc_force + blade_calc_force (classdef.f90:1704-1896, 4607-4671),
+    !! from the load velocities of forces_launch (after vlm_wait)
     type(c_ptr), intent(in) :: ctx
     type(rotor_class), intent(inout) :: rotor(:)
     integer, intent(in) :: nr, ir
     real(dp), intent(in) :: density, dt
-    integer :: jr, ib, i, j, is, ic, n
-    integer(c_int) :: tir(nr + 1), tset(nr + 1), tfl(nr + 1)
-    real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :), v(:, :, :, :)
+    integer :: ib, i, j, is, ic
     real(dp) :: vtc(rotor(ir)%nc, rotor(ir)%ns), gec(rotor(ir)%nc, rotor(ir)%ns)
     real(dp) :: ges(rotor(ir)%nc, rotor(ir)%ns), q(rotor(ir)%ns)
     real(dp) :: inv, sgn, vts, vi(3)
 
     associate (r => rotor(ir))
-      n = r%nc*r%ns*r%nb
-      allocate (cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb), v(3, r%nc, r%ns, r%nb))
-      call rotor_cp_arrays(r, cp, ncap)
-      ! velocity of the wing's own rings less every rotor's bound vortices
-      ! (classdef.f90:1376-1418, 1734), as one device-side sequence
-      do jr = 1, nr
-        tir(jr) = rotor(jr)%dev; tset(jr) = VLM_SET_WING_BOUND; tfl(jr) = 0
-      enddo
-      tir(nr + 1) = r%dev; tset(nr + 1) = VLM_SET_WING; tfl(nr + 1) = VLM_TERM_NEGATE
-      call vlm_check(ctx, vlm_induced_velocity_terms(ctx, int(nr + 1, c_int), tir, tset, tfl, &
-        & int(n, c_int64_t), cp, v, 0_c_int), 'load velocity')
       do ib = 1, r%nbConvect
         do j = 1, r%ns
           do i = 1, r%nc
-            r%blade(ib)%wiP(i, j)%velCPTotal = r%blade(ib)%wiP(i, j)%velCP + v(:, i, j, ib)
+            r%blade(ib)%wiP(i, j)%velCPTotal = r%blade(ib)%wiP(i, j)%velCP + r%vload(:, i, j, ib)
           enddo
         enddo
       enddo
@@ -1119,7 +1141,6 @@ contains
           r%blade(ib)%wiP%velCPTotal(3) = r%blade(1)%wiP%velCPTotal(3)
         enddo
       endif
-      deallocate (cp, ncap, v)
 
       ! rotor_calc_secAlpha / blade_calc_secAlpha (classdef.f90:4766-4784,
       ! 2234-2265); the vertical axis is the blade's zAxisAziFlap
@@ -1298,7 +1319,7 @@ contains
     real(dp), intent(inout) :: ct_hist(ct_cap, hist_len)
     type(volcanor_stats), intent(out) :: stats
     type(switches_class) :: sw
-    type(rotor_class), allocatable :: rotor(:)
+    type(rotor_class), allocatable, target :: rotor(:)
     type(c_ptr) :: ctx
     integer :: nt, nr, ir, ib, iter, rn, rf, uforce(16), nforce
     real(dp) :: dt, density, velSound, kinematicVisc, t
@@ -1307,7 +1328,7 @@ contains
     type(vlm_rotor_desc) :: d
     integer(c_int) :: rowNear, rowFar, dev
     integer(c_int64_t) :: c0, c1, c2, crate
-    logical :: write_out
+    logical :: write_out, loads
     integer :: ntStart
 
     call system_clock(c0, crate)
@@ -1368,6 +1389,8 @@ contains
     enddo
     t = 0._dp
     call compute_rhs_and_solve(ctx, rotor, nr, .false.)
+    call vlm_check(ctx, vlm_wait(ctx), 'wait')
+    call finish_solve(rotor, nr)
     do ir = 1, nr
       call vlm_check(ctx, vlm_rotor_init_cursors(ctx, rotor(ir)%dev), 'init_cursors')
       if (rotor(ir)%nNwake > 0) &
@@ -1385,6 +1408,10 @@ contains
       enddo
     endif
     if (sw%rotorForcePlot /= 0) then
+      do ir = 1, nr
+        call forces_launch(ctx, rotor, nr, ir)
+      enddo
+      call vlm_check(ctx, vlm_wait(ctx), 'wait')
       do ir = 1, nr
         call compute_forces(ctx, rotor, nr, ir, density, dt)
         if (ir <= ct_cap .and. hist_len >= 1) ct_hist(ir, 1) = rotor_CT(rotor(ir))
@@ -1431,18 +1458,30 @@ contains
         enddo
       endif
       call compute_rhs_and_solve(ctx, rotor, nr, sw%wakeSuppress == 0)
-      if (sw%rotorForcePlot /= 0) then
-        if (mod(iter, sw%rotorForcePlot) == 0) then
-          do ir = 1, nr
-            call compute_forces(ctx, rotor, nr, ir, density, dt)
-            if (ir <= ct_cap .and. iter + 1 <= hist_len) ct_hist(ir, iter + 1) = rotor_CT(rotor(ir))
-            if (write_out) call force_line(uforce(ir), rotor(ir), iter)
-          enddo
-        endif
+      loads = .false.
+      if (sw%rotorForcePlot /= 0) loads = mod(iter, sw%rotorForcePlot) == 0
+      if (loads) then
+        do ir = 1, nr
+          call forces_launch(ctx, rotor, nr, ir)
+        enddo
+      endif
+      ! The wake passes are enqueued before the host waits for the
+      ! circulation and the load velocities; the loads (main.f90:632-719)
+      ! depend only on the wing and its circulation, which the convection
+      ! does not change, so they are computed while the passes run.
+      if (sw%wakeSuppress == 0) &
+        & call vlm_check(ctx, vlm_convect(ctx, int(sw%fdScheme, c_int), int(iter, c_int), dt), &
+        & 'convect')
+      call vlm_check(ctx, vlm_wait(ctx), 'wait')
+      call finish_solve(rotor, nr)
+      if (loads) then
+        do ir = 1, nr
+          call compute_forces(ctx, rotor, nr, ir, density, dt)
+          if (ir <= ct_cap .and. iter + 1 <= hist_len) ct_hist(ir, iter + 1) = rotor_CT(rotor(ir))
+          if (write_out) call force_line(uforce(ir), rotor(ir), iter)
+        enddo
       endif
       if (sw%wakeSuppress == 0) then
-        call vlm_check(ctx, vlm_convect(ctx, int(sw%fdScheme, c_int), int(iter, c_int), dt), &
-          & 'convect')
         if (sw%wakeStrain == 1) then
           do ir = 1, nr
             if (rotor(ir)%nNwake > 0) &

@@ -67,6 +67,9 @@ _SIGS = {
     "vlm_rotor_convectwake": [_P, _I, _D, _I],
     "vlm_rotor_induced_velocity": [_P, _I, _I, _L, _P, _P, _I],
     "vlm_induced_velocity_terms": [_P, _I, _P, _P, _P, _L, _P, _P, _I],
+    "vlm_induced_velocity_terms_deferred": [_P, _I, _P, _P, _P, _L, _P, _P, _I],
+    "vlm_rotor_solve_deferred": [_P, _I, _P, _P, _I, _D, _P],
+    "vlm_wait": [_P],
     "vlm_wake_velocity": [_P, _I],
     "vlm_convect": [_P, _I, _I, _D],
     "vlm_rotor_get_wake": [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P],
@@ -128,6 +131,7 @@ class Context:
     """One HIP device context (vlm_ctx)."""
 
     def __init__(self, device: int = 0):
+        self._pending = []  # outputs of deferred calls, alive until filled
         self.lib = load_library()
         h = C.c_void_p()
         rc = self.lib.vlm_create(C.byref(h), device)
@@ -139,6 +143,7 @@ class Context:
         if getattr(self, "h", None):
             self.lib.vlm_destroy(self.h)
             self.h = None
+        self._pending = []
 
     def __del__(self):
         try:
@@ -314,6 +319,34 @@ class Context:
         self._chk(self.lib.vlm_rotor_solve(self.h, ir, _ptr(rhs), _ptr(g), max_iter, tol,
                                            C.byref(it)))
         return g, it.value
+
+    # -- deferred results: outputs are filled by wait() (or any call that
+    #    synchronises); the arrays are kept alive here until then
+    def rotor_solve_deferred(self, ir, rhs, max_iter=3, tol=1e-15):
+        rhs = _f64(rhs).ravel()
+        g = np.zeros_like(rhs)
+        it = np.zeros(1, dtype=np.int32)
+        self._chk(self.lib.vlm_rotor_solve_deferred(self.h, ir, _ptr(rhs), _ptr(g), max_iter,
+                                                    tol, _ptr(it)))
+        self._pending.extend([g, it])
+        return g, it
+
+    def induced_velocity_terms_deferred(self, terms, p, v=None):
+        p = _f64(p, (-1, 3))
+        acc = v is not None
+        out = _f64(v, (-1, 3)).copy() if acc else np.zeros_like(p)
+        ir = np.array([t[0] for t in terms], dtype=np.int32)
+        st = np.array([t[1] for t in terms], dtype=np.int32)
+        fl = np.array([TERM_NEGATE if t[2] else 0 for t in terms], dtype=np.int32)
+        self._chk(self.lib.vlm_induced_velocity_terms_deferred(
+            self.h, len(terms), _ptr(ir), _ptr(st), _ptr(fl), p.shape[0], _ptr(p), _ptr(out),
+            int(acc)))
+        self._pending.append(out)
+        return out
+
+    def wait(self):
+        self._chk(self.lib.vlm_wait(self.h))
+        self._pending = []
 
     def lu_factor(self, A):
         A = np.asfortranarray(A, dtype=np.float64).copy(order="F")
