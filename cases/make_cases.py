@@ -144,14 +144,14 @@ def make_all(out):
 
     # reference fixture case: Katz & Plotkin AR-4 wing
     d = case("katz_plotkin_AR4")
-    write_config(os.path.join(d, "config.nml"), nt=160, dt=0.00625, nr=1)
+    write_config(os.path.join(d, "config.nml"), nt=160, dt=0.00625, nr=1, wakePlot=32)
     write_geom(os.path.join(d, "geom01.nml"), nc=4, ns=26, spanSpacing=2, nNwake=160,
                spanwiseCore=0.04, streamwiseCoreVec=0.04)
 
     # reference fixture case: 5-bladed axisymmetric rotor with a PLOT3D blade
     d = case("elevate")
     write_config(os.path.join(d, "config.nml"), nt=150, dt=-0.04, nr=1,
-                 kinematicVisc=0.0000181, wakeDissipation=1)
+                 kinematicVisc=0.0000181, wakeDissipation=1, wakePlot=-1)
     write_geom(os.path.join(d, "geom01.nml"), nb=5, nc=16, ns=16, spanSpacing=2,
                geometryFile="elevate.xyz", nNwake=30, span=5.0, Omega=89.98, theta0=0.0,
                axisymmetrySwitch=1, velBody=[0.0, 0.0, 10.0], apparentViscCoeff=5000.0,

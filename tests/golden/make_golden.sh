@@ -13,6 +13,8 @@ cp "$REF/katzNplotkin-AR04.case/referenceResults/r01ForceNonDim.csv.ref" "$HERE/
 cp "$REF/katzNplotkin-AR04.case/referenceResults/r01b01ForceDist00160.csv.ref" "$HERE/katz_plotkin_AR4_ForceDist00160.csv"
 cp "$REF/elevateTest.case/referenceResults/r01ForceNonDim.csv.ref" "$HERE/elevate_ForceNonDim.csv"
 cp "$REF/elevateTest.case/referenceResults/r01b01ForceDist00150.csv.ref" "$HERE/elevate_ForceDist00150.csv"
+cp "$REF/katzNplotkin-AR04.case/referenceResults/r01Params.json.ref" "$HERE/katz_plotkin_AR4_Params.json"
+cp "$REF/elevateTest.case/referenceResults/r01Params.json.ref" "$HERE/elevate_Params.json"
 cp "$REF/elevateTest.case/elevate.xyz" "$HERE/elevate.xyz"
 make -C "$ROOT/oracle" ref
 python3 "$ROOT/cases/make_cases.py"

@@ -81,6 +81,23 @@ def test_fixture_sectional_loads(case, fixture):
         np.testing.assert_allclose(got[k], ref[k], rtol=1e-5, atol=1e-6, err_msg=k)
 
 
+@pytest.mark.parametrize("case,fixture", [("katz_plotkin_AR4", "katz_plotkin_AR4_Params.json"),
+                                          ("elevate", "elevate_Params.json")])
+def test_fixture_params(case, fixture):
+    """testParams of the reference's Python tests: every key of the
+    reference's rXXParams.json, floats at places=6, the rest equal."""
+    import json
+    ref = json.load(open(os.path.join(GOLD, fixture)))
+    run(case)
+    got = json.load(open(os.path.join(CASES, case + ".case", "Results", "r01Params.json")))
+    for k, v in ref.items():
+        assert k in got, k
+        if isinstance(v, float):
+            assert round(got[k] - v, 6) == 0, (k, got[k], v)
+        else:
+            assert got[k] == v, (k, got[k], v)
+
+
 def test_elevate_fixture_ct_history():
     """tests/test_elevate.py of the reference (axisymmetric rotor, PLOT3D
     blade, dissipation, truncated far wake)."""

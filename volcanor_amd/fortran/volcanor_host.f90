@@ -101,6 +101,11 @@ module volcanor_host
     real(dp), allocatable :: streamwiseCoreVec(:)
     real(dp), allocatable :: gamVec(:), gamVecPrev(:), RHS(:)
     real(dp), allocatable :: vload(:, :, :, :)  !! load velocities at the CPs (deferred download)
+    ! inputs kept only for rXXParams.json (libPostprocess.f90:52-141)
+    integer :: imagePlane = 0, imageRotorNum = 0
+    real(dp) :: thetaTwist = 0._dp, initWakeVel = 0._dp
+    real(dp) :: flapInitial = 0._dp, dflapInitial = 0._dp, Iflap = 0._dp, cflap = 0._dp
+    real(dp) :: kflap = 0._dp, MflapConstant = 0._dp, dpitch = 0._dp
     integer(c_int) :: solve_iters = 0
     real(dp) :: lift(3) = 0._dp, drag(3) = 0._dp, forceInertial(3) = 0._dp
     real(dp) :: liftUnsteady(3) = 0._dp
@@ -470,6 +475,10 @@ contains
     endif
     r%rollupStartRadius = rollupStartRadius; r%rollupEndRadius = rollupEndRadius
     r%psiStart = psiStart
+    r%imagePlane = imagePlane; r%imageRotorNum = imageRotorNum
+    r%thetaTwist = thetaTwist; r%initWakeVel = initWakeVel
+    r%flapInitial = flapInitial; r%dflapInitial = dflapInitial; r%Iflap = Iflap
+    r%cflap = cflap; r%kflap = kflap; r%MflapConstant = MflapConstant; r%dpitch = dpitch
   end subroutine rotor_readGeom
 
   subroutine to_chords_revs(r, n, dt)
@@ -689,6 +698,11 @@ contains
       call to_chords_revs(r, nt, dt)
     endif
     if (sw%slowStart /= 0) call to_chords_revs(r, sw%slowStartNt, dt)
+    ! plot intervals in chords or revolutions become steps (classdef.f90:3001-3003);
+    ! they are reported in rXXParams.json, no wake or grid plots are written
+    call to_chords_revs(r, sw%wakeTipPlot, dt)
+    call to_chords_revs(r, sw%wakePlot, dt)
+    call to_chords_revs(r, sw%gridPlot, dt)
     call to_chords_revs(r, r%wakeTruncateNt, dt)
     call to_chords_revs(r, r%nNwake, dt)
     if (r%wakeTruncateNt > 0) r%wakeTruncateNt = max(r%wakeTruncateNt, r%nNwake + 1)
@@ -731,6 +745,7 @@ contains
     r%pts = r%pts*degToRad
     r%preconeAngle = r%preconeAngle*degToRad
     r%psiStart = r%psiStart*degToRad
+    r%thetaTwist = r%thetaTwist*degToRad
     r%spanwiseCore = r%spanwiseCore*r%chord
     r%streamwiseCoreVec = r%streamwiseCoreVec*r%chord
     r%rollupStart = ceiling(r%rollupStartRadius*r%ns)
@@ -1307,6 +1322,66 @@ contains
     close (u)
   end subroutine dist_file
 
+  subroutine params_file(path, r, nt, dt, nr, density, velSound, sw)
+    !! rXXParams.json with the keys and value conventions of params2file
+    !! (libPostprocess.f90:52-141): angles in degrees except phi/theta/psi,
+    !! counts after the init-time conversions.  Written as a JSON object the
+    !! reference's tests/parseResults.py getParams reads.
+    character(len=*), intent(in) :: path
+    type(rotor_class), intent(in) :: r
+    integer, intent(in) :: nt, nr
+    real(dp), intent(in) :: dt, density, velSound
+    type(switches_class), intent(in) :: sw
+    integer :: u
+    open (newunit=u, file=path, status='replace', action='write')
+    write (u, '(A)') '{'
+    call ji('nt', nt); call jr('dt', dt); call ji('nr', nr)
+    call ji('restartWriteNt', sw%restartWriteNt); call ji('restartFromNt', sw%restartFromNt)
+    call ji('ntSub', sw%ntSub); call ji('ntSubInit', sw%ntSubInit)
+    call jr('density', density); call jr('velSound', velSound)
+    call ji('wakePlot', sw%wakePlot); call ji('wakeTipPlot', sw%wakeTipPlot)
+    call ji('rotorForcePlot', sw%rotorForcePlot); call ji('gridPlot', sw%gridPlot)
+    call ji('wakeDissipation', sw%wakeDissipation); call ji('wakeStrain', sw%wakeStrain)
+    call ji('wakeBurst', sw%wakeBurst); call ji('slowStart', sw%slowStart)
+    call ji('slowStartNt', sw%slowStartNt); call ji('fdScheme', sw%fdScheme)
+    call ji('initWakeVelNt', sw%initWakeVelNt); call ji('probe', sw%probe)
+    call ji('nb', r%nb); call ji('propConvention', r%propConvention)
+    write (u, '(A)') ' "geometryFile": "'//r%geometryFile//'",'
+    call ji('spanSpacing', r%spanSpacing); call ji('chordSpacing', r%chordSpacing)
+    call ji('nCamberFiles', 0); call ji('surfaceType', r%surfaceType)
+    call ji('imagePlane', r%imagePlane); call ji('imageRotorNum', r%imageRotorNum)
+    call ji('nc', r%nc); call ji('ns', r%ns); call ji('nNwake', r%nNwake)
+    call jr('radius', r%radius); call jr('root_cut', r%root_cut); call jr('chord', r%chord)
+    call jr('preconeAngle', r%preconeAngle*radToDeg); call jr('Omega', r%Omega)
+    call jr('phi', r%pts(1)); call jr('theta', r%pts(2)); call jr('psi', r%pts(3))
+    call jr('theta0', r%controlPitch(1)*radToDeg); call jr('thetaC', r%controlPitch(2)*radToDeg)
+    call jr('thetaS', r%controlPitch(3)*radToDeg); call jr('thetaTwist', r%thetaTwist*radToDeg)
+    call jr('u', r%velBody(1)); call jr('v', r%velBody(2)); call jr('w', r%velBody(3))
+    call jr('nonDimForceDenom', r%nonDimforceDenominator); call jr('alpha0', 0._dp)
+    call ji('wakeTruncateNt', r%wakeTruncateNt); call ji('prescWakeNt', 0)
+    call jr('initWakeVel', r%initWakeVel); call jr('psiStart', r%psiStart*radToDeg)
+    call jr('forceCalcSwitch', r%forceCalcSwitch*radToDeg)
+    call jr('apparentViscCoeff', r%apparentViscCoeff); call jr('decayCoeff', r%decayCoeff)
+    call ji('axisymmetrySwitch', r%axisymmetrySwitch); call ji('bladeDynamicsSwitch', 0)
+    call jr('flapInitial', r%flapInitial); call jr('dFlapInitial', r%dflapInitial)
+    call jr('Iflap', r%Iflap); call jr('cflap', r%cflap); call jr('kflap', r%kflap)
+    call jr('MflapConstant', r%MflapConstant); call ji('pitchDynamicsSwitch', 0)
+    write (u, '(A,ES24.16E3)') ' "dpitch": ', r%dpitch
+    write (u, '(A)') '}'
+    close (u)
+  contains
+    subroutine ji(k, v)
+      character(len=*), intent(in) :: k
+      integer, intent(in) :: v
+      write (u, '(A,I0,A)') ' "'//k//'": ', v, ','
+    end subroutine ji
+    subroutine jr(k, v)
+      character(len=*), intent(in) :: k
+      real(dp), intent(in) :: v
+      write (u, '(A,ES24.16E3,A)') ' "'//k//'": ', v, ','
+    end subroutine jr
+  end subroutine params_file
+
   ! ------------------------------------------------------------------
   ! The time loop, main.f90:1-1459
   ! ------------------------------------------------------------------
@@ -1401,6 +1476,8 @@ contains
       ir = int(c_mkdir(trim(dir)//'/Results'//c_null_char, int(o'755', c_int)))
       do ir = 1, nr
         write (rc2, '(I0.2)') ir
+        call params_file(trim(dir)//'/Results/r'//rc2//'Params.json', rotor(ir), nt, dt, nr, &
+          & density, velSound, sw)
         open (newunit=uforce(ir), file=trim(dir)//'/Results/r'//rc2//'ForceNonDim.csv', &
           & status='replace', action='write')
         write (uforce(ir), '(A5,9(A15))') 'iter', 'CL/CT', 'CD/CQ', 'CLu', 'CDi', 'CD0', 'CDu', &
