@@ -10,6 +10,7 @@ PLOT3D blade file) used by the tests and the benchmark.
   cfg5_rotor_wing    three-bladed rotor in forward flight above an AR-8 wing
   refprobe_rotor     2 x (2 x 3) rotor in forward flight, 14 steps: the case on
                      which oracle/refprobe.f90 dumps the reference's wake state
+  probes_wing        a short wing run with six velocity probes (probes.in)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
   elevate            the reference's 5-bladed axisymmetric test rotor
   (the last two reproduce tests/katzNplotkin-AR04.case and
@@ -141,6 +142,18 @@ def make_all(out):
     write_config(os.path.join(d, "config.nml"), nt=200, dt=0.025, nr=1)
     write_geom(os.path.join(d, "geom01.nml"), nc=4, ns=13, nNwake=200, spanwiseCore=0.03,
                streamwiseCoreVec=0.03)
+
+    # velocity probes on a short wing run: three fixed and three moving probes
+    d = case("probes_wing")
+    write_config(os.path.join(d, "config.nml"), nt=20, dt=0.025, nr=1, probe=1)
+    write_geom(os.path.join(d, "geom01.nml"), nc=4, ns=13, nNwake=20, spanwiseCore=0.03,
+               streamwiseCoreVec=0.03)
+    with open(os.path.join(d, "probes.in"), "w") as f:
+        f.write("6\n")
+        for p in ((0.5, 0.3, 0.2, 0.0, 0.0, 0.0), (2.0, 1.0, -0.1, 0.0, 0.0, 0.0),
+                  (1.0, 2.1, 0.0, 0.0, 0.0, 0.0), (0.2, 0.5, 0.15, 2.0, 0.0, 0.0),
+                  (-0.5, 1.5, -0.3, 5.0, 0.5, 0.1), (3.0, 0.0, 0.5, 1.0, 0.0, -0.2)):
+            f.write(" ".join(repr(x) for x in p) + "\n")
 
     # reference fixture case: Katz & Plotkin AR-4 wing
     d = case("katz_plotkin_AR4")

@@ -46,3 +46,10 @@ rm -rf "$tmp"
 #    truncation within it) and rotor above a wing (half a revolution)
 run_ref cfg4_quad_short cfg4_quad_short_ForceNonDim.csv
 run_ref cfg5_rotor_wing_short cfg5_rotor_wing_short_ForceNonDim.csv
+# 5. velocity probes: the reference's probes2file output of a short wing run
+tmp=$(mktemp -d)
+cp "$ROOT/cases/probes_wing.case/"* "$tmp/"
+mkdir -p "$tmp/Results" "$tmp/Restart"
+(cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
+for s in 00001 00010 00020; do cp "$tmp/Results/probes$s.csv" "$HERE/probes_wing_probes$s.csv"; done
+rm -rf "$tmp"

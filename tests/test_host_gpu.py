@@ -98,6 +98,24 @@ def test_fixture_params(case, fixture):
             assert got[k] == v, (k, got[k], v)
 
 
+@pytest.fixture(scope="module")
+def probes_run():
+    run("probes_wing")
+    return os.path.join(CASES, "probes_wing.case", "Results")
+
+
+@pytest.mark.parametrize("step", ["00001", "00010", "00020"])
+def test_velocity_probes_match_reference_solver(step, probes_run):
+    """probes2file of the reference solver (libPostprocess.f90:333-361) on a
+    short wing run with three fixed and three moving probes: velocity and
+    position of every probe, to the 7 printed digits."""
+    ref = read_table(os.path.join(GOLD, f"probes_wing_probes{step}.csv"))
+    got = read_table(os.path.join(probes_run, f"probes{step}.csv"))
+    assert list(got) == ["u", "v", "w", "x", "y", "z"] == list(ref)
+    for k in got:
+        np.testing.assert_allclose(got[k], ref[k], rtol=2e-6, atol=2e-7, err_msg=k)
+
+
 def test_elevate_fixture_ct_history():
     """tests/test_elevate.py of the reference (axisymmetric rotor, PLOT3D
     blade, dissipation, truncated far wake)."""

@@ -1403,8 +1403,11 @@ contains
     type(vlm_rotor_desc) :: d
     integer(c_int) :: rowNear, rowFar, dev
     integer(c_int64_t) :: c0, c1, c2, crate
-    logical :: write_out, loads
-    integer :: ntStart
+    logical :: write_out, loads, probing
+    integer :: ntStart, nprobe, uprobe, k
+    real(dp), allocatable :: probe(:, :), probeVel(:, :)
+    real(c_double), allocatable, target :: probeLoc(:, :), probeOut(:, :)
+    integer(c_int), allocatable :: pir(:), pset(:), pfl(:)
 
     call system_clock(c0, crate)
     call read_config(dir, nt, dt, nr, density, velSound, kinematicVisc, sw)
@@ -1414,6 +1417,20 @@ contains
     if (sw%initWakeVelNt /= 0) error stop 'ERROR: initWakeVel is not supported'
     if (sw%restartFromNt /= 0) error stop 'ERROR: restart is not supported'
     if (nr > 16) error stop 'ERROR: at most 16 rotors'
+    ! velocity probes (main.f90:83-93): read only for probe = 1, used at every
+    ! step that is a multiple of probe; any other value would leave the
+    ! reference's probe arrays unallocated
+    nprobe = 0
+    if (sw%probe /= 0) then
+      if (sw%probe /= 1) error stop 'ERROR: probe must be 0 or 1'
+      open (newunit=uprobe, file=trim(dir)//'/probes.in', status='old', action='read')
+      read (uprobe, *) nprobe
+      allocate (probe(3, nprobe), probeVel(3, nprobe), probeLoc(3, nprobe), probeOut(3, nprobe))
+      do k = 1, nprobe
+        read (uprobe, *) probe(:, k), probeVel(:, k)
+      enddo
+      close (uprobe)
+    endif
     allocate (rotor(nr))
     do ir = 1, nr
       write (rc2, '(I0.2)') ir
@@ -1542,6 +1559,27 @@ contains
           call forces_launch(ctx, rotor, nr, ir)
         enddo
       endif
+      ! velocity probes (probes2file, libPostprocess.f90:333-361): the probe's
+      ! own velocity plus every rotor's wing and wake velocity at its moving
+      ! position, on the wake before convection (main.f90:792-797)
+      probing = .false.
+      if (sw%probe /= 0 .and. nprobe > 0) probing = mod(iter, sw%probe) == 0
+      if (probing) then
+        do k = 1, nprobe
+          probeLoc(:, k) = probe(:, k) + probeVel(:, k)*t
+        enddo
+        probeOut = probeVel
+        if (.not. allocated(pir)) allocate (pir(2*nr), pset(2*nr), pfl(2*nr))
+        k = 0
+        do ir = 1, nr
+          k = k + 1; pir(k) = rotor(ir)%dev; pset(k) = VLM_SET_WING; pfl(k) = 0
+          if (rotor(ir)%nNwake > 0 .and. sw%wakeSuppress == 0) then
+            k = k + 1; pir(k) = rotor(ir)%dev; pset(k) = VLM_SET_WAKE; pfl(k) = 0
+          endif
+        enddo
+        call vlm_check(ctx, vlm_induced_velocity_terms_deferred(ctx, int(k, c_int), pir, &
+          & pset, pfl, int(nprobe, c_int64_t), probeLoc, c_loc(probeOut), 1_c_int), 'probes')
+      endif
       ! The wake passes are enqueued before the host waits for the
       ! circulation and the load velocities; the loads (main.f90:632-719)
       ! depend only on the wing and its circulation, which the convection
@@ -1551,6 +1589,16 @@ contains
         & 'convect')
       call vlm_check(ctx, vlm_wait(ctx), 'wait')
       call finish_solve(rotor, nr)
+      if (probing .and. verbose >= 0) then
+        write (ts5, '(I0.5)') iter
+        open (newunit=uprobe, file=trim(dir)//'/Results/probes'//ts5//'.csv', &
+          & status='replace', action='write')
+        write (uprobe, '(6(A15))') 'u', 'v', 'w', 'x', 'y', 'z'
+        do k = 1, nprobe
+          write (uprobe, '(6(E15.7))') probeOut(:, k), probeLoc(:, k)
+        enddo
+        close (uprobe)
+      endif
       if (loads) then
         do ir = 1, nr
           call compute_forces(ctx, rotor, nr, ir, density, dt)
