@@ -11,6 +11,7 @@ PLOT3D blade file) used by the tests and the benchmark.
   refprobe_rotor     2 x (2 x 3) rotor in forward flight, 14 steps: the case on
                      which oracle/refprobe.f90 dumps the reference's wake state
   probes_wing        a short wing run with six velocity probes (probes.in)
+  scheme_*_wing/rotor  FT1 and AB2 on a wing and a forward-flight rotor
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
   elevate            the reference's 5-bladed axisymmetric test rotor
   (the last two reproduce tests/katzNplotkin-AR04.case and
@@ -212,6 +213,25 @@ def make_all(out):
                        hubCoords=[hx, hy, 0.0], velBody=[0.0, 0.0, 0.0], wakeTruncateNt=tr4,
                        apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
                        streamwiseCoreVec=ROTOR_CORE)
+
+    # the other wake schemes (FT1, AB2) against the reference solver: a wing
+    # and a two-bladed rotor in forward flight, both reaching a far wake with
+    # rollup and truncation within the run.  PC2 is not among them: the
+    # reference's vel_order2_Fwake (libCommon.f90:236-258) writes elements 1
+    # and 0 of its zero-length result while the far-wake slice is empty, which
+    # corrupts the heap (free(): invalid pointer at step 2).
+    for sch, tag in ((0, "ft1"), (2, "ab2")):
+        d = case(f"scheme_{tag}_wing")
+        write_config(os.path.join(d, "config.nml"), nt=60, dt=0.025, nr=1, fdScheme=sch)
+        write_geom(os.path.join(d, "geom01.nml"), nc=2, ns=8, nNwake=20, wakeTruncateNt=45,
+                   spanwiseCore=0.03, streamwiseCoreVec=0.03)
+        d = case(f"scheme_{tag}_rotor")
+        write_config(os.path.join(d, "config.nml"), nt=72, dt=math.radians(5.0) / om, nr=1,
+                     velSound=340.0, fdScheme=sch, wakeDissipation=1, wakeStrain=1)
+        write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=1, ns=10, nNwake=24,
+                   wakeTruncateNt=60, span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25,
+                   velBody=[-13.6, 0.0, -2.0], apparentViscCoeff=ROTOR_AVC,
+                   spanwiseCore=ROTOR_CORE, streamwiseCoreVec=ROTOR_CORE)
 
     # cfg5: rotor in forward flight above a wing (15 revolutions); the
     # _short variant (half a revolution) is compared with the reference.  The

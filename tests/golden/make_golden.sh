@@ -46,7 +46,12 @@ rm -rf "$tmp"
 #    truncation within it) and rotor above a wing (half a revolution)
 run_ref cfg4_quad_short cfg4_quad_short_ForceNonDim.csv
 run_ref cfg5_rotor_wing_short cfg5_rotor_wing_short_ForceNonDim.csv
-# 5. velocity probes: the reference's probes2file output of a short wing run
+# 5. the FT1 and AB2 wake schemes on a wing and a forward-flight rotor
+for tag in ft1 ab2; do
+  run_ref scheme_${tag}_wing scheme_${tag}_wing_ForceNonDim.csv
+  run_ref scheme_${tag}_rotor scheme_${tag}_rotor_ForceNonDim.csv
+done
+# 6. velocity probes: the reference's probes2file output of a short wing run
 tmp=$(mktemp -d)
 cp "$ROOT/cases/probes_wing.case/"* "$tmp/"
 mkdir -p "$tmp/Results" "$tmp/Restart"
