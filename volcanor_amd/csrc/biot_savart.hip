@@ -63,12 +63,13 @@ __device__ __forceinline__ double zero_unless(bool c, double x)
     return __hiloint2double(c ? hi : 0, __double2loint(x));
 }
 
-// FP64 pair; segment record (ax ay az g | r0x r0y r0z qq), qq = (rc |r0|)^4.
+// FP64 pair; segment record (ax ay az g | r0x r0y r0z qq | n0), qq = (rc |r0|)^4,
+// n0 = |r0|^2 (so r0 . r2 = r0 . r1 - n0).
 // The reference's r0 . (r1/|r1| - r2/|r2|) / sqrt(qq + c2^2) is evaluated as
 // d1 / sqrt(n1 den) - d2 / sqrt(n2 den): two reciprocal square roots per
 // pair instead of three; the unitVec guards (|r| <= eps -> 0) stay on n1, n2.
 __device__ __forceinline__ void pair64(double px, double py, double pz, double4 sa, double4 sb,
-                                       double &vx, double &vy, double &vz)
+                                       double n0, double &vx, double &vy, double &vz)
 {
     const double r1x = px - sa.x, r1y = py - sa.y, r1z = pz - sa.z;
     const double r0x = sb.x, r0y = sb.y, r0z = sb.z;
@@ -80,7 +81,7 @@ __device__ __forceinline__ void pair64(double px, double py, double pz, double4 
     const double n1 = r1x * r1x + r1y * r1y + r1z * r1z;
     const double n2 = r2x * r2x + r2y * r2y + r2z * r2z;
     const double d1 = r0x * r1x + r0y * r1y + r0z * r1z;
-    const double d2 = r0x * r2x + r0y * r2y + r0z * r2z;
+    const double d2 = d1 - n0;
     const double den = __builtin_fma(c2, c2, sb.w);
     const double j1 = zero_unless(n1 > EPS2, rsq64(n1 * den));
     const double j2 = zero_unless(n2 > EPS2, rsq64(n2 * den));
@@ -111,7 +112,7 @@ __device__ __forceinline__ bool pair32(float phx, float phy, float phz, float pl
     const float n2 = r2x * r2x + r2y * r2y + r2z * r2z;
     const bool nearp = (c2 < NEAR_CORE2 * s1.w) && (n1 + n2 < s2.w);
     const float d1 = r0x * r1x + r0y * r1y + r0z * r1z;
-    const float d2 = r0x * r2x + r0y * r2y + r0z * r2z;
+    const float d2 = __builtin_fmaf(s2.w, -0.25f, d1);  // d1 - |r0|^2
     const float den = __builtin_fmaf(c2, c2, s1.w * s1.w);
     const float j1 = n1 > EPS2F ? __builtin_amdgcn_rsqf(n1 * den) : 0.0f;
     const float j2 = n2 > EPS2F ? __builtin_amdgcn_rsqf(n2 * den) : 0.0f;
@@ -152,6 +153,7 @@ __global__ __launch_bounds__(TILE) void bs_fp64_kernel(BsArgs a)
 {
     __shared__ double4 sa[TILE];
     __shared__ double4 sb[TILE];
+    __shared__ double sn[TILE];
     __shared__ int item_sh;
     const int tid = threadIdx.x;
     const int64_t nseg = *a.nseg;
@@ -180,10 +182,11 @@ __global__ __launch_bounds__(TILE) void bs_fp64_kernel(BsArgs a)
                 sa[tid] = make_double4(a.s64[s], a.s64[S + s], a.s64[2 * S + s], a.s64[3 * S + s]);
                 sb[tid] = make_double4(a.s64[4 * S + s], a.s64[5 * S + s], a.s64[6 * S + s],
                                        a.s64[7 * S + s]);
+                sn[tid] = a.s64[8 * S + s];
             }
             __syncthreads();
 #pragma unroll 2
-            for (int k = 0; k < cnt; ++k) pair64(px, py, pz, sa[k], sb[k], vx, vy, vz);
+            for (int k = 0; k < cnt; ++k) pair64(px, py, pz, sa[k], sb[k], sn[k], vx, vy, vz);
         }
         if (live) {
             double *o = a.part + (size_t)chunk * 3 * a.m;
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(TILE) void bs_mixed_kernel(BsArgs a)
                            make_double4(a.s64[s], a.s64[S + s], a.s64[2 * S + s], a.s64[3 * S + s]),
                            make_double4(a.s64[4 * S + s], a.s64[5 * S + s], a.s64[6 * S + s],
                                         a.s64[7 * S + s]),
-                           nx, ny, nz);
+                           a.s64[8 * S + s], nx, ny, nz);
                 }
             }
             kahan_add(Sx, Cx, tx);
@@ -331,8 +334,10 @@ __global__ __launch_bounds__(SM_BLOCK) void bs_small_kernel(const double *s64, i
         const double4 sa = make_double4(s64[sg], s64[S + sg], s64[2 * S + sg], s64[3 * S + sg]);
         const double4 sb =
             make_double4(s64[4 * S + sg], s64[5 * S + sg], s64[6 * S + sg], s64[7 * S + sg]);
+        const double n0 = s64[8 * S + sg];
 #pragma unroll
-        for (int q = 0; q < SM_T; ++q) pair64(px[q], py[q], pz[q], sa, sb, vx[q], vy[q], vz[q]);
+        for (int q = 0; q < SM_T; ++q)
+            pair64(px[q], py[q], pz[q], sa, sb, n0, vx[q], vy[q], vz[q]);
     }
 #pragma unroll
     for (int q = 0; q < SM_T; ++q) {

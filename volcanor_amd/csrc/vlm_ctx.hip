@@ -46,6 +46,7 @@ __global__ void compact_kernel(const Slot *slots, const int *flags, const int *i
     d64[5 * cap + k] = r0y;
     d64[6 * cap + k] = r0z;
     d64[7 * cap + k] = q * q;
+    d64[8 * cap + k] = n0;
     const float hx = (float)s.ax, hy = (float)s.ay, hz = (float)s.az;
     d32[k] = hx;
     d32[cap + k] = hy;
@@ -101,8 +102,8 @@ int compact_slots(vlm_ctx *ctx, int64_t nslots)
 {
     const int64_t cap = nslots > 0 ? nslots : 1;
     if (ctx->seg.cap < cap) {
-        VLM_HIP(ctx->seg.d64.reserve((size_t)8 * cap));
-        VLM_HIP(ctx->seg.d32.reserve((size_t)12 * cap));
+        VLM_HIP(ctx->seg.d64.reserve((size_t)SEG64_N * cap));
+        VLM_HIP(ctx->seg.d32.reserve((size_t)SEG32_N * cap));
         ctx->seg.cap = cap;
     }
     return compact_into(ctx, ctx->slots.p, ctx->flags.p, nslots, ctx->seg.d64, ctx->seg.d32,
@@ -169,8 +170,8 @@ int upload_fset(vlm_ctx *ctx, int64_t nfil, const double *a, const double *b,
                        raw.p, raw.p + 3 * nfil, raw.p + 6 * nfil, raw.p + 7 * nfil,
                        ctx->slots.p, ctx->flags.p);
     VLM_HIP(hipGetLastError());
-    VLM_HIP(d64.reserve((size_t)8 * nfil));
-    VLM_HIP(d32.reserve((size_t)12 * nfil));
+    VLM_HIP(d64.reserve((size_t)SEG64_N * nfil));
+    VLM_HIP(d32.reserve((size_t)SEG32_N * nfil));
     VLM_TRY(compact_into(ctx, ctx->slots.p, ctx->flags.p, nfil, d64, d32, count, nfil));
     VLM_TRY(sync_stream(ctx));  // raw is released on return
     return VLM_OK;
@@ -384,8 +385,8 @@ int vlm_filament_vind(vlm_ctx *ctx, int64_t nfil, const double *a, const double 
     FSet fs;
     if (nfil > 0) VLM_TRY(upload_fset(ctx, nfil, a, b, gam, rc, fs.d64, fs.d32, fs.count));
     else {
-        VLM_HIP(fs.d64.reserve(8));
-        VLM_HIP(fs.d32.reserve(12));
+        VLM_HIP(fs.d64.reserve(SEG64_N));
+        VLM_HIP(fs.d32.reserve(SEG32_N));
         VLM_TRY(compact_into(ctx, nullptr, nullptr, 0, fs.d64, fs.d32, fs.count, 1));
     }
     VLM_HIP(ctx->tgt.reserve((size_t)3 * m));

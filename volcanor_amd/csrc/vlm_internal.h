@@ -42,11 +42,14 @@ struct DBuf {
 };
 
 // Compacted straight-segment list: structure of arrays (stride cap).
-//   d64 (8):  ax ay az g | r0x r0y r0z q^2        g = gam/(4 pi), r0 = b - a,
-//                                                 q = rc^2 |r0|^2
+//   d64 (9):  ax ay az g | r0x r0y r0z q^2 | n0   g = gam/(4 pi), r0 = b - a,
+//                                                 q = rc^2 |r0|^2, n0 = |r0|^2
 //   d32 (12): ahx ahy ahz g | alx aly alz q | r0x r0y r0z 4|r0|^2
 //             (a = a_hi + a_lo split into two floats)
 // count: device int64 with the number of valid segments.
+constexpr int SEG64_N = 9;   // doubles per FP64 segment record
+constexpr int SEG32_N = 12;  // floats per FP32 segment record
+
 struct SegList {
     DBuf<double> d64;
     DBuf<float> d32;
