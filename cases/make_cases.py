@@ -12,6 +12,7 @@ PLOT3D blade file) used by the tests and the benchmark.
                      which oracle/refprobe.f90 dumps the reference's wake state
   probes_wing        a short wing run with six velocity probes (probes.in)
   scheme_*_wing/rotor  FT1 and AB2 on a wing and a forward-flight rotor
+  cfg2_rotor_farwake_2rev  cfg2 with a 72-row near wake and a far wake (strain)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
   elevate            the reference's 5-bladed axisymmetric test rotor
   (the last two reproduce tests/katzNplotkin-AR04.case and
@@ -185,6 +186,17 @@ def make_all(out):
                    chord=c, Omega=om, theta0=8.0, pivotLE=0.25, geometryFile="blade.xyz",
                    velBody=[0.0, 0.0, 0.0], apparentViscCoeff=ROTOR_AVC,
                    spanwiseCore=ROTOR_CORE, streamwiseCoreVec=ROTOR_CORE)
+    # cfg2 with a one-revolution near wake rolled up into a far wake (the
+    # variant that exercises strain, which acts on the far wake only); two
+    # revolutions against the reference solver
+    d = case("cfg2_rotor_farwake_2rev")
+    write_config(os.path.join(d, "config.nml"), nt=-2, dt=0.0, nr=1, velSound=340.0,
+                 wakeDissipation=1, wakeStrain=1)
+    twisted_blade_plot3d(os.path.join(d, "blade.xyz"), 1, 20, R, c, -8.0)
+    write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=1, ns=20, nNwake=72, wakeTruncateNt=-2,
+               span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25, geometryFile="blade.xyz",
+               velBody=[0.0, 0.0, 0.0], apparentViscCoeff=ROTOR_AVC,
+               spanwiseCore=ROTOR_CORE, streamwiseCoreVec=ROTOR_CORE)
 
     # small forward-flight rotor for the per-node wake comparison with the
     # reference (oracle/refprobe.f90): far wake, rollup, truncation shift,

@@ -32,6 +32,7 @@ run_ref() {  # case-name  output-name
 }
 [ -e "$HERE/cfg1_wing_ForceNonDim.csv" ] || run_ref cfg1_wing cfg1_wing_ForceNonDim.csv
 run_ref cfg2_rotor_2rev cfg2_rotor_2rev_ForceNonDim.csv
+run_ref cfg2_rotor_farwake_2rev cfg2_rotor_farwake_2rev_ForceNonDim.csv
 # 3. per-step wing and wake state of the reference (oracle/_ref/refprobe,
 #    our driver linked against the reference objects) on the small
 #    forward-flight rotor: ring corners, core radii, circulation, velocity

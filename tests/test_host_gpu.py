@@ -144,6 +144,16 @@ def test_cfg2_two_revs_matches_reference_solver():
     np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
 
 
+def test_cfg2_far_wake_matches_reference_solver():
+    """cfg2 with a one-revolution near wake: from step 72 on, the oldest row
+    rolls up into a far wake every step and strain (far wake only) stretches
+    its cores; two revolutions against the reference solver."""
+    ref = read_force(os.path.join(GOLD, "cfg2_rotor_farwake_2rev_ForceNonDim.csv"))[:, 0]
+    ct, _ = run("cfg2_rotor_farwake_2rev")
+    assert len(ct[0]) == len(ref) == 145
+    np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
+
+
 def test_cfg2_mixed_precision_ct_within_1e4():
     """Mixed FP32/FP64 Biot-Savart against the FP64 path, two revolutions."""
     a, _ = run("cfg2_rotor_2rev", "fp64")
