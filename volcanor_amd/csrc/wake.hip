@@ -1200,7 +1200,6 @@ int learned_pass(vlm_ctx *ctx)
     VLM_HIP(hipGetLastError());
     const bool warm = P.steps < P.warmup;
     P.steps += 1;
-    bool full = warm;
     int64_t ms = 0, phase = 0, stride = 1;
     const double *judged_sums = nullptr;
     if (!warm) {
@@ -1260,7 +1259,6 @@ int learned_pass(vlm_ctx *ctx)
         P.sum_base2 += h2;
         P.judged += 1;
         if (P.last_err > P.tol) {
-            full = true;
             P.fallbacks += 1;
             VLM_TRY(full_pass(ctx, m, nmax));
         } else {
