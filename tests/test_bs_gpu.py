@@ -95,6 +95,52 @@ def test_mixed_precision_helix_error_bound():
     assert np.max(err) / rms > 1e-10   # FP32 pair arithmetic did run
 
 
+def test_mixed_precision_near_core_targets():
+    """Targets around isolated segments, radial distance h from 0 to 4 core
+    radii, along-segment position from -0.2 to 1.2 of the segment, plus
+    targets exactly on the end points.  Inside the core (h < rc and
+    |r1|^2 + |r2|^2 < 4 |r0|^2) the mixed kernel recomputes the pair in
+    FP64, so it agrees with the FP64 oracle far below the FP32 error;
+    outside, the FP32 pair's error stays within 5e-6 of the peak speed (a
+    numpy emulation of the FP32 pair gives at most 1.5e-6 at h < rc and
+    7e-7 at rc <= h < 2 rc, for segment length / core radius from 0.1 to
+    10)."""
+    rng = np.random.default_rng(77)
+    ns, m = 64, 4096
+    # segments 1000 apart: the FP32 far field of the other segments (whose
+    # d1/|r1| - d2/|r2| cancels to a relative error of about eps d / |r0|)
+    # stays near 1e-10 of the peak; at 10 apart it reaches 1e-8
+    a = 1000.0 * np.stack(np.meshgrid(*[np.arange(4)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    a = a + rng.uniform(-1, 1, size=a.shape)
+    d = rng.normal(size=(ns, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    L = rng.uniform(0.05, 0.5, ns)
+    rc = L * rng.uniform(0.1, 2.0, ns)
+    b = a + L[:, None] * d
+    g = rng.normal(size=ns)
+    k = np.arange(m) % ns
+    t = rng.uniform(-0.2, 1.2, m)
+    h = rc[k] * rng.uniform(0, 4, m)
+    t[:ns], h[:ns] = 0.0, 0.0          # on the start points
+    t[ns:2 * ns], h[ns:2 * ns] = 1.0, 0.0   # on the end points
+    e = rng.normal(size=(m, 3))
+    e -= np.sum(e * d[k], 1)[:, None] * d[k]
+    e /= np.linalg.norm(e, axis=1)[:, None]
+    p = a[k] + t[:, None] * (b[k] - a[k]) + h[:, None] * e
+    w = orc.filament_sum(a, b, g, rc, p, threads=8)
+    v = _ctx(1).filament_vind(a, b, g, rc, p)
+    peak = np.max(np.linalg.norm(w, axis=1))
+    err = np.linalg.norm(v - w, axis=1) / peak
+    # the kernel's FP64 region: h < rc and |r1|^2 + |r2|^2 < 4 |r0|^2 (with
+    # a margin for the FP32 evaluation of both tests)
+    n12 = np.sum((p - a[k])**2, 1) + np.sum((p - b[k])**2, 1)
+    core = (h < 0.999 * rc[k]) & (n12 < 0.999 * 4 * L[k]**2)
+    assert core.sum() > 200
+    assert np.max(err[core]) < 1e-9, np.max(err[core])   # 3.5e-7 in FP32 (emulated)
+    assert np.max(err) < 5e-6, np.max(err)
+    assert np.max(err[~core]) > 1e-12   # FP32 pair arithmetic did run
+
+
 def test_resident_sets_and_profile():
     rng = np.random.default_rng(4)
     a, b = rng.uniform(-2, 2, size=(2, 4096, 3))

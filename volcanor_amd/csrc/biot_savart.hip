@@ -25,8 +25,9 @@
 //           (r1 = (p_hi - a_hi) + (p_lo - a_lo), accurate to FP32 relative
 //           precision of r1 itself), v_rsq_f32, per-tile FP32 sums folded
 //           into a Kahan-compensated accumulator.  Pairs with the target
-//           within two core radii of the segment are recomputed in FP64
-//           (the near-singular core and target-on-filament terms).
+//           inside the core (within one core radius of the segment) are
+//           recomputed in FP64 (the regularised core and target-on-filament
+//           terms, where the reference's exact-zero guards act).
 //           Launches with fewer than 1024 targets (collocation points, judge
 //           samples) run the FP64 small-target kernel in both modes.
 #include "vlm_internal.h"
@@ -37,7 +38,7 @@ namespace {
 constexpr int BS_BLOCK = 256;       // wake passes: many targets
 constexpr double EPS2 = kEps * kEps;
 constexpr float EPS2F = 4.93038065763132e-32f;
-constexpr float NEAR_CORE2 = 4.0f;  // h < 2 rc, and |r1|^2 + |r2|^2 < 4 |r0|^2 (stored)
+constexpr float NEAR_CORE2 = 1.0f;  // h < rc, and |r1|^2 + |r2|^2 < 4 |r0|^2 (stored)
 
 __device__ __forceinline__ double rsq64(double x)
 {
