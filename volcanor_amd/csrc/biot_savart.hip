@@ -55,8 +55,8 @@ __device__ __forceinline__ double rsq64(double x)
 // x where c holds, else a value that acts as zero here: only the high dword
 // (sign, exponent, top of the mantissa) is cleared, one select instead of
 // two.  What is left is a positive number below 2^-1042.  It only enters
-// products with a factor below 2^-50 (d1 when |r1| <= eps, c when
-// |r1 x r2| <= eps), which underflow to exactly 0, so the sums are bitwise
+// products with the components of c = r0 x r1, below 2^-51 when
+// |r1 x r2| <= eps, which underflow to exactly 0, so the sums are bitwise
 // those of a true zero.
 __device__ __forceinline__ double zero_unless(bool c, double x)
 {
@@ -68,7 +68,13 @@ __device__ __forceinline__ double zero_unless(bool c, double x)
 // n0 = |r0|^2 (so r0 . r2 = r0 . r1 - n0).
 // The reference's r0 . (r1/|r1| - r2/|r2|) / sqrt(qq + c2^2) is evaluated as
 // d1 / sqrt(n1 den) - d2 / sqrt(n2 den): two reciprocal square roots per
-// pair instead of three; the unitVec guards (|r| <= eps -> 0) stay on n1, n2.
+// pair instead of three.  The pair guard |r1 x r2|^2 <= eps^2 is exact.  The
+// unitVec guard (|r| <= eps -> that term 0) is a clamp n >= eps^2, one
+// v_max instead of a compare and a select: on an end point (r = 0) the term
+// is d * j = 0 exactly, as in the reference; for 0 < |r| <= eps,
+// |r1 x r2| <= |r0| |r| lets the pair guard zero the pair whenever
+// |r0| <= 1, and for longer segments the term left over is of the order of
+// the pair's own (eps-sized) contribution.
 __device__ __forceinline__ void pair64(double px, double py, double pz, double4 sa, double4 sb,
                                        double n0, double &vx, double &vy, double &vz)
 {
@@ -84,8 +90,8 @@ __device__ __forceinline__ void pair64(double px, double py, double pz, double4 
     const double d1 = r0x * r1x + r0y * r1y + r0z * r1z;
     const double d2 = d1 - n0;
     const double den = __builtin_fma(c2, c2, sb.w);
-    const double j1 = zero_unless(n1 > EPS2, rsq64(n1 * den));
-    const double j2 = zero_unless(n2 > EPS2, rsq64(n2 * den));
+    const double j1 = rsq64(__builtin_fmax(n1, EPS2) * den);
+    const double j2 = rsq64(__builtin_fmax(n2, EPS2) * den);
     const double s = zero_unless(c2 > EPS2, sa.w * (d1 * j1 - d2 * j2));
     vx += cx * s;
     vy += cy * s;
@@ -115,8 +121,8 @@ __device__ __forceinline__ bool pair32(float phx, float phy, float phz, float pl
     const float d1 = r0x * r1x + r0y * r1y + r0z * r1z;
     const float d2 = __builtin_fmaf(s2.w, -0.25f, d1);  // d1 - |r0|^2
     const float den = __builtin_fmaf(c2, c2, s1.w * s1.w);
-    const float j1 = n1 > EPS2F ? __builtin_amdgcn_rsqf(n1 * den) : 0.0f;
-    const float j2 = n2 > EPS2F ? __builtin_amdgcn_rsqf(n2 * den) : 0.0f;
+    const float j1 = __builtin_amdgcn_rsqf(__builtin_fmaxf(n1, EPS2F) * den);
+    const float j2 = __builtin_amdgcn_rsqf(__builtin_fmaxf(n2, EPS2F) * den);
     float s = s0.w * (d1 * j1 - d2 * j2);
     s = (c2 > EPS2F && !nearp) ? s : 0.0f;
     vx += cx * s;
