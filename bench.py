@@ -394,7 +394,9 @@ def main():
                     help="comma list of rotor2 variants run beside fp64 (mixed, learned_mixed)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--learned", action="store_true")
-    ap.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--tol", type=float, default=5e-4,
+                    help="judge tolerance of the learned pass: RMS error / RMS speed on the "
+                         "judge sample above which the pass is recomputed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-twin", action="store_true",
                     help="skip the perturbed FP64 twin run (chaos floor of CT)")
