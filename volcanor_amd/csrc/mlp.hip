@@ -435,6 +435,7 @@ int vlm_mlp_num_params(void) { return MLP_NP; }
 int vlm_mlp_set_params(vlm_ctx *ctx, const float *params)
 {
     if (!ctx || !params) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     Mlp *M = get_mlp(ctx);
     if (!M) return fail(ctx, VLM_ERR_ALLOC, "mlp allocation failed");
     VLM_HIP(hipMemcpy(M->params.p, params, MLP_NP * sizeof(float), hipMemcpyHostToDevice));
@@ -448,6 +449,7 @@ int vlm_mlp_set_params(vlm_ctx *ctx, const float *params)
 int vlm_mlp_get_params(vlm_ctx *ctx, float *params)
 {
     if (!ctx || !params) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     Mlp *M = get_mlp(ctx);
     if (!M) return fail(ctx, VLM_ERR_ALLOC, "mlp allocation failed");
     VLM_TRY(sync_stream(ctx));
@@ -458,6 +460,7 @@ int vlm_mlp_get_params(vlm_ctx *ctx, float *params)
 int vlm_mlp_forward(vlm_ctx *ctx, int64_t m, const float *x, float *y)
 {
     if (!ctx || m < 0 || (m > 0 && (!x || !y))) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     Mlp *M = get_mlp(ctx);
     if (!M) return fail(ctx, VLM_ERR_ALLOC, "mlp allocation failed");
     if (m == 0) return VLM_OK;
@@ -476,6 +479,7 @@ int vlm_mlp_loss_grad(vlm_ctx *ctx, int64_t m, const float *x, const float *t, f
                       float *loss)
 {
     if (!ctx || m <= 0 || !x || !t) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     Mlp *M = get_mlp(ctx);
     if (!M) return fail(ctx, VLM_ERR_ALLOC, "mlp allocation failed");
     VLM_HIP(M->xin.reserve((size_t)m * (MLP_IN + MLP_OUT)));
@@ -496,6 +500,7 @@ int vlm_mlp_train_step(vlm_ctx *ctx, int64_t m, const float *x, const float *t, 
                        float *loss)
 {
     if (!ctx || m <= 0 || !x || !t) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     Mlp *M = get_mlp(ctx);
     if (!M) return fail(ctx, VLM_ERR_ALLOC, "mlp allocation failed");
     VLM_HIP(M->xin.reserve((size_t)m * (MLP_IN + MLP_OUT)));

@@ -302,6 +302,7 @@ extern "C" {
 int vlm_rotor_calc_aic(vlm_ctx *ctx, int ir, const double *cp, const double *ncap, double *Aout)
 {
     if (!ctx || ir < 0 || ir >= (int)ctx->rotors.size()) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (!cp || !ncap) return fail(ctx, VLM_ERR_ARG, "null buffer");
     Rotor &r = *ctx->rotors[ir];
     const int n = r.nring;
@@ -342,6 +343,7 @@ int vlm_rotor_solve(vlm_ctx *ctx, int ir, const double *rhs, double *gam, int ma
                     double tol, int *iters)
 {
     if (!ctx || ir < 0 || ir >= (int)ctx->rotors.size()) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (!rhs || !gam) return fail(ctx, VLM_ERR_ARG, "null buffer");
     if (!ctx->lu[ir]) return fail(ctx, VLM_ERR_STATE, "vlm_rotor_calc_aic not called");
     const int n = ctx->aic_n[ir];
@@ -353,6 +355,7 @@ int vlm_rotor_solve_deferred(vlm_ctx *ctx, int ir, const double *rhs, double *ga
                              double tol, int *iters)
 {
     if (!ctx || ir < 0 || ir >= (int)ctx->rotors.size()) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (!rhs || !gam) return fail(ctx, VLM_ERR_ARG, "null buffer");
     if (!ctx->lu[ir]) return fail(ctx, VLM_ERR_STATE, "vlm_rotor_calc_aic not called");
     const int n = ctx->aic_n[ir];
@@ -363,6 +366,7 @@ int vlm_rotor_solve_deferred(vlm_ctx *ctx, int ir, const double *rhs, double *ga
 int vlm_lu_factor(vlm_ctx *ctx, int n, double *A, int *ipiv)
 {
     if (!ctx || n <= 0 || !A || !ipiv) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     DBuf<double> dA;
     DBuf<int> dp;
     VLM_HIP(dA.reserve((size_t)n * n));
@@ -382,6 +386,7 @@ int vlm_lu_solve_refine(vlm_ctx *ctx, int n, const double *A, const double *LU, 
                         const double *rhs, double *x, int max_iter, double tol, int *iters)
 {
     if (!ctx || n <= 0 || !A || !LU || !ipiv || !rhs || !x) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     DBuf<double> dA, dLU;
     DBuf<int> dp;
     VLM_HIP(dA.reserve((size_t)n * n));

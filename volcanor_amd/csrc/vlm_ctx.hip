@@ -445,6 +445,7 @@ int vlm_fset_vind(vlm_ctx *ctx, int fset, int pset)
     if (fset < 0 || fset >= (int)ctx->fsets.size() || pset < 0 ||
         pset >= (int)ctx->psets.size())
         return fail(ctx, VLM_ERR_ARG, "unknown filament or point set");
+    VLM_HIP(hipSetDevice(ctx->device));
     FSet *fs = ctx->fsets[fset];
     PSet *ps = ctx->psets[pset];
     return bs_eval(ctx, fs->d64.p, fs->d32.p, fs->n, fs->count.p, fs->n, ps->p.p, ps->m, ps->m,
@@ -456,6 +457,7 @@ int vlm_pset_velocity(vlm_ctx *ctx, int pset, double *v)
     if (!ctx || !v) return VLM_ERR_ARG;
     if (pset < 0 || pset >= (int)ctx->psets.size())
         return fail(ctx, VLM_ERR_ARG, "unknown point set");
+    VLM_HIP(hipSetDevice(ctx->device));
     PSet *ps = ctx->psets[pset];
     return points_to_host(ctx, ps->m, ps->v.p, ps->m, v);
 }

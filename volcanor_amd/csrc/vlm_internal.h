@@ -189,6 +189,9 @@ int hip_fail(vlm_ctx *ctx, hipError_t e, const char *what);
         if (e_ != hipSuccess) return vlm::hip_fail(ctx, e_, #call); \
     } while (0)
 
+// Every entry point that touches the GPU makes the context's device current.
+#define VLM_DEVICE(ctx) VLM_HIP(hipSetDevice((ctx)->device))
+
 #define VLM_TRY(call)                 \
     do {                              \
         int rc_ = (call);             \

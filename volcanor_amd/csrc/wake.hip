@@ -1227,10 +1227,10 @@ int learned_pass(vlm_ctx *ctx)
         // the sums travel to pinned memory behind an event; the host waits
         // for them only after the training below is enqueued, so the GPU
         // trains while the host decides
-        if (!ctx->judge_ev) {
+        if (!ctx->judge_ev)
             VLM_HIP(hipEventCreateWithFlags(&ctx->judge_ev, hipEventDisableTiming));
+        if (!ctx->judge_pin)
             VLM_HIP(hipHostMalloc(reinterpret_cast<void **>(&ctx->judge_pin), 4 * sizeof(double)));
-        }
         VLM_HIP(hipMemcpyAsync(ctx->judge_pin, P.scal.p + 1, 3 * sizeof(double),
                                hipMemcpyDeviceToHost, ctx->stream));
         VLM_HIP(hipEventRecord(ctx->judge_ev, ctx->stream));
@@ -1387,6 +1387,7 @@ int vlm_rotor_set_wing(vlm_ctx *ctx, int ir, const double *corners, const double
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (corners) VLM_TRY(h2d(ctx, r->wc.p, corners, sizeof(double) * 12 * r->nring));
     if (rc) VLM_TRY(h2d(ctx, r->wrc.p, rc, sizeof(double) * 4 * r->nring));
     if (gam) VLM_TRY(h2d(ctx, r->wg.p, gam, sizeof(double) * r->nring));
@@ -1433,6 +1434,7 @@ int vlm_rotor_assignshed(vlm_ctx *ctx, int ir, int edge)
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (r->d.nNwake <= 0) return VLM_OK;
     if (edge != VLM_EDGE_LE && edge != VLM_EDGE_TE)
         return fail(ctx, VLM_ERR_ARG, "edge must be VLM_EDGE_LE or VLM_EDGE_TE");
@@ -1453,6 +1455,7 @@ int vlm_rotor_age_wake(vlm_ctx *ctx, int ir, double dt)
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (r->d.nNwake <= 0 || r->rowNear > r->d.nNwake) return VLM_OK;
     const int64_t R = r->d.nNwake - (r->rowNear - 1);
     const int64_t F = r->d.nFwake - (r->rowFar - 1);
@@ -1467,6 +1470,7 @@ int vlm_rotor_dissipate_wake(vlm_ctx *ctx, int ir, double dt, double kinematicVi
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (r->d.nNwake <= 0 || r->rowNear > r->d.nNwake) return VLM_OK;
     const double oseen = 1.2564;
     const double inc = 4.0 * oseen * apparentViscCoeff * kinematicVisc * dt;
@@ -1487,6 +1491,7 @@ int vlm_rotor_strain_wake(vlm_ctx *ctx, int ir)
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     const int64_t F = r->d.nFwake - (r->rowFar - 1);
     if (r->d.nNwake <= 0 || F <= 0) return VLM_OK;
     hipLaunchKernelGGL(strain_kernel, dim3(nblk(F * r->d.nb)), dim3(256), 0, ctx->stream,
@@ -1499,6 +1504,7 @@ int vlm_rotor_rollup(vlm_ctx *ctx, int ir)
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (r->d.nNwake <= 0) return VLM_OK;
     int fi = r->rowFar - 2;  // rowFarNext - 1, 0-based
     if (r->d.nFwake > 0 && fi < 0) {
@@ -1526,6 +1532,7 @@ int vlm_rotor_convectwake(vlm_ctx *ctx, int ir, double dt, int wakeType)
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (wakeType != VLM_WAKE_C && wakeType != VLM_WAKE_P)
         return fail(ctx, VLM_ERR_ARG, "wakeType must be VLM_WAKE_C or VLM_WAKE_P");
     return rotor_convect(ctx, *r, dt, wakeType == VLM_WAKE_P);
@@ -1536,6 +1543,7 @@ int vlm_rotor_induced_velocity(vlm_ctx *ctx, int ir, int set, int64_t m, const d
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (set < VLM_SET_WING || set > VLM_SET_WAKE_PRED) return fail(ctx, VLM_ERR_ARG, "bad set");
     if (m <= 0) return VLM_OK;
     if (!p || !v) return fail(ctx, VLM_ERR_ARG, "null buffer");
@@ -1555,6 +1563,7 @@ static int induced_terms(vlm_ctx *ctx, int nterms, const int *ir, const int *set
                          int accumulate, bool deferred)
 {
     if (!ctx) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (nterms < 0 || (nterms > 0 && (!ir || !set || !flags)))
         return fail(ctx, VLM_ERR_ARG, "bad term list");
     for (int k = 0; k < nterms; ++k) {
@@ -1605,6 +1614,7 @@ int vlm_induced_velocity_terms_deferred(vlm_ctx *ctx, int nterms, const int *ir,
 int vlm_wake_velocity(vlm_ctx *ctx, int wakeType)
 {
     if (!ctx) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (wakeType != VLM_WAKE_C && wakeType != VLM_WAKE_P)
         return fail(ctx, VLM_ERR_ARG, "wakeType must be VLM_WAKE_C or VLM_WAKE_P");
     return wake_pass(ctx, wakeType == VLM_WAKE_P);
@@ -1613,6 +1623,7 @@ int vlm_wake_velocity(vlm_ctx *ctx, int wakeType)
 int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
 {
     if (!ctx) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (scheme < VLM_SCHEME_FT1 || scheme > VLM_SCHEME_AM2)
         return fail(ctx, VLM_ERR_ARG, "fdScheme must be 0..3");
     VLM_TRY(wake_pass(ctx, 0));
@@ -1698,6 +1709,7 @@ int vlm_rotor_get_wake(vlm_ctx *ctx, int ir, int wakeType, double *nodes, double
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     const int pred = wakeType == VLM_WAKE_P;
     const size_t nb = r->d.nb, nN = r->d.nNwake, ns = r->d.ns, nF = r->d.nFwake;
     VLM_TRY(sync_stream(ctx));
@@ -1741,6 +1753,7 @@ int vlm_rotor_set_wake(vlm_ctx *ctx, int ir, const double *nodes, const double *
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     const size_t nb = r->d.nb, nN = r->d.nNwake, ns = r->d.ns, nF = r->d.nFwake;
     if (rowNear < 1 || rowNear > (int)nN + 1 || rowFar < 1 || rowFar > (int)nF + 1)
         return fail(ctx, VLM_ERR_ARG, "cursor out of range");
@@ -1781,6 +1794,7 @@ int vlm_rotor_get_velocity(vlm_ctx *ctx, int ir, int which, double *velN, double
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (which < VLM_VEL || which > VLM_VEL_STEP) return fail(ctx, VLM_ERR_ARG, "bad which");
     VLM_TRY(sync_stream(ctx));
     if (velN && r->nvn() > 0)
@@ -1810,6 +1824,7 @@ int vlm_set_predictor(vlm_ctx *ctx, int mode, double tol, int sample_stride, int
                       int train_iters, double lref)
 {
     if (!ctx) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     if (mode != 0 && mode != 1) return fail(ctx, VLM_ERR_ARG, "predictor mode must be 0 or 1");
     if (sample_stride < 1 || warmup_steps < 0 || train_iters < 0 || !(lref > 0) || !(tol > 0))
         return fail(ctx, VLM_ERR_ARG, "bad predictor settings");
@@ -1829,6 +1844,7 @@ int vlm_predictor_stats(vlm_ctx *ctx, int64_t *full, int64_t *learned, int64_t *
                         double *last_err, double *last_loss)
 {
     if (!ctx) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
     Predictor &P = ctx->pred;
     if (full) *full = P.full;
     if (learned) *learned = P.learned;

@@ -165,6 +165,7 @@ class Context:
 
     def synchronize(self):
         self._chk(self.lib.vlm_synchronize(self.h))
+        self._pending = []
 
     # -- explicit filament lists -----------------------------------------
     def filament_vind(self, a, b, gam, rc, p, v=None):
@@ -402,7 +403,7 @@ class Context:
                                               C.byref(loss)))
         return loss.value
 
-    def set_predictor(self, mode, tol=1e-4, sample_stride=32, warmup=8, train_iters=4, lref=1.0):
+    def set_predictor(self, mode, tol=5e-4, sample_stride=32, warmup=8, train_iters=4, lref=1.0):
         self._chk(self.lib.vlm_set_predictor(self.h, mode, tol, sample_stride, warmup,
                                              train_iters, lref))
 
@@ -414,7 +415,6 @@ class Context:
         return dict(full=a.value, learned=b.value, fallbacks=c.value, last_err=e.value,
                     last_loss=l.value)
 
-    # -- timing ----------------------------------------------------------
     def predictor_errors(self):
         """Relative RMS error of the learned pass 2 and of the network-free
         substitute on the judge samples, and the number of judged steps."""
@@ -422,6 +422,7 @@ class Context:
         self._chk(self.lib.vlm_predictor_errors(self.h, C.byref(a), C.byref(b), C.byref(n)))
         return {"mlp": a.value, "base": b.value, "judged": n.value}
 
+    # -- timing ----------------------------------------------------------
     def profile_enable(self, on=True):
         self._chk(self.lib.vlm_profile_enable(self.h, int(on)))
 
