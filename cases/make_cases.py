@@ -11,7 +11,8 @@ PLOT3D blade file) used by the tests and the benchmark.
   refprobe_rotor     2 x (2 x 3) rotor in forward flight, 14 steps: the case on
                      which oracle/refprobe.f90 dumps the reference's wake state
   probes_wing        a short wing run with six velocity probes (probes.in)
-  scheme_*_wing/rotor  FT1 and AB2 on a wing and a forward-flight rotor
+  scheme_*_wing/rotor  FT1, AB2, AM3 and AM4 on a wing and a forward-flight
+                     rotor
   cfg2_rotor_farwake_2rev  cfg2 with a 72-row near wake and a far wake (strain)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
   elevate            the reference's 5-bladed axisymmetric test rotor
@@ -226,13 +227,13 @@ def make_all(out):
                        apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
                        streamwiseCoreVec=ROTOR_CORE)
 
-    # the other wake schemes (FT1, AB2) against the reference solver: a wing
+    # the other wake schemes (FT1, AB2, AM3, AM4) against the reference solver: a wing
     # and a two-bladed rotor in forward flight, both reaching a far wake with
     # rollup and truncation within the run.  PC2 is not among them: the
     # reference's vel_order2_Fwake (libCommon.f90:236-258) writes elements 1
     # and 0 of its zero-length result while the far-wake slice is empty, which
     # corrupts the heap (free(): invalid pointer at step 2).
-    for sch, tag in ((0, "ft1"), (2, "ab2")):
+    for sch, tag in ((0, "ft1"), (2, "ab2"), (4, "am3"), (5, "am4")):
         d = case(f"scheme_{tag}_wing")
         write_config(os.path.join(d, "config.nml"), nt=60, dt=0.025, nr=1, fdScheme=sch)
         write_geom(os.path.join(d, "geom01.nml"), nc=2, ns=8, nNwake=20, wakeTruncateNt=45,

@@ -72,7 +72,7 @@ enum { VLM_WAKE_C = 0, VLM_WAKE_P = 1 };     /* current / predicted wake */
 enum { VLM_EDGE_LE = 0, VLM_EDGE_TE = 1 };   /* assignshed edge */
 enum {                                        /* fdScheme, main.f90:844 */
     VLM_SCHEME_FT1 = 0, VLM_SCHEME_PC2 = 1, VLM_SCHEME_AB2 = 2,
-    VLM_SCHEME_AM2 = 3
+    VLM_SCHEME_AM2 = 3, VLM_SCHEME_AM3 = 4, VLM_SCHEME_AM4 = 5
 };
 enum {                                        /* vlm_rotor_get_velocity */
     VLM_VEL = 0, VLM_VEL_PREV = 1, VLM_VEL_PRED = 2, VLM_VEL_STEP = 3
@@ -176,8 +176,9 @@ int vlm_induced_velocity_terms(vlm_ctx *ctx, int nterms, const int *ir, const in
 int vlm_wake_velocity(vlm_ctx *ctx, int wakeType);
 
 /* One whole convection block, main.f90:800-1406: pass 1, the scheme
- * algebra of fdScheme (FT1, PC2, AB2 or AM2), pass 2 where the scheme has
- * one, and the final convection of every rotor. */
+ * algebra of fdScheme (FT1, PC2, AB2, AM2, AM3 or AM4), pass 2 where the
+ * scheme has one, and the final convection of every rotor.  The learned
+ * pass 2 (vlm_set_predictor) replaces pass 2 of AM2 only. */
 int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt);
 
 /* State access (tests, restart files, plots). */

@@ -47,8 +47,8 @@ rm -rf "$tmp"
 #    truncation within it) and rotor above a wing (half a revolution)
 run_ref cfg4_quad_short cfg4_quad_short_ForceNonDim.csv
 run_ref cfg5_rotor_wing_short cfg5_rotor_wing_short_ForceNonDim.csv
-# 5. the FT1 and AB2 wake schemes on a wing and a forward-flight rotor
-for tag in ft1 ab2; do
+# 5. the FT1, AB2, AM3 and AM4 wake schemes on a wing and a forward-flight rotor
+for tag in ft1 ab2 am3 am4; do
   run_ref scheme_${tag}_wing scheme_${tag}_wing_ForceNonDim.csv
   run_ref scheme_${tag}_rotor scheme_${tag}_rotor_ForceNonDim.csv
 done

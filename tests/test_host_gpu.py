@@ -184,13 +184,14 @@ def test_learned_predictor_gain_over_extrapolation():
     assert np.max(np.abs(ct[0] - ref)) < 1e-3 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("name", ["scheme_ft1_wing", "scheme_ft1_rotor", "scheme_ab2_wing",
-                                  "scheme_ab2_rotor"])
+@pytest.mark.parametrize("name", [f"scheme_{s}_{g}" for s in ("ft1", "ab2", "am3", "am4")
+                                  for g in ("wing", "rotor")])
 def test_other_schemes_match_reference_solver(name):
-    """FT1 (main.f90:846-859) and AB2 (951-1000) on a wing and on a rotor in
-    forward flight with dissipation and strain, each reaching a far wake
-    with rollup and truncation: the CL/CT history against the reference
-    solver's (7 printed significant digits)."""
+    """FT1 (main.f90:846-859), AB2 (951-1000), AM3 (1117-1248, whose
+    start-up test iter == 0 is never true) and AM4 (1250-1404) on a wing and
+    on a rotor in forward flight with dissipation and strain, each reaching
+    a far wake with rollup and truncation: the CL/CT history against the
+    reference solver's (7 printed significant digits)."""
     ref = read_force(os.path.join(GOLD, name + "_ForceNonDim.csv"))[:, 0]
     ct, _ = run(name)
     assert len(ct[0]) == len(ref)

@@ -62,6 +62,10 @@ struct Slot {
     double ax, ay, az, bx, by, bz, gam, rc;
 };
 
+// velocity arrays per rotor: the four of include/vlm_c.h, then two more
+// histories used only by AM3 and AM4
+constexpr int VEL_H2 = 4, VEL_H3 = 5, NVEL = 6;
+
 struct Rotor {
     vlm_rotor_desc d{};
     int rowNear = 0, rowFar = 0;  // 1-based, as in the reference
@@ -78,8 +82,9 @@ struct Rotor {
     DBuf<double> ffc, pffc;       // 6*nb*nF
     DBuf<double> fgam, frc, frc0, fl0, flc, fage;  // nb*nF
     DBuf<double> ftmp;            // shift staging, 11*nb*nF
-    // velocities, logical rows: [VLM_VEL, PREV, PRED, STEP]
-    DBuf<double> velN[4], velF[4];
+    // velocities, logical rows: [VLM_VEL, PREV, PRED, STEP] and the older
+    // histories of AM3/AM4 (velNwake2, velNwake3; PREV is velNwake1)
+    DBuf<double> velN[NVEL], velF[NVEL];
     size_t nvn() const { return 3u * d.nb * (size_t)d.nNwake * (d.ns + 1); }
     size_t nvf() const { return 3u * d.nb * (size_t)(d.nFwake > 0 ? d.nFwake : 0); }
 };

@@ -661,6 +661,56 @@ __global__ void scheme_kernel(double *v, double *v1, double *vp, double *vs, int
     }
 }
 
+// AM3 and AM4 velocity algebra (main.f90:1117-1404) on whole logical-row
+// arrays: v = velNwake, v1..v3 = velNwake1..3, vp = velNwakePredicted,
+// vs = velNwakeStep.
+enum AmOp {
+    AM3_PRED, AM3_CORR, AM3_ROTATE,  // 1160-1171, 1219-1235
+    AM4_PRED, AM4_CORR, AM4_ROTATE,  // 1307-1320, 1365-1385
+    AM_START1, AM_START2, AM_START3  // start-up copies v1|v2|v3 = v
+};
+
+__global__ void am_kernel(double *v, double *v1, double *v2, double *v3, const double *vp,
+                          double *vs, int64_t n, int op)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    switch (op) {
+    case AM3_PRED:
+        vs[k] = v[k];
+        v[k] = (23.0 * v[k] - 16.0 * v2[k] + 5.0 * v1[k]) / 12.0;
+        break;
+    case AM3_CORR:
+        v[k] = (5.0 * vp[k] + 8.0 * vs[k] - v2[k]) / 12.0;
+        break;
+    case AM3_ROTATE:
+        v1[k] = v2[k];
+        v2[k] = vs[k];
+        break;
+    case AM4_PRED:
+        vs[k] = v[k];
+        v[k] = (55.0 * v[k] - 59.0 * v3[k] + 37.0 * v2[k] - 9.0 * v1[k]) / 24.0;
+        break;
+    case AM4_CORR:
+        v[k] = (9.0 * vp[k] + 19.0 * vs[k] - 5.0 * v3[k] + v2[k]) / 24.0;
+        break;
+    case AM4_ROTATE:
+        v1[k] = v2[k];
+        v2[k] = v3[k];
+        v3[k] = vs[k];
+        break;
+    case AM_START1:
+        v1[k] = v[k];
+        break;
+    case AM_START2:
+        v2[k] = v[k];
+        break;
+    default:
+        v3[k] = v[k];
+        break;
+    }
+}
+
 // vel_order2_Nwake / vel_order2_Fwake (libCommon.f90:213-258) on the
 // active window, written to `out` (logical rows).
 __global__ void order2_kernel(const double *vn, const double *vnp1, double *out, int nbc,
@@ -830,6 +880,24 @@ int launch_scheme(vlm_ctx *ctx, Rotor &r, int op)
         hipLaunchKernelGGL(scheme_kernel, dim3(nblk(nf)), dim3(256), 0, ctx->stream,
                            r.velF[VLM_VEL].p, r.velF[VLM_VEL_PREV].p, r.velF[VLM_VEL_PRED].p,
                            r.velF[VLM_VEL_STEP].p, nf, op);
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+int launch_am(vlm_ctx *ctx, Rotor &r, int op)
+{
+    const int64_t nn = (int64_t)3 * r.d.nbConvect * r.d.nNwake * (r.d.ns + 1);
+    const int64_t nf = (int64_t)3 * r.d.nbConvect * r.d.nFwake;
+    if (nn > 0)
+        hipLaunchKernelGGL(am_kernel, dim3(nblk(nn)), dim3(256), 0, ctx->stream,
+                           r.velN[VLM_VEL].p, r.velN[VLM_VEL_PREV].p, r.velN[VEL_H2].p,
+                           r.velN[VEL_H3].p, r.velN[VLM_VEL_PRED].p, r.velN[VLM_VEL_STEP].p, nn,
+                           op);
+    if (nf > 0)
+        hipLaunchKernelGGL(am_kernel, dim3(nblk(nf)), dim3(256), 0, ctx->stream,
+                           r.velF[VLM_VEL].p, r.velF[VLM_VEL_PREV].p, r.velF[VEL_H2].p,
+                           r.velF[VEL_H3].p, r.velF[VLM_VEL_PRED].p, r.velF[VLM_VEL_STEP].p, nf,
+                           op);
     VLM_HIP(hipGetLastError());
     return VLM_OK;
 }
@@ -1327,7 +1395,7 @@ int vlm_rotor_create(vlm_ctx *ctx, const vlm_rotor_desc *d, double spanwiseCore,
     R(r->flc, nb * nF);
     R(r->fage, nb * nF);
     R(r->ftmp, 12 * nb * nF);
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NVEL; ++k) {
         R(r->velN[k], r->nvn());
         R(r->velF[k], r->nvf());
     }
@@ -1341,7 +1409,7 @@ int vlm_rotor_create(vlm_ctx *ctx, const vlm_rotor_desc *d, double spanwiseCore,
                                        &r->le,   &r->ple,  &r->gam,  &r->rc4,  &r->age,
                                        &r->ffc,  &r->pffc, &r->fgam, &r->frc,  &r->frc0,
                                        &r->fl0,  &r->flc,  &r->fage, &r->ftmp};
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NVEL; ++k) {
         all.push_back(&r->velN[k]);
         all.push_back(&r->velF[k]);
     }
@@ -1624,10 +1692,42 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
 {
     if (!ctx) return VLM_ERR_ARG;
     VLM_DEVICE(ctx);
-    if (scheme < VLM_SCHEME_FT1 || scheme > VLM_SCHEME_AM2)
-        return fail(ctx, VLM_ERR_ARG, "fdScheme must be 0..3");
+    if (scheme < VLM_SCHEME_FT1 || scheme > VLM_SCHEME_AM4)
+        return fail(ctx, VLM_ERR_ARG, "fdScheme must be 0..5");
     VLM_TRY(wake_pass(ctx, 0));
     auto active = [](Rotor *r) { return r->d.nNwake > 0; };
+    if (scheme == VLM_SCHEME_AM3 || scheme == VLM_SCHEME_AM4) {
+        // start-up steps convect with pass 1 and keep it as history.  AM3's
+        // first start-up branch tests iter == 0, which the time loop never
+        // reaches (main.f90:1118), so its step 1 runs the full scheme on the
+        // zero-initialised histories.
+        const bool am3 = scheme == VLM_SCHEME_AM3;
+        int start = -1;
+        if (am3 && iter == 2) start = AM_START2;
+        if (!am3 && iter >= 1 && iter <= 3) start = AM_START1 + (iter - 1);
+        if (start >= 0) {
+            for (auto *r : ctx->rotors)
+                if (active(r)) {
+                    VLM_TRY(rotor_convect(ctx, *r, dt, 0));
+                    VLM_TRY(launch_am(ctx, *r, start));
+                }
+            return VLM_OK;
+        }
+        for (auto *r : ctx->rotors)
+            if (active(r)) {
+                VLM_TRY(copy_predicted(ctx, *r));
+                VLM_TRY(launch_am(ctx, *r, am3 ? AM3_PRED : AM4_PRED));
+                VLM_TRY(rotor_convect(ctx, *r, dt, 1));
+            }
+        VLM_TRY(wake_pass(ctx, 1));
+        for (auto *r : ctx->rotors)
+            if (active(r)) {
+                VLM_TRY(launch_am(ctx, *r, am3 ? AM3_CORR : AM4_CORR));
+                VLM_TRY(rotor_convect(ctx, *r, dt, 0));
+                VLM_TRY(launch_am(ctx, *r, am3 ? AM3_ROTATE : AM4_ROTATE));
+            }
+        return VLM_OK;
+    }
     if (scheme == VLM_SCHEME_FT1) {
         for (auto *r : ctx->rotors)
             if (active(r)) VLM_TRY(rotor_convect(ctx, *r, dt, 0));

@@ -1412,7 +1412,7 @@ contains
     call system_clock(c0, crate)
     call read_config(dir, nt, dt, nr, density, velSound, kinematicVisc, sw)
     if (sw%slowStart /= 0) error stop 'ERROR: slowStart is not supported'
-    if (sw%fdScheme < 0 .or. sw%fdScheme > 3) error stop 'ERROR: fdScheme must be 0..3'
+    if (sw%fdScheme < 0 .or. sw%fdScheme > 5) error stop 'ERROR: fdScheme must be 0..5'
     if (sw%ntSub /= 0 .or. sw%ntSubInit /= 0) error stop 'ERROR: sub-iterations are not supported'
     if (sw%initWakeVelNt /= 0) error stop 'ERROR: initWakeVel is not supported'
     if (sw%restartFromNt /= 0) error stop 'ERROR: restart is not supported'

@@ -20,7 +20,7 @@ SET_WING, SET_WING_BOUND, SET_WAKE, SET_WAKE_PRED = 0, 1, 2, 3
 WAKE_C, WAKE_P = 0, 1
 EDGE_LE, EDGE_TE = 0, 1
 TERM_NEGATE = 1
-SCHEME_FT1, SCHEME_PC2, SCHEME_AB2, SCHEME_AM2 = 0, 1, 2, 3
+SCHEME_FT1, SCHEME_PC2, SCHEME_AB2, SCHEME_AM2, SCHEME_AM3, SCHEME_AM4 = 0, 1, 2, 3, 4, 5
 VEL, VEL_PREV, VEL_PRED, VEL_STEP = 0, 1, 2, 3
 
 
