@@ -14,6 +14,7 @@ PLOT3D blade file) used by the tests and the benchmark.
   scheme_*_wing/rotor  FT1, AB2, AM3 and AM4 on a wing and a forward-flight
                      rotor
   cfg2_rotor_farwake_2rev  cfg2 with a 72-row near wake and a far wake (strain)
+  initwake_*_rotor   the start-up wake velocity initWakeVel under AM2 and AM4
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
   elevate            the reference's 5-bladed axisymmetric test rotor
   (the last two reproduce tests/katzNplotkin-AR04.case and
@@ -245,6 +246,20 @@ def make_all(out):
                    wakeTruncateNt=60, span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25,
                    velBody=[-13.6, 0.0, -2.0], apparentViscCoeff=ROTOR_AVC,
                    spanwiseCore=ROTOR_CORE, streamwiseCoreVec=ROTOR_CORE)
+
+    # initWakeVel: for the first 30 of 48 steps every pass adds the
+    # reference's start-up wake velocity (main.f90:829-838, pass-2 blocks),
+    # with a 12-row near wake so that far-wake rows exist while it applies;
+    # under AM2 and under AM4
+    for sch, tag in ((3, "am2"), (5, "am4")):
+        d = case(f"initwake_{tag}_rotor")
+        write_config(os.path.join(d, "config.nml"), nt=48, dt=math.radians(5.0) / om, nr=1,
+                     velSound=340.0, fdScheme=sch, wakeDissipation=1, wakeStrain=1,
+                     initWakeVelNt=30)
+        write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=1, ns=10, nNwake=12,
+                   wakeTruncateNt=40, span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25,
+                   velBody=[-13.6, 0.0, -2.0], apparentViscCoeff=ROTOR_AVC,
+                   spanwiseCore=ROTOR_CORE, streamwiseCoreVec=ROTOR_CORE, initWakeVel=2.0)
 
     # cfg5: rotor in forward flight above a wing (15 revolutions); the
     # _short variant (half a revolution) is compared with the reference.  The

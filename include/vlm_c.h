@@ -175,6 +175,16 @@ int vlm_induced_velocity_terms(vlm_ctx *ctx, int nterms, const int *ir, const in
  * 1052-1086). */
 int vlm_wake_velocity(vlm_ctx *ctx, int wakeType);
 
+/* Initial wake velocity (classdef.f90:394 initWakeVel; libCommon.f90:106
+ * initWakeVelNt).  While iter < initWakeVelNt, vlm_convect adds
+ * initWakeVel * shaftAxis to the pass-1 near-wake velocity and subtracts it
+ * from the pass-1 far-wake velocity (main.f90:829-838), and subtracts it
+ * from both pass-2 velocities (902-913, 1070-1081, 1201-1210, 1353-1362):
+ * the reference's signs, kept as they are.  The axis is the one of
+ * vlm_rotor_set_axis. */
+int vlm_rotor_set_init_wake_vel(vlm_ctx *ctx, int ir, double initWakeVel,
+                                int initWakeVelNt);
+
 /* One whole convection block, main.f90:800-1406: pass 1, the scheme
  * algebra of fdScheme (FT1, PC2, AB2, AM2, AM3 or AM4), pass 2 where the
  * scheme has one, and the final convection of every rotor.  The learned

@@ -52,6 +52,10 @@ for tag in ft1 ab2 am3 am4; do
   run_ref scheme_${tag}_wing scheme_${tag}_wing_ForceNonDim.csv
   run_ref scheme_${tag}_rotor scheme_${tag}_rotor_ForceNonDim.csv
 done
+# 5b. the start-up wake velocity initWakeVel under AM2 and AM4
+for tag in am2 am4; do
+  run_ref initwake_${tag}_rotor initwake_${tag}_rotor_ForceNonDim.csv
+done
 # 6. velocity probes: the reference's probes2file output of a short wing run
 tmp=$(mktemp -d)
 cp "$ROOT/cases/probes_wing.case/"* "$tmp/"

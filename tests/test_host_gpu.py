@@ -198,6 +198,19 @@ def test_other_schemes_match_reference_solver(name):
     np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
 
 
+@pytest.mark.parametrize("name", ["initwake_am2_rotor", "initwake_am4_rotor"])
+def test_initial_wake_velocity_matches_reference_solver(name):
+    """initWakeVel for the first 30 of 48 steps (main.f90:829-838 in pass 1:
+    added to the near wake, subtracted from the far wake; 1070-1081 and
+    1353-1362 in pass 2: subtracted from both) on a forward-flight rotor
+    whose far wake forms at step 12, under AM2 and AM4.  Without the
+    start-up velocity the reference's history differs by 2%."""
+    ref = read_force(os.path.join(GOLD, name + "_ForceNonDim.csv"))[:, 0]
+    ct, _ = run(name)
+    assert len(ct[0]) == len(ref)
+    np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
+
+
 @pytest.mark.parametrize("name,nr", [("cfg4_quad_short", 4), ("cfg5_rotor_wing_short", 2)])
 def test_multi_rotor_matches_reference_solver(name, nr):
     """Several lifting surfaces coupled through the right-hand side (every

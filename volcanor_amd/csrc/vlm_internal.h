@@ -71,6 +71,8 @@ struct Rotor {
     int rowNear = 0, rowFar = 0;  // 1-based, as in the reference
     int head = 0;                 // physical slot of logical near-wake row 0
     double shaft[3] = {0, 0, 1}, hub[3] = {0, 0, 0};
+    double init_wake_vel = 0.0;   // initWakeVel, applied while iter < init_wake_vel_nt
+    int init_wake_vel_nt = 0;
     int nring = 0;                // nb*nc*ns wing rings
     // wing (uploaded by the host after every rigid motion)
     DBuf<double> wc, wrc, wg;     // 12*nring, 4*nring, nring

@@ -711,6 +711,22 @@ __global__ void am_kernel(double *v, double *v1, double *v2, double *v3, const d
     }
 }
 
+// vel(:, row0:nrows-1, :) += a for blades 0..nbc-1 of a logical-row
+// velocity array (cols = ns+1 for the near wake, 1 for the far wake).
+__global__ void add_uniform_kernel(double *vel, int nbc, int nrows, int cols, int row0,
+                                   double ax, double ay, double az)
+{
+    const int64_t per = (int64_t)(nrows - row0) * cols;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * nbc) return;
+    const int b = (int)(e / per);
+    const int64_t k = e - b * per;
+    double *V = vel + 3 * (((size_t)b * nrows + row0) * cols + k);
+    V[0] += ax;
+    V[1] += ay;
+    V[2] += az;
+}
+
 // vel_order2_Nwake / vel_order2_Fwake (libCommon.f90:213-258) on the
 // active window, written to `out` (logical rows).
 __global__ void order2_kernel(const double *vn, const double *vnp1, double *out, int nbc,
@@ -898,6 +914,32 @@ int launch_am(vlm_ctx *ctx, Rotor &r, int op)
                            r.velF[VLM_VEL].p, r.velF[VLM_VEL_PREV].p, r.velF[VEL_H2].p,
                            r.velF[VEL_H3].p, r.velF[VLM_VEL_PRED].p, r.velF[VLM_VEL_STEP].p, nf,
                            op);
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+// initWakeVel of pass 1 (main.f90:829-838: + near, - far) or of pass 2
+// (902-913 and the AM blocks: - near, - far) on the active windows.
+int add_init_wake_vel(vlm_ctx *ctx, Rotor &r, int iter, bool pass2)
+{
+    if (iter >= r.init_wake_vel_nt || r.init_wake_vel == 0.0) return VLM_OK;
+    const int slot = pass2 ? VLM_VEL_PRED : VLM_VEL;
+    const double u = r.init_wake_vel;
+    const double sn = pass2 ? -1.0 : 1.0;
+    const int nbc = r.d.nbConvect;
+    const int rn = r.rowNear - 1, rf = r.rowFar - 1;
+    if (r.d.nNwake > rn) {
+        const int64_t n = (int64_t)nbc * (r.d.nNwake - rn) * (r.d.ns + 1);
+        hipLaunchKernelGGL(add_uniform_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+                           r.velN[slot].p, nbc, r.d.nNwake, r.d.ns + 1, rn, sn * u * r.shaft[0],
+                           sn * u * r.shaft[1], sn * u * r.shaft[2]);
+    }
+    if (r.d.nFwake > rf) {
+        const int64_t n = (int64_t)nbc * (r.d.nFwake - rf);
+        hipLaunchKernelGGL(add_uniform_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+                           r.velF[slot].p, nbc, r.d.nFwake, 1, rf, -u * r.shaft[0],
+                           -u * r.shaft[1], -u * r.shaft[2]);
+    }
     VLM_HIP(hipGetLastError());
     return VLM_OK;
 }
@@ -1471,6 +1513,16 @@ int vlm_rotor_set_axis(vlm_ctx *ctx, int ir, const double *shaft, const double *
     return VLM_OK;
 }
 
+int vlm_rotor_set_init_wake_vel(vlm_ctx *ctx, int ir, double initWakeVel, int initWakeVelNt)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (!std::isfinite(initWakeVel)) return fail(ctx, VLM_ERR_ARG, "initWakeVel not finite");
+    r->init_wake_vel = initWakeVel;
+    r->init_wake_vel_nt = initWakeVelNt;
+    return VLM_OK;
+}
+
 int vlm_rotor_init_cursors(vlm_ctx *ctx, int ir)
 {
     Rotor *r = get_rotor(ctx, ir);
@@ -1696,6 +1748,12 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
         return fail(ctx, VLM_ERR_ARG, "fdScheme must be 0..5");
     VLM_TRY(wake_pass(ctx, 0));
     auto active = [](Rotor *r) { return r->d.nNwake > 0; };
+    auto init_vel = [&](bool pass2) -> int {
+        for (auto *r : ctx->rotors)
+            if (active(r)) VLM_TRY(add_init_wake_vel(ctx, *r, iter, pass2));
+        return VLM_OK;
+    };
+    VLM_TRY(init_vel(false));
     if (scheme == VLM_SCHEME_AM3 || scheme == VLM_SCHEME_AM4) {
         // start-up steps convect with pass 1 and keep it as history.  AM3's
         // first start-up branch tests iter == 0, which the time loop never
@@ -1720,6 +1778,7 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
                 VLM_TRY(rotor_convect(ctx, *r, dt, 1));
             }
         VLM_TRY(wake_pass(ctx, 1));
+        VLM_TRY(init_vel(true));
         for (auto *r : ctx->rotors)
             if (active(r)) {
                 VLM_TRY(launch_am(ctx, *r, am3 ? AM3_CORR : AM4_CORR));
@@ -1740,6 +1799,7 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
                 VLM_TRY(rotor_convect(ctx, *r, dt, 1));
             }
         VLM_TRY(wake_pass(ctx, 1));
+        VLM_TRY(init_vel(true));
         for (auto *r : ctx->rotors) {
             if (!active(r)) continue;
             const int rows = r->d.nNwake - (r->rowNear - 1);
@@ -1795,6 +1855,7 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
         }
     if (ctx->pred.mode == 1) VLM_TRY(learned_pass(ctx));
     else VLM_TRY(wake_pass(ctx, 1));
+    VLM_TRY(init_vel(true));
     for (auto *r : ctx->rotors)
         if (active(r)) {
             VLM_TRY(launch_scheme(ctx, *r, 2));

@@ -87,6 +87,14 @@ module vlm_c
       real(c_double), intent(in) :: shaft(3), hub(3)
     end function
 
+    integer(c_int) function vlm_rotor_set_init_wake_vel(ctx, ir, initWakeVel, initWakeVelNt) &
+        bind(C, name='vlm_rotor_set_init_wake_vel')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir, initWakeVelNt
+      real(c_double), value :: initWakeVel
+    end function
+
     integer(c_int) function vlm_rotor_init_cursors(ctx, ir) bind(C, name='vlm_rotor_init_cursors')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

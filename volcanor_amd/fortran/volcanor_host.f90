@@ -937,10 +937,9 @@ contains
   ! ------------------------------------------------------------------
   ! Device hand-off of the wing
   ! ------------------------------------------------------------------
-  subroutine rotor_upload_wing(ctx, r, with_gam)
+  subroutine rotor_upload_wing(ctx, r)
     type(c_ptr), intent(in) :: ctx
     type(rotor_class), intent(in) :: r
-    logical, intent(in) :: with_gam
     real(c_double) :: wc(3, 4, r%nc, r%ns, r%nb), wr(4, r%nc, r%ns, r%nb)
     real(c_double) :: wg(r%nc, r%ns, r%nb)
     integer :: ib, i, j
@@ -953,11 +952,7 @@ contains
         enddo
       enddo
     enddo
-    if (with_gam) then
-      call vlm_check(ctx, vlm_rotor_set_wing(ctx, r%dev, wc, wr, wg), 'rotor_upload_wing')
-    else
-      call vlm_check(ctx, vlm_rotor_set_wing(ctx, r%dev, wc, wr, wg), 'rotor_upload_wing')
-    endif
+    call vlm_check(ctx, vlm_rotor_set_wing(ctx, r%dev, wc, wr, wg), 'rotor_upload_wing')
     call vlm_check(ctx, vlm_rotor_set_axis(ctx, r%dev, r%shaftAxis, r%hubCoords), &
       & 'rotor_set_axis')
   end subroutine rotor_upload_wing
@@ -1414,7 +1409,6 @@ contains
     if (sw%slowStart /= 0) error stop 'ERROR: slowStart is not supported'
     if (sw%fdScheme < 0 .or. sw%fdScheme > 5) error stop 'ERROR: fdScheme must be 0..5'
     if (sw%ntSub /= 0 .or. sw%ntSubInit /= 0) error stop 'ERROR: sub-iterations are not supported'
-    if (sw%initWakeVelNt /= 0) error stop 'ERROR: initWakeVel is not supported'
     if (sw%restartFromNt /= 0) error stop 'ERROR: restart is not supported'
     if (nr > 16) error stop 'ERROR: at most 16 rotors'
     ! velocity probes (main.f90:83-93): read only for probe = 1, used at every
@@ -1457,6 +1451,8 @@ contains
         call vlm_check(ctx, vlm_rotor_create(ctx, d, r%spanwiseCore, r%streamwiseCoreVec, dev), &
           & 'rotor_create')
         r%dev = dev
+        call vlm_check(ctx, vlm_rotor_set_init_wake_vel(ctx, r%dev, r%initWakeVel, &
+          & int(sw%initWakeVelNt, c_int)), 'set_init_wake_vel')
         ! initial pitch, main.f90:43-58
         block
           integer :: ib
@@ -1465,7 +1461,7 @@ contains
             call blade_rot_pitch(r%blade(ib), sign(1._dp, r%Omega)*r%blade(ib)%theta)
           enddo
         end block
-        call rotor_upload_wing(ctx, r, .true.)
+        call rotor_upload_wing(ctx, r)
         block
           real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :)
           allocate (cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb))
@@ -1539,7 +1535,7 @@ contains
         call rotor_move(rotor(ir), rotor(ir)%velBody*dt)
         call rotor_rot_pts(rotor(ir), rotor(ir)%omegaBody*dt, rotor(ir)%cgCoords)
         call rotor_rot_advance(rotor(ir), rotor(ir)%omegaSlow*dt, .false.)
-        call rotor_upload_wing(ctx, rotor(ir), .true.)
+        call rotor_upload_wing(ctx, rotor(ir))
       enddo
       if (sw%wakeSuppress == 0) then
         do ir = 1, nr

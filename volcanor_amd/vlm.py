@@ -56,6 +56,7 @@ _SIGS = {
     "vlm_rotor_create": [_P, C.POINTER(RotorDesc), _D, _P, C.POINTER(_I)],
     "vlm_rotor_set_wing": [_P, _I, _P, _P, _P],
     "vlm_rotor_set_axis": [_P, _I, _P, _P],
+    "vlm_rotor_set_init_wake_vel": [_P, _I, _D, _I],
     "vlm_rotor_init_cursors": [_P, _I],
     "vlm_rotor_begin_step": [_P, _I, _I],
     "vlm_rotor_cursors": [_P, _I, C.POINTER(_I), C.POINTER(_I)],
@@ -222,6 +223,10 @@ class Context:
     def rotor_set_axis(self, ir, shaft, hub):
         s, h = _f64(shaft).ravel(), _f64(hub).ravel()
         self._chk(self.lib.vlm_rotor_set_axis(self.h, ir, _ptr(s), _ptr(h)))
+
+    def rotor_set_init_wake_vel(self, ir, init_wake_vel, init_wake_vel_nt):
+        self._chk(self.lib.vlm_rotor_set_init_wake_vel(self.h, ir, init_wake_vel,
+                                                       init_wake_vel_nt))
 
     def rotor_init_cursors(self, ir):
         self._chk(self.lib.vlm_rotor_init_cursors(self.h, ir))
