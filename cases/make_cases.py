@@ -15,6 +15,8 @@ PLOT3D blade file) used by the tests and the benchmark.
                      rotor
   cfg2_rotor_farwake_2rev  cfg2 with a 72-row near wake and a far wake (strain)
   initwake_*_rotor   the start-up wake velocity initWakeVel under AM2 and AM4
+  slowstart*_rotor   the three slow-start ramps of the rotor speed
+  subiter_rotor_wing cfg5_rotor_wing_short with sub-iterations (ntSubInit 5, ntSub 3)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
   elevate            the reference's 5-bladed axisymmetric test rotor
   (the last two reproduce tests/katzNplotkin-AR04.case and
@@ -261,6 +263,17 @@ def make_all(out):
                    velBody=[-13.6, 0.0, -2.0], apparentViscCoeff=ROTOR_AVC,
                    spanwiseCore=ROTOR_CORE, streamwiseCoreVec=ROTOR_CORE, initWakeVel=2.0)
 
+    # slow start (main.f90:97-105, 428-452): the rotor speed ramps from rest
+    # over 20 steps, linearly (1), as tanh(5 n/N) (2) or as a centred tanh (3)
+    for k in (1, 2, 3):
+        d = case(f"slowstart{k}_rotor")
+        write_config(os.path.join(d, "config.nml"), nt=48, dt=math.radians(5.0) / om, nr=1,
+                     velSound=340.0, slowStart=k, slowStartNt=20)
+        write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=1, ns=10, nNwake=48, span=R,
+                   chord=c, Omega=om, theta0=8.0, pivotLE=0.25, velBody=[0.0, 0.0, 0.0],
+                   apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+                   streamwiseCoreVec=ROTOR_CORE)
+
     # cfg5: rotor in forward flight above a wing (15 revolutions); the
     # _short variant (half a revolution) is compared with the reference.  The
     # reference solver turns NaN from the first step whenever two coupled
@@ -268,9 +281,12 @@ def make_all(out):
     # each alone, or both with 30, run normally), so the short parity case
     # gives the wing 30 spanwise panels.
     dt5 = math.radians(5.0) / om
-    for name, nt5, ns_w in (("cfg5_rotor_wing", 1080, 40), ("cfg5_rotor_wing_short", 36, 30)):
+    for name, nt5, ns_w, opts in (("cfg5_rotor_wing", 1080, 40, {}),
+                                  ("cfg5_rotor_wing_short", 36, 30, {}),
+                                  ("subiter_rotor_wing", 36, 30, dict(ntSubInit=5, ntSub=3))):
         d = case(name)
-        write_config(os.path.join(d, "config.nml"), nt=nt5, dt=dt5, nr=2, velSound=340.0)
+        write_config(os.path.join(d, "config.nml"), nt=nt5, dt=dt5, nr=2, velSound=340.0,
+                     **opts)
         write_geom(os.path.join(d, "geom01.nml"), nb=3, nc=1, ns=30, nNwake=nt5, span=R,
                    chord=c, Omega=om, theta0=10.0, pivotLE=0.25, velBody=[-13.6, 0.0, 0.0],
                    apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,

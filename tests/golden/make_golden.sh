@@ -56,6 +56,11 @@ done
 for tag in am2 am4; do
   run_ref initwake_${tag}_rotor initwake_${tag}_rotor_ForceNonDim.csv
 done
+# 5c. slow start and sub-iterations
+for k in 1 2 3; do
+  run_ref slowstart${k}_rotor slowstart${k}_rotor_ForceNonDim.csv
+done
+run_ref subiter_rotor_wing subiter_rotor_wing_ForceNonDim.csv
 # 6. velocity probes: the reference's probes2file output of a short wing run
 tmp=$(mktemp -d)
 cp "$ROOT/cases/probes_wing.case/"* "$tmp/"

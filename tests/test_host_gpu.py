@@ -211,6 +211,28 @@ def test_initial_wake_velocity_matches_reference_solver(name):
     np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
 
 
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_slow_start_matches_reference_solver(k):
+    """slowStart 1-3 (main.f90:97-105, 428-452): the initial solution at
+    rest, then the rotor speed ramps over 20 steps, linearly, as
+    tanh(5 n/N) or as a centred tanh; hover rotor, 48 steps."""
+    ref = read_force(os.path.join(GOLD, f"slowstart{k}_rotor_ForceNonDim.csv"))[:, 0]
+    ct, _ = run(f"slowstart{k}_rotor")
+    assert len(ct[0]) == len(ref)
+    np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
+
+
+def test_sub_iterations_match_reference_solver():
+    """ntSubInit = 5 and ntSub = 3 (main.f90:120-211, 522-615) on the rotor
+    above a wing: the Jacobi coupling of the two surfaces is iterated to the
+    limit every step (the reference's residual never reaches eps here).
+    Without the sub-iterations the wing's history differs by 56%."""
+    ref = reference_histories("subiter_rotor_wing", 2)
+    ct, _ = run("subiter_rotor_wing", nrotors=2)
+    assert ct.shape == ref.shape
+    np.testing.assert_allclose(ct, ref, rtol=2e-6, atol=2e-9)
+
+
 @pytest.mark.parametrize("name,nr", [("cfg4_quad_short", 4), ("cfg5_rotor_wing_short", 2)])
 def test_multi_rotor_matches_reference_solver(name, nr):
     """Several lifting surfaces coupled through the right-hand side (every

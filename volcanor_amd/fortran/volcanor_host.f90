@@ -248,6 +248,25 @@ contains
     T(3, 3) = cf*ct
   end function euler_gb
 
+  pure function slow_omega(sw, iter, Omega) result(w)
+    !! Rotor speed of the slow start, main.f90:428-452 (the reference's
+    !! default-real conversions kept; they are exact for step counts)
+    type(switches_class), intent(in) :: sw
+    integer, intent(in) :: iter
+    real(dp), intent(in) :: Omega
+    real(dp) :: w
+    select case (sw%slowStart)
+    case (1)
+      w = min(real(sw%slowStartNt), real(iter + 1))*Omega/sw%slowStartNt
+    case (2)
+      w = tanh(5._dp*iter/sw%slowStartNt)*Omega
+    case (3)
+      w = (tanh(6._dp*real(iter)/real(sw%slowStartNt) - 3._dp) + 1._dp)*0.5_dp*Omega
+    case default
+      w = Omega
+    end select
+  end function slow_omega
+
   pure function angle_tan(a, b) result(ang)
     !! Angle between two vectors by the tangent formula, libMath.f90:224-236
     real(dp), intent(in) :: a(3), b(3)
@@ -1073,6 +1092,33 @@ contains
     enddo
   end subroutine finish_solve
 
+  subroutine sub_iterate(ctx, rotor, nr, nsub, with_wake, running_max)
+    !! Sub-iterations of the RHS and the solve (main.f90:120-211 initial,
+    !! 522-615 every step): after a deferred compute_rhs_and_solve, wait,
+    !! map Γ, and solve again until max over rotors of
+    !! norm2(gamVec - gamVecPrev) <= eps, at most nsub more times.  The
+    !! per-step test of the reference exits inside the rotor loop as soon as
+    !! the running maximum is below eps (running_max); the initial test
+    !! takes the maximum over all rotors.
+    type(c_ptr), intent(in) :: ctx
+    type(rotor_class), intent(inout), target :: rotor(:)
+    integer, intent(in) :: nr, nsub
+    logical, intent(in) :: with_wake, running_max
+    integer :: i, ir
+    real(dp) :: res
+    do i = 0, nsub
+      call vlm_check(ctx, vlm_wait(ctx), 'wait')
+      call finish_solve(rotor, nr)
+      res = 0._dp
+      do ir = 1, nr
+        res = max(res, norm2(rotor(ir)%gamVec - rotor(ir)%gamVecPrev))
+        if (running_max .and. res <= eps) return
+      enddo
+      if (res <= eps) return
+      if (i < nsub) call compute_rhs_and_solve(ctx, rotor, nr, with_wake)
+    enddo
+  end subroutine sub_iterate
+
   subroutine blade_sec_resvel(b)
     !! Section resultant velocity, classdef.f90:2197-2232
     type(blade_class), intent(inout) :: b
@@ -1406,9 +1452,11 @@ contains
 
     call system_clock(c0, crate)
     call read_config(dir, nt, dt, nr, density, velSound, kinematicVisc, sw)
-    if (sw%slowStart /= 0) error stop 'ERROR: slowStart is not supported'
+    if (sw%slowStart < 0 .or. sw%slowStart > 3) &
+      & error stop 'ERROR: slowStart must be 0..3 (body dynamics are not supported)'
+    if (sw%slowStart > 0 .and. sw%slowStartNt <= 0) error stop 'ERROR: slowStartNt must be positive'
     if (sw%fdScheme < 0 .or. sw%fdScheme > 5) error stop 'ERROR: fdScheme must be 0..5'
-    if (sw%ntSub /= 0 .or. sw%ntSubInit /= 0) error stop 'ERROR: sub-iterations are not supported'
+    if (sw%ntSub < 0 .or. sw%ntSubInit < 0) error stop 'ERROR: ntSub and ntSubInit must be >= 0'
     if (sw%restartFromNt /= 0) error stop 'ERROR: restart is not supported'
     if (nr > 16) error stop 'ERROR: at most 16 rotors'
     ! velocity probes (main.f90:83-93): read only for probe = 1, used at every
@@ -1471,12 +1519,14 @@ contains
       end associate
     enddo
 
-    ! initial solution without wake
+    ! initial solution without wake; a slow start begins at rest (main.f90:97-105)
     do ir = 1, nr
       rotor(ir)%omegaSlow = rotor(ir)%Omega
+      if (sw%slowStart > 0) rotor(ir)%omegaSlow = 0._dp
     enddo
     t = 0._dp
     call compute_rhs_and_solve(ctx, rotor, nr, .false.)
+    if (sw%ntSubInit /= 0) call sub_iterate(ctx, rotor, nr, sw%ntSubInit, .false., .false.)
     call vlm_check(ctx, vlm_wait(ctx), 'wait')
     call finish_solve(rotor, nr)
     do ir = 1, nr
@@ -1529,7 +1579,7 @@ contains
         call vlm_check(ctx, vlm_rotor_begin_step(ctx, rotor(ir)%dev, int(iter, c_int)), 'begin_step')
         call vlm_check(ctx, vlm_rotor_cursors(ctx, rotor(ir)%dev, rowNear, rowFar), 'cursors')
         rotor(ir)%rowNear = rowNear; rotor(ir)%rowFar = rowFar
-        rotor(ir)%omegaSlow = rotor(ir)%Omega
+        rotor(ir)%omegaSlow = slow_omega(sw, iter, rotor(ir)%Omega)
       enddo
       do ir = 1, nr
         call rotor_move(rotor(ir), rotor(ir)%velBody*dt)
@@ -1548,6 +1598,8 @@ contains
         enddo
       endif
       call compute_rhs_and_solve(ctx, rotor, nr, sw%wakeSuppress == 0)
+      if (sw%ntSub /= 0) &
+        & call sub_iterate(ctx, rotor, nr, sw%ntSub, sw%wakeSuppress == 0, .true.)
       loads = .false.
       if (sw%rotorForcePlot /= 0) loads = mod(iter, sw%rotorForcePlot) == 0
       if (loads) then
