@@ -15,6 +15,7 @@ PLOT3D blade file) used by the tests and the benchmark.
                      rotor
   cfg2_rotor_farwake_2rev  cfg2 with a 72-row near wake and a far wake (strain)
   initwake_*_rotor   the start-up wake velocity initWakeVel under AM2 and AM4
+  tec_rotor          wing and wake plots (.tec) of a forward-flight rotor
   slowstart*_rotor   the three slow-start ramps of the rotor speed
   subiter_rotor_wing cfg5_rotor_wing_short with sub-iterations (ntSubInit 5, ntSub 3)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
@@ -262,6 +263,16 @@ def make_all(out):
                    wakeTruncateNt=40, span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25,
                    velBody=[-13.6, 0.0, -2.0], apparentViscCoeff=ROTOR_AVC,
                    spanwiseCore=ROTOR_CORE, streamwiseCoreVec=ROTOR_CORE, initWakeVel=2.0)
+
+    # wing and wake plots (.tec, geom2file) at steps 20 and 40 of a
+    # forward-flight rotor whose far wake forms at step 12
+    d = case("tec_rotor")
+    write_config(os.path.join(d, "config.nml"), nt=40, dt=math.radians(5.0) / om, nr=1,
+                 velSound=340.0, wakeDissipation=1, wakeStrain=1, wakePlot=20)
+    write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=2, ns=8, nNwake=12, wakeTruncateNt=36,
+               span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25, velBody=[-13.6, 0.0, -2.0],
+               apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+               streamwiseCoreVec=ROTOR_CORE)
 
     # slow start (main.f90:97-105, 428-452): the rotor speed ramps from rest
     # over 20 steps, linearly (1), as tanh(5 n/N) (2) or as a centred tanh (3)

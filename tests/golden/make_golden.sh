@@ -61,6 +61,16 @@ for k in 1 2 3; do
   run_ref slowstart${k}_rotor slowstart${k}_rotor_ForceNonDim.csv
 done
 run_ref subiter_rotor_wing subiter_rotor_wing_ForceNonDim.csv
+# 5d. wing and wake plots (.tec) of a forward-flight rotor at steps 20 and 40
+tmp=$(mktemp -d)
+cp "$ROOT/cases/tec_rotor.case/"* "$tmp/"
+mkdir -p "$tmp/Results" "$tmp/Restart"
+(cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 OMP_SCHEDULE=static \
+   "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
+for st in 00020 00040; do
+  cp "$tmp/Results/r01wingNwake$st.tec" "$HERE/tec_rotor_wingNwake$st.tec"
+done
+rm -rf "$tmp"
 # 6. velocity probes: the reference's probes2file output of a short wing run
 tmp=$(mktemp -d)
 cp "$ROOT/cases/probes_wing.case/"* "$tmp/"

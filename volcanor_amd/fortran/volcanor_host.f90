@@ -1092,6 +1092,129 @@ contains
     enddo
   end subroutine finish_solve
 
+  subroutine geom_file(ctx, dir, timestamp, r, wakeSuppress)
+    !! geom2file, libPostprocess.f90:143-331, for lifting surfaces without a
+    !! prescribed far wake: per blade the wing panel corners and -Γ, the
+    !! active near-wake lattice and -Γ, and the far-wake tip line and -Γ, as
+    !! Tecplot blocks in the reference's list-directed format.  The wake
+    !! comes from the device (vlm_rotor_get_wake).  The skew column is 0:
+    !! the reference fills the wake rings' skew only for skew plots.
+    type(c_ptr), intent(in) :: ctx
+    character(len=*), intent(in) :: dir, timestamp
+    type(rotor_class), intent(in) :: r
+    integer, intent(in) :: wakeSuppress
+    real(c_double), allocatable, target :: nodes(:, :, :, :), le(:, :, :), gam(:, :, :), &
+      & rc4(:, :, :, :), ffc(:, :, :, :), fgam(:, :), frc(:, :)
+    real(dp) :: wingMesh(3, r%nc + 1, r%ns + 1), wakeMesh(3, r%nNwake + 1, r%ns + 1)
+    real(dp) :: wakeTip(3, max(r%nFwake, 1) + 1), tip1(3)
+    character(len=5) :: nxChar, nyChar
+    character(len=2) :: bid
+    integer :: u, ib, i, j, nx, ny, nN, nF
+    logical :: wake
+
+    nN = r%nNwake; nF = r%nFwake
+    wake = wakeSuppress == 0 .and. nN > 0
+    if (wake) then
+      allocate (nodes(3, r%ns + 1, nN, r%nb), le(3, r%ns + 1, r%nb), gam(r%ns, nN, r%nb), &
+        & rc4(4, r%ns, nN, r%nb), ffc(3, 2, max(nF, 1), r%nb), fgam(max(nF, 1), r%nb), &
+        & frc(max(nF, 1), r%nb))
+      call vlm_check(ctx, vlm_rotor_get_wake(ctx, r%dev, VLM_WAKE_C, c_loc(nodes), c_loc(le), &
+        & c_loc(gam), c_loc(rc4), c_loc(ffc), c_loc(fgam), c_loc(frc)), 'get_wake')
+    endif
+    ! the reference appends (position='append'); within one run each file is
+    ! written once, so replacing gives its output also when a case is rerun
+    open (newunit=u, file=trim(dir)//'/Results/r'//r%id//'wingNwake'//timestamp//'.tec', &
+      & status='replace', action='write')
+    write (u, *) 'Title = "Wing and Wake"'
+    write (u, *) 'VARIABLES = "X" "Y" "Z" "GAM" "skew"'
+    do ib = 1, r%nb
+      write (bid, '(I0.2)') ib
+      nx = r%nc; ny = r%ns
+      write (nxChar, '(I5)') nx + 1
+      write (nyChar, '(I5)') ny + 1
+      do j = 1, ny
+        do i = 1, nx
+          wingMesh(:, i, j) = r%blade(ib)%wiP(i, j)%PC(:, 1)
+        enddo
+      enddo
+      do i = 1, nx
+        wingMesh(:, i, ny + 1) = r%blade(ib)%wiP(i, ny)%PC(:, 4)
+      enddo
+      do j = 1, ny
+        wingMesh(:, nx + 1, j) = r%blade(ib)%wiP(nx, j)%PC(:, 2)
+      enddo
+      wingMesh(:, nx + 1, ny + 1) = r%blade(ib)%wiP(nx, ny)%PC(:, 3)
+      write (u, *) 'Zone I='//trim(nxChar)//' J='//trim(nyChar)//' K=1  T="Blade '//bid//'"'
+      write (u, *) 'DATAPACKING=BLOCK'
+      write (u, *) 'VARLOCATION=([4]=CELLCENTERED,[5]=CELLCENTERED)'
+      write (u, *) ((wingMesh(1, i, j), i=1, nx + 1), j=1, ny + 1)
+      write (u, *) ((wingMesh(2, i, j), i=1, nx + 1), j=1, ny + 1)
+      write (u, *) ((wingMesh(3, i, j), i=1, nx + 1), j=1, ny + 1)
+      write (u, *) ((-1._dp*r%blade(ib)%wiP(i, j)%vr%gam, i=1, nx), j=1, ny)
+      write (u, *) ((r%blade(ib)%wiP(i, j)%vr%gam*0._dp, i=1, nx), j=1, ny)
+      if (.not. wake) cycle
+      ! near wake: corner 1 of the active rings (the shed line on the newest
+      ! row), the last column's corner 4, the oldest row's corners 2 and 3
+      nx = nN
+      write (nxChar, '(I5)') nx - (r%rowNear - 1) + 1
+      write (nyChar, '(I5)') ny + 1
+      do j = 1, ny + 1
+        wakeMesh(:, r%rowNear, j) = le(:, j, ib)
+        do i = r%rowNear + 1, nx + 1
+          wakeMesh(:, i, j) = nodes(:, j, i - 1, ib)
+        enddo
+      enddo
+      write (u, *) 'Zone I='//trim(nxChar)//' J='//trim(nyChar)//' K=1  T="NearWake"'
+      write (u, *) 'DATAPACKING=BLOCK'
+      write (u, *) 'VARLOCATION=([4]=CELLCENTERED, [5]=CELLCENTERED)'
+      write (u, *) ((wakeMesh(1, i, j), i=r%rowNear, nx + 1), j=1, ny + 1)
+      write (u, *) ((wakeMesh(2, i, j), i=r%rowNear, nx + 1), j=1, ny + 1)
+      write (u, *) ((wakeMesh(3, i, j), i=r%rowNear, nx + 1), j=1, ny + 1)
+      write (u, *) ((-1._dp*gam(j, i, ib), i=r%rowNear, nx), j=1, ny)
+      write (u, *) ((0._dp, i=r%rowNear, nx), j=1, ny)
+      ! far wake: the LE end of every active filament and the TE end of the
+      ! oldest, or the oldest near ring's corner 3 twice when there is none
+      tip1 = nodes(:, ny + 1, nN, ib)
+      if (r%rowFar <= nF) then
+        nx = nF
+        write (nxChar, '(I5)') nx - (r%rowFar - 1) + 1
+        do i = r%rowFar, nx
+          wakeTip(:, i) = ffc(:, 2, i, ib)
+        enddo
+        wakeTip(:, nx + 1) = ffc(:, 1, nF, ib)
+        write (u, *) 'Zone I='//trim(nxChar)//' J=1   K=1   T="FarWake"'
+        write (u, *) 'DATAPACKING=BLOCK'
+        write (u, *) 'VARLOCATION=([4]=CELLCENTERED ,[5]=CELLCENTERED)'
+        write (u, *) (wakeTip(1, i), i=r%rowFar, nx + 1)
+        write (u, *) (wakeTip(2, i), i=r%rowFar, nx + 1)
+        write (u, *) (wakeTip(3, i), i=r%rowFar, nx + 1)
+        write (u, *) (-1._dp*fgam(i, ib), i=r%rowFar, nx)
+        write (u, *) (frc(i, ib)*0._dp, i=r%rowFar, nx)
+      else
+        write (nxChar, '(I5)') 2
+        write (u, *) 'Zone I='//trim(nxChar)//' J=1   K=1   T="FarWake"'
+        write (u, *) 'DATAPACKING=BLOCK'
+        write (u, *) 'VARLOCATION=([4]=CELLCENTERED, [5]=CELLCENTERED)'
+        write (u, *) tip1(1), tip1(1)
+        write (u, *) tip1(2), tip1(2)
+        write (u, *) tip1(3), tip1(3)
+        write (u, *) 0._dp
+        write (u, *) 0._dp
+      endif
+      ! no prescribed far wake: the same redundant point
+      write (nxChar, '(I5)') 2
+      write (u, *) 'Zone I='//trim(nxChar)//' J=1   K=1   T="PrescFarWake"'
+      write (u, *) 'DATAPACKING=BLOCK'
+      write (u, *) 'VARLOCATION=([4]=CELLCENTERED, [5]=CELLCENTERED)'
+      write (u, *) tip1(1), tip1(1)
+      write (u, *) tip1(2), tip1(2)
+      write (u, *) tip1(3), tip1(3)
+      write (u, *) 0._dp
+      write (u, *) 0._dp
+    enddo
+    close (u)
+  end subroutine geom_file
+
   subroutine sub_iterate(ctx, rotor, nr, nsub, with_wake, running_max)
     !! Sub-iterations of the RHS and the solve (main.f90:120-211 initial,
     !! 522-615 every step): after a deferred compute_rhs_and_solve, wait,
@@ -1596,6 +1719,15 @@ contains
             & rotor(ir)%dev, dt, kinematicVisc, rotor(ir)%apparentViscCoeff, rotor(ir)%decayCoeff), &
             & 'dissipate_wake')
         enddo
+      endif
+      ! wing and wake plot, main.f90:509-513
+      if (write_out .and. sw%wakePlot /= 0) then
+        if (mod(iter, sw%wakePlot) == 0) then
+          write (ts5, '(I0.5)') iter
+          do ir = 1, nr
+            call geom_file(ctx, dir, ts5, rotor(ir), sw%wakeSuppress)
+          enddo
+        endif
       endif
       call compute_rhs_and_solve(ctx, rotor, nr, sw%wakeSuppress == 0)
       if (sw%ntSub /= 0) &
