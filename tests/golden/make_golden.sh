@@ -61,7 +61,8 @@ for k in 1 2 3; do
   run_ref slowstart${k}_rotor slowstart${k}_rotor_ForceNonDim.csv
 done
 run_ref subiter_rotor_wing subiter_rotor_wing_ForceNonDim.csv
-# 5d. wing and wake plots (.tec) of a forward-flight rotor at steps 20 and 40
+# 5d. wing and wake plots (.tec) of a forward-flight rotor at steps 20 and 40, and
+#     the force files of every force step (ForceNonDim, ForceDim, ForceDist)
 tmp=$(mktemp -d)
 cp "$ROOT/cases/tec_rotor.case/"* "$tmp/"
 mkdir -p "$tmp/Results" "$tmp/Restart"
@@ -69,6 +70,10 @@ mkdir -p "$tmp/Results" "$tmp/Restart"
    "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
 for st in 00020 00040; do
   cp "$tmp/Results/r01wingNwake$st.tec" "$HERE/tec_rotor_wingNwake$st.tec"
+done
+for f in ForceNonDim.csv ForceDim.csv b01ForceDist00000.csv b02ForceDist00001.csv \
+         b01ForceDist00040.csv b02ForceDist00040.csv bladedynamics.csv; do
+  cp "$tmp/Results/r01$f" "$HERE/tec_rotor_$f"
 done
 rm -rf "$tmp"
 # 6. velocity probes: the reference's probes2file output of a short wing run

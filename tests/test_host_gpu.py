@@ -257,6 +257,35 @@ def test_wake_plot_matches_reference_solver(step):
             np.testing.assert_array_equal(zg[7], zr[7])
 
 
+def test_force_outputs_match_reference_solver():
+    """force2file (libPostprocess.f90:814-882) at every force step of the
+    forward-flight rotor: every column of rXXForceNonDim.csv and
+    rXXForceDim.csv, the ForceDist tables of both blades at steps 0, 1 and
+    40, and the header of rXXbladedynamics.csv (init_plots, 10-49), against
+    the reference solver's files (7 printed significant digits; components
+    that are round-off in the reference, like 1e-18, within 2e-6 of the
+    column's scale)."""
+    res = os.path.join(CASES, "tec_rotor.case", "Results")
+    run("tec_rotor")
+
+    def same(name, mine):
+        ref, got = read_table(os.path.join(GOLD, "tec_rotor_" + name)), read_table(mine)
+        assert list(got) == list(ref), name
+        for k in ref:
+            assert len(got[k]) == len(ref[k]), (name, k)
+            scale = max(np.max(np.abs(ref[k])), 1e-300)
+            np.testing.assert_allclose(got[k], ref[k], rtol=2e-6, atol=2e-6 * scale,
+                                       err_msg=f"{name} {k}")
+
+    same("ForceNonDim.csv", os.path.join(res, "r01ForceNonDim.csv"))
+    same("ForceDim.csv", os.path.join(res, "r01ForceDim.csv"))
+    for f in ("b01ForceDist00000.csv", "b02ForceDist00001.csv", "b01ForceDist00040.csv",
+              "b02ForceDist00040.csv"):
+        same(f, os.path.join(res, "r01" + f))
+    assert (open(os.path.join(res, "r01bladedynamics.csv")).read()
+            == open(os.path.join(GOLD, "tec_rotor_bladedynamics.csv")).read())
+
+
 @pytest.mark.parametrize("k", [1, 2, 3])
 def test_slow_start_matches_reference_solver(k):
     """slowStart 1-3 (main.f90:97-105, 428-452): the initial solution at

@@ -1462,6 +1462,27 @@ contains
       & 0._dp, 0._dp, 0._dp, r%forceInertial(1)/d, r%forceInertial(2)/d, r%forceInertial(3)/d
   end subroutine force_line
 
+  subroutine force_files(uf, ud, dir, r, it)
+    !! force2file, libPostprocess.f90:814-882, at every force step: the
+    !! ForceNonDim and ForceDim rows and the ForceDist table of every
+    !! convected blade
+    integer, intent(in) :: uf, ud, it
+    character(len=*), intent(in) :: dir
+    type(rotor_class), intent(in) :: r
+    character(len=5) :: ts
+    character(len=2) :: bc2
+    integer :: ib
+    call force_line(uf, r, it)
+    write (ts, '(I0.5)') it
+    write (ud, '(A,11(E15.7))') ts, norm2(r%lift), norm2(r%drag), r%lift, r%drag, &
+      & r%forceInertial
+    do ib = 1, r%nbConvect
+      write (bc2, '(I0.2)') ib
+      call dist_file(trim(dir)//'/Results/r'//r%id//'b'//bc2//'ForceDist'//ts//'.csv', &
+        & r%blade(ib), r%hubCoords, r%ns)
+    enddo
+  end subroutine force_files
+
   subroutine dist_file(path, b, hub, ns)
     !! Sectional loads of one blade in the format of force2file's ForceDist
     !! table (libPostprocess.f90:848-881).  Drag is zero in this load model
@@ -1560,7 +1581,7 @@ contains
     type(switches_class) :: sw
     type(rotor_class), allocatable, target :: rotor(:)
     type(c_ptr) :: ctx
-    integer :: nt, nr, ir, ib, iter, rn, rf, uforce(16), nforce
+    integer :: nt, nr, ir, ib, iter, rn, rf, uforce(16), udim(16), nforce
     real(dp) :: dt, density, velSound, kinematicVisc, t
     character(len=2) :: rc2, bc2
     character(len=5) :: ts5
@@ -1668,6 +1689,18 @@ contains
           & status='replace', action='write')
         write (uforce(ir), '(A5,9(A15))') 'iter', 'CL/CT', 'CD/CQ', 'CLu', 'CDi', 'CD0', 'CDu', &
           & 'CFx', 'CFy', 'CFz'
+        ! init_plots, libPostprocess.f90:10-49
+        open (newunit=udim(ir), file=trim(dir)//'/Results/r'//rc2//'ForceDim.csv', &
+          & status='replace', action='write')
+        write (udim(ir), '(A5,11(A15))') 'iter', 'LiftMag', 'DragMag', 'Lx', 'Ly', 'Lz', &
+          & 'Dx', 'Dy', 'Dz', 'FInertx', 'FInerty', 'FInertz'
+        block
+          integer :: ud
+          open (newunit=ud, file=trim(dir)//'/Results/r'//rc2//'bladedynamics.csv', &
+            & status='replace', action='write')
+          write (ud, '(A5,5(A15))') 'iter', 'pitch', 'flap', 'dflap', 'w', 'omega'
+          close (ud)
+        end block
       enddo
     endif
     if (sw%rotorForcePlot /= 0) then
@@ -1678,7 +1711,7 @@ contains
       do ir = 1, nr
         call compute_forces(ctx, rotor, nr, ir, density, dt)
         if (ir <= ct_cap .and. hist_len >= 1) ct_hist(ir, 1) = rotor_CT(rotor(ir))
-        if (write_out) call force_line(uforce(ir), rotor(ir), 0)
+        if (write_out) call force_files(uforce(ir), udim(ir), dir, rotor(ir), 0)
       enddo
     endif
 
@@ -1783,7 +1816,7 @@ contains
         do ir = 1, nr
           call compute_forces(ctx, rotor, nr, ir, density, dt)
           if (ir <= ct_cap .and. iter + 1 <= hist_len) ct_hist(ir, iter + 1) = rotor_CT(rotor(ir))
-          if (write_out) call force_line(uforce(ir), rotor(ir), iter)
+          if (write_out) call force_files(uforce(ir), udim(ir), dir, rotor(ir), iter)
         enddo
       endif
       if (sw%wakeSuppress == 0) then
@@ -1826,17 +1859,9 @@ contains
         & stats%pred_judged), 'predictor_errors')
     end block
     if (write_out) then
-      ! sectional loads of the last step, named as force2file names them
-      ! (rXXbYYForceDistNNNNN.csv, libPostprocess.f90:848-852)
-      write (ts5, '(I0.5)') nt
       do ir = 1, nr
         close (uforce(ir))
-        write (rc2, '(I0.2)') ir
-        do ib = 1, rotor(ir)%nbConvect
-          write (bc2, '(I0.2)') ib
-          call dist_file(trim(dir)//'/Results/r'//rc2//'b'//bc2//'ForceDist'//ts5//'.csv', &
-            & rotor(ir)%blade(ib), rotor(ir)%hubCoords, rotor(ir)%ns)
-        enddo
+        close (udim(ir))
       enddo
     endif
     rn = 0; rf = 0
