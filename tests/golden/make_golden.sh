@@ -72,7 +72,8 @@ for st in 00020 00040; do
   cp "$tmp/Results/r01wingNwake$st.tec" "$HERE/tec_rotor_wingNwake$st.tec"
 done
 for f in ForceNonDim.csv ForceDim.csv b01ForceDist00000.csv b02ForceDist00001.csv \
-         b01ForceDist00040.csv b02ForceDist00040.csv bladedynamics.csv; do
+         b01ForceDist00040.csv b02ForceDist00040.csv bladedynamics.csv b01Surface.tec \
+         b02Surface.tec; do
   cp "$tmp/Results/r01$f" "$HERE/tec_rotor_$f"
 done
 rm -rf "$tmp"

@@ -257,6 +257,33 @@ def test_wake_plot_matches_reference_solver(step):
             np.testing.assert_array_equal(zg[7], zr[7])
 
 
+@pytest.mark.parametrize("blade", ["b01", "b02"])
+def test_surface_geometry_matches_reference_solver(blade):
+    """geomSurface2file (libPostprocess.f90:475-615, written once after the
+    initial pitch, main.f90:60-63): panel corners, vortex-ring corners, CPs
+    with nCap, tauCapChord and tauCapSpan, and the sectional CPs with their
+    normal and tangent vectors of both blades of the forward-flight rotor.
+    The text lines (titles, zone sizes, sharing lists) must be identical and
+    every number must agree."""
+    run("tec_rotor")
+
+    def split(path):
+        text, nums = [], []
+        for line in open(path):
+            w = line.strip()
+            if w and (w[0].isalpha() or w[0] == "_"):
+                text.append(w)
+            else:
+                nums.extend(float(x) for x in w.split())
+        return text, np.array(nums)
+
+    tr, vr = split(os.path.join(GOLD, f"tec_rotor_{blade}Surface.tec"))
+    tg, vg = split(os.path.join(CASES, "tec_rotor.case", "Results", f"r01{blade}Surface.tec"))
+    assert tg == tr
+    assert vg.shape == vr.shape
+    np.testing.assert_allclose(vg, vr, rtol=1e-12, atol=1e-14)
+
+
 def test_force_outputs_match_reference_solver():
     """force2file (libPostprocess.f90:814-882) at every force step of the
     forward-flight rotor: every column of rXXForceNonDim.csv and

@@ -1092,6 +1092,118 @@ contains
     enddo
   end subroutine finish_solve
 
+  subroutine surface_file(dir, r)
+    !! geomSurface2file, libPostprocess.f90:475-615: the blade geometry after
+    !! the initial pitch, one rXXbYYSurface.tec per blade (panel corners,
+    !! vortex-ring corners, CPs with nCap, tauCapChord and tauCapSpan, and
+    !! the sectional CPs with their normal and tangents)
+    character(len=*), intent(in) :: dir
+    type(rotor_class), intent(in) :: r
+    real(dp) :: mesh(3, r%nc + 1, r%ns + 1)
+    character(len=5) :: nxChar, nyChar
+    character(len=2) :: bid
+    integer :: u, ib, i, j, nx, ny, k
+
+    do ib = 1, r%nb
+      write (bid, '(I0.2)') ib
+      open (newunit=u, file=trim(dir)//'/Results/r'//r%id//'b'//bid//'Surface.tec', &
+        & status='replace', action='write')
+      write (u, *) 'Title="r'//r%id//'b'//bid//'"'
+      write (u, *) 'VARIABLES = "X" "Y" "Z" "nx" "ny" "nz"'
+      nx = r%nc; ny = r%ns
+      write (nxChar, '(I5)') nx + 1
+      write (nyChar, '(I5)') ny + 1
+      do k = 1, 2
+        ! k = 1: panel corners PC; k = 2: corners of the vortex rings
+        do j = 1, ny
+          do i = 1, nx
+            mesh(:, i, j) = corner(r%blade(ib)%wiP(i, j), 1, k)
+          enddo
+        enddo
+        do i = 1, nx
+          mesh(:, i, ny + 1) = corner(r%blade(ib)%wiP(i, ny), 4, k)
+        enddo
+        do j = 1, ny
+          mesh(:, nx + 1, j) = corner(r%blade(ib)%wiP(nx, j), 2, k)
+        enddo
+        mesh(:, nx + 1, ny + 1) = corner(r%blade(ib)%wiP(nx, ny), 3, k)
+        if (k == 1) then
+          write (u, *) 'Zone I='//trim(nxChar)//' J='//trim(nyChar)//' K=1 T="PC"'
+          write (u, *) 'DATAPACKING=BLOCK'
+        else
+          write (u, *) 'Zone I='//trim(nxChar)//' J='//trim(nyChar)//' K=1 T="VR"'
+          write (u, *) 'DATAPACKING=BLOCK'
+          write (u, *) 'VARSHARELIST=([4-6]=1)'
+        endif
+        write (u, *) ((mesh(1, i, j), i=1, nx + 1), j=1, ny + 1)
+        write (u, *) ((mesh(2, i, j), i=1, nx + 1), j=1, ny + 1)
+        write (u, *) ((mesh(3, i, j), i=1, nx + 1), j=1, ny + 1)
+        if (k == 1) then
+          write (u, *) ((0, i=1, nx + 1), j=1, ny + 1)
+          write (u, *) ((0, i=1, nx + 1), j=1, ny + 1)
+          write (u, *) ((0, i=1, nx + 1), j=1, ny + 1)
+        endif
+      enddo
+      write (nxChar, '(I5)') nx
+      write (nyChar, '(I5)') ny
+      write (u, *) 'Zone I='//trim(nxChar)//' J='//trim(nyChar)//' K=1  T="CP, nCap"'
+      write (u, *) 'DATAPACKING=BLOCK'
+      do k = 1, 3
+        write (u, *) ((r%blade(ib)%wiP(i, j)%CP(k), i=1, nx), j=1, ny)
+      enddo
+      do k = 1, 3
+        write (u, *) ((r%blade(ib)%wiP(i, j)%nCap(k), i=1, nx), j=1, ny)
+      enddo
+      ! zones 4 and 5 carry the same title in the reference, and the titles
+      ! continued over two source lines there carry two spaces before T=
+      write (u, *) 'Zone I='//trim(nxChar)//' J='//trim(nyChar)//' K=1  T="CP, tauCapChord"'
+      write (u, *) 'DATAPACKING=BLOCK'
+      write (u, *) 'VARSHARELIST=([1-3]=3)'
+      do k = 1, 3
+        write (u, *) ((r%blade(ib)%wiP(i, j)%tauCapChord(k), i=1, nx), j=1, ny)
+      enddo
+      write (u, *) 'Zone I='//trim(nxChar)//' J='//trim(nyChar)//' K=1  T="CP, tauCapChord"'
+      write (u, *) 'DATAPACKING=BLOCK'
+      write (u, *) 'VARSHARELIST=([1-3]=3)'
+      do k = 1, 3
+        write (u, *) ((r%blade(ib)%wiP(i, j)%tauCapSpan(k), i=1, nx), j=1, ny)
+      enddo
+      write (nxChar, '(I5)') ny
+      write (u, *) 'Zone I='//trim(nxChar)//' J=1 K=1 T="secCP, secNcap"'
+      write (u, *) 'DATAPACKING=BLOCK'
+      do k = 1, 3
+        write (u, *) (r%blade(ib)%secCP(k, i), i=1, ny)
+      enddo
+      do k = 1, 3
+        write (u, *) (r%blade(ib)%secNormalVec(k, i), i=1, ny)
+      enddo
+      write (u, *) 'Zone I='//trim(nxChar)//' J=1 K=1  T="secCP, secTauCapChord"'
+      write (u, *) 'DATAPACKING=BLOCK'
+      write (u, *) 'VARSHARELIST=([1-3]=6)'
+      do k = 1, 3
+        write (u, *) (r%blade(ib)%secTauCapChord(k, i), i=1, ny)
+      enddo
+      write (u, *) 'Zone I='//trim(nxChar)//' J=1 K=1  T="secCP, secTauCapSpan"'
+      write (u, *) 'DATAPACKING=BLOCK'
+      write (u, *) 'VARSHARELIST=([1-3]=6)'
+      do k = 1, 3
+        write (u, *) (r%blade(ib)%secTauCapSpan(k, i), i=1, ny)
+      enddo
+      close (u)
+    enddo
+  contains
+    pure function corner(w, c, k) result(x)
+      type(wingpanel_class), intent(in) :: w
+      integer, intent(in) :: c, k
+      real(dp) :: x(3)
+      if (k == 1) then
+        x = w%PC(:, c)
+      else
+        x = w%vr%fc(:, c)
+      endif
+    end function corner
+  end subroutine surface_file
+
   subroutine geom_file(ctx, dir, timestamp, r, wakeSuppress)
     !! geom2file, libPostprocess.f90:143-331, for lifting surfaces without a
     !! prescribed far wake: per blade the wing panel corners and -Γ, the
@@ -1683,6 +1795,7 @@ contains
       ir = int(c_mkdir(trim(dir)//'/Results'//c_null_char, int(o'755', c_int)))
       do ir = 1, nr
         write (rc2, '(I0.2)') ir
+        call surface_file(dir, rotor(ir))
         call params_file(trim(dir)//'/Results/r'//rc2//'Params.json', rotor(ir), nt, dt, nr, &
           & density, velSound, sw)
         open (newunit=uforce(ir), file=trim(dir)//'/Results/r'//rc2//'ForceNonDim.csv', &
