@@ -16,6 +16,7 @@ PLOT3D blade file) used by the tests and the benchmark.
   cfg2_rotor_farwake_2rev  cfg2 with a 72-row near wake and a far wake (strain)
   initwake_*_rotor   the start-up wake velocity initWakeVel under AM2 and AM4
   tec_rotor          wing and wake plots (.tec) of a forward-flight rotor
+  plots_rotor_wing   inflow and bound-circulation plots of a rotor above a wing
   slowstart*_rotor   the three slow-start ramps of the rotor speed
   subiter_rotor_wing cfg5_rotor_wing_short with sub-iterations (ntSubInit 5, ntSub 3)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
@@ -80,7 +81,8 @@ def write_geom(path, **g):
              forceCalcSwitch=0, nAirfoils=0, apparentViscCoeff=5.0, decayCoeff=0.0,
              wakeTruncateNt=0, prescWakeAfterTruncNt=0, prescWakeGenNt=0, spanwiseCore=0.03,
              streamwiseCoreVec=0.03, rollupStartRadius=0.75, rollupEndRadius=1.0,
-             initWakeVel=0.0, psiStart=0.0, skewLimit=0.5)
+             initWakeVel=0.0, psiStart=0.0, skewLimit=0.5, inflowPlotSwitch=0,
+             gammaPlotSwitch=0)
     d.update(g)
     groups = [
         ("SURFACE", ["surfaceType", "imagePlane", "imageRotorNum"]),
@@ -113,7 +115,8 @@ def write_geom(path, **g):
                 "bodyDynamicsIOVars = 1\n/\n")
         f.write("&WINDFRAME\ndragUnitVec = 0.0, 0.0, 0.0\nsideUnitVec = 0.0, 0.0, 0.0\n"
                 "liftUnitVec = 0.0, 0.0, 0.0\n/\n")
-        f.write("&PLOTS\ninflowPlotSwitch = 0\ngammaPlotSwitch = 0\nskewPlotSwitch = 0\n/\n")
+        f.write(f"&PLOTS\ninflowPlotSwitch = {d['inflowPlotSwitch']}\n"
+                f"gammaPlotSwitch = {d['gammaPlotSwitch']}\nskewPlotSwitch = 0\n/\n")
 
 
 def twisted_blade_plot3d(path, nc, ns, R, chord, twist_deg, omega_sign=1.0, r_zero=0.75,
@@ -273,6 +276,22 @@ def make_all(out):
                span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25, velBody=[-13.6, 0.0, -2.0],
                apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
                streamwiseCoreVec=ROTOR_CORE)
+
+    # inflow and bound-circulation plots of a rotor above a wing: the rotor
+    # plots every 12 steps, the wing every 24; at step 24 the reference
+    # writes rotor 1's circulation twice (gamma2file is called with rotor 1)
+    d = case("plots_rotor_wing")
+    write_config(os.path.join(d, "config.nml"), nt=24, dt=math.radians(5.0) / om, nr=2,
+                 velSound=340.0)
+    write_geom(os.path.join(d, "geom01.nml"), nb=3, nc=1, ns=30, nNwake=24, span=R, chord=c,
+               Omega=om, theta0=10.0, pivotLE=0.25, velBody=[-13.6, 0.0, 0.0],
+               apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+               streamwiseCoreVec=ROTOR_CORE, inflowPlotSwitch=12, gammaPlotSwitch=12)
+    write_geom(os.path.join(d, "geom02.nml"), nb=1, nc=4, ns=30, nNwake=24, span=2.0,
+               chord=0.25, Omega=0.0, theta0=3.0, hubCoords=[0.0, 0.0, -0.5],
+               fromCoords=[0.0, 1.0, 0.0], velBody=[-13.6, 0.0, 0.0],
+               apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+               streamwiseCoreVec=ROTOR_CORE, inflowPlotSwitch=24, gammaPlotSwitch=24)
 
     # slow start (main.f90:97-105, 428-452): the rotor speed ramps from rest
     # over 20 steps, linearly (1), as tanh(5 n/N) (2) or as a centred tanh (3)

@@ -77,6 +77,16 @@ for f in ForceNonDim.csv ForceDim.csv b01ForceDist00000.csv b02ForceDist00001.cs
   cp "$tmp/Results/r01$f" "$HERE/tec_rotor_$f"
 done
 rm -rf "$tmp"
+# 5e. inflow and bound-circulation plots of a rotor above a wing
+tmp=$(mktemp -d)
+cp "$ROOT/cases/plots_rotor_wing.case/"* "$tmp/"
+mkdir -p "$tmp/Results" "$tmp/Restart"
+(cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 OMP_SCHEDULE=static \
+   "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
+for f in "$tmp"/Results/*.curve; do
+  cp "$f" "$HERE/plots_rotor_wing_$(basename "$f")"
+done
+rm -rf "$tmp"
 # 6. velocity probes: the reference's probes2file output of a short wing run
 tmp=$(mktemp -d)
 cp "$ROOT/cases/probes_wing.case/"* "$tmp/"

@@ -257,6 +257,38 @@ def test_wake_plot_matches_reference_solver(step):
             np.testing.assert_array_equal(zg[7], zr[7])
 
 
+def test_inflow_and_circulation_plots_match_reference_solver():
+    """inflow2file (libPostprocess.f90:900-946: every rotor's wing less its
+    bound vortices plus its wake, along -z at the section CPs; the span
+    column is secCP . (hubCoords - yAxis) as the reference computes it) and
+    gamma2file (949-978) on a rotor above a wing: the rotor plots every 12
+    steps, the wing every 24.  At step 24 the reference calls gamma2file
+    with rotor 1 for both switches (main.f90:777-783), so rotor 1's files
+    hold two copies; the wing's own circulation is never written."""
+    import glob
+    res = os.path.join(CASES, "plots_rotor_wing.case", "Results")
+    run("plots_rotor_wing", nrotors=2)
+    refs = sorted(glob.glob(os.path.join(GOLD, "plots_rotor_wing_*.curve")))
+    names = sorted(os.path.basename(f) for f in glob.glob(os.path.join(res, "*.curve")))
+    assert names == [os.path.basename(f)[len("plots_rotor_wing_"):] for f in refs]
+    for f in refs:
+        mine = os.path.join(res, os.path.basename(f)[len("plots_rotor_wing_"):])
+        if "inflowDist" in f:
+            r, g = read_table(f), read_table(mine)
+            assert list(g) == list(r) == ["secSpan", "inflowVel"]
+            for k in r:
+                np.testing.assert_allclose(g[k], r[k], rtol=2e-6, atol=2e-7, err_msg=f)
+        else:
+            lr = [l.split() for l in open(f)]
+            lg = [l.split() for l in open(mine)]
+            assert [x for x in lg if x[0] == "#"] == [x for x in lr if x[0] == "#"]
+            vr = np.array([[float(v) for v in x] for x in lr if x[0] != "#"])
+            vg = np.array([[float(v) for v in x] for x in lg if x[0] != "#"])
+            assert vg.shape == vr.shape
+            np.testing.assert_allclose(vg, vr, rtol=1e-10, atol=1e-12 * np.max(np.abs(vr)),
+                                       err_msg=f)
+
+
 @pytest.mark.parametrize("blade", ["b01", "b02"])
 def test_surface_geometry_matches_reference_solver(blade):
     """geomSurface2file (libPostprocess.f90:475-615, written once after the

@@ -104,6 +104,8 @@ module volcanor_host
     ! inputs kept only for rXXParams.json (libPostprocess.f90:52-141)
     integer :: imagePlane = 0, imageRotorNum = 0
     real(dp) :: thetaTwist = 0._dp, initWakeVel = 0._dp
+    integer :: inflowPlotSwitch = 0, gammaPlotSwitch = 0
+    real(dp), allocatable :: inflowV(:, :, :)
     real(dp) :: flapInitial = 0._dp, dflapInitial = 0._dp, Iflap = 0._dp, cflap = 0._dp
     real(dp) :: kflap = 0._dp, MflapConstant = 0._dp, dpitch = 0._dp
     integer(c_int) :: solve_iters = 0
@@ -496,6 +498,8 @@ contains
     r%psiStart = psiStart
     r%imagePlane = imagePlane; r%imageRotorNum = imageRotorNum
     r%thetaTwist = thetaTwist; r%initWakeVel = initWakeVel
+    r%inflowPlotSwitch = inflowPlotSwitch; r%gammaPlotSwitch = gammaPlotSwitch
+    if (skewPlotSwitch /= 0) error stop 'ERROR: skew plots are not supported'
     r%flapInitial = flapInitial; r%dflapInitial = dflapInitial; r%Iflap = Iflap
     r%cflap = cflap; r%kflap = kflap; r%MflapConstant = MflapConstant; r%dpitch = dpitch
   end subroutine rotor_readGeom
@@ -724,6 +728,8 @@ contains
     call to_chords_revs(r, sw%gridPlot, dt)
     call to_chords_revs(r, r%wakeTruncateNt, dt)
     call to_chords_revs(r, r%nNwake, dt)
+    call to_chords_revs(r, r%inflowPlotSwitch, dt)
+    call to_chords_revs(r, r%gammaPlotSwitch, dt)
     if (r%wakeTruncateNt > 0) r%wakeTruncateNt = max(r%wakeTruncateNt, r%nNwake + 1)
     if (r%surfaceType == 0) r%surfaceType = 1
     if (r%nNwake > 0 .and. r%nNwake < 2) error stop 'ERROR: Atleast 2 near wake rows mandatory'
@@ -1091,6 +1097,89 @@ contains
       call rotor_map_gam(rotor(ir))
     enddo
   end subroutine finish_solve
+
+  subroutine inflow_launch(ctx, rotor, nr, ir, wakeSuppress)
+    !! Enqueues inflow2file's velocities (libPostprocess.f90:900-946) at the
+    !! section CPs of rotor ir: every rotor's wing less its bound vortices,
+    !! plus its wake; they reach rotor(ir)%inflowV at vlm_wait
+    type(c_ptr), intent(in) :: ctx
+    type(rotor_class), intent(inout), target :: rotor(:)
+    integer, intent(in) :: nr, ir, wakeSuppress
+    integer(c_int) :: tir(3*nr), tset(3*nr), tfl(3*nr)
+    real(c_double) :: p(3, rotor(ir)%ns, rotor(ir)%nb)
+    integer :: jr, k, ib
+    associate (r => rotor(ir))
+      if (.not. allocated(r%inflowV)) allocate (r%inflowV(3, r%ns, r%nb))
+      do ib = 1, r%nb
+        p(:, :, ib) = r%blade(ib)%secCP
+      enddo
+      ! -(sum of bound vortices), then the wings and the wakes
+      k = 0
+      do jr = 1, nr
+        k = k + 1; tir(k) = rotor(jr)%dev; tset(k) = VLM_SET_WING_BOUND; tfl(k) = 0
+      enddo
+      do jr = 1, nr
+        k = k + 1; tir(k) = rotor(jr)%dev; tset(k) = VLM_SET_WING; tfl(k) = 0
+        if (jr == 1) tfl(k) = VLM_TERM_NEGATE
+      enddo
+      do jr = 1, nr
+        if (rotor(jr)%nNwake > 0 .and. wakeSuppress == 0) then
+          k = k + 1; tir(k) = rotor(jr)%dev; tset(k) = VLM_SET_WAKE; tfl(k) = 0
+        endif
+      enddo
+      call vlm_check(ctx, vlm_induced_velocity_terms_deferred(ctx, int(k, c_int), tir, tset, &
+        & tfl, int(r%ns*r%nb, c_int64_t), p, c_loc(r%inflowV), 0_c_int), 'inflow')
+    end associate
+  end subroutine inflow_launch
+
+  subroutine inflow_file(dir, timestamp, r)
+    !! inflow2file's rXXbYYinflowDistNNNNN.curve: the inflow along -z at each
+    !! section CP.  The span column is secCP . (hubCoords - yAxis), as the
+    !! reference computes it.
+    character(len=*), intent(in) :: dir, timestamp
+    type(rotor_class), intent(in) :: r
+    character(len=2) :: bid
+    integer :: u, ib, is
+    do ib = 1, r%nb
+      write (bid, '(I0.2)') ib
+      open (newunit=u, file=trim(dir)//'/Results/r'//r%id//'b'//bid//'inflowDist'// &
+        & timestamp//'.curve', status='replace', action='write')
+      write (u, '(2(A15))') 'secSpan', 'inflowVel'
+      do is = 1, r%ns
+        write (u, '(2(F15.7))') dot_product(r%blade(ib)%secCP(:, is), r%hubCoords - &
+          & r%blade(ib)%yAxis), -r%inflowV(3, is, ib)
+      enddo
+      close (u)
+    enddo
+  end subroutine inflow_file
+
+  subroutine gamma_file(dir, timestamp, r, first)
+    !! gamma2file (libPostprocess.f90:949-978): the spanwise bound
+    !! circulation -sum over the chord of Γ per blade.  The reference calls
+    !! it with rotor 1 for every rotor whose switch fires (main.f90:777-783)
+    !! and appends, so a step's later calls add copies to the same files.
+    character(len=*), intent(in) :: dir, timestamp
+    type(rotor_class), intent(in) :: r
+    logical, intent(in) :: first
+    character(len=2) :: bid
+    integer :: u, ib, is
+    do ib = 1, r%nb
+      write (bid, '(I0.2)') ib
+      if (first) then
+        open (newunit=u, file=trim(dir)//'/Results/r'//r%id//'b'//bid//'gammaDist'// &
+          & timestamp//'.curve', status='replace', action='write')
+      else
+        open (newunit=u, file=trim(dir)//'/Results/r'//r%id//'b'//bid//'gammaDist'// &
+          & timestamp//'.curve', position='append', action='write')
+      endif
+      write (u, *) '# Row01'
+      do is = 1, r%ns
+        write (u, *) dot_product(r%blade(ib)%wiP(1, is)%CP - r%hubCoords, r%blade(ib)%yAxis), &
+          & -1._dp*sum(r%blade(ib)%wiP(:, is)%vr%gam)
+      enddo
+      close (u)
+    enddo
+  end subroutine gamma_file
 
   subroutine surface_file(dir, r)
     !! geomSurface2file, libPostprocess.f90:475-615: the blade geometry after
@@ -1906,6 +1995,12 @@ contains
         call vlm_check(ctx, vlm_induced_velocity_terms_deferred(ctx, int(k, c_int), pir, &
           & pset, pfl, int(nprobe, c_int64_t), probeLoc, c_loc(probeOut), 1_c_int), 'probes')
       endif
+      ! inflow plots (main.f90:767-774), on the wake before convection
+      do ir = 1, nr
+        if (.not. write_out .or. rotor(ir)%inflowPlotSwitch == 0) cycle
+        if (mod(iter, rotor(ir)%inflowPlotSwitch) /= 0) cycle
+        call inflow_launch(ctx, rotor, nr, ir, sw%wakeSuppress)
+      enddo
       ! The wake passes are enqueued before the host waits for the
       ! circulation and the load velocities; the loads (main.f90:632-719)
       ! depend only on the wing and its circulation, which the convection
@@ -1915,6 +2010,20 @@ contains
         & 'convect')
       call vlm_check(ctx, vlm_wait(ctx), 'wait')
       call finish_solve(rotor, nr)
+      if (write_out) then
+        write (ts5, '(I0.5)') iter
+        do ir = 1, nr
+          if (rotor(ir)%inflowPlotSwitch == 0) cycle
+          if (mod(iter, rotor(ir)%inflowPlotSwitch) == 0) call inflow_file(dir, ts5, rotor(ir))
+        enddo
+        k = 0
+        do ir = 1, nr
+          if (rotor(ir)%gammaPlotSwitch == 0) cycle
+          if (mod(iter, rotor(ir)%gammaPlotSwitch) /= 0) cycle
+          call gamma_file(dir, ts5, rotor(1), k == 0)
+          k = k + 1
+        enddo
+      endif
       if (probing .and. verbose >= 0) then
         write (ts5, '(I0.5)') iter
         open (newunit=uprobe, file=trim(dir)//'/Results/probes'//ts5//'.csv', &
