@@ -17,6 +17,7 @@ PLOT3D blade file) used by the tests and the benchmark.
   initwake_*_rotor   the start-up wake velocity initWakeVel under AM2 and AM4
   tec_rotor          wing and wake plots (.tec) of a forward-flight rotor
   plots_rotor_wing   inflow and bound-circulation plots of a rotor above a wing
+  grid_rotor         filament dumps and gridconfig.nml for the grid velocity field
   slowstart*_rotor   the three slow-start ramps of the rotor speed
   subiter_rotor_wing cfg5_rotor_wing_short with sub-iterations (ntSubInit 5, ntSub 3)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
@@ -276,6 +277,23 @@ def make_all(out):
                span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25, velBody=[-13.6, 0.0, -2.0],
                apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
                streamwiseCoreVec=ROTOR_CORE)
+
+    # filament dumps for gridgen at steps 20 and 40 of the same rotor, and
+    # the grid the velocity field is evaluated on (gridconfig.nml 0.2)
+    d = case("grid_rotor")
+    write_config(os.path.join(d, "config.nml"), nt=40, dt=math.radians(5.0) / om, nr=1,
+                 velSound=340.0, wakeDissipation=1, wakeStrain=1, gridPlot=20)
+    write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=2, ns=8, nNwake=12, wakeTruncateNt=36,
+               span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25, velBody=[-13.6, 0.0, -2.0],
+               apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+               streamwiseCoreVec=ROTOR_CORE)
+    with open(os.path.join(d, "gridconfig.nml"), "w") as f:
+        f.write("&VERSION\nfileFormatVersion = 0.2\n/\n")
+        f.write(f"&INPUTS\nnx = 7\nny = 7\nnz = 7\n"
+                f"xyzMin = {fmt([-1.2 * R, -1.2 * R, -0.6 * R])}\n"
+                f"xyzMax = {fmt([1.2 * R, 1.2 * R, 0.2 * R])}\n"
+                f"vel = 13.6, 0.0, 2.0\nfileRangeStart = 20\nfileRangeStep = 20\n"
+                f"fileRangeEnd = 40\n/\n")
 
     # inflow and bound-circulation plots of a rotor above a wing: the rotor
     # plots every 12 steps, the wing every 24; at step 24 the reference

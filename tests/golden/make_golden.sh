@@ -87,6 +87,17 @@ for f in "$tmp"/Results/*.curve; do
   cp "$f" "$HERE/plots_rotor_wing_$(basename "$f")"
 done
 rm -rf "$tmp"
+# 5f. the velocity field on a grid: the reference's gridgen on the reference's
+#     filament dumps of the grid_rotor case (dump sizes are in the test)
+tmp=$(mktemp -d)
+cp "$ROOT/cases/grid_rotor.case/"* "$tmp/"
+mkdir -p "$tmp/Results" "$tmp/Restart"
+(cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 OMP_SCHEDULE=static \
+   "$ROOT/oracle/_ref/volcanor" > run.log 2>&1 && "$ROOT/oracle/_ref/gridgen" > gridgen.log 2>&1)
+for st in 00020 00040; do
+  cp "$tmp/Results/grid$st.tec" "$HERE/grid_rotor_grid$st.tec"
+done
+rm -rf "$tmp"
 # 6. velocity probes: the reference's probes2file output of a short wing run
 tmp=$(mktemp -d)
 cp "$ROOT/cases/probes_wing.case/"* "$tmp/"

@@ -257,6 +257,44 @@ def test_wake_plot_matches_reference_solver(step):
             np.testing.assert_array_equal(zg[7], zr[7])
 
 
+def test_grid_velocity_field_matches_reference_gridgen():
+    """The Eulerian grid (SURVEY's gridgen row): the host writes
+    filaments2file's dumps at steps 20 and 40 (libPostprocess.f90:363-473),
+    byte for byte the size of the reference's dumps of the same steps (the
+    same unformatted records), and volcanor_amd/bin/gridgen_gpu evaluates
+    every filament at the 6x6x6 cell centres on the GPU, with the
+    reference's wing-loop assignment (gridgen.f90:122: only the last wing
+    ring counts) and the free stream added.  Compared with the grid files
+    of the reference's own gridgen run on the reference's dumps."""
+    import subprocess
+    run("grid_rotor")
+    case = os.path.join(CASES, "grid_rotor.case")
+    res = os.path.join(case, "Results")
+    ref_dump_bytes = {"00020": 93000, "00040": 96328}
+    for st, n in ref_dump_bytes.items():
+        assert os.path.getsize(os.path.join(res, f"filaments{st}.dat")) == n
+    subprocess.run([os.path.join(ROOT, "volcanor_amd", "bin", "gridgen_gpu"), case], check=True,
+                   timeout=120, stdout=subprocess.DEVNULL)
+
+    def split(path):
+        text, nums = [], []
+        for line in open(path):
+            w = line.strip()
+            if w and w[0].isalpha():
+                text.append(w)
+            else:
+                nums.extend(float(x) for x in w.split())
+        return text, np.array(nums)
+
+    for st in ref_dump_bytes:
+        tr, vr = split(os.path.join(GOLD, f"grid_rotor_grid{st}.tec"))
+        tg, vg = split(os.path.join(res, f"grid{st}.tec"))
+        assert tg == tr and vg.shape == vr.shape == (3 * 343 + 3 * 216,)
+        np.testing.assert_array_equal(vg[:3 * 343], vr[:3 * 343])      # grid nodes
+        np.testing.assert_allclose(vg[3 * 343:], vr[3 * 343:], rtol=1e-10,
+                                   atol=1e-12 * np.max(np.abs(vr[3 * 343:])))
+
+
 def test_inflow_and_circulation_plots_match_reference_solver():
     """inflow2file (libPostprocess.f90:900-946: every rotor's wing less its
     bound vortices plus its wake, along -z at the section CPs; the span

@@ -61,6 +61,16 @@ module vlm_c
       type(c_ptr), value :: ctx
     end function
 
+    integer(c_int) function vlm_filament_vind(ctx, nfil, a, b, gam, rc, m, p, v, acc) &
+        bind(C, name='vlm_filament_vind')
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nfil, m
+      real(c_double), intent(in) :: a(3, *), b(3, *), gam(*), rc(*), p(3, *)
+      real(c_double), intent(inout) :: v(3, *)
+      integer(c_int), value :: acc
+    end function
+
     integer(c_int) function vlm_rotor_create(ctx, desc, spanwiseCore, scv, ir) &
         bind(C, name='vlm_rotor_create')
       import :: c_int, c_ptr, c_double, vlm_rotor_desc

@@ -1098,6 +1098,122 @@ contains
     enddo
   end subroutine finish_solve
 
+  subroutine filaments_file(ctx, dir, timestamp, rotor, nr)
+    !! filaments2file (libPostprocess.f90:363-473), the input of gridgen:
+    !! every wing ring, every near-wake ring (all nNwake rows), the oldest
+    !! near row's trailing filaments (vf(2), with -Γ) and the active far-wake
+    !! filaments, as the reference's unformatted records of its vr_class and
+    !! vf_class (a filament: fc(3,2), l0, lc, rVc0, rVc, age, ageAzimuthal; a
+    !! ring: 4 filaments, gam, skew).  gridgen reads only fc, rVc and gam;
+    !! the lengths, ages and skew are written as 0.
+    type(c_ptr), intent(in) :: ctx
+    character(len=*), intent(in) :: dir, timestamp
+    type(rotor_class), intent(in) :: rotor(:)
+    integer, intent(in) :: nr
+    type :: dump_vf
+      real(dp) :: fc(3, 2) = 0._dp
+      real(dp) :: l0 = 0._dp, lc = 0._dp, rVc0 = 0._dp, rVc = 0._dp
+      real(dp) :: age = 0._dp, ageAzimuthal = 0._dp
+    end type dump_vf
+    type :: dump_vr
+      type(dump_vf) :: vf(4)
+      real(dp) :: gam = 0._dp, skew = 0._dp
+    end type dump_vr
+    type(dump_vr), allocatable :: vrWing(:), vrNwake(:)
+    type(dump_vf), allocatable :: vfNwakeTE(:), vfFwake(:)
+    real(dp), allocatable :: gamNwakeTE(:), gamFwake(:)
+    real(c_double), allocatable, target :: nodes(:, :, :, :), le(:, :, :), gam(:, :, :), &
+      & rc4(:, :, :, :), ffc(:, :, :, :), fgam(:, :), frc(:, :)
+    integer :: nW, nN, nT, nF, ir, ib, irow, icol, k, iw, in, it, jf, u
+    real(dp) :: c(3, 4)
+
+    do ir = 1, nr
+      if (rotor(ir)%rowFar > rotor(ir)%nFwake) &
+        & error stop 'ERROR: Use filaments2file() only after development of far wake'
+    enddo
+    nW = 0; nN = 0; nT = 0; nF = 0
+    do ir = 1, nr
+      nW = nW + rotor(ir)%nb*rotor(ir)%nc*rotor(ir)%ns
+      nN = nN + rotor(ir)%nb*rotor(ir)%nNwake*rotor(ir)%ns
+      nT = nT + rotor(ir)%nb*rotor(ir)%ns
+      nF = nF + (rotor(ir)%nFwake - rotor(ir)%rowFar + 1)*rotor(ir)%nb
+    enddo
+    allocate (vrWing(nW), vrNwake(nN), vfNwakeTE(nT), vfFwake(nF), gamNwakeTE(nT), gamFwake(nF))
+    iw = 0; in = 0; it = 0; jf = 0
+    do ir = 1, nr
+      associate (r => rotor(ir))
+        do ib = 1, r%nb
+          do icol = 1, r%ns
+            do irow = 1, r%nc
+              iw = iw + 1
+              do k = 1, 4
+                vrWing(iw)%vf(k)%fc(:, 1) = r%blade(ib)%wiP(irow, icol)%vr%fc(:, k)
+                vrWing(iw)%vf(k)%fc(:, 2) = r%blade(ib)%wiP(irow, icol)%vr%fc(:, mod(k, 4) + 1)
+                vrWing(iw)%vf(k)%rVc = r%blade(ib)%wiP(irow, icol)%vr%rVc(k)
+              enddo
+              vrWing(iw)%gam = r%blade(ib)%wiP(irow, icol)%vr%gam
+            enddo
+          enddo
+        enddo
+        allocate (nodes(3, r%ns + 1, r%nNwake, r%nb), le(3, r%ns + 1, r%nb), &
+          & gam(r%ns, r%nNwake, r%nb), rc4(4, r%ns, r%nNwake, r%nb), &
+          & ffc(3, 2, r%nFwake, r%nb), fgam(r%nFwake, r%nb), frc(r%nFwake, r%nb))
+        call vlm_check(ctx, vlm_rotor_get_wake(ctx, r%dev, VLM_WAKE_C, c_loc(nodes), &
+          & c_loc(le), c_loc(gam), c_loc(rc4), c_loc(ffc), c_loc(fgam), c_loc(frc)), 'get_wake')
+        do ib = 1, r%nb
+          do icol = 1, r%ns
+            do irow = 1, r%nNwake
+              in = in + 1
+              if (irow < r%rowNear) cycle
+              ! ring corners: c1 and c4 on the upstream row (the shed line
+              ! on the newest row), c2 and c3 on this row
+              if (irow == r%rowNear) then
+                c(:, 1) = le(:, icol, ib); c(:, 4) = le(:, icol + 1, ib)
+              else
+                c(:, 1) = nodes(:, icol, irow - 1, ib); c(:, 4) = nodes(:, icol + 1, irow - 1, ib)
+              endif
+              c(:, 2) = nodes(:, icol, irow, ib); c(:, 3) = nodes(:, icol + 1, irow, ib)
+              do k = 1, 4
+                vrNwake(in)%vf(k)%fc(:, 1) = c(:, k)
+                vrNwake(in)%vf(k)%fc(:, 2) = c(:, mod(k, 4) + 1)
+                vrNwake(in)%vf(k)%rVc = rc4(k, icol, irow, ib)
+              enddo
+              vrNwake(in)%gam = gam(icol, irow, ib)
+            enddo
+          enddo
+        enddo
+        do ib = 1, r%nb
+          do icol = 1, r%ns
+            it = it + 1
+            vfNwakeTE(it)%fc(:, 1) = nodes(:, icol, r%nNwake, ib)
+            vfNwakeTE(it)%fc(:, 2) = nodes(:, icol + 1, r%nNwake, ib)
+            vfNwakeTE(it)%rVc = rc4(2, icol, r%nNwake, ib)
+            gamNwakeTE(it) = gam(icol, r%nNwake, ib)*(-1._dp)
+          enddo
+        enddo
+        do ib = 1, r%nb
+          do irow = r%rowFar, r%nFwake
+            jf = jf + 1
+            vfFwake(jf)%fc = ffc(:, :, irow, ib)
+            vfFwake(jf)%rVc = frc(irow, ib)
+            gamFwake(jf) = fgam(irow, ib)
+          enddo
+        enddo
+        deallocate (nodes, le, gam, rc4, ffc, fgam, frc)
+      end associate
+    enddo
+    open (newunit=u, file=trim(dir)//'/Results/filaments'//timestamp//'.dat', &
+      & status='replace', action='write', form='unformatted')
+    write (u) nW
+    write (u) nN
+    write (u) nT
+    write (u) nF
+    write (u) vrWing, vrNwake
+    write (u) vfNwakeTE, gamNwakeTE
+    write (u) vfFwake, gamFwake
+    close (u)
+  end subroutine filaments_file
+
   subroutine inflow_launch(ctx, rotor, nr, ir, wakeSuppress)
     !! Enqueues inflow2file's velocities (libPostprocess.f90:900-946) at the
     !! section CPs of rotor ir: every rotor's wing less its bound vortices,
@@ -1994,6 +2110,16 @@ contains
         enddo
         call vlm_check(ctx, vlm_induced_velocity_terms_deferred(ctx, int(k, c_int), pir, &
           & pset, pfl, int(nprobe, c_int64_t), probeLoc, c_loc(probeOut), 1_c_int), 'probes')
+      endif
+      ! filament dumps for gridgen (main.f90:785-790): the wake before
+      ! convection with this step's circulation
+      if (write_out .and. sw%gridPlot /= 0) then
+        if (mod(iter, sw%gridPlot) == 0) then
+          call vlm_check(ctx, vlm_wait(ctx), 'wait')
+          call finish_solve(rotor, nr)
+          write (ts5, '(I0.5)') iter
+          call filaments_file(ctx, dir, ts5, rotor, nr)
+        endif
       endif
       ! inflow plots (main.f90:767-774), on the wake before convection
       do ir = 1, nr
