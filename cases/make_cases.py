@@ -18,6 +18,7 @@ PLOT3D blade file) used by the tests and the benchmark.
   tec_rotor          wing and wake plots (.tec) of a forward-flight rotor
   plots_rotor_wing   inflow and bound-circulation plots of a rotor above a wing
   grid_rotor         filament dumps and gridconfig.nml for the grid velocity field
+  burst_rotor        far-wake burst every step with a small skew limit
   slowstart*_rotor   the three slow-start ramps of the rotor speed
   subiter_rotor_wing cfg5_rotor_wing_short with sub-iterations (ntSubInit 5, ntSub 3)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
@@ -277,6 +278,16 @@ def make_all(out):
                span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25, velBody=[-13.6, 0.0, -2.0],
                apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
                streamwiseCoreVec=ROTOR_CORE)
+
+    # far-wake burst every step (main.f90:490-496) with a small skew limit,
+    # strain off so that the burst cores persist
+    d = case("burst_rotor")
+    write_config(os.path.join(d, "config.nml"), nt=48, dt=math.radians(5.0) / om, nr=1,
+                 velSound=340.0, wakeDissipation=1, wakeBurst=1)
+    write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=1, ns=10, nNwake=12, wakeTruncateNt=44,
+               span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25, velBody=[-13.6, 0.0, -2.0],
+               apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+               streamwiseCoreVec=ROTOR_CORE, skewLimit=0.02)
 
     # filament dumps for gridgen at steps 20 and 40 of the same rotor, and
     # the grid the velocity field is evaluated on (gridconfig.nml 0.2)

@@ -142,7 +142,16 @@ int vlm_rotor_age_wake(vlm_ctx *ctx, int ir, double dt);      /* 4331-4354 */
 int vlm_rotor_dissipate_wake(vlm_ctx *ctx, int ir, double dt,
                              double kinematicVisc, double apparentViscCoeff,
                              double decayCoeff);              /* 4356-4408 */
-int vlm_rotor_strain_wake(vlm_ctx *ctx, int ir);              /* 4410-4422 */
+int vlm_rotor_strain_wake(vlm_ctx *ctx, int ir);              /* Far-wake burst (rotor_burst_wake, classdef.f90:4911-4917 and
+ * blade_burst_wake 2306-2339, called every wakeBurst steps, main.f90:490-496):
+ * for every blade, a far filament takes the core radius largeCoreRadius
+ * (the reference passes the chord) when the skew |acos(cos) - pi| / pi of
+ * the angle between it and a neighbouring far filament, reversed, reaches
+ * skewLimit. */
+int vlm_rotor_burst_wake(vlm_ctx *ctx, int ir, double skewLimit,
+                         double largeCoreRadius);
+
+/* 4410-4422 */
 int vlm_rotor_rollup(vlm_ctx *ctx, int ir);                   /* 4515-4605 + shift 4481-4513 */
 int vlm_rotor_convectwake(vlm_ctx *ctx, int ir, double dt,
                           int wakeType);                      /* 1515-1702, 4786-4830 */

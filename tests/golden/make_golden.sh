@@ -61,6 +61,7 @@ for k in 1 2 3; do
   run_ref slowstart${k}_rotor slowstart${k}_rotor_ForceNonDim.csv
 done
 run_ref subiter_rotor_wing subiter_rotor_wing_ForceNonDim.csv
+run_ref burst_rotor burst_rotor_ForceNonDim.csv
 # 5d. wing and wake plots (.tec) of a forward-flight rotor at steps 20 and 40, and
 #     the force files of every force step (ForceNonDim, ForceDim, ForceDist)
 tmp=$(mktemp -d)

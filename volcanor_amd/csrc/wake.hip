@@ -471,6 +471,35 @@ __global__ void strain_kernel(RV v)
     v.frc[q] = v.frc0[q] * sqrt(v.fl0[q] / v.flc[q]);
 }
 
+// Skew of the far-wake pair (i, i+1): the angle between filament i
+// (fc1 -> fc2) and filament i+1 reversed, as |acos(cos) - pi| / pi
+// (blade_burst_wake, classdef.f90:2322-2338; getAngleCos, libMath.f90:238-247).
+__device__ double far_pair_skew(const double *ffc, size_t q)
+{
+    const double *f = ffc + 6 * q, *g = ffc + 6 * (q + 1);
+    const double ax = f[3] - f[0], ay = f[4] - f[1], az = f[5] - f[2];
+    const double bx = g[0] - g[3], by = g[1] - g[4], bz = g[2] - g[5];
+    const double d = ax * bx + ay * by + az * bz;
+    const double n = sqrt((ax * ax + ay * ay + az * az) * (bx * bx + by * by + bz * bz));
+    return fabs(acos(d / n) - kPi) / kPi;
+}
+
+// rotor_burst_wake (classdef.f90:4911-4917): a far filament takes the large
+// core radius when the pair with either neighbour reaches the skew limit.
+// One thread per filament, so no two threads write one core radius.
+__global__ void burst_kernel(RV v, double skew_limit, double large_core)
+{
+    const int F = v.nF - v.rf;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (F <= 0 || e >= v.nb * F) return;
+    const int i = v.rf + e % F;
+    const size_t q = (size_t)(e / F) * v.nF + i;
+    bool burst = false;
+    if (i > v.rf) burst = far_pair_skew(v.ffc, q - 1) >= skew_limit;
+    if (i + 1 < v.nF) burst = burst || far_pair_skew(v.ffc, q) >= skew_limit;
+    if (burst) v.frc[q] = large_core;
+}
+
 // rotor_shiftFwake (classdef.f90:4500-4513) in two passes via staging.
 __global__ void far_stage_kernel(RV v, double *tmp)
 {
@@ -1616,6 +1645,19 @@ int vlm_rotor_strain_wake(vlm_ctx *ctx, int ir)
     if (r->d.nNwake <= 0 || F <= 0) return VLM_OK;
     hipLaunchKernelGGL(strain_kernel, dim3(nblk(F * r->d.nb)), dim3(256), 0, ctx->stream,
                        view(*r));
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
+int vlm_rotor_burst_wake(vlm_ctx *ctx, int ir, double skewLimit, double largeCoreRadius)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
+    const int64_t F = r->d.nFwake - (r->rowFar - 1);
+    if (r->d.nNwake <= 0 || F <= 0) return VLM_OK;
+    hipLaunchKernelGGL(burst_kernel, dim3(nblk(F * r->d.nb)), dim3(256), 0, ctx->stream,
+                       view(*r), skewLimit, largeCoreRadius);
     VLM_HIP(hipGetLastError());
     return VLM_OK;
 }

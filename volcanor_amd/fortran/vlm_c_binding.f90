@@ -152,6 +152,14 @@ module vlm_c
       integer(c_int), value :: ir
     end function
 
+    integer(c_int) function vlm_rotor_burst_wake(ctx, ir, skewLimit, largeCore) &
+        bind(C, name='vlm_rotor_burst_wake')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir
+      real(c_double), value :: skewLimit, largeCore
+    end function
+
     integer(c_int) function vlm_rotor_rollup(ctx, ir) bind(C, name='vlm_rotor_rollup')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

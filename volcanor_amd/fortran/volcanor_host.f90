@@ -105,6 +105,7 @@ module volcanor_host
     integer :: imagePlane = 0, imageRotorNum = 0
     real(dp) :: thetaTwist = 0._dp, initWakeVel = 0._dp
     integer :: inflowPlotSwitch = 0, gammaPlotSwitch = 0
+    real(dp) :: skewLimit = 0._dp
     real(dp), allocatable :: inflowV(:, :, :)
     real(dp) :: flapInitial = 0._dp, dflapInitial = 0._dp, Iflap = 0._dp, cflap = 0._dp
     real(dp) :: kflap = 0._dp, MflapConstant = 0._dp, dpitch = 0._dp
@@ -499,6 +500,7 @@ contains
     r%imagePlane = imagePlane; r%imageRotorNum = imageRotorNum
     r%thetaTwist = thetaTwist; r%initWakeVel = initWakeVel
     r%inflowPlotSwitch = inflowPlotSwitch; r%gammaPlotSwitch = gammaPlotSwitch
+    r%skewLimit = skewLimit
     if (skewPlotSwitch /= 0) error stop 'ERROR: skew plots are not supported'
     r%flapInitial = flapInitial; r%dflapInitial = dflapInitial; r%Iflap = Iflap
     r%cflap = cflap; r%kflap = kflap; r%MflapConstant = MflapConstant; r%dpitch = dpitch
@@ -2069,6 +2071,11 @@ contains
           if (sw%wakeDissipation == 1) call vlm_check(ctx, vlm_rotor_dissipate_wake(ctx, &
             & rotor(ir)%dev, dt, kinematicVisc, rotor(ir)%apparentViscCoeff, rotor(ir)%decayCoeff), &
             & 'dissipate_wake')
+          ! far-wake burst every wakeBurst steps (main.f90:490-496)
+          if (sw%wakeBurst /= 0) then
+            if (mod(iter, sw%wakeBurst) == 0) call vlm_check(ctx, vlm_rotor_burst_wake(ctx, &
+              & rotor(ir)%dev, rotor(ir)%skewLimit, rotor(ir)%chord), 'burst_wake')
+          endif
         enddo
       endif
       ! wing and wake plot, main.f90:509-513

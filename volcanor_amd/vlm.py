@@ -64,6 +64,7 @@ _SIGS = {
     "vlm_rotor_age_wake": [_P, _I, _D],
     "vlm_rotor_dissipate_wake": [_P, _I, _D, _D, _D, _D],
     "vlm_rotor_strain_wake": [_P, _I],
+    "vlm_rotor_burst_wake": [_P, _I, _D, _D],
     "vlm_rotor_rollup": [_P, _I],
     "vlm_rotor_convectwake": [_P, _I, _D, _I],
     "vlm_rotor_induced_velocity": [_P, _I, _I, _L, _P, _P, _I],
@@ -250,6 +251,9 @@ class Context:
 
     def rotor_strain_wake(self, ir):
         self._chk(self.lib.vlm_rotor_strain_wake(self.h, ir))
+
+    def rotor_burst_wake(self, ir, skew_limit, large_core):
+        self._chk(self.lib.vlm_rotor_burst_wake(self.h, ir, skew_limit, large_core))
 
     def rotor_rollup(self, ir):
         self._chk(self.lib.vlm_rotor_rollup(self.h, ir))
