@@ -19,6 +19,7 @@ PLOT3D blade file) used by the tests and the benchmark.
   plots_rotor_wing   inflow and bound-circulation plots of a rotor above a wing
   grid_rotor         filament dumps and gridconfig.nml for the grid velocity field
   burst_rotor        far-wake burst every step with a small skew limit
+  traj_wing          the AR-4 wing on a custom trajectory (trajectory01.in)
   slowstart*_rotor   the three slow-start ramps of the rotor speed
   subiter_rotor_wing cfg5_rotor_wing_short with sub-iterations (ntSubInit 5, ntSub 3)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
@@ -278,6 +279,22 @@ def make_all(out):
                span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25, velBody=[-13.6, 0.0, -2.0],
                apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
                streamwiseCoreVec=ROTOR_CORE)
+
+    # custom trajectory (classdef.f90:3064-3080, main.f90:112-117 and
+    # 419-426): the AR-4 wing's body velocity and rate follow
+    # trajectory01.in, one line per step: speed varying by 20%, a plunge
+    # velocity and a pitch rate
+    nt_t = 60
+    d = case("traj_wing")
+    write_config(os.path.join(d, "config.nml"), nt=nt_t, dt=0.025, nr=1)
+    write_geom(os.path.join(d, "geom01.nml"), nc=4, ns=13, nNwake=200,
+               customTrajectorySwitch=1)
+    with open(os.path.join(d, "trajectory01.in"), "w") as f:
+        for i in range(1, nt_t + 1):
+            u = -10.0 * (1.0 + 0.2 * math.sin(2.0 * math.pi * i / 30.0))
+            w = -0.5 * math.sin(2.0 * math.pi * i / 20.0)
+            q = 0.1 * math.cos(2.0 * math.pi * i / 30.0)
+            f.write(f"{u:.15g} 0.0 {w:.15g} 0.0 {q:.15g} 0.0\n")
 
     # far-wake burst every step (main.f90:490-496) with a small skew limit,
     # strain off so that the burst cores persist

@@ -62,6 +62,7 @@ for k in 1 2 3; do
 done
 run_ref subiter_rotor_wing subiter_rotor_wing_ForceNonDim.csv
 run_ref burst_rotor burst_rotor_ForceNonDim.csv
+run_ref traj_wing traj_wing_ForceNonDim.csv
 # 5d. wing and wake plots (.tec) of a forward-flight rotor at steps 20 and 40, and
 #     the force files of every force step (ForceNonDim, ForceDim, ForceDist)
 tmp=$(mktemp -d)

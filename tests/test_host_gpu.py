@@ -383,6 +383,18 @@ def test_force_outputs_match_reference_solver():
             == open(os.path.join(GOLD, "tec_rotor_bladedynamics.csv")).read())
 
 
+def test_custom_trajectory_matches_reference_solver():
+    """customTrajectorySwitch = 1 (classdef.f90:3064-3080; main.f90:112-117
+    and 419-426): the AR-4 wing's body velocity and rate come from
+    trajectory01.in, one line per step (speed varying by 20%, a plunge
+    velocity, a pitch rate about the c.g.).  Without the trajectory the
+    reference's CL history differs by more than 100%."""
+    ref = read_force(os.path.join(GOLD, "traj_wing_ForceNonDim.csv"))[:, 0]
+    ct, _ = run("traj_wing")
+    assert len(ct[0]) == len(ref)
+    np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
+
+
 def test_wake_burst_matches_reference_solver():
     """wakeBurst = 1 with skewLimit 0.02 (main.f90:490-496, classdef.f90:
     2306-2339): every step, far filaments whose angle with a neighbour

@@ -104,8 +104,9 @@ module volcanor_host
     ! inputs kept only for rXXParams.json (libPostprocess.f90:52-141)
     integer :: imagePlane = 0, imageRotorNum = 0
     real(dp) :: thetaTwist = 0._dp, initWakeVel = 0._dp
-    integer :: inflowPlotSwitch = 0, gammaPlotSwitch = 0
+    integer :: inflowPlotSwitch = 0, gammaPlotSwitch = 0, customTrajectorySwitch = 0
     real(dp) :: skewLimit = 0._dp
+    real(dp), allocatable :: velBodyHistory(:, :), omegaBodyHistory(:, :)
     real(dp), allocatable :: inflowV(:, :, :)
     real(dp) :: flapInitial = 0._dp, dflapInitial = 0._dp, Iflap = 0._dp, cflap = 0._dp
     real(dp) :: kflap = 0._dp, MflapConstant = 0._dp, dpitch = 0._dp
@@ -473,7 +474,9 @@ contains
     if (forceCalcSwitch /= 0) error stop 'ERROR: only forceCalcSwitch = 0 is supported'
     if (bladeDynamicsSwitch /= 0 .or. bodyDynamicsSwitch /= 0 .or. pitchDynamicsSwitch /= 0) &
       error stop 'ERROR: blade, pitch and body dynamics are not supported'
-    if (customTrajectorySwitch /= 0) error stop 'ERROR: custom trajectories are not supported'
+    if (customTrajectorySwitch /= 0 .and. customTrajectorySwitch /= 1) &
+      & error stop 'ERROR: customTrajectorySwitch must be 0 or 1'
+    r%customTrajectorySwitch = customTrajectorySwitch
     if (prescWakeAfterTruncNt /= 0) error stop 'ERROR: the prescribed far wake is not supported'
 
     r%surfaceType = surfaceType
@@ -740,6 +743,19 @@ contains
       r%nFwake = nt - r%nNwake
     else
       r%nFwake = r%wakeTruncateNt - r%nNwake
+    endif
+    ! custom trajectory: body velocity and rate of every step from
+    ! trajectoryXX.in, nt lines of six values (classdef.f90:3064-3080)
+    if (r%customTrajectorySwitch == 1) then
+      block
+        integer :: u, i
+        allocate (r%velBodyHistory(3, nt), r%omegaBodyHistory(3, nt))
+        open (newunit=u, file=trim(dir)//'/trajectory'//r%id//'.in', status='old', action='read')
+        do i = 1, nt
+          read (u, *) r%velBodyHistory(:, i), r%omegaBodyHistory(:, i)
+        enddo
+        close (u)
+      end block
     endif
     r%nbConvect = r%nb
     if (r%axisymmetrySwitch == 1) r%nbConvect = 1
@@ -1982,8 +1998,13 @@ contains
       end associate
     enddo
 
-    ! initial solution without wake; a slow start begins at rest (main.f90:97-105)
+    ! initial solution without wake; a slow start begins at rest (main.f90:97-105);
+    ! a custom trajectory starts from its first line (112-117)
     do ir = 1, nr
+      if (rotor(ir)%customTrajectorySwitch == 1) then
+        rotor(ir)%velBody = rotor(ir)%velBodyHistory(:, 1)
+        rotor(ir)%omegaBody = rotor(ir)%omegaBodyHistory(:, 1)
+      endif
       rotor(ir)%omegaSlow = rotor(ir)%Omega
       if (sw%slowStart > 0) rotor(ir)%omegaSlow = 0._dp
     enddo
@@ -2055,6 +2076,10 @@ contains
         call vlm_check(ctx, vlm_rotor_begin_step(ctx, rotor(ir)%dev, int(iter, c_int)), 'begin_step')
         call vlm_check(ctx, vlm_rotor_cursors(ctx, rotor(ir)%dev, rowNear, rowFar), 'cursors')
         rotor(ir)%rowNear = rowNear; rotor(ir)%rowFar = rowFar
+        if (rotor(ir)%customTrajectorySwitch == 1) then   ! main.f90:419-426
+          rotor(ir)%velBody = rotor(ir)%velBodyHistory(:, iter)
+          rotor(ir)%omegaBody = rotor(ir)%omegaBodyHistory(:, iter)
+        endif
         rotor(ir)%omegaSlow = slow_omega(sw, iter, rotor(ir)%Omega)
       enddo
       do ir = 1, nr
