@@ -20,6 +20,7 @@ PLOT3D blade file) used by the tests and the benchmark.
   grid_rotor         filament dumps and gridconfig.nml for the grid velocity field
   burst_rotor        far-wake burst every step with a small skew limit
   traj_wing          the AR-4 wing on a custom trajectory (trajectory01.in)
+  camber_wing        the AR-4 wing with a camber line
   slowstart*_rotor   the three slow-start ramps of the rotor speed
   subiter_rotor_wing cfg5_rotor_wing_short with sub-iterations (ntSubInit 5, ntSub 3)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
@@ -85,7 +86,7 @@ def write_geom(path, **g):
              wakeTruncateNt=0, prescWakeAfterTruncNt=0, prescWakeGenNt=0, spanwiseCore=0.03,
              streamwiseCoreVec=0.03, rollupStartRadius=0.75, rollupEndRadius=1.0,
              initWakeVel=0.0, psiStart=0.0, skewLimit=0.5, inflowPlotSwitch=0,
-             gammaPlotSwitch=0)
+             gammaPlotSwitch=0, camberSectionLimit=[1.0], camberFile=["0"])
     d.update(g)
     groups = [
         ("SURFACE", ["surfaceType", "imagePlane", "imageRotorNum"]),
@@ -108,7 +109,8 @@ def write_geom(path, **g):
         for name, keys in groups:
             f.write(f"&{name}\n")
             if name == "CAMBERSECTIONS":
-                f.write("camberSectionLimit = 1.0\ncamberFile = 0\n")
+                f.write(f"camberSectionLimit = {fmt(d['camberSectionLimit'])}\n"
+                        f"camberFile = {fmt(d['camberFile'])}\n")
             for k in keys:
                 f.write(f"{k} = {fmt(d[k])}\n")
             f.write("/\n")
@@ -295,6 +297,22 @@ def make_all(out):
             w = -0.5 * math.sin(2.0 * math.pi * i / 20.0)
             q = 0.1 * math.cos(2.0 * math.pi * i / 30.0)
             f.write(f"{u:.15g} 0.0 {w:.15g} 0.0 {q:.15g} 0.0\n")
+
+    # camber lines (rotor_getCamber, classdef.f90:4060-4112): the AR-4 wing
+    # with a 4% parabolic camber.  One file only: the reference sizes its
+    # camber arrays from this%nCamberFiles before setting it
+    # (classdef.f90:2618 against 2668), so with two or more files its
+    # namelist read aborts.
+    d = case("camber_wing")
+    write_config(os.path.join(d, "config.nml"), nt=60, dt=0.025, nr=1)
+    write_geom(os.path.join(d, "geom01.nml"), nc=4, ns=13, nNwake=200, nCamberFiles=1,
+               camberSectionLimit=[1.0], camberFile=["'camber1.dat'"])
+    for name, m, n in (("camber1.dat", 0.04, 21),):
+        with open(os.path.join(d, name), "w") as f:
+            f.write(f"{n}\n")
+            for k in range(n):
+                xk = 2.0 * k / (n - 1)           # chord of 2 units, normalised on reading
+                f.write(f"{xk:.15g} {4.0 * m * xk * (1.0 - xk / 2.0):.15g}\n")
 
     # far-wake burst every step (main.f90:490-496) with a small skew limit,
     # strain off so that the burst cores persist

@@ -63,6 +63,7 @@ done
 run_ref subiter_rotor_wing subiter_rotor_wing_ForceNonDim.csv
 run_ref burst_rotor burst_rotor_ForceNonDim.csv
 run_ref traj_wing traj_wing_ForceNonDim.csv
+run_ref camber_wing camber_wing_ForceNonDim.csv
 # 5d. wing and wake plots (.tec) of a forward-flight rotor at steps 20 and 40, and
 #     the force files of every force step (ForceNonDim, ForceDim, ForceDist)
 tmp=$(mktemp -d)
@@ -99,6 +100,13 @@ mkdir -p "$tmp/Results" "$tmp/Restart"
 for st in 00020 00040; do
   cp "$tmp/Results/grid$st.tec" "$HERE/grid_rotor_grid$st.tec"
 done
+rm -rf "$tmp"
+# 5g. the cambered wing's blade surface file
+tmp=$(mktemp -d)
+cp "$ROOT/cases/camber_wing.case/"* "$tmp/"
+mkdir -p "$tmp/Results" "$tmp/Restart"
+(cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
+cp "$tmp/Results/r01b01Surface.tec" "$HERE/camber_wing_b01Surface.tec"
 rm -rf "$tmp"
 # 6. velocity probes: the reference's probes2file output of a short wing run
 tmp=$(mktemp -d)

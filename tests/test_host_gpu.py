@@ -395,6 +395,26 @@ def test_custom_trajectory_matches_reference_solver():
     np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
 
 
+def test_camber_line_matches_reference_solver():
+    """A camber file (rotor_getCamber, classdef.f90:4060-4112, with the
+    order-2 interp1d of libMath.f90:520-575): the AR-4 wing with a 4%
+    parabolic camber line.  The cambered grid is checked node by node
+    through the blade surface file, and the CL history against the
+    reference solver's (the camber raises the final CL from 0.33 to 0.57)."""
+    ref = read_force(os.path.join(GOLD, "camber_wing_ForceNonDim.csv"))[:, 0]
+    ct, _ = run("camber_wing")
+    assert len(ct[0]) == len(ref)
+    np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
+
+    def nums(path):
+        return np.array([float(x) for l in open(path) if l.strip() and not l.strip()[0].isalpha()
+                         for x in l.split()])
+
+    np.testing.assert_allclose(
+        nums(os.path.join(CASES, "camber_wing.case", "Results", "r01b01Surface.tec")),
+        nums(os.path.join(GOLD, "camber_wing_b01Surface.tec")), rtol=1e-12, atol=1e-14)
+
+
 def test_wake_burst_matches_reference_solver():
     """wakeBurst = 1 with skewLimit 0.02 (main.f90:490-496, classdef.f90:
     2306-2339): every step, far filaments whose angle with a neighbour

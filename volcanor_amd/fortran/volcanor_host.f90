@@ -105,6 +105,9 @@ module volcanor_host
     integer :: imagePlane = 0, imageRotorNum = 0
     real(dp) :: thetaTwist = 0._dp, initWakeVel = 0._dp
     integer :: inflowPlotSwitch = 0, gammaPlotSwitch = 0, customTrajectorySwitch = 0
+    integer :: nCamberFiles = 0
+    real(dp) :: camberSectionLimit(8) = 0._dp
+    character(len=30) :: camberFile(8) = '0'
     real(dp) :: skewLimit = 0._dp
     real(dp), allocatable :: velBodyHistory(:, :), omegaBodyHistory(:, :)
     real(dp), allocatable :: inflowV(:, :, :)
@@ -251,6 +254,87 @@ contains
     T(3, 2) = sf*ct
     T(3, 3) = cf*ct
   end function euler_gb
+
+  function wing_camber(r, dir, x, y) result(z)
+    !! rotor_getCamber (classdef.f90:4060-4112): z of the wing grid from the
+    !! camber lines (files of <n> then n lines "x z", normalised by the last
+    !! x; '0' is a flat line).  Node column j uses the first file whose
+    !! sectionLimit is >= y(j)/span (else the previous column's file), and
+    !! z = chord * interp1d((x - x(1))/chord) of order 2, with chord and span
+    !! the signed extents of x and y, as the reference computes them.
+    type(rotor_class), intent(in) :: r
+    character(len=*), intent(in) :: dir
+    real(dp), intent(in) :: x(:), y(:)
+    real(dp) :: z(size(x), size(y)), chord, span
+    real(dp), allocatable :: xc(:, :), zc(:, :)
+    integer :: np(r%nCamberFiles), j, i, k, u, f
+    allocate (xc(5000, r%nCamberFiles), zc(5000, r%nCamberFiles))
+    chord = x(size(x)) - x(1)
+    span = y(size(y)) - y(1)
+    do k = 1, r%nCamberFiles
+      if (r%camberFile(k)(1:1) == '0') then
+        np(k) = 2
+        xc(1:2, k) = [0._dp, 1._dp]
+        zc(1:2, k) = 0._dp
+      else
+        open (newunit=u, file=trim(dir)//'/'//trim(r%camberFile(k)), status='old', action='read')
+        read (u, *) np(k)
+        do i = 1, np(k)
+          read (u, *) xc(i, k), zc(i, k)
+        enddo
+        close (u)
+        zc(:, k) = zc(:, k)/xc(np(k), k)
+        xc(:, k) = xc(:, k)/xc(np(k), k)
+      endif
+    enddo
+    f = 1
+    do j = 1, size(y)
+      do k = 1, r%nCamberFiles
+        if (y(j)/span <= r%camberSectionLimit(k)) then
+          f = k
+          exit
+        endif
+      enddo
+      do i = 1, size(x)
+        z(i, j) = chord*lagrange2((x(i) - x(1))/chord, xc(1:np(f), f), zc(1:np(f), f))
+      enddo
+    enddo
+  end function wing_camber
+
+  pure function lagrange2(xq, x, y) result(v)
+    !! interp1d of order 2 (libMath.f90:520-575): the value at the first node
+    !! with x >= xq when xq is within eps of it, 0 when both bracketing
+    !! values vanish, else the quadratic Lagrange polynomial through that
+    !! node, its predecessor and, on the first interval, its successor or,
+    !! after it, the node before the predecessor (linear with two nodes)
+    real(dp), intent(in) :: xq, x(:), y(:)
+    real(dp) :: v, l0, l1, l2
+    integer :: ix, i
+    ix = size(x)
+    do i = 1, size(x)
+      if (xq <= x(i)) then
+        ix = i
+        exit
+      endif
+    enddo
+    if (abs(xq - x(ix)) <= eps) then
+      v = y(ix)
+    elseif (norm2(y(ix - 1:ix)) <= eps) then
+      v = 0._dp
+    elseif (size(x) == 2) then
+      v = y(ix - 1) + (xq - x(ix - 1))*(y(ix) - y(ix - 1))/(x(ix) - x(ix - 1))
+    elseif (ix == 2) then
+      l0 = (xq - x(ix))*(xq - x(ix + 1))/((x(ix - 1) - x(ix))*(x(ix - 1) - x(ix + 1)))
+      l1 = (xq - x(ix - 1))*(xq - x(ix + 1))/((x(ix) - x(ix - 1))*(x(ix) - x(ix + 1)))
+      l2 = (xq - x(ix - 1))*(xq - x(ix))/((x(ix + 1) - x(ix - 1))*(x(ix + 1) - x(ix)))
+      v = y(ix - 1)*l0 + y(ix)*l1 + y(ix + 1)*l2
+    else
+      l0 = (xq - x(ix - 1))*(xq - x(ix))/((x(ix - 2) - x(ix - 1))*(x(ix - 2) - x(ix)))
+      l1 = (xq - x(ix - 2))*(xq - x(ix))/((x(ix - 1) - x(ix - 2))*(x(ix - 1) - x(ix)))
+      l2 = (xq - x(ix - 2))*(xq - x(ix - 1))/((x(ix) - x(ix - 2))*(x(ix) - x(ix - 1)))
+      v = y(ix - 2)*l0 + y(ix - 1)*l1 + y(ix)*l2
+    endif
+  end function lagrange2
 
   pure function slow_omega(sw, iter, Omega) result(w)
     !! Rotor speed of the slow start, main.f90:428-452 (the reference's
@@ -469,7 +553,9 @@ contains
 
     if (surfaceType < 0) error stop 'ERROR: image surfaces are not supported'
     if (surfaceType == 2) error stop 'ERROR: non-lifting surfaces are not supported'
-    if (nCamberFiles > 0) error stop 'ERROR: camber files are not supported'
+    if (nCamberFiles > 8) error stop 'ERROR: at most 8 camber files'
+    r%nCamberFiles = nCamberFiles
+    r%camberSectionLimit = camberSectionLimit; r%camberFile = camberFile
     if (ductSwitch /= 0) error stop 'ERROR: ducts are not supported'
     if (forceCalcSwitch /= 0) error stop 'ERROR: only forceCalcSwitch = 0 is supported'
     if (bladeDynamicsSwitch /= 0 .or. bodyDynamicsSwitch /= 0 .or. pitchDynamicsSwitch /= 0) &
@@ -807,6 +893,7 @@ contains
           grid(:, i, j) = [xVec(i), yVec(j), 0._dp]
         enddo
       enddo
+      if (r%nCamberFiles > 0) grid(3, :, :) = wing_camber(r, dir, xVec, yVec)
     else
       open (newunit=u, file=trim(dir)//'/'//trim(r%geometryFile), status='old', action='read')
       read (u, *) nx, ny, nz
