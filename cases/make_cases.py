@@ -21,6 +21,7 @@ PLOT3D blade file) used by the tests and the benchmark.
   burst_rotor        far-wake burst every step with a small skew limit
   traj_wing          the AR-4 wing on a custom trajectory (trajectory01.in)
   camber_wing        the AR-4 wing with a camber line
+  skew_rotor         near-wake skew plots of a forward-flight rotor
   slowstart*_rotor   the three slow-start ramps of the rotor speed
   subiter_rotor_wing cfg5_rotor_wing_short with sub-iterations (ntSubInit 5, ntSub 3)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
@@ -86,7 +87,7 @@ def write_geom(path, **g):
              wakeTruncateNt=0, prescWakeAfterTruncNt=0, prescWakeGenNt=0, spanwiseCore=0.03,
              streamwiseCoreVec=0.03, rollupStartRadius=0.75, rollupEndRadius=1.0,
              initWakeVel=0.0, psiStart=0.0, skewLimit=0.5, inflowPlotSwitch=0,
-             gammaPlotSwitch=0, camberSectionLimit=[1.0], camberFile=["0"])
+             gammaPlotSwitch=0, skewPlotSwitch=0, camberSectionLimit=[1.0], camberFile=["0"])
     d.update(g)
     groups = [
         ("SURFACE", ["surfaceType", "imagePlane", "imageRotorNum"]),
@@ -121,7 +122,8 @@ def write_geom(path, **g):
         f.write("&WINDFRAME\ndragUnitVec = 0.0, 0.0, 0.0\nsideUnitVec = 0.0, 0.0, 0.0\n"
                 "liftUnitVec = 0.0, 0.0, 0.0\n/\n")
         f.write(f"&PLOTS\ninflowPlotSwitch = {d['inflowPlotSwitch']}\n"
-                f"gammaPlotSwitch = {d['gammaPlotSwitch']}\nskewPlotSwitch = 0\n/\n")
+                f"gammaPlotSwitch = {d['gammaPlotSwitch']}\n"
+                f"skewPlotSwitch = {d['skewPlotSwitch']}\n/\n")
 
 
 def twisted_blade_plot3d(path, nc, ns, R, chord, twist_deg, omega_sign=1.0, r_zero=0.75,
@@ -297,6 +299,15 @@ def make_all(out):
             w = -0.5 * math.sin(2.0 * math.pi * i / 20.0)
             q = 0.1 * math.cos(2.0 * math.pi * i / 30.0)
             f.write(f"{u:.15g} 0.0 {w:.15g} 0.0 {q:.15g} 0.0\n")
+
+    # near-wake skew plots at steps 12 and 24 of a forward-flight rotor
+    d = case("skew_rotor")
+    write_config(os.path.join(d, "config.nml"), nt=24, dt=math.radians(5.0) / om, nr=1,
+                 velSound=340.0, wakeDissipation=1)
+    write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=1, ns=10, nNwake=30, span=R, chord=c,
+               Omega=om, theta0=8.0, pivotLE=0.25, velBody=[-13.6, 0.0, -2.0],
+               apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+               streamwiseCoreVec=ROTOR_CORE, skewPlotSwitch=12)
 
     # camber lines (rotor_getCamber, classdef.f90:4060-4112): the AR-4 wing
     # with a 4% parabolic camber.  One file only: the reference sizes its

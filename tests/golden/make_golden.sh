@@ -108,6 +108,15 @@ mkdir -p "$tmp/Results" "$tmp/Restart"
 (cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
 cp "$tmp/Results/r01b01Surface.tec" "$HERE/camber_wing_b01Surface.tec"
 rm -rf "$tmp"
+# 5h. near-wake skew plots of a forward-flight rotor
+tmp=$(mktemp -d)
+cp "$ROOT/cases/skew_rotor.case/"* "$tmp/"
+mkdir -p "$tmp/Results" "$tmp/Restart"
+(cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
+for f in "$tmp"/Results/*skew*.csv; do
+  cp "$f" "$HERE/skew_rotor_$(basename "$f")"
+done
+rm -rf "$tmp"
 # 6. velocity probes: the reference's probes2file output of a short wing run
 tmp=$(mktemp -d)
 cp "$ROOT/cases/probes_wing.case/"* "$tmp/"

@@ -395,6 +395,21 @@ def test_custom_trajectory_matches_reference_solver():
     np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
 
 
+@pytest.mark.parametrize("f", ["r01b01skew00012.csv", "r01b02skew00012.csv",
+                               "r01b01skew00024.csv", "r01b02skew00024.csv"])
+def test_skew_plots_match_reference_solver(f):
+    """calc_skew and skew2file (classdef.f90:704-747, libPostprocess.f90:
+    786-812) at steps 12 and 24 of a forward-flight rotor: the maximum and
+    the spanwise mean of the bimedian skew of every active near-wake row,
+    from the device wake, against the reference solver's files."""
+    run("skew_rotor")
+    r = read_table(os.path.join(GOLD, "skew_rotor_" + f))
+    g = read_table(os.path.join(CASES, "skew_rotor.case", "Results", f))
+    assert list(g) == list(r) == ["max", "avg"]
+    for k in r:
+        np.testing.assert_allclose(g[k], r[k], rtol=2e-6, atol=1e-9, err_msg=k)
+
+
 def test_camber_line_matches_reference_solver():
     """A camber file (rotor_getCamber, classdef.f90:4060-4112, with the
     order-2 interp1d of libMath.f90:520-575): the AR-4 wing with a 4%

@@ -105,7 +105,7 @@ module volcanor_host
     integer :: imagePlane = 0, imageRotorNum = 0
     real(dp) :: thetaTwist = 0._dp, initWakeVel = 0._dp
     integer :: inflowPlotSwitch = 0, gammaPlotSwitch = 0, customTrajectorySwitch = 0
-    integer :: nCamberFiles = 0
+    integer :: nCamberFiles = 0, skewPlotSwitch = 0
     real(dp) :: camberSectionLimit(8) = 0._dp
     character(len=30) :: camberFile(8) = '0'
     real(dp) :: skewLimit = 0._dp
@@ -590,7 +590,7 @@ contains
     r%thetaTwist = thetaTwist; r%initWakeVel = initWakeVel
     r%inflowPlotSwitch = inflowPlotSwitch; r%gammaPlotSwitch = gammaPlotSwitch
     r%skewLimit = skewLimit
-    if (skewPlotSwitch /= 0) error stop 'ERROR: skew plots are not supported'
+    r%skewPlotSwitch = skewPlotSwitch
     r%flapInitial = flapInitial; r%dflapInitial = dflapInitial; r%Iflap = Iflap
     r%cflap = cflap; r%kflap = kflap; r%MflapConstant = MflapConstant; r%dpitch = dpitch
   end subroutine rotor_readGeom
@@ -821,6 +821,7 @@ contains
     call to_chords_revs(r, r%nNwake, dt)
     call to_chords_revs(r, r%inflowPlotSwitch, dt)
     call to_chords_revs(r, r%gammaPlotSwitch, dt)
+    call to_chords_revs(r, r%skewPlotSwitch, dt)
     if (r%wakeTruncateNt > 0) r%wakeTruncateNt = max(r%wakeTruncateNt, r%nNwake + 1)
     if (r%surfaceType == 0) r%surfaceType = 1
     if (r%nNwake > 0 .and. r%nNwake < 2) error stop 'ERROR: Atleast 2 near wake rows mandatory'
@@ -1202,6 +1203,60 @@ contains
       call rotor_map_gam(rotor(ir))
     enddo
   end subroutine finish_solve
+
+  subroutine skew_file(ctx, dir, timestamp, r)
+    !! calc_skew and skew2file (classdef.f90:704-747 and 4919-4937,
+    !! libPostprocess.f90:786-812): the skew of every active near-wake ring,
+    !! |x1 . x2| / sqrt(|x1|^2 |x2|^2) of its bimedians x1 = p3+p4-p1-p2 and
+    !! x2 = p4+p1-p2-p3 (0 where |Γ| <= eps), on the convected blades and
+    !! copied to the others of an axisymmetric rotor; per blade, the maximum
+    !! and the mean over the span of each row from the oldest to rowNear
+    type(c_ptr), intent(in) :: ctx
+    character(len=*), intent(in) :: dir, timestamp
+    type(rotor_class), intent(in) :: r
+    real(c_double), allocatable, target :: nodes(:, :, :, :), le(:, :, :), gam(:, :, :), &
+      & rc4(:, :, :, :), ffc(:, :, :, :), fgam(:, :), frc(:, :)
+    real(dp) :: skew(r%nNwake, r%ns, r%nb), p(3, 4), x1(3), x2(3)
+    character(len=2) :: bid
+    integer :: ib, irow, icol, u, nF
+    nF = max(r%nFwake, 1)
+    allocate (nodes(3, r%ns + 1, r%nNwake, r%nb), le(3, r%ns + 1, r%nb), &
+      & gam(r%ns, r%nNwake, r%nb), rc4(4, r%ns, r%nNwake, r%nb), ffc(3, 2, nF, r%nb), &
+      & fgam(nF, r%nb), frc(nF, r%nb))
+    call vlm_check(ctx, vlm_rotor_get_wake(ctx, r%dev, VLM_WAKE_C, c_loc(nodes), c_loc(le), &
+      & c_loc(gam), c_loc(rc4), c_loc(ffc), c_loc(fgam), c_loc(frc)), 'get_wake')
+    skew = 0._dp
+    do ib = 1, r%nbConvect
+      do icol = 1, r%ns
+        do irow = r%rowNear, r%nNwake
+          if (abs(gam(icol, irow, ib)) <= eps) cycle
+          if (irow == r%rowNear) then
+            p(:, 1) = le(:, icol, ib); p(:, 4) = le(:, icol + 1, ib)
+          else
+            p(:, 1) = nodes(:, icol, irow - 1, ib); p(:, 4) = nodes(:, icol + 1, irow - 1, ib)
+          endif
+          p(:, 2) = nodes(:, icol, irow, ib); p(:, 3) = nodes(:, icol + 1, irow, ib)
+          x1 = p(:, 3) + p(:, 4) - p(:, 1) - p(:, 2)
+          x2 = p(:, 4) + p(:, 1) - p(:, 2) - p(:, 3)
+          skew(irow, icol, ib) = abs(dot_product(x1, x2)/ &
+            & sqrt(dot_product(x1, x1)*dot_product(x2, x2)))
+        enddo
+      enddo
+    enddo
+    do ib = r%nbConvect + 1, r%nb
+      skew(:, :, ib) = skew(:, :, 1)
+    enddo
+    do ib = 1, r%nb
+      write (bid, '(I0.2)') ib
+      open (newunit=u, file=trim(dir)//'/Results/r'//r%id//'b'//bid//'skew'//timestamp// &
+        & '.csv', status='replace', action='write')
+      write (u, '(2(A15))') 'max', 'avg'
+      do irow = r%nNwake, r%rowNear, -1
+        write (u, '(2(E15.7))') maxval(skew(irow, :, ib)), sum(skew(irow, :, ib))/r%ns
+      enddo
+      close (u)
+    enddo
+  end subroutine skew_file
 
   subroutine filaments_file(ctx, dir, timestamp, rotor, nr)
     !! filaments2file (libPostprocess.f90:363-473), the input of gridgen:
@@ -2187,6 +2242,13 @@ contains
           if (sw%wakeBurst /= 0) then
             if (mod(iter, sw%wakeBurst) == 0) call vlm_check(ctx, vlm_rotor_burst_wake(ctx, &
               & rotor(ir)%dev, rotor(ir)%skewLimit, rotor(ir)%chord), 'burst_wake')
+          endif
+          ! near-wake skew plot (main.f90:497-502)
+          if (write_out .and. rotor(ir)%skewPlotSwitch /= 0) then
+            if (mod(iter, rotor(ir)%skewPlotSwitch) == 0) then
+              write (ts5, '(I0.5)') iter
+              call skew_file(ctx, dir, ts5, rotor(ir))
+            endif
           endif
         enddo
       endif
