@@ -22,6 +22,7 @@ PLOT3D blade file) used by the tests and the benchmark.
   traj_wing          the AR-4 wing on a custom trajectory (trajectory01.in)
   camber_wing        the AR-4 wing with a camber line
   skew_rotor         near-wake skew plots of a forward-flight rotor
+  restart_rotor      restart files of a forward-flight rotor
   slowstart*_rotor   the three slow-start ramps of the rotor speed
   subiter_rotor_wing cfg5_rotor_wing_short with sub-iterations (ntSubInit 5, ntSub 3)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
@@ -124,6 +125,13 @@ def write_geom(path, **g):
         f.write(f"&PLOTS\ninflowPlotSwitch = {d['inflowPlotSwitch']}\n"
                 f"gammaPlotSwitch = {d['gammaPlotSwitch']}\n"
                 f"skewPlotSwitch = {d['skewPlotSwitch']}\n/\n")
+        if d["nAirfoils"] > 0:
+            # section tables named '0' are not read (classdef.f90:3697-3699);
+            # with forceCalcSwitch = 0 alpha0 never enters the loads
+            n = d["nAirfoils"]
+            none = ["'0'"] * n
+            f.write(f"&AIRFOILS\nairfoilSectionLimit = {fmt([1.0] * n)}\n"
+                    f"alpha0 = {fmt([0.0] * n)}\nairfoilFile = {fmt(none)}\n/\n")
 
 
 def twisted_blade_plot3d(path, nc, ns, R, chord, twist_deg, omega_sign=1.0, r_zero=0.75,
@@ -299,6 +307,22 @@ def make_all(out):
             w = -0.5 * math.sin(2.0 * math.pi * i / 20.0)
             q = 0.1 * math.cos(2.0 * math.pi * i / 30.0)
             f.write(f"{u:.15g} 0.0 {w:.15g} 0.0 {q:.15g} 0.0\n")
+
+    # restart files at steps 20 and 40 of a forward-flight rotor (far wake
+    # from step 12); the test resumes a copy from step 20.  The reference's
+    # restart write lists every scheme's arrays and the airfoil alpha0
+    # (classdef.f90:2485-2505), so it succeeds only under AM4, the one
+    # scheme that allocates them all, and with nAirfoils > 0 (one dummy
+    # table, unused by forceCalcSwitch = 0)
+    d = case("restart_rotor")
+    write_config(os.path.join(d, "config.nml"), nt=40, dt=math.radians(5.0) / om, nr=1,
+                 velSound=340.0, fdScheme=5, wakeDissipation=1, wakeStrain=1,
+                 restartWriteNt=20)
+    write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=2, ns=8, nNwake=12, wakeTruncateNt=36,
+               nAirfoils=1,
+               span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25, velBody=[-13.6, 0.0, -2.0],
+               apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+               streamwiseCoreVec=ROTOR_CORE)
 
     # near-wake skew plots at steps 12 and 24 of a forward-flight rotor
     d = case("skew_rotor")

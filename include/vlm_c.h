@@ -200,6 +200,16 @@ int vlm_rotor_set_init_wake_vel(vlm_ctx *ctx, int ir, double initWakeVel,
  * pass 2 (vlm_set_predictor) replaces pass 2 of AM2 only. */
 int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt);
 
+/* Restart snapshot of a rotor's device state: every wing, near-wake,
+ * far-wake and velocity-history array and the wake cursors, as one opaque
+ * byte string of vlm_rotor_state_bytes bytes.  vlm_rotor_set_state takes a
+ * snapshot of the same case (sizes are checked) and resumes from it.  The
+ * reference writes its whole rotor array for this (main.f90:390-398,
+ * 1442-1450); the host writes these bytes beside its own state. */
+int vlm_rotor_state_bytes(vlm_ctx *ctx, int ir, int64_t *bytes);
+int vlm_rotor_get_state(vlm_ctx *ctx, int ir, void *buf, int64_t bytes);
+int vlm_rotor_set_state(vlm_ctx *ctx, int ir, const void *buf, int64_t bytes);
+
 /* State access (tests, restart files, plots). */
 int vlm_rotor_get_wake(vlm_ctx *ctx, int ir, int wakeType, double *nodes,
                        double *le, double *gam, double *rc4, double *far_fc,

@@ -64,6 +64,7 @@ run_ref subiter_rotor_wing subiter_rotor_wing_ForceNonDim.csv
 run_ref burst_rotor burst_rotor_ForceNonDim.csv
 run_ref traj_wing traj_wing_ForceNonDim.csv
 run_ref camber_wing camber_wing_ForceNonDim.csv
+run_ref restart_rotor restart_rotor_ForceNonDim.csv
 # 5d. wing and wake plots (.tec) of a forward-flight rotor at steps 20 and 40, and
 #     the force files of every force step (ForceNonDim, ForceDim, ForceDist)
 tmp=$(mktemp -d)

@@ -410,6 +410,39 @@ def test_skew_plots_match_reference_solver(f):
         np.testing.assert_allclose(g[k], r[k], rtol=2e-6, atol=1e-9, err_msg=k)
 
 
+def test_restart_resumes_bitwise():
+    """restartWriteNt / restartFromNt (main.f90:390-398, 1442-1450): the run
+    writes its state at steps 20 and 40; a copy of the case resumed from
+    step 20 reproduces steps 21-40 of the uninterrupted run bit for bit
+    (host frames, panels and circulation, and the device wake, cursors and
+    velocity histories of vlm_rotor_get_state).  The history also matches
+    the reference solver's uninterrupted run."""
+    import shutil
+    import tempfile
+    from volcanor_amd import host
+    case = os.path.join(CASES, "restart_rotor.case")
+    ref = read_force(os.path.join(GOLD, "restart_rotor_ForceNonDim.csv"))[:, 0]
+    ct, _ = run("restart_rotor")
+    np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
+    assert os.path.exists(os.path.join(case, "Restart", "restart00020.dat"))
+    tmp = tempfile.mkdtemp(prefix="vlmrestart_")
+    try:
+        for f in os.listdir(case):
+            if os.path.isfile(os.path.join(case, f)):
+                shutil.copy(os.path.join(case, f), tmp)
+        os.makedirs(os.path.join(tmp, "Restart"))
+        shutil.copy(os.path.join(case, "Restart", "restart00020.dat"), os.path.join(tmp, "Restart"))
+        cfg = os.path.join(tmp, "config.nml")
+        text = open(cfg).read()
+        assert "restartFromNt = 0" in text
+        open(cfg, "w").write(text.replace("restartFromNt = 0", "restartFromNt = 20"))
+        ct2, _ = host.run_case(tmp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    assert ct2.shape == ct.shape
+    np.testing.assert_array_equal(ct2[0][21:], ct[0][21:])
+
+
 def test_camber_line_matches_reference_solver():
     """A camber file (rotor_getCamber, classdef.f90:4060-4112, with the
     order-2 interp1d of libMath.f90:520-575): the AR-4 wing with a 4%

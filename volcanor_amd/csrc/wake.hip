@@ -1649,6 +1649,79 @@ int vlm_rotor_strain_wake(vlm_ctx *ctx, int ir)
     return VLM_OK;
 }
 
+// Every device array of a rotor's state, in a fixed order (restart files).
+static std::vector<DBuf<double> *> state_bufs(Rotor &r)
+{
+    std::vector<DBuf<double> *> v = {&r.wc,  &r.wrc, &r.wg,  &r.node, &r.pnode, &r.le,
+                                     &r.ple, &r.gam, &r.rc4, &r.age,  &r.ffc,   &r.pffc,
+                                     &r.fgam, &r.frc, &r.frc0, &r.fl0, &r.flc,  &r.fage};
+    for (int k = 0; k < NVEL; ++k) {
+        v.push_back(&r.velN[k]);
+        v.push_back(&r.velF[k]);
+    }
+    return v;
+}
+
+static int64_t state_bytes(Rotor &r)
+{
+    int64_t n = 4 * (int64_t)sizeof(int64_t);
+    for (auto *b : state_bufs(r)) n += (int64_t)(b->n * sizeof(double));
+    return n;
+}
+
+int vlm_rotor_state_bytes(vlm_ctx *ctx, int ir, int64_t *bytes)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r || !bytes) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor or null pointer")
+                                 : VLM_ERR_ARG;
+    *bytes = state_bytes(*r);
+    return VLM_OK;
+}
+
+int vlm_rotor_get_state(vlm_ctx *ctx, int ir, void *buf, int64_t bytes)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r || !buf) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor or null buffer")
+                               : VLM_ERR_ARG;
+    if (bytes != state_bytes(*r)) return fail(ctx, VLM_ERR_ARG, "state size mismatch");
+    VLM_DEVICE(ctx);
+    VLM_TRY(sync_stream(ctx));
+    int64_t *h = static_cast<int64_t *>(buf);
+    h[0] = r->rowNear;
+    h[1] = r->rowFar;
+    h[2] = r->head;
+    h[3] = 0;
+    char *p = static_cast<char *>(buf) + 4 * sizeof(int64_t);
+    for (auto *b : state_bufs(*r)) {
+        if (b->n) VLM_HIP(hipMemcpy(p, b->p, b->n * sizeof(double), hipMemcpyDeviceToHost));
+        p += b->n * sizeof(double);
+    }
+    return VLM_OK;
+}
+
+int vlm_rotor_set_state(vlm_ctx *ctx, int ir, const void *buf, int64_t bytes)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r || !buf) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor or null buffer")
+                               : VLM_ERR_ARG;
+    if (bytes != state_bytes(*r)) return fail(ctx, VLM_ERR_ARG, "state size mismatch");
+    const int64_t *h = static_cast<const int64_t *>(buf);
+    if (h[0] < 1 || h[0] > r->d.nNwake + 1 || h[1] < 1 || h[1] > r->d.nFwake + 1 || h[2] < 0 ||
+        h[2] >= (r->d.nNwake > 0 ? r->d.nNwake : 1))
+        return fail(ctx, VLM_ERR_ARG, "state cursors out of range");
+    VLM_DEVICE(ctx);
+    VLM_TRY(sync_stream(ctx));
+    const char *p = static_cast<const char *>(buf) + 4 * sizeof(int64_t);
+    for (auto *b : state_bufs(*r)) {
+        if (b->n) VLM_HIP(hipMemcpy(b->p, p, b->n * sizeof(double), hipMemcpyHostToDevice));
+        p += b->n * sizeof(double);
+    }
+    r->rowNear = (int)h[0];
+    r->rowFar = (int)h[1];
+    r->head = (int)h[2];
+    return VLM_OK;
+}
+
 int vlm_rotor_burst_wake(vlm_ctx *ctx, int ir, double skewLimit, double largeCoreRadius)
 {
     Rotor *r = get_rotor(ctx, ir);

@@ -160,6 +160,32 @@ module vlm_c
       real(c_double), value :: skewLimit, largeCore
     end function
 
+    integer(c_int) function vlm_rotor_state_bytes(ctx, ir, bytes) &
+        bind(C, name='vlm_rotor_state_bytes')
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir
+      integer(c_int64_t), intent(out) :: bytes
+    end function
+
+    integer(c_int) function vlm_rotor_get_state(ctx, ir, buf, bytes) &
+        bind(C, name='vlm_rotor_get_state')
+      import :: c_int, c_int64_t, c_ptr, c_int8_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir
+      integer(c_int8_t), intent(out) :: buf(*)
+      integer(c_int64_t), value :: bytes
+    end function
+
+    integer(c_int) function vlm_rotor_set_state(ctx, ir, buf, bytes) &
+        bind(C, name='vlm_rotor_set_state')
+      import :: c_int, c_int64_t, c_ptr, c_int8_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir
+      integer(c_int8_t), intent(in) :: buf(*)
+      integer(c_int64_t), value :: bytes
+    end function
+
     integer(c_int) function vlm_rotor_rollup(ctx, ir) bind(C, name='vlm_rotor_rollup')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -1204,6 +1204,78 @@ contains
     enddo
   end subroutine finish_solve
 
+  subroutine restart_io(ctx, u, rotor, nr, t, writing)
+    !! One restart record set (the reference writes t and its whole rotor
+    !! array, main.f90:1442-1450, and reads them back, 390-398): here t,
+    !! every host field a step changes (rigid-motion frames, blade panels
+    !! and sections, circulation, loads) and the device state snapshot of
+    !! each rotor (vlm_rotor_get_state / vlm_rotor_set_state).
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: u, nr
+    type(rotor_class), intent(inout) :: rotor(:)
+    real(dp), intent(inout) :: t
+    logical, intent(in) :: writing
+    integer(c_int8_t), allocatable :: blob(:)
+    integer(c_int64_t) :: nbytes, nfile
+    integer :: ir, ib
+    if (writing) then
+      write (u) t
+    else
+      read (u) t
+    endif
+    do ir = 1, nr
+      associate (r => rotor(ir))
+        if (writing) then
+          write (u) r%rowNear, r%rowFar, r%omegaSlow, r%psi, r%shaftAxis, r%hubCoords, &
+            & r%cgCoords, r%fromCoords, r%pts, r%xAxisBody, r%yAxisBody, r%zAxisBody, &
+            & r%velBody, r%omegaBody, r%controlPitch, r%gamVec, r%gamVecPrev, r%RHS, r%lift, &
+            & r%drag, r%forceInertial, r%liftUnsteady
+        else
+          read (u) r%rowNear, r%rowFar, r%omegaSlow, r%psi, r%shaftAxis, r%hubCoords, &
+            & r%cgCoords, r%fromCoords, r%pts, r%xAxisBody, r%yAxisBody, r%zAxisBody, &
+            & r%velBody, r%omegaBody, r%controlPitch, r%gamVec, r%gamVecPrev, r%RHS, r%lift, &
+            & r%drag, r%forceInertial, r%liftUnsteady
+        endif
+        do ib = 1, r%nb
+          associate (b => r%blade(ib))
+            if (writing) then
+              write (u) b%wiP, b%xAxis, b%yAxis, b%zAxis, b%xAxisAzi, b%yAxisAzi, b%zAxisAzi, &
+                & b%xAxisAziFlap, b%yAxisAziFlap, b%zAxisAziFlap, b%theta, b%psi, b%pivotLE, &
+                & b%flapOrigin, b%lift, b%drag, b%forceInertial, b%liftUnsteady
+              write (u) b%secCP, b%secTauCapChord, b%secTauCapSpan, b%secNormalVec, &
+                & b%secChordwiseResVel, b%secLift, b%secLiftDir, b%secDragDir, &
+                & b%secForceInertial, b%secLiftUnsteady, b%secLiftInPlane, b%secLiftOutPlane, &
+                & b%secChord, b%secArea, b%secCL, b%secCLu, b%secAlpha, b%secMflapArm, &
+                & b%secPhi, b%secTheta, b%secViz, b%secVix
+            else
+              read (u) b%wiP, b%xAxis, b%yAxis, b%zAxis, b%xAxisAzi, b%yAxisAzi, b%zAxisAzi, &
+                & b%xAxisAziFlap, b%yAxisAziFlap, b%zAxisAziFlap, b%theta, b%psi, b%pivotLE, &
+                & b%flapOrigin, b%lift, b%drag, b%forceInertial, b%liftUnsteady
+              read (u) b%secCP, b%secTauCapChord, b%secTauCapSpan, b%secNormalVec, &
+                & b%secChordwiseResVel, b%secLift, b%secLiftDir, b%secDragDir, &
+                & b%secForceInertial, b%secLiftUnsteady, b%secLiftInPlane, b%secLiftOutPlane, &
+                & b%secChord, b%secArea, b%secCL, b%secCLu, b%secAlpha, b%secMflapArm, &
+                & b%secPhi, b%secTheta, b%secViz, b%secVix
+            endif
+          end associate
+        enddo
+        call vlm_check(ctx, vlm_rotor_state_bytes(ctx, r%dev, nbytes), 'state_bytes')
+        allocate (blob(nbytes))
+        if (writing) then
+          call vlm_check(ctx, vlm_rotor_get_state(ctx, r%dev, blob, nbytes), 'get_state')
+          write (u) nbytes
+          write (u) blob
+        else
+          read (u) nfile
+          if (nfile /= nbytes) error stop 'ERROR: restart file does not match this case'
+          read (u) blob
+          call vlm_check(ctx, vlm_rotor_set_state(ctx, r%dev, blob, nbytes), 'set_state')
+        endif
+        deallocate (blob)
+      end associate
+    enddo
+  end subroutine restart_io
+
   subroutine skew_file(ctx, dir, timestamp, r)
     !! calc_skew and skew2file (classdef.f90:704-747 and 4919-4937,
     !! libPostprocess.f90:786-812): the skew of every active near-wake ring,
@@ -2078,7 +2150,10 @@ contains
     if (sw%slowStart > 0 .and. sw%slowStartNt <= 0) error stop 'ERROR: slowStartNt must be positive'
     if (sw%fdScheme < 0 .or. sw%fdScheme > 5) error stop 'ERROR: fdScheme must be 0..5'
     if (sw%ntSub < 0 .or. sw%ntSubInit < 0) error stop 'ERROR: ntSub and ntSubInit must be >= 0'
-    if (sw%restartFromNt /= 0) error stop 'ERROR: restart is not supported'
+    if (sw%restartFromNt < 0 .or. sw%restartWriteNt < 0) &
+      & error stop 'ERROR: restartFromNt and restartWriteNt must be >= 0'
+    if (sw%restartFromNt > 0 .and. predictor /= 0) &
+      & error stop 'ERROR: a restart does not carry the learned predictor; run without it'
     if (nr > 16) error stop 'ERROR: at most 16 rotors'
     ! velocity probes (main.f90:83-93): read only for probe = 1, used at every
     ! step that is a multiple of probe; any other value would leave the
@@ -2201,7 +2276,20 @@ contains
     call system_clock(c1)
     c2 = c1
     ntStart = 1
-    do iter = 1, nt
+    ! resume from a restart file (main.f90:390-398): the state at the end of
+    ! step restartFromNt, read after the initial solution as there
+    if (sw%restartFromNt > 0) then
+      block
+        integer :: ur
+        write (ts5, '(I0.5)') sw%restartFromNt
+        open (newunit=ur, file=trim(dir)//'/Restart/restart'//ts5//'.dat', status='old', &
+          & action='read', form='unformatted')
+        call restart_io(ctx, ur, rotor, nr, t, .false.)
+        close (ur)
+      end block
+      ntStart = sw%restartFromNt + 1
+    endif
+    do iter = ntStart, nt
       if (iter == nt - tail_steps + 1) then
         call vlm_check(ctx, vlm_synchronize(ctx), 'synchronize')
         if (tail_steps > 0) then
@@ -2361,6 +2449,20 @@ contains
           if (rowNear == 1) call vlm_check(ctx, vlm_rotor_rollup(ctx, rotor(ir)%dev), 'rollup')
           call vlm_check(ctx, vlm_rotor_assignshed(ctx, rotor(ir)%dev, VLM_EDGE_TE), 'assignshed TE')
         enddo
+      endif
+      ! restart file at the end of the step (main.f90:1442-1450)
+      if (sw%restartWriteNt /= 0 .and. verbose >= 0) then
+        if (mod(iter, sw%restartWriteNt) == 0) then
+          block
+            integer :: uw
+            ir = int(c_mkdir(trim(dir)//'/Restart'//c_null_char, int(o'755', c_int)))
+            write (ts5, '(I0.5)') iter
+            open (newunit=uw, file=trim(dir)//'/Restart/restart'//ts5//'.dat', &
+              & status='replace', action='write', form='unformatted')
+            call restart_io(ctx, uw, rotor, nr, t, .true.)
+            close (uw)
+          end block
+        endif
       endif
     enddo
     call vlm_check(ctx, vlm_synchronize(ctx), 'synchronize')

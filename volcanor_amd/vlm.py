@@ -65,6 +65,9 @@ _SIGS = {
     "vlm_rotor_dissipate_wake": [_P, _I, _D, _D, _D, _D],
     "vlm_rotor_strain_wake": [_P, _I],
     "vlm_rotor_burst_wake": [_P, _I, _D, _D],
+    "vlm_rotor_state_bytes": [_P, _I, C.POINTER(_L)],
+    "vlm_rotor_get_state": [_P, _I, _P, _L],
+    "vlm_rotor_set_state": [_P, _I, _P, _L],
     "vlm_rotor_rollup": [_P, _I],
     "vlm_rotor_convectwake": [_P, _I, _D, _I],
     "vlm_rotor_induced_velocity": [_P, _I, _I, _L, _P, _P, _I],
@@ -254,6 +257,18 @@ class Context:
 
     def rotor_burst_wake(self, ir, skew_limit, large_core):
         self._chk(self.lib.vlm_rotor_burst_wake(self.h, ir, skew_limit, large_core))
+
+    def rotor_get_state(self, ir):
+        """Opaque snapshot (bytes) of the rotor's device state, for restarts."""
+        n = C.c_int64()
+        self._chk(self.lib.vlm_rotor_state_bytes(self.h, ir, C.byref(n)))
+        buf = np.zeros(n.value, dtype=np.uint8)
+        self._chk(self.lib.vlm_rotor_get_state(self.h, ir, _ptr(buf), n.value))
+        return buf
+
+    def rotor_set_state(self, ir, buf):
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        self._chk(self.lib.vlm_rotor_set_state(self.h, ir, _ptr(buf), buf.size))
 
     def rotor_rollup(self, ir):
         self._chk(self.lib.vlm_rotor_rollup(self.h, ir))
