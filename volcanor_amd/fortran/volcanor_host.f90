@@ -111,6 +111,9 @@ module volcanor_host
     real(dp) :: skewLimit = 0._dp
     real(dp), allocatable :: velBodyHistory(:, :), omegaBodyHistory(:, :)
     real(dp), allocatable :: inflowV(:, :, :)
+    ! azimuthal age of the near-wake rows and far-wake slots, for the tip
+    ! plot: the same in every column, filament and blade, so one value per row
+    real(dp), allocatable :: ageAzN(:), ageAzF(:)
     real(dp) :: flapInitial = 0._dp, dflapInitial = 0._dp, Iflap = 0._dp, cflap = 0._dp
     real(dp) :: kflap = 0._dp, MflapConstant = 0._dp, dpitch = 0._dp
     integer(c_int) :: solve_iters = 0
@@ -831,6 +834,8 @@ contains
     else
       r%nFwake = r%wakeTruncateNt - r%nNwake
     endif
+    allocate (r%ageAzN(max(r%nNwake, 1)), r%ageAzF(max(r%nFwake, 1)))
+    r%ageAzN = 0._dp; r%ageAzF = 0._dp
     ! custom trajectory: body velocity and rate of every step from
     ! trajectoryXX.in, nt lines of six values (classdef.f90:3064-3080)
     if (r%customTrajectorySwitch == 1) then
@@ -1229,12 +1234,12 @@ contains
           write (u) r%rowNear, r%rowFar, r%omegaSlow, r%psi, r%shaftAxis, r%hubCoords, &
             & r%cgCoords, r%fromCoords, r%pts, r%xAxisBody, r%yAxisBody, r%zAxisBody, &
             & r%velBody, r%omegaBody, r%controlPitch, r%gamVec, r%gamVecPrev, r%RHS, r%lift, &
-            & r%drag, r%forceInertial, r%liftUnsteady
+            & r%drag, r%forceInertial, r%liftUnsteady, r%ageAzN, r%ageAzF
         else
           read (u) r%rowNear, r%rowFar, r%omegaSlow, r%psi, r%shaftAxis, r%hubCoords, &
             & r%cgCoords, r%fromCoords, r%pts, r%xAxisBody, r%yAxisBody, r%zAxisBody, &
             & r%velBody, r%omegaBody, r%controlPitch, r%gamVec, r%gamVecPrev, r%RHS, r%lift, &
-            & r%drag, r%forceInertial, r%liftUnsteady
+            & r%drag, r%forceInertial, r%liftUnsteady, r%ageAzN, r%ageAzF
         endif
         do ib = 1, r%nb
           associate (b => r%blade(ib))
@@ -1640,6 +1645,183 @@ contains
       endif
     end function corner
   end subroutine surface_file
+
+  subroutine age_azimuthal(r, dt)
+    !! The azimuthal part of rotor_age_wake (classdef.f90:4343-4352): every
+    !! active near-wake row and far-wake slot ages by dt*omegaSlow.
+    type(rotor_class), intent(inout) :: r
+    real(dp), intent(in) :: dt
+    if (r%nNwake > 0) r%ageAzN(r%rowNear:r%nNwake) = r%ageAzN(r%rowNear:r%nNwake) + dt*r%omegaSlow
+    if (r%nFwake > 0) r%ageAzF(r%rowFar:r%nFwake) = r%ageAzF(r%rowFar:r%nFwake) + dt*r%omegaSlow
+  end subroutine age_azimuthal
+
+  subroutine shift_azimuthal(r, rowFar)
+    !! The azimuthal ages through rollup: a full far wake shifts by one slot
+    !! (shiftFwake, classdef.f90:4575-4580 and 4500-4513), then the near wake
+    !! by one row (shiftwake, 4604 and 4484-4498).  Neither shift resets the
+    !! first slot's azimuthal age (only age), and the rolled-up filament keeps
+    !! its slot's.
+    type(rotor_class), intent(inout) :: r
+    integer(c_int), intent(in) :: rowFar
+    integer :: n
+    n = r%nFwake
+    if (n > 0 .and. rowFar == 1) r%ageAzF(2:n) = r%ageAzF(1:n - 1)
+    n = r%nNwake
+    r%ageAzN(2:n) = r%ageAzN(1:n - 1)
+  end subroutine shift_azimuthal
+
+  subroutine tip_file(ctx, dir, timestamp, r)
+    !! tip2file, libPostprocess.f90:617-784: per blade the wing mesh with -Γ
+    !! and azimuthal age (0 on the wing), the Γ-weighted centroid of corner 4
+    !! over the rollup columns of every active near-wake row (corner 3 for
+    !! the oldest row's far edge) with the row's extreme Γ, and the far-wake
+    !! tip line, as rXXtipNNNNN.tec; the tip points also as rXXtipXYZNNNNN.csv.
+    !! The wake comes from the device (vlm_rotor_get_wake).
+    type(c_ptr), intent(in) :: ctx
+    character(len=*), intent(in) :: dir, timestamp
+    type(rotor_class), intent(in) :: r
+    real(c_double), allocatable, target :: nodes(:, :, :, :), le(:, :, :), gam(:, :, :), &
+      & rc4(:, :, :, :), ffc(:, :, :, :), fgam(:, :), frc(:, :)
+    real(dp) :: wingMesh(3, r%nc + 1, r%ns + 1), nWakeTip(3, r%nNwake + 1)
+    real(dp) :: gamRollup(r%nNwake), fWakeTip(3, max(r%nFwake, 1) + 1), gamSum, c(3)
+    character(len=5) :: nxChar, nyChar
+    integer :: u, u2, ib, i, j, nx, ny, nN, nF
+    logical :: lowest
+
+    nN = r%nNwake; nF = r%nFwake
+    allocate (nodes(3, r%ns + 1, nN, r%nb), le(3, r%ns + 1, r%nb), gam(r%ns, nN, r%nb), &
+      & rc4(4, r%ns, nN, r%nb), ffc(3, 2, max(nF, 1), r%nb), fgam(max(nF, 1), r%nb), &
+      & frc(max(nF, 1), r%nb))
+    call vlm_check(ctx, vlm_rotor_get_wake(ctx, r%dev, VLM_WAKE_C, c_loc(nodes), c_loc(le), &
+      & c_loc(gam), c_loc(rc4), c_loc(ffc), c_loc(fgam), c_loc(frc)), 'get_wake')
+    ! Γ is negative for positive Omega*pitch, so the extreme is the lowest
+    lowest = sign(1._dp, r%Omega*r%controlPitch(1)) > eps
+    open (newunit=u, file=trim(dir)//'/Results/r'//r%id//'tip'//timestamp//'.tec', &
+      & status='replace', action='write')
+    open (newunit=u2, file=trim(dir)//'/Results/r'//r%id//'tipXYZ'//timestamp//'.csv', &
+      & status='replace', action='write')
+    write (u, *) 'Title = "Wing and Tip"'
+    write (u, *) 'VARIABLES = "X" "Y" "Z" "GAM" "ageAzimuthal"'
+    do ib = 1, r%nb
+      nx = r%nc; ny = r%ns
+      write (nxChar, '(I5)') nx + 1
+      write (nyChar, '(I5)') ny + 1
+      do j = 1, ny
+        do i = 1, nx
+          wingMesh(:, i, j) = r%blade(ib)%wiP(i, j)%PC(:, 1)
+        enddo
+      enddo
+      do i = 1, nx
+        wingMesh(:, i, ny + 1) = r%blade(ib)%wiP(i, ny)%PC(:, 4)
+      enddo
+      do j = 1, ny
+        wingMesh(:, nx + 1, j) = r%blade(ib)%wiP(nx, j)%PC(:, 2)
+      enddo
+      wingMesh(:, nx + 1, ny + 1) = r%blade(ib)%wiP(nx, ny)%PC(:, 3)
+      write (u, *) 'Zone I='//trim(nxChar)//' J='//trim(nyChar)//' K=1  T="Blade"'
+      write (u, *) 'DATAPACKING=BLOCK'
+      write (u, *) 'VARLOCATION=([4,5]=CELLCENTERED)'
+      write (u, *) ((wingMesh(1, i, j), i=1, nx + 1), j=1, ny + 1)
+      write (u, *) ((wingMesh(2, i, j), i=1, nx + 1), j=1, ny + 1)
+      write (u, *) ((wingMesh(3, i, j), i=1, nx + 1), j=1, ny + 1)
+      write (u, *) ((-1._dp*r%blade(ib)%wiP(i, j)%vr%gam, i=1, nx), j=1, ny)
+      write (u, *) ((0._dp*radToDeg, i=1, nx), j=1, ny)
+      ! near wake: corner 4 of ring (i, j) is LE node j+1 of row i, the shed
+      ! line on the newest row and row i-1's trailing nodes behind it
+      nx = nN
+      write (nxChar, '(I5)') nx - (r%rowNear - 1) + 1
+      do i = r%rowNear, nx
+        gamRollup(i) = gam(ny, i, ib)
+        gamSum = 0._dp
+        nWakeTip(:, i) = 0._dp
+        do j = r%rollupStart, r%rollupEnd
+          nWakeTip(:, i) = nWakeTip(:, i) + corner4(i, j)*gam(j, i, ib)
+          gamSum = gamSum + gam(j, i, ib)
+          if (lowest) then
+            if (gam(j, i, ib) < gamRollup(i)) gamRollup(i) = gam(j, i, ib)
+          else
+            if (gam(j, i, ib) > gamRollup(i)) gamRollup(i) = gam(j, i, ib)
+          endif
+        enddo
+        if (abs(gamSum) > eps) then
+          nWakeTip(:, i) = nWakeTip(:, i)/gamSum
+        else
+          nWakeTip(:, i) = corner4(i, r%rollupEnd)
+        endif
+      enddo
+      ! the oldest row's corner 3 (trailing node j+1)
+      gamSum = 0._dp
+      nWakeTip(:, nx + 1) = 0._dp
+      do j = r%rollupStart, r%rollupEnd
+        nWakeTip(:, nx + 1) = nWakeTip(:, nx + 1) + nodes(:, j + 1, nx, ib)*gam(j, nx, ib)
+        gamSum = gamSum + gam(j, nx, ib)
+      enddo
+      if (abs(gamSum) > eps) then
+        nWakeTip(:, nx + 1) = nWakeTip(:, nx + 1)/gamSum
+      else
+        nWakeTip(:, nx + 1) = nodes(:, r%rollupEnd + 1, nx, ib)
+      endif
+      write (u, *) 'Zone I='//trim(nxChar)//' J=1    K=1  T="NearWake"'
+      write (u, *) 'DATAPACKING=BLOCK'
+      write (u, *) 'VARLOCATION=([4,5]=CELLCENTERED)'
+      write (u, *) (nWakeTip(1, i), i=r%rowNear, nx + 1)
+      write (u, *) (nWakeTip(2, i), i=r%rowNear, nx + 1)
+      write (u, *) (nWakeTip(3, i), i=r%rowNear, nx + 1)
+      write (u, *) (-1._dp*gamRollup(i), i=r%rowNear, nx)
+      write (u, *) (r%ageAzN(i)*radToDeg, i=r%rowNear, nx)
+      write (u2, *) 'X        Y         Z'
+      do i = r%rowNear, nx + 1
+        write (u2, *) nWakeTip(1, i), nWakeTip(2, i), nWakeTip(3, i)
+      enddo
+      ! far wake: the LE end of every active filament and the TE end of the
+      ! oldest, or the oldest near ring's corner 3 twice when there is none
+      if (r%rowFar <= nF) then
+        nx = nF
+        write (nxChar, '(I5)') nx - (r%rowFar - 1) + 1
+        do i = r%rowFar, nx
+          fWakeTip(:, i) = ffc(:, 2, i, ib)
+        enddo
+        fWakeTip(:, nx + 1) = ffc(:, 1, nF, ib)
+        write (u, *) 'Zone I='//trim(nxChar)//' J=1   K=1   T="FarWake"'
+        write (u, *) 'DATAPACKING=BLOCK'
+        write (u, *) 'VARLOCATION=([4,5]=CELLCENTERED)'
+        write (u, *) (fWakeTip(1, i), i=r%rowFar, nx + 1)
+        write (u, *) (fWakeTip(2, i), i=r%rowFar, nx + 1)
+        write (u, *) (fWakeTip(3, i), i=r%rowFar, nx + 1)
+        write (u, *) (-1._dp*fgam(i, ib), i=r%rowFar, nx)
+        write (u, *) (r%ageAzF(i)*radToDeg, i=r%rowFar, nx)
+        do i = r%rowFar, nx + 1
+          write (u2, *) fWakeTip(1, i), fWakeTip(2, i), fWakeTip(3, i)
+        enddo
+      else
+        c = nodes(:, ny + 1, nN, ib)
+        write (nxChar, '(I5)') 2
+        write (u, *) 'Zone I='//trim(nxChar)//' J=1   K=1   T="FarWake"'
+        write (u, *) 'DATAPACKING=BLOCK'
+        write (u, *) 'VARLOCATION=([4,5]=CELLCENTERED)'
+        write (u, *) c(1), c(1)
+        write (u, *) c(2), c(2)
+        write (u, *) c(3), c(3)
+        write (u, *) 0._dp
+        write (u, *) 0._dp
+      endif
+    enddo
+    close (u)
+    close (u2)
+
+  contains
+
+    function corner4(i, j) result(p)
+      integer, intent(in) :: i, j
+      real(dp) :: p(3)
+      if (i == r%rowNear) then
+        p = le(:, j + 1, ib)
+      else
+        p = nodes(:, j + 1, i - 1, ib)
+      endif
+    end function corner4
+
+  end subroutine tip_file
 
   subroutine geom_file(ctx, dir, timestamp, r, wakeSuppress)
     !! geom2file, libPostprocess.f90:143-331, for lifting surfaces without a
@@ -2323,6 +2505,7 @@ contains
           if (rotor(ir)%nNwake <= 0) cycle
           call vlm_check(ctx, vlm_rotor_assignshed(ctx, rotor(ir)%dev, VLM_EDGE_LE), 'assignshed LE')
           call vlm_check(ctx, vlm_rotor_age_wake(ctx, rotor(ir)%dev, dt), 'age_wake')
+          call age_azimuthal(rotor(ir), dt)
           if (sw%wakeDissipation == 1) call vlm_check(ctx, vlm_rotor_dissipate_wake(ctx, &
             & rotor(ir)%dev, dt, kinematicVisc, rotor(ir)%apparentViscCoeff, rotor(ir)%decayCoeff), &
             & 'dissipate_wake')
@@ -2346,6 +2529,15 @@ contains
           write (ts5, '(I0.5)') iter
           do ir = 1, nr
             call geom_file(ctx, dir, ts5, rotor(ir), sw%wakeSuppress)
+          enddo
+        endif
+      endif
+      ! wake tip plot, main.f90:515-518
+      if (write_out .and. sw%wakeTipPlot /= 0 .and. sw%wakeSuppress == 0) then
+        if (mod(iter, sw%wakeTipPlot) == 0) then
+          write (ts5, '(I0.5)') iter
+          do ir = 1, nr
+            if (rotor(ir)%nNwake > 0) call tip_file(ctx, dir, ts5, rotor(ir))
           enddo
         endif
       endif
@@ -2446,7 +2638,10 @@ contains
         do ir = 1, nr
           if (rotor(ir)%nNwake <= 0) cycle
           call vlm_check(ctx, vlm_rotor_cursors(ctx, rotor(ir)%dev, rowNear, rowFar), 'cursors')
-          if (rowNear == 1) call vlm_check(ctx, vlm_rotor_rollup(ctx, rotor(ir)%dev), 'rollup')
+          if (rowNear == 1) then
+            call vlm_check(ctx, vlm_rotor_rollup(ctx, rotor(ir)%dev), 'rollup')
+            call shift_azimuthal(rotor(ir), rowFar)
+          endif
           call vlm_check(ctx, vlm_rotor_assignshed(ctx, rotor(ir)%dev, VLM_EDGE_TE), 'assignshed TE')
         enddo
       endif
