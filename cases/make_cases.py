@@ -23,6 +23,7 @@ PLOT3D blade file) used by the tests and the benchmark.
   camber_wing        the AR-4 wing with a camber line
   skew_rotor         near-wake skew plots of a forward-flight rotor
   restart_rotor      restart files of a forward-flight rotor
+  restart_rotor_wing restart files of the rotor above the wing (AM2, sub-iterations)
   tip_rotor          wake tip plots of a forward-flight rotor in slow start
   slowstart*_rotor   the three slow-start ramps of the rotor speed
   subiter_rotor_wing cfg5_rotor_wing_short with sub-iterations (ntSubInit 5, ntSub 3)
@@ -438,6 +439,22 @@ def make_all(out):
                    fromCoords=[0.0, 1.0, 0.0], velBody=[-13.6, 0.0, 0.0],
                    apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
                    streamwiseCoreVec=ROTOR_CORE)
+
+    # restart files of the rotor above the wing under AM2 with sub-iterations,
+    # both surfaces rolling up into a truncated far wake; written at step 18
+    # (the reference's restart write aborts for AM2, see restart_rotor)
+    d = case("restart_rotor_wing")
+    write_config(os.path.join(d, "config.nml"), nt=36, dt=dt5, nr=2, velSound=340.0,
+                 ntSubInit=5, ntSub=3, wakeDissipation=1, wakeStrain=1, restartWriteNt=18)
+    write_geom(os.path.join(d, "geom01.nml"), nb=3, nc=1, ns=30, nNwake=12, wakeTruncateNt=30,
+               span=R, chord=c, Omega=om, theta0=10.0, pivotLE=0.25, velBody=[-13.6, 0.0, 0.0],
+               apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+               streamwiseCoreVec=ROTOR_CORE)
+    write_geom(os.path.join(d, "geom02.nml"), nb=1, nc=4, ns=30, nNwake=12, wakeTruncateNt=30,
+               span=2.0, chord=0.25, Omega=0.0, theta0=3.0, hubCoords=[0.0, 0.0, -0.5],
+               fromCoords=[0.0, 1.0, 0.0], velBody=[-13.6, 0.0, 0.0],
+               apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+               streamwiseCoreVec=ROTOR_CORE)
 
 
 if __name__ == "__main__":

@@ -65,6 +65,17 @@ run_ref burst_rotor burst_rotor_ForceNonDim.csv
 run_ref traj_wing traj_wing_ForceNonDim.csv
 run_ref camber_wing camber_wing_ForceNonDim.csv
 run_ref restart_rotor restart_rotor_ForceNonDim.csv
+# the rotor above the wing under AM2, with the restart write switched off:
+# the reference's restart write aborts for every scheme but AM4
+tmp=$(mktemp -d)
+cp "$ROOT/cases/restart_rotor_wing.case/"* "$tmp/"
+sed -i 's/restartWriteNt = 18/restartWriteNt = 0/' "$tmp/config.nml"
+mkdir -p "$tmp/Results" "$tmp/Restart"
+(cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 OMP_SCHEDULE=static \
+   "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
+cp "$tmp/Results/r01ForceNonDim.csv" "$HERE/restart_rotor_wing_ForceNonDim.csv"
+cp "$tmp/Results/r02ForceNonDim.csv" "$HERE/restart_rotor_wing_r02ForceNonDim.csv"
+rm -rf "$tmp"
 # 5d. wing and wake plots (.tec) of a forward-flight rotor at steps 20 and 40, and
 #     the force files of every force step (ForceNonDim, ForceDim, ForceDist)
 tmp=$(mktemp -d)

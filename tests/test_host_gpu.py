@@ -449,6 +449,44 @@ def test_skew_plots_match_reference_solver(f):
         np.testing.assert_allclose(g[k], r[k], rtol=2e-6, atol=1e-9, err_msg=k)
 
 
+def resume(name, step, nrotors=1):
+    """Run a copy of the case resumed from its restart file of `step`."""
+    import shutil
+    import tempfile
+    from volcanor_amd import host
+    case = os.path.join(CASES, name + ".case")
+    tmp = tempfile.mkdtemp(prefix="vlmrestart_")
+    try:
+        for f in os.listdir(case):
+            if os.path.isfile(os.path.join(case, f)):
+                shutil.copy(os.path.join(case, f), tmp)
+        os.makedirs(os.path.join(tmp, "Restart"))
+        shutil.copy(os.path.join(case, "Restart", f"restart{step:05d}.dat"),
+                    os.path.join(tmp, "Restart"))
+        cfg = os.path.join(tmp, "config.nml")
+        text = open(cfg).read()
+        assert "restartFromNt = 0" in text
+        open(cfg, "w").write(text.replace("restartFromNt = 0", f"restartFromNt = {step}"))
+        ct, _ = host.run_case(tmp, nrotors=nrotors)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return ct
+
+
+def test_restart_two_surfaces_resume_bitwise():
+    """Restart of the rotor above the wing under AM2 with sub-iterations,
+    both surfaces with a truncated far wake: resumed from step 18, both
+    histories of steps 19-36 equal the uninterrupted run's bit for bit.  The
+    uninterrupted run matches the reference solver's run of the case with
+    the restart write switched off (its write aborts under AM2)."""
+    ct, _ = run("restart_rotor_wing", nrotors=2)
+    np.testing.assert_allclose(ct, reference_histories("restart_rotor_wing", 2), rtol=2e-6,
+                               atol=1e-9)
+    ct2 = resume("restart_rotor_wing", 18, nrotors=2)
+    assert ct2.shape == ct.shape
+    np.testing.assert_array_equal(ct2[:, 19:], ct[:, 19:])
+
+
 def test_restart_resumes_bitwise():
     """restartWriteNt / restartFromNt (main.f90:390-398, 1442-1450): the run
     writes its state at steps 20 and 40; a copy of the case resumed from
@@ -456,28 +494,12 @@ def test_restart_resumes_bitwise():
     (host frames, panels and circulation, and the device wake, cursors and
     velocity histories of vlm_rotor_get_state).  The history also matches
     the reference solver's uninterrupted run."""
-    import shutil
-    import tempfile
-    from volcanor_amd import host
     case = os.path.join(CASES, "restart_rotor.case")
     ref = read_force(os.path.join(GOLD, "restart_rotor_ForceNonDim.csv"))[:, 0]
     ct, _ = run("restart_rotor")
     np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
     assert os.path.exists(os.path.join(case, "Restart", "restart00020.dat"))
-    tmp = tempfile.mkdtemp(prefix="vlmrestart_")
-    try:
-        for f in os.listdir(case):
-            if os.path.isfile(os.path.join(case, f)):
-                shutil.copy(os.path.join(case, f), tmp)
-        os.makedirs(os.path.join(tmp, "Restart"))
-        shutil.copy(os.path.join(case, "Restart", "restart00020.dat"), os.path.join(tmp, "Restart"))
-        cfg = os.path.join(tmp, "config.nml")
-        text = open(cfg).read()
-        assert "restartFromNt = 0" in text
-        open(cfg, "w").write(text.replace("restartFromNt = 0", "restartFromNt = 20"))
-        ct2, _ = host.run_case(tmp)
-    finally:
-        shutil.rmtree(tmp, ignore_errors=True)
+    ct2 = resume("restart_rotor", 20)
     assert ct2.shape == ct.shape
     np.testing.assert_array_equal(ct2[0][21:], ct[0][21:])
 
