@@ -2419,7 +2419,11 @@ contains
     enddo
     nforce = 0
     if (write_out) then
-      ir = int(c_mkdir(trim(dir)//'/Results'//c_null_char, int(o'755', c_int)))
+      block
+        integer(c_int) :: mk
+        ! an existing directory is fine; the opens below report any failure
+        mk = c_mkdir(trim(dir)//'/Results'//c_null_char, int(o'755', c_int))
+      end block
       do ir = 1, nr
         write (rc2, '(I0.2)') ir
         call surface_file(dir, rotor(ir))
@@ -2650,7 +2654,9 @@ contains
         if (mod(iter, sw%restartWriteNt) == 0) then
           block
             integer :: uw
-            ir = int(c_mkdir(trim(dir)//'/Restart'//c_null_char, int(o'755', c_int)))
+            integer(c_int) :: mk
+            ! an existing directory is fine; the open below reports any failure
+            mk = c_mkdir(trim(dir)//'/Restart'//c_null_char, int(o'755', c_int))
             write (ts5, '(I0.5)') iter
             open (newunit=uw, file=trim(dir)//'/Restart/restart'//ts5//'.dat', &
               & status='replace', action='write', form='unformatted')
