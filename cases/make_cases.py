@@ -24,7 +24,8 @@ PLOT3D blade file) used by the tests and the benchmark.
   skew_rotor         near-wake skew plots of a forward-flight rotor
   restart_rotor      restart files of a forward-flight rotor
   restart_rotor_wing restart files of the rotor above the wing (AM2, sub-iterations)
-  tip_rotor          wake tip plots of a forward-flight rotor in slow start
+  tip_rotor(_cw)     wake tip plots of a forward-flight rotor in slow start, turning
+                     anticlockwise and clockwise
   slowstart*_rotor   the three slow-start ramps of the rotor speed
   subiter_rotor_wing cfg5_rotor_wing_short with sub-iterations (ntSubInit 5, ntSub 3)
   katz_plotkin_AR4   the reference's AR-4 wing test case (4 x 26, 160 steps)
@@ -313,14 +314,18 @@ def make_all(out):
     # wake tip plots every 12 steps of a forward-flight rotor in slow start
     # (omegaSlow ramps over 20 steps): far wake from step 12, full at step
     # 30 (wakeTruncateNt 30), then shifted on every rollup
-    d = case("tip_rotor")
-    write_config(os.path.join(d, "config.nml"), nt=48, dt=math.radians(5.0) / om, nr=1,
-                 velSound=340.0, wakeDissipation=1, wakeStrain=1, wakeTipPlot=12,
-                 slowStart=1, slowStartNt=20)
-    write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=2, ns=8, nNwake=12, wakeTruncateNt=30,
-               span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25, velBody=[-13.6, 0.0, -2.0],
-               apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
-               streamwiseCoreVec=ROTOR_CORE)
+    # tip_rotor_cw: the same rotor turning clockwise (negative Omega): Γ
+    # changes sign, so the tip plot takes each row's highest Γ, and the
+    # azimuthal ages are negative
+    for name, sgn in (("tip_rotor", 1.0), ("tip_rotor_cw", -1.0)):
+        d = case(name)
+        write_config(os.path.join(d, "config.nml"), nt=48, dt=math.radians(5.0) / om, nr=1,
+                     velSound=340.0, wakeDissipation=1, wakeStrain=1, wakeTipPlot=12,
+                     slowStart=1, slowStartNt=20)
+        write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=2, ns=8, nNwake=12,
+                   wakeTruncateNt=30, span=R, chord=c, Omega=sgn * om, theta0=8.0,
+                   pivotLE=0.25, velBody=[-13.6, 0.0, -2.0], apparentViscCoeff=ROTOR_AVC,
+                   spanwiseCore=ROTOR_CORE, streamwiseCoreVec=ROTOR_CORE)
 
     # restart files at steps 20 and 40 of a forward-flight rotor (far wake
     # from step 12); the test resumes a copy from step 20.  The reference's

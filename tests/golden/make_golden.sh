@@ -129,16 +129,18 @@ for f in "$tmp"/Results/*skew*.csv; do
   cp "$f" "$HERE/skew_rotor_$(basename "$f")"
 done
 rm -rf "$tmp"
-# 5h. wake tip plots of a forward-flight rotor in slow start
-tmp=$(mktemp -d)
-cp "$ROOT/cases/tip_rotor.case/"* "$tmp/"
-mkdir -p "$tmp/Results" "$tmp/Restart"
-(cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
-cp "$tmp/Results/r01ForceNonDim.csv" "$HERE/tip_rotor_ForceNonDim.csv"
-for f in "$tmp"/Results/r01tip*; do
-  cp "$f" "$HERE/tip_rotor_$(basename "$f")"
+# 5h. wake tip plots of a forward-flight rotor in slow start, both directions
+for c in tip_rotor tip_rotor_cw; do
+  tmp=$(mktemp -d)
+  cp "$ROOT/cases/$c.case/"* "$tmp/"
+  mkdir -p "$tmp/Results" "$tmp/Restart"
+  (cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
+  cp "$tmp/Results/r01ForceNonDim.csv" "$HERE/${c}_ForceNonDim.csv"
+  for f in "$tmp"/Results/r01tip*; do
+    cp "$f" "$HERE/${c}_$(basename "$f")"
+  done
+  rm -rf "$tmp"
 done
-rm -rf "$tmp"
 # 6. velocity probes: the reference's probes2file output of a short wing run
 tmp=$(mktemp -d)
 cp "$ROOT/cases/probes_wing.case/"* "$tmp/"

@@ -257,7 +257,8 @@ def test_wake_plot_matches_reference_solver(step):
             np.testing.assert_array_equal(zg[7], zr[7])
 
 
-def test_wake_tip_plots_match_reference_solver():
+@pytest.mark.parametrize("name", ["tip_rotor", "tip_rotor_cw"])
+def test_wake_tip_plots_match_reference_solver(name):
     """tip2file (libPostprocess.f90:617-784) every 12 steps of a
     forward-flight rotor in slow start whose far wake forms at step 12, fills
     at step 30 and then shifts on every rollup.  Per blade: the wing mesh and
@@ -265,14 +266,16 @@ def test_wake_tip_plots_match_reference_solver():
     row's extreme Γ, and the far-wake tip line, each with its azimuthal age
     in degrees (grown by dt*omegaSlow on active slots, carried through both
     shifts and never reset), compared zone by zone, and the tipXYZ point
-    lists.  The CL/CT history matches as well."""
-    ref_ct = read_force(os.path.join(GOLD, "tip_rotor_ForceNonDim.csv"))[:, 0]
-    ct, _ = run("tip_rotor")
+    lists.  The CL/CT history matches as well.  tip_rotor_cw turns the other
+    way: Γ changes sign, each row's highest Γ is taken, and the ages are
+    negative."""
+    ref_ct = read_force(os.path.join(GOLD, f"{name}_ForceNonDim.csv"))[:, 0]
+    ct, _ = run(name)
     np.testing.assert_allclose(ct[0], ref_ct, rtol=2e-6, atol=1e-9)
-    res = os.path.join(CASES, "tip_rotor.case", "Results")
+    res = os.path.join(CASES, f"{name}.case", "Results")
     ages = []
     for step in ("00012", "00024", "00036", "00048"):
-        ref = read_tec(os.path.join(GOLD, f"tip_rotor_r01tip{step}.tec"))
+        ref = read_tec(os.path.join(GOLD, f"{name}_r01tip{step}.tec"))
         got = read_tec(os.path.join(res, f"r01tip{step}.tec"))
         assert [z[:3] for z in got] == [z[:3] for z in ref]
         for zg, zr in zip(got, ref):
@@ -287,13 +290,14 @@ def test_wake_tip_plots_match_reference_solver():
             return sum(r[0] == "X" for r in rows), \
                 np.array([[float(x) for x in r] for r in rows if r[0] != "X"])
         hg, pg = xyz(os.path.join(res, f"r01tipXYZ{step}.csv"))
-        hr, pr = xyz(os.path.join(GOLD, f"tip_rotor_r01tipXYZ{step}.csv"))
+        hr, pr = xyz(os.path.join(GOLD, f"{name}_r01tipXYZ{step}.csv"))
         assert hg == hr and pg.shape == pr.shape
         np.testing.assert_allclose(pg, pr, rtol=1e-10, atol=1e-12, err_msg=step)
     # the far-wake ages vary along the line at step 36 (blade 1's zone);
     # by step 48 the shifts have copied slot 1's age into every slot
+    sgn = -1.0 if name.endswith("_cw") else 1.0
     assert len(ages[4]) == 18 and np.ptp(ages[4]) > 0
-    assert np.ptp(ages[6]) == 0 and ages[6][0] > ages[4][-1]
+    assert np.ptp(ages[6]) == 0 and sgn * ages[6][0] > sgn * ages[4][-1] > 0
 
 
 def test_grid_velocity_field_matches_reference_gridgen():
