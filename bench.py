@@ -86,15 +86,35 @@ def measured_traffic(key):
         return None
 
 
+_FMA_STREAM = []
+
+
+def fma_stream():
+    """Sustained FP64 FMA rate of this device (TFLOP/s) and the relative time
+    of a v_rsq_f64 stream, measured once per process after the timed work."""
+    if not _FMA_STREAM:
+        from volcanor_amd import vlm
+        _FMA_STREAM.extend(vlm.Context(0).fp64_issue_rate())
+    return _FMA_STREAM
+
+
 def roofline(kernel_ms, launches, pairs, peak, kernel, traffic_key=None):
     avg_ms = kernel_ms / max(launches, 1)
     flops_launch = F_PAIR * pairs / max(launches, 1)
     achieved = flops_launch / (avg_ms * 1e-3) if avg_ms > 0 else 0.0
-    return {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
-            "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": measured_traffic(traffic_key) if traffic_key else None,
-            "kernel": kernel, "kernel_avg_ms": avg_ms, "launches": int(launches),
-            "flop_per_pair": F_PAIR, "pairs_per_launch": pairs / max(launches, 1)}
+    out = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
+           "unit": "TFLOP/s", "frac": achieved / peak,
+           "traffic": measured_traffic(traffic_key) if traffic_key else None,
+           "kernel": kernel, "kernel_avg_ms": avg_ms, "launches": int(launches),
+           "flop_per_pair": F_PAIR, "pairs_per_launch": pairs / max(launches, 1)}
+    if peak == PEAK_FP64_VALU:
+        # the published peak assumes every slot an FMA at the peak clock; a
+        # plain FMA stream shows what this device sustains
+        tf, rsq = fma_stream()
+        out["fma_stream_tflops"] = tf
+        out["frac_of_fma_stream"] = achieved / 1e12 / tf
+        out["rsq_f64_time_over_fma"] = rsq
+    return out
 
 
 def bench_microbench(args):

@@ -310,6 +310,12 @@ int vlm_profile_get(vlm_ctx *ctx, double *bs_ms, int64_t *bs_launches,
  * their nominal flops (2 n^3 / 3 each). */
 int vlm_profile_get_lu(vlm_ctx *ctx, double *ms, int64_t *factorisations,
                        double *flops);
+/* Calibration of the roofline (no counterpart in the reference): the
+ * sustained FP64 rate of this device on a stream of independent FMAs
+ * (8 chains per lane, 16 workgroups of 256 per CU, after one untimed round),
+ * in TFLOP/s, and the time of the same stream of v_rsq_f64 relative to it.
+ * Allocates a scratch buffer of 8 bytes per launched lane for the call. */
+int vlm_fp64_issue_rate(vlm_ctx *ctx, double *fma_tflops, double *rsq_over_fma);
 
 #ifdef __cplusplus
 }

@@ -157,3 +157,12 @@ def test_resident_sets_and_profile():
     v = ctx.pset_velocity(ps, m)
     w = orc.filament_sum(a, b, g, rc, p, threads=8)
     np.testing.assert_allclose(v, w, rtol=1e-10, atol=1e-12 * np.max(np.abs(w)))
+
+
+def test_fp64_issue_rate_calibration():
+    """The FMA-stream calibration behind the bench's frac_of_fma_stream: a
+    sustained FP64 rate below the published 78.6 TFLOP/s and well above a
+    quarter of it, and a v_rsq_f64 stream slower than the FMA stream."""
+    tf, rsq = _ctx().fp64_issue_rate()
+    assert 20.0 < tf <= 80.0, tf
+    assert 1.0 < rsq < 16.0, rsq

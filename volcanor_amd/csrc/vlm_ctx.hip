@@ -525,3 +525,77 @@ int vlm_profile_get_lu(vlm_ctx *ctx, double *ms, int64_t *factorisations, double
 }
 
 }  // extern "C"
+
+// Calibration streams for vlm_fp64_issue_rate: every lane runs 8 independent
+// chains of CALIB_ITERS dependent operations (FMA, or v_rsq_f64) and stores
+// one value at its global index, so the output holds exactly one double per
+// launched lane.
+namespace {
+constexpr int CALIB_ITERS = 4096;
+constexpr int CALIB_CHAINS = 8;
+
+__global__ void __launch_bounds__(256) calib_fma_kernel(double *out, double s)
+{
+    double a[CALIB_CHAINS];
+    for (int c = 0; c < CALIB_CHAINS; ++c) a[c] = s + c + threadIdx.x * 1e-9;
+    for (int i = 0; i < CALIB_ITERS; ++i)
+#pragma unroll
+        for (int c = 0; c < CALIB_CHAINS; ++c) a[c] = __builtin_fma(a[c], 0.999999, 1e-7);
+    double t = 0.0;
+    for (int c = 0; c < CALIB_CHAINS; ++c) t += a[c];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(256) calib_rsq_kernel(double *out, double s)
+{
+    double a[CALIB_CHAINS];
+    for (int c = 0; c < CALIB_CHAINS; ++c) a[c] = s + c + threadIdx.x * 1e-9;
+    for (int i = 0; i < CALIB_ITERS; ++i)
+#pragma unroll
+        for (int c = 0; c < CALIB_CHAINS; ++c) a[c] = __builtin_amdgcn_rsq(a[c]);
+    double t = 0.0;
+    for (int c = 0; c < CALIB_CHAINS; ++c) t += a[c];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = t;
+}
+}  // namespace
+
+extern "C" {
+
+int vlm_fp64_issue_rate(vlm_ctx *ctx, double *fma_tflops, double *rsq_over_fma)
+{
+    if (!ctx || !fma_tflops || !rsq_over_fma) return VLM_ERR_ARG;
+    VLM_DEVICE(ctx);
+    VLM_TRY(sync_stream(ctx));
+    int cus = 0;
+    VLM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const int threads = 256, blocks = cus * 16;
+    double *out = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    float ms_f = 0.f, ms_r = 0.f;
+    hipError_t err = hipMalloc(&out, sizeof(double) * (size_t)blocks * threads);
+    for (int i = 0; i < 3 && err == hipSuccess; ++i) err = hipEventCreate(&ev[i]);
+    // the first round is untimed (clocks ramp up), the second is timed
+    for (int round = 0; round < 2 && err == hipSuccess; ++round) {
+        err = hipEventRecord(ev[0], ctx->stream);
+        if (err != hipSuccess) break;
+        hipLaunchKernelGGL(calib_fma_kernel, dim3(blocks), dim3(threads), 0, ctx->stream, out, 1.5);
+        err = hipEventRecord(ev[1], ctx->stream);
+        if (err != hipSuccess) break;
+        hipLaunchKernelGGL(calib_rsq_kernel, dim3(blocks), dim3(threads), 0, ctx->stream, out, 1.5);
+        err = hipEventRecord(ev[2], ctx->stream);
+        if (err == hipSuccess) err = hipGetLastError();
+        if (err == hipSuccess) err = hipEventSynchronize(ev[2]);
+    }
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms_f, ev[0], ev[1]);
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms_r, ev[1], ev[2]);
+    for (int i = 0; i < 3; ++i)
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+    if (out) (void)hipFree(out);
+    if (err != hipSuccess) return vlm::hip_fail(ctx, err, "vlm_fp64_issue_rate");
+    const double fmas = 64.0 * (double)blocks * (threads / 64) * CALIB_ITERS * CALIB_CHAINS;
+    *fma_tflops = 2.0 * fmas / (ms_f * 1e-3) / 1e12;
+    *rsq_over_fma = (double)ms_r / (double)ms_f;
+    return VLM_OK;
+}
+
+}  // extern "C"

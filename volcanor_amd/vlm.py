@@ -98,6 +98,7 @@ _SIGS = {
     "vlm_profile_reset": [_P],
     "vlm_profile_get": [_P, C.POINTER(_D), C.POINTER(_L), C.POINTER(_D)],
     "vlm_profile_get_lu": [_P, C.POINTER(_D), C.POINTER(_L), C.POINTER(_D)],
+    "vlm_fp64_issue_rate": [_P, C.POINTER(_D), C.POINTER(_D)],
 }
 
 _lib = None
@@ -457,6 +458,13 @@ class Context:
         ms, n, pairs = C.c_double(), C.c_int64(), C.c_double()
         self._chk(self.lib.vlm_profile_get(self.h, C.byref(ms), C.byref(n), C.byref(pairs)))
         return ms.value, n.value, pairs.value
+
+    def fp64_issue_rate(self):
+        """(TFLOP/s of a stream of independent FP64 FMAs on this device,
+        time of the same stream of v_rsq_f64 relative to it)."""
+        t, r = C.c_double(), C.c_double()
+        self._chk(self.lib.vlm_fp64_issue_rate(self.h, C.byref(t), C.byref(r)))
+        return t.value, r.value
 
     def profile_get_lu(self):
         """(device ms, factorisations, nominal flops) of LU factorisations."""
