@@ -2,19 +2,22 @@
 """Benchmark of the MI355X free-wake hot path.  Prints one JSON line.
 
 Workloads
-  rotor2 (default)  two-bladed hover rotor, 1 x 20 panels per blade, 5 deg
-                    azimuth steps, 10 revolutions (720 free-wake steps,
-                    cases/cfg2_rotor.case) run by the Fortran host on the GPU.
-                    A step is one free-wake timestep (RHS + circulation solve
-                    + loads + AM2 predictor-corrector convection with both
-                    Biot-Savart passes).  W warm-up steps run untimed, the
-                    last K steps are timed (default: the 10th revolution).
+  quad (default)    quad-rotor (cases/cfg4_quad.case): 4 three-bladed rotors x
+                    1 x 30 panels, alternating rotation, 10 deg steps, AM2
+                    free wake, 15-revolution truncated lattice, run by the
+                    Fortran host on the GPU.  A step is one free-wake timestep
+                    (rigid motion, shed, RHS + circulation solve + loads, AM2
+                    predictor-corrector convection with both Biot-Savart
+                    passes, rollup).  The run always covers all 370 steps and
+                    times the last K (default 20: steps 351-370, the 10th
+                    revolution); the steps before build the wake that defines
+                    the workload, so --warmup does not shorten the case.
                     Variants: plain FP64 (the value), mixed FP32/FP64, mixed
                     with the learned pass 2, and an FP64 twin with a 1e-9 deg
                     cyclic pitch (the chaos floor of CT).
-  quad              quad-rotor (cases/cfg4_quad.case), 4 three-bladed rotors
-                    x 30 panels, 10 deg steps; steps 351-370 (10 revolutions)
-                    timed; same variants.
+  rotor2            two-bladed hover rotor (cases/cfg2_rotor.case), 1 x 20
+                    panels per blade, 5 deg steps, 720 steps (10
+                    revolutions), the last revolution timed; same variants.
   rotorwing         three-bladed rotor in forward flight above an AR-8 wing
                     (cases/cfg5_rotor_wing.case), 1080 steps (15
                     revolutions), the last revolution timed; same variants.
@@ -23,7 +26,7 @@ Workloads
   lu                dense FP64 LU with partial pivoting, N = 2176.
   hbmcopy           1 GiB device copy (HBM counter calibration).
 
-  python bench.py [--workload rotor2|quad|rotorwing|microbench|lu|hbmcopy] [--steps K]
+  python bench.py [--workload quad|rotor2|rotorwing|microbench|lu|hbmcopy] [--steps K]
                   [--warmup W]
 """
 import argparse
@@ -229,19 +232,35 @@ WORKLOADS = {
                   "0.5 R below the hub, 5 deg steps, AM2 free wakes, 15 revolutions; the last "
                   "revolution timed"),
 }
+# steps of the case the reference solver runs for the CPU baseline (about
+# 20 s on 16 host threads)
+CPU_STEPS = {"rotor2": 100, "quad": 26, "rotorwing": 0}
 # The reference solver turns NaN from the first step on cases whose coupled
 # surfaces have different spanwise panel counts (cfg5: 30 and 40), so no
 # reference run is timed for them.
 NO_REFERENCE = {"rotorwing"}
 
 
+def host_cores():
+    """Physical cores of this host (lscpu: sockets x cores per socket)."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = dict(l.split(":", 1) for l in out.splitlines() if ":" in l)
+        return int(kv["Socket(s)"].strip()) * int(kv["Core(s) per socket"].strip())
+    except (OSError, KeyError, ValueError, subprocess.SubprocessError):
+        return None
+
+
 def bench_rotor(args):
     from volcanor_amd import host
     case_name, nt_case, K_def, steps_rev, nrot, desc = WORKLOADS[args.workload]
     case = os.path.join(ROOT, "cases", case_name)
-    K = args.steps if args.steps else K_def
-    W = args.warmup if args.warmup is not None else nt_case - K
-    nt = W + K
+    # The workload is the case's own step range: the run always covers all
+    # nt_case steps and times the last K of them (the steps before build the
+    # wake whose size defines the workload), so --warmup cannot shorten it.
+    K = min(args.steps if args.steps else K_def, nt_case)
+    nt = nt_case
+    W = nt - K
 
     runs = {}
     plan = (("fp64", "fp64", False), ("mixed", "mixed", False), ("learned_mixed", "mixed", True))
@@ -273,27 +292,42 @@ def bench_rotor(args):
         dev = np.abs(r["ct"] - ref) / np.abs(ref).max(axis=1, keepdims=True)
         r["ct_dev_per_rev"] = [float(dev[:, 1 + steps_rev * k: 1 + steps_rev * (k + 1)].max())
                                for k in range((dev.shape[1] - 1) // steps_rev)]
-    # primary path: mixed precision when its CT history holds 1e-4 of FP64
-    prim = "mixed" if "mixed" in runs and runs["mixed"]["ct_rel"] < 1e-4 else "fp64"
+    # CT averaged over the last full revolution, per rotor: the observable
+    # that the wake's chaotic amplification of round-off leaves most stable
+    last = slice(nt + 1 - steps_rev, nt + 1)
+    ref_mean = ref[:, last].mean(axis=1)
+    for r in runs.values():
+        m = r["ct"][:, last].mean(axis=1)
+        r["ct_mean_last_rev"] = [float(x) for x in m]
+        r["ct_mean_last_rev_rel_dev"] = float(np.max(np.abs(m - ref_mean) /
+                                                     np.maximum(np.abs(ref_mean), 1e-30)))
+    # value: the plain FP64 path, every Biot-Savart launch of the timed steps
+    prim = "fp64"
     P = runs[prim]["st"]
-    value = P.bs_pairs / P.wall_steps_tail
-    traffic_key = None
-    if prim == "fp64" and args.workload == "rotor2" and K == 72 and nt == nt_case:
-        traffic_key = "rotor_tail144:bs_fp64_kernel<256>"
+    value = P.bs_pairs_all / P.wall_steps_tail
+    traffic_key = f"{args.workload}_tail{K}:bs_fp64_kernel<256>"
     out = {
         "metric": "Biot-Savart pair-interactions/s",
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": P.wall_steps_tail * 1e3 / K, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32+f64" if prim == "mixed" else "f64",
+        "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": desc, "case": f"cases/{case_name}", "timed_steps": K,
-                   "total_steps": nt, "precision": prim},
-        "roofline": roofline(P.bs_kernel_ms, P.bs_launches, P.bs_pairs,
-                             PEAK_FP32_VALU if prim == "mixed" else PEAK_FP64_VALU,
-                             "bs_mixed_kernel<256>" if prim == "mixed" else "bs_fp64_kernel<256>",
-                             traffic_key),
+                   "timed_step_range": [nt - K + 1, nt], "total_steps": nt,
+                   "precision": prim,
+                   "warmup_note": (f"the {W} steps before the timed ones build the wake that "
+                                   f"defines the workload; --warmup "
+                                   f"{args.warmup if args.warmup is not None else '(unset)'} "
+                                   "does not shorten the case")},
+        "roofline": roofline(P.bs_kernel_ms, P.bs_launches, P.bs_pairs, PEAK_FP64_VALU,
+                             "bs_fp64_kernel<256>", traffic_key),
+        "pairs_timed": {"all_launches": P.bs_pairs_all, "launches": int(P.bs_launches_all),
+                        "work_queue_kernel": P.bs_pairs,
+                        "work_queue_launches": int(P.bs_launches)},
         "variants": {k: {"s_per_step": r["st"].wall_steps_tail / K,
-                         "pairs_per_s": r["st"].bs_pairs / r["st"].wall_steps_tail,
+                         "pairs_per_s": r["st"].bs_pairs_all / r["st"].wall_steps_tail,
+                         "ct_mean_last_rev": r["ct_mean_last_rev"],
+                         "ct_mean_last_rev_rel_dev_vs_fp64": r["ct_mean_last_rev_rel_dev"],
                          "run_wall_s": r["wall"], "ct_final": [float(x) for x in r["ct"][:, -1]],
                          "ct_max_rel_dev_vs_fp64": r["ct_rel"],
                          "ct_dev_per_rev_vs_fp64": r["ct_dev_per_rev"],
@@ -322,6 +356,9 @@ def bench_rotor(args):
             "ct_dev_per_rev": L["ct_dev_per_rev"],
             "fp64_twin_ct_dev_per_rev": runs.get("fp64_twin_thetaC_1e-9", {}).get("ct_dev_per_rev"),
             "revs_within_1e-4": sum(1 for x in L["ct_dev_per_rev"] if x <= 1e-4),
+            "ct_mean_last_rev_rel_dev": L["ct_mean_last_rev_rel_dev"],
+            "fp64_twin_ct_mean_last_rev_rel_dev":
+                runs.get("fp64_twin_thetaC_1e-9", {}).get("ct_mean_last_rev_rel_dev"),
             "fallback_rate": (L["st"].pred_fallbacks / L["st"].pred_judged
                               if L["st"].pred_judged > 0 else None),
             "network_gain_over_extrapolation":
@@ -332,16 +369,18 @@ def bench_rotor(args):
                                     "(coupled surfaces with different spanwise panel counts), "
                                     "so it is not timed")
     elif not args.no_cpu and os.path.exists(REF_BIN):
-        S = args.cpu_steps
+        S = args.cpu_steps or CPU_STEPS[args.workload]
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         _, sg = host.run_case(case, precision="fp64", nt=S, tail_steps=S, nrotors=nrot)
         wall_cpu = run_reference(case, S, threads)
+        phys = host_cores()
         out["cpu_baseline"] = {
-            "value": sg.bs_pairs / wall_cpu, "unit": "pairs/s", "cores": threads,
-            "kind": "reference",
+            "value": sg.bs_pairs_all / wall_cpu, "unit": "pairs/s", "cores": threads,
+            "kind": "reference", "host_physical_cores": phys,
             "sample": f"reference solver built from its sources, first {S} steps of the same "
-                      f"case ({wall_cpu:.1f} s); pairs counted as on the GPU over those steps "
-                      f"({sg.bs_pairs:.3e}); GPU FP64 on the same steps took "
+                      f"case ({wall_cpu:.1f} s, {threads} OpenMP threads on a host with "
+                      f"{phys} physical cores); pairs counted as on the GPU over those steps, "
+                      f"every launch ({sg.bs_pairs_all:.3e}); GPU FP64 on the same steps took "
                       f"{sg.wall_steps_tail:.3f} s"}
     print(json.dumps(out), flush=True)
 
@@ -409,9 +448,9 @@ def main():
     ap.add_argument("--steps", type=int, default=0)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", choices=["rotor2", "quad", "rotorwing", "microbench", "lu",
-                                           "hbmcopy"], default="rotor2")
+                                           "hbmcopy"], default="quad")
     ap.add_argument("--variants", default="mixed,learned_mixed",
-                    help="comma list of rotor2 variants run beside fp64 (mixed, learned_mixed)")
+                    help="comma list of variants run beside fp64 (mixed, learned_mixed)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--learned", action="store_true")
     ap.add_argument("--tol", type=float, default=5e-4,
@@ -421,7 +460,8 @@ def main():
     ap.add_argument("--no-twin", action="store_true",
                     help="skip the perturbed FP64 twin run (chaos floor of CT)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="steps of the case the reference solver runs (0: per-workload default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); a multi-GPU "

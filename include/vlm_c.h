@@ -305,6 +305,10 @@ int vlm_profile_enable(vlm_ctx *ctx, int enable);
 int vlm_profile_reset(vlm_ctx *ctx);
 int vlm_profile_get(vlm_ctx *ctx, double *bs_ms, int64_t *bs_launches,
                     double *pairs);
+/* Pair count of every Biot-Savart launch since the last reset, the
+ * small-target launches (fewer than 1024 targets: collocation points,
+ * bound vortices, loads) included, and the number of those launches. */
+int vlm_profile_get_all(vlm_ctx *ctx, double *pairs, int64_t *launches);
 /* Device time of the LU factorisations (lu_factor on the device, events
  * around the whole factorisation) since the last reset, their count and
  * their nominal flops (2 n^3 / 3 each). */

@@ -22,17 +22,18 @@ def _cmp(a, b, scale, tol, what):
 
 
 def compare(sa, sb, case, tol=1e-11):
-    for ra, rb in zip(sa, sb):
+    for k, (ra, rb) in enumerate(zip(sa, sb)):
+        c = case.surface(k)
         assert ra["cursors"] == rb["cursors"]
         rn, rf = ra["cursors"][0] - 1, ra["cursors"][1] - 1
-        if rn < case.nNwake:
+        if rn < c.nNwake:
             sc = np.max(np.abs(ra["nodes"][:, rn:])) + 1.0
             _cmp(ra["nodes"][:, rn:], rb["nodes"][:, rn:], sc, tol, "near nodes")
             _cmp(ra["le"], rb["le"], sc, tol, "shed line")
             sg = np.max(np.abs(ra["gam"][:, rn:])) + 1e-30
             _cmp(ra["gam"][:, rn:], rb["gam"][:, rn:], sg, tol, "near gam")
             _cmp(ra["rc4"][:, rn:], rb["rc4"][:, rn:], 1.0, tol, "near cores")
-        if rf < case.nFwake:
+        if rf < c.nFwake:
             sc = np.max(np.abs(ra["far_fc"][:, rf:])) + 1.0
             _cmp(ra["far_fc"][:, rf:], rb["far_fc"][:, rf:], sc, tol, "far ends")
             _cmp(ra["far_gam"][:, rf:], rb["far_gam"][:, rf:], 1.0, tol, "far gam")
@@ -75,14 +76,71 @@ def test_wake_steps_match_oracle(name):
         compare(H.snapshot(dev, case, ia), H.snapshot(ref, case, ib), case)
 
 
-def test_mixed_precision_wake_close_to_fp64():
+def test_mixed_mode_small_launches_stay_fp64():
+    """In MIXED mode, launches with fewer than 1024 targets (here every wake
+    pass: at most 84 nodes) run the FP64 small-target kernel, so the wake
+    stays at FP64 parity with the oracle and no work-queue launch happens."""
     case = H.Case(**CASES["rotor_am2"])
     dev, ref = _ctx(precision=1), orc.System()
+    dev.profile_enable(True)
+    dev.profile_reset()
     ia, ib = H.setup(dev, case), H.setup(ref, case)
     for it in range(1, case.nt + 1):
         H.step(dev, case, ia, it)
         H.step(ref, case, ib, it)
-    compare(H.snapshot(dev, case, ia), H.snapshot(ref, case, ib), case, tol=2e-5)
+    compare(H.snapshot(dev, case, ia), H.snapshot(ref, case, ib), case, tol=1e-11)
+    assert dev.profile_get()[1] == 0
+    assert dev.profile_get_all()[1] > 0
+
+
+def test_mixed_precision_work_queue_wake_close_to_oracle():
+    """MIXED mode on a wake large enough for the work-queue kernel
+    (bs_mixed_kernel: from step 40 the 2 x 13-node rows give >= 1024
+    targets): a two-bladed rotor in climb, 2 x 12 panels, 60 near rows,
+    60 AM2 steps, against the FP64 oracle step by step.  Bound: node
+    positions within 1e-9 and velocities within 2e-6 of the largest; the
+    FP64 path is held to 1e-11 by test_wake_steps_match_oracle's cases."""
+    case = H.Case(nb=2, nc=1, ns=12, nNwake=60, nFwake=0, nt=60, scheme=3, omega=30.0,
+                  vel=(0.0, 0.0, 1.0), rollup=(9, 12))
+    dev, ref = _ctx(precision=1), orc.System()
+    dev.profile_enable(True)
+    dev.profile_reset()
+    ia, ib = H.setup(dev, case), H.setup(ref, case)
+    worst_node = worst_vel = 0.0
+    for it in range(1, case.nt + 1):
+        H.step(dev, case, ia, it)
+        H.step(ref, case, ib, it)
+        ra, rb = H.snapshot(dev, case, ia)[0], H.snapshot(ref, case, ib)[0]
+        assert ra["cursors"] == rb["cursors"]
+        rn = ra["cursors"][0] - 1
+        sc = np.max(np.abs(rb["nodes"][:, rn:])) + 1.0
+        worst_node = max(worst_node, np.max(np.abs(ra["nodes"][:, rn:] - rb["nodes"][:, rn:])) / sc)
+        for k in ("vn", "vn1"):
+            sv = np.max(np.abs(rb[k])) + 1e-30
+            worst_vel = max(worst_vel, np.max(np.abs(ra[k] - rb[k])) / sv)
+    launches = dev.profile_get()[1]
+    print(f"mixed work-queue wake: node err {worst_node:.2e}, velocity err {worst_vel:.2e}, "
+          f"work-queue launches {launches}")
+    assert launches >= 2 * 20          # both AM2 passes of the last 20 steps at least
+    assert worst_node <= 1e-9 and worst_vel <= 2e-6
+
+
+def test_surfaces_with_different_spanwise_counts_match_oracle():
+    """A three-bladed rotor with 30 spanwise panels above a 4 x 40 wing
+    (cfg5's discretisation), both with free wakes that roll up into far
+    wakes, coupled through every Biot-Savart pass (main.f90:549-561,
+    libCommon.f90:114-211), step by step against the oracle."""
+    case = H.Case(scheme=3, nt=9, dt=0.005, vel=(13.6, 0.0, 0.0), surfaces=[
+        dict(nb=3, nc=1, ns=30, nNwake=5, nFwake=3, omega=136.0, span=1.0, root=0.1,
+             chord=1 / 6, pitch=10.0, rollup=(22, 30), hub=(0.0, 0.0, 0.0)),
+        dict(nb=1, nc=4, ns=40, nNwake=5, nFwake=3, omega=0.0, span=2.0, root=-1.0,
+             chord=0.25, pitch=3.0, rollup=(30, 40), hub=(0.0, 0.0, -0.5))])
+    dev, ref = _ctx(), orc.System()
+    ia, ib = H.setup(dev, case), H.setup(ref, case)
+    for it in range(1, case.nt + 1):
+        H.step(dev, case, ia, it)
+        H.step(ref, case, ib, it)
+        compare(H.snapshot(dev, case, ia), H.snapshot(ref, case, ib), case)
 
 
 def test_induced_velocity_sets_match_oracle():

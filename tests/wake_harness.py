@@ -11,14 +11,31 @@ from tests.geometry import blade_grid, rings_from_grids, rot_z
 
 
 class Case:
+    """surfaces: optional list of per-rotor overrides (nb, nc, ns, nNwake,
+    nFwake, omega, span, chord, pitch, rollup, hub, ...); rotor ir then
+    uses the case's values with surfaces[ir] applied (nr = len(surfaces))."""
+
     def __init__(self, nb=2, nc=2, ns=4, nNwake=6, nFwake=4, nt=14, scheme=3, axisym=False,
                  dissipation=False, strain=False, omega=0.0, vel=(10.0, 0.0, 0.0), dt=0.02,
                  span=1.0, root=0.2, chord=0.25, pitch=6.0, rollup=(3, 4), rollupMin=1,
-                 spanwiseCore=0.02, scv=None, bug_compat=True, nr=1, hub_dx=0.0):
+                 spanwiseCore=0.02, scv=None, bug_compat=True, nr=1, hub_dx=0.0, hub=None,
+                 surfaces=None):
         self.__dict__.update(locals())
         del self.__dict__["self"]
         if scv is None:
             self.scv = np.full(ns + 1, 0.02)
+        if surfaces is not None:
+            self.nr = len(surfaces)
+
+    def surface(self, ir):
+        """The case as rotor ir sees it."""
+        if self.surfaces is None:
+            return self
+        d = {k: v for k, v in self.__dict__.items() if k not in ("surfaces", "nr", "scv")}
+        d.update(self.surfaces[ir])
+        if "scv" not in self.surfaces[ir]:
+            d["scv"] = np.full(d["ns"] + 1, 0.02)
+        return Case(**d)
 
     @property
     def dims(self):
@@ -28,7 +45,8 @@ class Case:
 def blade_grids(case, t, ir=0):
     """Rigid motion at time t: blades rotate about z through the hub and
     the whole rotor translates with -vel."""
-    hub = np.array([case.hub_dx * ir, 0.0, 0.0])
+    case = case.surface(ir)
+    hub = np.array(case.hub if case.hub is not None else [case.hub_dx * ir, 0.0, 0.0], float)
     g0 = blade_grid(case.nc, case.ns, chord=case.chord, span=case.span,
                     root=case.root, pitch_deg=case.pitch, seed=100 + ir)
     grids = []
@@ -41,6 +59,7 @@ def blade_grids(case, t, ir=0):
 def wing_gamma(case, it, ir=0):
     """Smooth prescribed circulation, equal on every blade (as map_gam
     copies blade 1 under axisymmetry)."""
+    case = case.surface(ir)
     ic = np.arange(case.nc)[None, :]
     js = np.arange(case.ns)[:, None]
     g = -(1.0 + 0.3 * np.sin(0.7 * it + 0.5 * ir)) * np.sin(np.pi * (js + 0.5) / case.ns) * (ic + 1) / case.nc
@@ -50,11 +69,12 @@ def wing_gamma(case, it, ir=0):
 def setup(sys_, case):
     irs = []
     for ir in range(case.nr):
-        k = sys_.rotor_create(case.nb, case.nc, case.ns, case.nNwake, case.nFwake,
-                              nbConvect=1 if case.axisym else case.nb,
-                              rollupStart=case.rollup[0], rollupEnd=case.rollup[1],
-                              rollupMin=case.rollupMin, spanwiseCore=case.spanwiseCore,
-                              streamwiseCoreVec=case.scv)
+        c = case.surface(ir)
+        k = sys_.rotor_create(c.nb, c.nc, c.ns, c.nNwake, c.nFwake,
+                              nbConvect=1 if c.axisym else c.nb,
+                              rollupStart=c.rollup[0], rollupEnd=c.rollup[1],
+                              rollupMin=c.rollupMin, spanwiseCore=c.spanwiseCore,
+                              streamwiseCoreVec=c.scv)
         irs.append(k)
         grids, hub = blade_grids(case, 0.0, ir)
         corners = rings_from_grids(grids)
@@ -94,9 +114,10 @@ def step(sys_, case, irs, it):
 
 def snapshot(sys_, case, irs):
     out = []
-    for ir in irs:
-        w = sys_.rotor_get_wake(ir, case.dims)
-        vn, vf = sys_.rotor_get_velocity(ir, 0, case.dims)
-        vn1, vf1 = sys_.rotor_get_velocity(ir, 1, case.dims)
+    for k, ir in enumerate(irs):
+        dims = case.surface(k).dims
+        w = sys_.rotor_get_wake(ir, dims)
+        vn, vf = sys_.rotor_get_velocity(ir, 0, dims)
+        vn1, vf1 = sys_.rotor_get_velocity(ir, 1, dims)
         out.append(dict(cursors=sys_.rotor_cursors(ir), vn=vn, vf=vf, vn1=vn1, vf1=vf1, **w))
     return out

@@ -408,6 +408,26 @@ int resident_blocks(vlm_ctx *ctx, K kernel, int block, int *cache)
     return *cache;
 }
 
+// Records one profiled launch: the pair count is m times the device
+// segment count, copied to the pinned slot of this record without a stream
+// synchronisation and resolved at read-out (vlm_profile_get).
+int profile_count(vlm_ctx *ctx, const long long *nseg_dev, int64_t m, int kind, const Timer *tm)
+{
+    Timer r{};
+    if (tm) r = *tm;
+    r.kind = kind;
+    r.pairs = -1.0;
+    const size_t slot = ctx->timers.size();
+    if (slot < ctx->prof_cap) {
+        VLM_HIP(hipMemcpyAsync(ctx->prof_nseg + slot, nseg_dev, sizeof(long long),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        r.pairs = (double)m;
+    }
+    ctx->timers.push_back(r);
+    if (kind == 0) ctx->prof_launches += 1;
+    return VLM_OK;
+}
+
 }  // namespace
 
 int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segstride,
@@ -429,6 +449,7 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
                                0, ctx->stream, seg64, segstride, nseg_dev, nchunks, t, m, tstride,
                                ctx->part.p);
             VLM_HIP(hipGetLastError());
+            if (ctx->profile) VLM_TRY(profile_count(ctx, nseg_dev, m, 1, nullptr));
         }
         hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + RED_BLOCK - 1) / RED_BLOCK), 3),
                            dim3(RED_BLOCK), 0,
@@ -489,17 +510,7 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
         VLM_HIP(hipGetLastError());
         if (timed) {
             VLM_HIP(hipEventRecord(tm.b, ctx->stream));
-            tm.pairs = -1.0;
-            ctx->timers.push_back(tm);
-            ctx->prof_launches += 1;
-            // pair count = m * device segment count, copied to pinned memory
-            // without a stream synchronisation and resolved at read-out
-            const size_t slot = ctx->timers.size() - 1;
-            if (slot < ctx->prof_cap) {
-                VLM_HIP(hipMemcpyAsync(ctx->prof_nseg + slot, nseg_dev, sizeof(long long),
-                                       hipMemcpyDeviceToHost, ctx->stream));
-                ctx->timers.back().pairs = (double)m;
-            }
+            VLM_TRY(profile_count(ctx, nseg_dev, m, 0, &tm));
         }
     }
     hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + RED_BLOCK - 1) / RED_BLOCK), 3),

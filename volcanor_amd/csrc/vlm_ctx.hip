@@ -324,8 +324,8 @@ int vlm_destroy(vlm_ctx *ctx)
     for (auto *p : ctx->piv) (void)hipFree(p);
     for (auto *v : {&ctx->timers, &ctx->lu_timers})
         for (auto &t : *v) {
-            (void)hipEventDestroy(t.a);
-            (void)hipEventDestroy(t.b);
+            if (t.a) (void)hipEventDestroy(t.a);
+            if (t.b) (void)hipEventDestroy(t.b);
         }
     if (ctx->prof_nseg) (void)hipHostFree(ctx->prof_nseg);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
@@ -474,8 +474,8 @@ int vlm_profile_reset(vlm_ctx *ctx)
     if (!ctx) return VLM_ERR_ARG;
     VLM_TRY(sync_stream(ctx));
     for (auto &t : ctx->timers) {
-        (void)hipEventDestroy(t.a);
-        (void)hipEventDestroy(t.b);
+        if (t.a) (void)hipEventDestroy(t.a);
+        if (t.b) (void)hipEventDestroy(t.b);
     }
     ctx->timers.clear();
     for (auto &t : ctx->lu_timers) {
@@ -494,16 +494,32 @@ int vlm_profile_get(vlm_ctx *ctx, double *bs_ms, int64_t *bs_launches, double *p
     if (!ctx) return VLM_ERR_ARG;
     VLM_TRY(sync_stream(ctx));
     double ms = 0.0, pr = 0.0;
+    int64_t n = 0;
     for (size_t i = 0; i < ctx->timers.size(); ++i) {
+        if (ctx->timers[i].kind != 0) continue;
         float e = 0.f;
         VLM_HIP(hipEventElapsedTime(&e, ctx->timers[i].a, ctx->timers[i].b));
         ms += e;
+        ++n;
         if (i < ctx->prof_cap && ctx->timers[i].pairs > 0)
             pr += ctx->timers[i].pairs * (double)ctx->prof_nseg[i];
     }
     if (bs_ms) *bs_ms = ms;
-    if (bs_launches) *bs_launches = (int64_t)ctx->timers.size();
+    if (bs_launches) *bs_launches = n;
     if (pairs) *pairs = pr;
+    return VLM_OK;
+}
+
+int vlm_profile_get_all(vlm_ctx *ctx, double *pairs, int64_t *launches)
+{
+    if (!ctx) return VLM_ERR_ARG;
+    VLM_TRY(sync_stream(ctx));
+    double pr = 0.0;
+    for (size_t i = 0; i < ctx->timers.size(); ++i)
+        if (i < ctx->prof_cap && ctx->timers[i].pairs > 0)
+            pr += ctx->timers[i].pairs * (double)ctx->prof_nseg[i];
+    if (pairs) *pairs = pr;
+    if (launches) *launches = (int64_t)ctx->timers.size();
     return VLM_OK;
 }
 

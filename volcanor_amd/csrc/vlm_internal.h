@@ -91,9 +91,14 @@ struct Rotor {
     size_t nvf() const { return 3u * d.nb * (size_t)(d.nFwake > 0 ? d.nFwake : 0); }
 };
 
+// One profiled Biot-Savart launch: events around the work-queue kernel
+// (a, b; null for the small-target kernel, which is counted but not timed)
+// and the target count; pairs = targets x the device segment count, which
+// is copied to the pinned slot of the same index without a synchronisation.
 struct Timer {
     hipEvent_t a, b;
     double pairs;
+    int kind = 0;  // 0 work-queue kernel (timed), 1 small-target kernel
 };
 
 struct Mlp;

@@ -332,6 +332,14 @@ module vlm_c
       real(c_double), intent(out) :: ms, pairs
       integer(c_int64_t), intent(out) :: launches
     end function
+
+    integer(c_int) function vlm_profile_get_all(ctx, pairs, launches) &
+        bind(C, name='vlm_profile_get_all')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: pairs
+      integer(c_int64_t), intent(out) :: launches
+    end function
   end interface
 
 contains

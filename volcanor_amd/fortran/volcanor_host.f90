@@ -133,6 +133,8 @@ module volcanor_host
     real(c_double) :: pred_last_err
     real(c_double) :: pred_mlp_rms, pred_base_rms
     integer(c_int64_t) :: pred_judged
+    real(c_double) :: bs_pairs_all         !! every Biot-Savart launch of the tail
+    integer(c_int64_t) :: bs_launches_all
   end type volcanor_stats
 
   public :: volcanor_run, run_case
@@ -2682,9 +2684,12 @@ contains
     stats%nt = nt
     stats%tail_steps = tail_steps
     stats%bs_kernel_ms = 0._dp; stats%bs_pairs = 0._dp; stats%bs_launches = 0
+    stats%bs_pairs_all = 0._dp; stats%bs_launches_all = 0
     if (tail_steps > 0) then
       call vlm_check(ctx, vlm_profile_get(ctx, stats%bs_kernel_ms, stats%bs_launches, &
         & stats%bs_pairs), 'profile_get')
+      call vlm_check(ctx, vlm_profile_get_all(ctx, stats%bs_pairs_all, stats%bs_launches_all), &
+        & 'profile_get_all')
       call vlm_check(ctx, vlm_profile_enable(ctx, 0_c_int), 'profile_enable')
     endif
     block

@@ -24,7 +24,8 @@ class Stats(C.Structure):
                 ("pred_full", C.c_int64), ("pred_learned", C.c_int64),
                 ("pred_fallbacks", C.c_int64), ("pred_last_err", C.c_double),
                 ("pred_mlp_rms", C.c_double), ("pred_base_rms", C.c_double),
-                ("pred_judged", C.c_int64)]
+                ("pred_judged", C.c_int64), ("bs_pairs_all", C.c_double),
+                ("bs_launches_all", C.c_int64)]
 
 
 _host = None

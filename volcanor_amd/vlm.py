@@ -97,6 +97,7 @@ _SIGS = {
     "vlm_profile_enable": [_P, _I],
     "vlm_profile_reset": [_P],
     "vlm_profile_get": [_P, C.POINTER(_D), C.POINTER(_L), C.POINTER(_D)],
+    "vlm_profile_get_all": [_P, C.POINTER(_D), C.POINTER(_L)],
     "vlm_profile_get_lu": [_P, C.POINTER(_D), C.POINTER(_L), C.POINTER(_D)],
     "vlm_fp64_issue_rate": [_P, C.POINTER(_D), C.POINTER(_D)],
 }
@@ -458,6 +459,13 @@ class Context:
         ms, n, pairs = C.c_double(), C.c_int64(), C.c_double()
         self._chk(self.lib.vlm_profile_get(self.h, C.byref(ms), C.byref(n), C.byref(pairs)))
         return ms.value, n.value, pairs.value
+
+    def profile_get_all(self):
+        """(pairs, launches) of every Biot-Savart launch since the reset,
+        small-target launches included."""
+        pairs, n = C.c_double(), C.c_int64()
+        self._chk(self.lib.vlm_profile_get_all(self.h, C.byref(pairs), C.byref(n)))
+        return pairs.value, n.value
 
     def fp64_issue_rate(self):
         """(TFLOP/s of a stream of independent FP64 FMAs on this device,
