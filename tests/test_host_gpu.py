@@ -576,6 +576,53 @@ def test_multi_rotor_matches_reference_solver(name, nr):
     np.testing.assert_allclose(ct, ref, rtol=2e-6, atol=2e-9)
 
 
+def test_error_after_device_setup_returns_and_next_run_works():
+    """A restart file that does not exist is found missing after the device
+    context and the rotors are set up: volcanor_run returns an error with
+    the message, destroys that context, and the next run in the same
+    process gives the reference history."""
+    import shutil
+    import tempfile
+    from volcanor_amd import host, vlm
+    d = tempfile.mkdtemp(prefix="vlmerr_")
+    try:
+        src = os.path.join(CASES, "cfg1_wing.case")
+        for f in os.listdir(src):
+            shutil.copy(os.path.join(src, f), d)
+        cfg = os.path.join(d, "config.nml")
+        text = open(cfg).read()
+        assert "restartFromNt = 0" in text
+        with open(cfg, "w") as fh:
+            fh.write(text.replace("restartFromNt = 0", "restartFromNt = 7"))
+        with pytest.raises(vlm.VLMError, match="restart00007.dat"):
+            host.run_case(d, nt=10)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    ref = read_force(os.path.join(GOLD, "cfg1_wing_ForceNonDim.csv"))[:, 0]
+    ct, _ = run("cfg1_wing", nt=20)
+    np.testing.assert_allclose(ct[0], ref[:21], rtol=1e-6, atol=1e-7)
+
+
+def test_cfg5_as_specified_one_revolution():
+    """cfg5 as specified: the three-bladed rotor (30 spanwise panels) in
+    forward flight 0.5 R above the 4 x 40 wing, coupled through the
+    right-hand side and every wake pass, for one revolution (72 steps).
+    The reference solver turns NaN on surfaces with unequal spanwise counts,
+    so this run is held to properties: every CT/CL finite, the rotor's CT
+    positive and below 0.05, the wing's CL within (-2, 2), and the MIXED
+    path within 1e-5 of FP64 relative to the largest value (one revolution
+    is inside the horizon where the two stay close).  The wake-level parity
+    of unequal spanwise counts against the oracle is
+    test_wake_gpu.test_surfaces_with_different_spanwise_counts_match_oracle."""
+    ct, st = run("cfg5_rotor_wing", nt=72, nrotors=2)
+    assert ct.shape == (2, 73) and np.all(np.isfinite(ct))
+    assert np.all(ct[0, 1:] > 0.0) and np.all(ct[0, 1:] < 0.05)
+    assert np.all(np.abs(ct[1]) < 2.0)
+    cm, _ = run("cfg5_rotor_wing", precision="mixed", nt=72, nrotors=2)
+    scale = np.max(np.abs(ct), axis=1, keepdims=True)
+    assert np.max(np.abs(cm - ct) / scale) < 1e-5
+
+
 @pytest.mark.skip(reason="this run aborted in the host (SIGABRT, no GPU fault) with its message "
                          "lost to output capture; the cause is not found, and the host now stops "
                          "with an error for restarts combined with slowStart or wakeTipPlot")

@@ -119,7 +119,7 @@ struct Predictor {
     // network-free substitute, and squared exact speed
     double sum_err2 = 0.0, sum_base2 = 0.0, sum_ref2 = 0.0;
     int64_t buf_cap = 16384, buf_n = 0, buf_head = 0;
-    DBuf<float> X, Y, bx, bt;
+    DBuf<float> X, Y, bx, bt, tb;  // tb: minibatch targets in units of the buffer RMS
     DBuf<double> vn, ts, vs, e2, scal;
 };
 

@@ -1180,14 +1180,44 @@ __global__ __launch_bounds__(1024) void vref_kernel(const double *vn, int64_t m,
     if (threadIdx.x == 0) vref[0] = fmax(sqrt(sh[0] / (double)(m > 0 ? m : 1)), 1e-12);
 }
 
-// v = v_pass1 + y * vref for every target
-__global__ void apply_mlp_kernel(const double *vn, const float *Y, int64_t m, const double *vref,
+// v = v_pass1 + y * sigma * vref for every target, sigma = scal[4] the RMS
+// of the replay buffer's targets: the network predicts the correction in
+// units of its typical size.  With sigma = 0 (every recorded pass 2 equal
+// to pass 1) the prediction is the pass-1 velocity itself.
+__global__ void apply_mlp_kernel(const double *vn, const float *Y, int64_t m, const double *scal,
                                  double *out)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m) return;
-    const double s = vref[0];
-    for (int d = 0; d < 3; ++d) out[d * m + e] = vn[d * m + e] + (double)Y[3 * e + d] * s;
+    const double s = scal[0] * scal[4];
+    for (int d = 0; d < 3; ++d)
+        out[d * m + e] = vn[d * m + e] + (s > 0.0 ? (double)Y[3 * e + d] * s : 0.0);
+}
+
+// sigma = RMS of the n x 3 buffer targets (single block, fixed order) into
+// scal[4]; 0 for an empty buffer.
+__global__ __launch_bounds__(1024) void target_scale_kernel(const float *bt, int64_t n,
+                                                            double *scal)
+{
+    __shared__ double sh[1024];
+    double s = 0.0;
+    for (int64_t k = threadIdx.x; k < 3 * n; k += blockDim.x) s += (double)bt[k] * bt[k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) scal[4] = n > 0 ? sqrt(sh[0] / (double)(3 * n)) : 0.0;
+}
+
+// Minibatch targets in units of sigma (zero when sigma is zero).
+__global__ void scale_targets_kernel(const float *bt, int64_t n3, const double *scal, float *out)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n3) return;
+    const double sg = scal[4];
+    out[k] = sg > 0.0 ? (float)((double)bt[k] / sg) : 0.f;
 }
 
 // Sampled targets: e = phase + k*stride.  Gather their positions.
@@ -1312,7 +1342,8 @@ int learned_pass(vlm_ctx *ctx)
     VLM_HIP(P.vn.reserve((size_t)3 * m));
     VLM_HIP(P.X.reserve((size_t)45 * m));
     VLM_HIP(P.Y.reserve((size_t)3 * m));
-    VLM_HIP(P.scal.reserve(4));
+    VLM_HIP(P.scal.reserve(8));
+    VLM_HIP(P.tb.reserve((size_t)3 * P.batch));
     if (P.bx.n < (size_t)45 * P.buf_cap) {
         VLM_HIP(P.bx.reserve((size_t)45 * P.buf_cap));
         VLM_HIP(P.bt.reserve((size_t)3 * P.buf_cap));
@@ -1336,6 +1367,9 @@ int learned_pass(vlm_ctx *ctx)
                            P.X.p, off, 1.0 / P.lref, P.scal.p);
         off += n;
     }
+    // scale of the corrections seen so far (the buffer before this step)
+    hipLaunchKernelGGL(target_scale_kernel, dim3(1), dim3(1024), 0, ctx->stream, P.bt.p,
+                       P.buf_n, P.scal.p);
     VLM_HIP(hipGetLastError());
     const bool warm = P.steps < P.warmup;
     P.steps += 1;
@@ -1386,7 +1420,10 @@ int learned_pass(vlm_ctx *ctx)
         const int64_t B = P.buf_n < P.batch ? P.buf_n : P.batch;
         const int64_t start = (P.train_calls * 7919) % (P.buf_n - B + 1);
         P.train_calls += 1;
-        VLM_TRY(mlp_train_dev(ctx, B, P.bx.p + 45 * start, P.bt.p + 3 * start, 1, P.lr, nullptr,
+        hipLaunchKernelGGL(scale_targets_kernel, dim3(nblk(3 * B)), dim3(256), 0, ctx->stream,
+                           P.bt.p + 3 * start, 3 * B, P.scal.p, P.tb.p);
+        VLM_HIP(hipGetLastError());
+        VLM_TRY(mlp_train_dev(ctx, B, P.bx.p + 45 * start, P.tb.p, 1, P.lr, nullptr,
                               it == P.train_iters - 1));
     }
     if (judged_sums) {

@@ -345,13 +345,14 @@ module vlm_c
 contains
 
   subroutine vlm_check(ctx, rc, where)
-    !! Turns a non-zero status into `error stop` with the library message,
-    !! preserving the reference's abort-on-error semantics at top level.
+    !! Turns a non-zero status into a stop of the run with the library
+    !! message (host_fail).
     type(c_ptr), intent(in) :: ctx
     integer(c_int), intent(in) :: rc
     character(len=*), intent(in) :: where
     character(kind=c_char), pointer :: msg(:)
     character(len=512) :: text
+    character(len=700) :: line
     integer :: i
 
     if (rc == VLM_OK) return
@@ -363,8 +364,23 @@ contains
         text(i:i) = msg(i)
       enddo
     endif
-    write (*, '(A,A,A,I0,A,A)') 'ERROR in ', where, ' (code ', rc, '): ', trim(text)
-    error stop 'libvlm call failed'
+    write (line, '(A,A,A,I0,A,A)') 'ERROR in ', where, ' (code ', rc, '): ', trim(text)
+    call host_fail(trim(line))
   end subroutine vlm_check
+
+  subroutine host_fail(msg)
+    !! Stops the run with msg: the caller of volcanor_run gets status 1 and
+    !! the message (volcanor_last_error); a program stops with status 1, as
+    !! the reference's `error stop` does (host_guard.c).
+    character(len=*), intent(in) :: msg
+    interface
+      subroutine volcanor_host_fail(m, n) bind(C, name='volcanor_host_fail')
+        import :: c_char, c_int
+        character(kind=c_char), intent(in) :: m(*)
+        integer(c_int), value :: n
+      end subroutine
+    end interface
+    call volcanor_host_fail(msg, int(len(msg), c_int))
+  end subroutine host_fail
 
 end module vlm_c

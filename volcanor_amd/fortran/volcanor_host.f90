@@ -137,7 +137,11 @@ module volcanor_host
     integer(c_int64_t) :: bs_launches_all
   end type volcanor_stats
 
-  public :: volcanor_run, run_case
+  public :: volcanor_run_impl, volcanor_host_cleanup, run_case
+
+  !! libvlm context of the run in progress (destroyed by
+  !! volcanor_host_cleanup when an error stops the run)
+  type(c_ptr), save :: active_ctx = c_null_ptr
 
   interface
     integer(c_int) function c_mkdir(path, mode) bind(C, name='mkdir')
@@ -225,7 +229,7 @@ contains
       enddo
       x = x0 + (x1 - x0)*tan(t)/tan(1.2_dp)
     case default
-      error stop 'ERROR: spacing must be 1..4'
+      call host_fail('ERROR: spacing must be 1..4')
     end select
   end function spaced
 
@@ -282,7 +286,7 @@ contains
         xc(1:2, k) = [0._dp, 1._dp]
         zc(1:2, k) = 0._dp
       else
-        open (newunit=u, file=trim(dir)//'/'//trim(r%camberFile(k)), status='old', action='read')
+        call open_old(u, trim(dir)//'/'//trim(r%camberFile(k)))
         read (u, *) np(k)
         do i = 1, np(k)
           read (u, *) xc(i, k), zc(i, k)
@@ -433,13 +437,42 @@ contains
         endif
       endif
     enddo
-    if (idx < 1) error stop 'ERROR: interpolation point out of range'
+    if (idx < 1) call host_fail('ERROR: interpolation point out of range')
     v = y(idx) + (y(idx + 1) - y(idx))/(x(idx + 1) - x(idx))*(q - x(idx))
   end function pwl
 
   ! ------------------------------------------------------------------
   ! Inputs
   ! ------------------------------------------------------------------
+  subroutine open_old(u, path, unformatted)
+    !! Opens an existing input file for reading; a missing or unreadable
+    !! file stops the run with the runtime's message.
+    integer, intent(out) :: u
+    character(len=*), intent(in) :: path
+    logical, intent(in), optional :: unformatted
+    integer :: ios
+    character(len=256) :: emsg
+    logical :: unf
+    unf = .false.
+    if (present(unformatted)) unf = unformatted
+    emsg = ''
+    if (unf) then
+      open (newunit=u, file=path, status='old', action='read', form='unformatted', &
+        & iostat=ios, iomsg=emsg)
+    else
+      open (newunit=u, file=path, status='old', action='read', iostat=ios, iomsg=emsg)
+    endif
+    call io_check(ios, emsg, path)
+  end subroutine open_old
+
+  subroutine io_check(ios, emsg, what)
+    !! Stops the run with the runtime's message when an input open or
+    !! namelist read failed.
+    integer, intent(in) :: ios
+    character(len=*), intent(in) :: emsg, what
+    if (ios /= 0) call host_fail('ERROR: reading '//what//': '//trim(emsg))
+  end subroutine io_check
+
   subroutine read_config(dir, nt, dt, nr, density, velSound, kinematicVisc, sw)
     !! config.nml v0.5 (libCommon.f90:51-108); absent keys default to 0
     character(len=*), intent(in) :: dir
@@ -450,7 +483,8 @@ contains
     integer :: restartWriteNt, restartFromNt, ntSub, ntSubInit, wakePlot, wakeTipPlot, &
       & rotorForcePlot, gridPlot, wakeDissipation, wakeStrain, wakeBurst, wakeSuppress, &
       & slowStart, slowStartNt, fdScheme, initWakeVelNt, probe
-    integer :: u
+    integer :: u, ios
+    character(len=256) :: emsg
     namelist /VERSION/ fileFormatVersion
     namelist /PARAMS/ nt, dt, nr, density, velSound, kinematicVisc
     namelist /OPTIONS/ restartWriteNt, restartFromNt, ntSub, ntSubInit, &
@@ -462,12 +496,17 @@ contains
     wakeTipPlot = 0; rotorForcePlot = 0; gridPlot = 0; wakeDissipation = 0; wakeStrain = 0
     wakeBurst = 0; wakeSuppress = 0; slowStart = 0; slowStartNt = 0; fdScheme = 3
     initWakeVelNt = 0; probe = 0
-    open (newunit=u, file=trim(dir)//'/config.nml', status='old', action='read')
-    read (u, nml=VERSION)
+    open (newunit=u, file=trim(dir)//'/config.nml', status='old', action='read', iostat=ios, &
+      & iomsg=emsg)
+    call io_check(ios, emsg, trim(dir)//'/config.nml')
+    read (u, nml=VERSION, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, 'config.nml &VERSION')
     if (adjustl(fileFormatVersion) /= '0.5') &
-      error stop 'ERROR: config.nml template version does not match'
-    read (u, nml=PARAMS)
-    read (u, nml=OPTIONS)
+      call host_fail('ERROR: config.nml template version does not match')
+    read (u, nml=PARAMS, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, 'config.nml &PARAMS')
+    read (u, nml=OPTIONS, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, 'config.nml &OPTIONS')
     close (u)
     sw%restartWriteNt = restartWriteNt; sw%restartFromNt = restartFromNt
     sw%ntSub = ntSub; sw%ntSubInit = ntSubInit; sw%wakePlot = wakePlot
@@ -500,7 +539,8 @@ contains
     integer :: bladeDynamicsSwitch, pitchDynamicsSwitch, bodyDynamicsSwitch, &
       & bodyDynamicsIOVars, inflowPlotSwitch, gammaPlotSwitch, skewPlotSwitch
     real(dp) :: flapInitial, dflapInitial, Iflap, cflap, kflap, MflapConstant, dpitch
-    integer :: u
+    integer :: u, ios
+    character(len=256) :: emsg
     namelist /VERSION/ fileFormatVersion
     namelist /SURFACE/ surfaceType, imagePlane, imageRotorNum
     namelist /PANELS/ nb, propConvention, spanSpacing, chordSpacing, geometryFile, &
@@ -539,36 +579,48 @@ contains
     flapInitial = 0._dp; dflapInitial = 0._dp; Iflap = 0._dp; cflap = 0._dp; kflap = 0._dp
     MflapConstant = 0._dp; dpitch = 0._dp
 
-    open (newunit=u, file=trim(dir)//'/'//trim(fname), status='old', action='read')
-    read (u, nml=VERSION)
+    open (newunit=u, file=trim(dir)//'/'//trim(fname), status='old', action='read', &
+      & iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, trim(dir)//'/'//trim(fname))
+    read (u, nml=VERSION, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, trim(fname)//' &VERSION')
     if (adjustl(fileFormatVersion) /= '0.15') &
-      error stop 'ERROR: geomXX.nml template version does not match'
-    read (u, nml=SURFACE)
-    read (u, nml=PANELS)
-    read (u, nml=CAMBERSECTIONS)
-    read (u, nml=ORIENT)
-    read (u, nml=GEOMPARAMS)
+      call host_fail('ERROR: geomXX.nml template version does not match')
+    read (u, nml=SURFACE, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, trim(fname)//' &SURFACE')
+    read (u, nml=PANELS, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, trim(fname)//' &PANELS')
+    read (u, nml=CAMBERSECTIONS, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, trim(fname)//' &CAMBERSECTIONS')
+    read (u, nml=ORIENT, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, trim(fname)//' &ORIENT')
+    read (u, nml=GEOMPARAMS, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, trim(fname)//' &GEOMPARAMS')
     allocate (streamwiseCoreVec(ns + 1))
     streamwiseCoreVec = 0._dp
-    read (u, nml=WAKEPARAMS)
-    read (u, nml=DYNAMICS)
-    read (u, nml=WINDFRAME)
-    read (u, nml=PLOTS)
+    read (u, nml=WAKEPARAMS, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, trim(fname)//' &WAKEPARAMS')
+    read (u, nml=DYNAMICS, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, trim(fname)//' &DYNAMICS')
+    read (u, nml=WINDFRAME, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, trim(fname)//' &WINDFRAME')
+    read (u, nml=PLOTS, iostat=ios, iomsg=emsg)
+    call io_check(ios, emsg, trim(fname)//' &PLOTS')
     close (u)
 
-    if (surfaceType < 0) error stop 'ERROR: image surfaces are not supported'
-    if (surfaceType == 2) error stop 'ERROR: non-lifting surfaces are not supported'
-    if (nCamberFiles > 8) error stop 'ERROR: at most 8 camber files'
+    if (surfaceType < 0) call host_fail('ERROR: image surfaces are not supported')
+    if (surfaceType == 2) call host_fail('ERROR: non-lifting surfaces are not supported')
+    if (nCamberFiles > 8) call host_fail('ERROR: at most 8 camber files')
     r%nCamberFiles = nCamberFiles
     r%camberSectionLimit = camberSectionLimit; r%camberFile = camberFile
-    if (ductSwitch /= 0) error stop 'ERROR: ducts are not supported'
-    if (forceCalcSwitch /= 0) error stop 'ERROR: only forceCalcSwitch = 0 is supported'
+    if (ductSwitch /= 0) call host_fail('ERROR: ducts are not supported')
+    if (forceCalcSwitch /= 0) call host_fail('ERROR: only forceCalcSwitch = 0 is supported')
     if (bladeDynamicsSwitch /= 0 .or. bodyDynamicsSwitch /= 0 .or. pitchDynamicsSwitch /= 0) &
-      error stop 'ERROR: blade, pitch and body dynamics are not supported'
+      call host_fail('ERROR: blade, pitch and body dynamics are not supported')
     if (customTrajectorySwitch /= 0 .and. customTrajectorySwitch /= 1) &
-      & error stop 'ERROR: customTrajectorySwitch must be 0 or 1'
+      & call host_fail('ERROR: customTrajectorySwitch must be 0 or 1')
     r%customTrajectorySwitch = customTrajectorySwitch
-    if (prescWakeAfterTruncNt /= 0) error stop 'ERROR: the prescribed far wake is not supported'
+    if (prescWakeAfterTruncNt /= 0) call host_fail('ERROR: the prescribed far wake is not supported')
 
     r%surfaceType = surfaceType
     r%nb = nb; r%propConvention = propConvention; r%spanSpacing = spanSpacing
@@ -829,7 +881,7 @@ contains
     call to_chords_revs(r, r%skewPlotSwitch, dt)
     if (r%wakeTruncateNt > 0) r%wakeTruncateNt = max(r%wakeTruncateNt, r%nNwake + 1)
     if (r%surfaceType == 0) r%surfaceType = 1
-    if (r%nNwake > 0 .and. r%nNwake < 2) error stop 'ERROR: Atleast 2 near wake rows mandatory'
+    if (r%nNwake > 0 .and. r%nNwake < 2) call host_fail('ERROR: Atleast 2 near wake rows mandatory')
     r%nNwake = min(r%nNwake, nt)
     if (r%wakeTruncateNt == 0) then
       r%nFwake = nt - r%nNwake
@@ -844,7 +896,7 @@ contains
       block
         integer :: u, i
         allocate (r%velBodyHistory(3, nt), r%omegaBodyHistory(3, nt))
-        open (newunit=u, file=trim(dir)//'/trajectory'//r%id//'.in', status='old', action='read')
+        call open_old(u, trim(dir)//'/trajectory'//r%id//'.in')
         do i = 1, nt
           read (u, *) r%velBodyHistory(:, i), r%omegaBodyHistory(:, i)
         enddo
@@ -903,10 +955,10 @@ contains
       enddo
       if (r%nCamberFiles > 0) grid(3, :, :) = wing_camber(r, dir, xVec, yVec)
     else
-      open (newunit=u, file=trim(dir)//'/'//trim(r%geometryFile), status='old', action='read')
+      call open_old(u, trim(dir)//'/'//trim(r%geometryFile))
       read (u, *) nx, ny, nz
       if (nz > 1 .or. nx /= r%nc + 1 .or. ny /= r%ns + 1) &
-        error stop 'ERROR: Wrong or conflicting data in PLOT3D file'
+        call host_fail('ERROR: Wrong or conflicting data in PLOT3D file')
       read (u, *) ((grid(1, i, j), i=1, nx), j=1, ny), ((grid(2, i, j), i=1, nx), j=1, ny), &
         & ((grid(3, i, j), i=1, nx), j=1, ny)
       close (u)
@@ -1274,7 +1326,7 @@ contains
           write (u) blob
         else
           read (u) nfile
-          if (nfile /= nbytes) error stop 'ERROR: restart file does not match this case'
+          if (nfile /= nbytes) call host_fail('ERROR: restart file does not match this case')
           read (u) blob
           call vlm_check(ctx, vlm_rotor_set_state(ctx, r%dev, blob, nbytes), 'set_state')
         endif
@@ -1368,7 +1420,7 @@ contains
 
     do ir = 1, nr
       if (rotor(ir)%rowFar > rotor(ir)%nFwake) &
-        & error stop 'ERROR: Use filaments2file() only after development of far wake'
+        & call host_fail('ERROR: Use filaments2file() only after development of far wake')
     enddo
     nW = 0; nN = 0; nT = 0; nF = 0
     do ir = 1, nr
@@ -2330,27 +2382,27 @@ contains
     call system_clock(c0, crate)
     call read_config(dir, nt, dt, nr, density, velSound, kinematicVisc, sw)
     if (sw%slowStart < 0 .or. sw%slowStart > 3) &
-      & error stop 'ERROR: slowStart must be 0..3 (body dynamics are not supported)'
-    if (sw%slowStart > 0 .and. sw%slowStartNt <= 0) error stop 'ERROR: slowStartNt must be positive'
+      & call host_fail('ERROR: slowStart must be 0..3 (body dynamics are not supported)')
+    if (sw%slowStart > 0 .and. sw%slowStartNt <= 0) call host_fail('ERROR: slowStartNt must be positive')
     ! a restart run with both of these aborted in the host once, cause not
     ! yet found; restarts are verified without them
     if ((sw%restartWriteNt /= 0 .or. sw%restartFromNt /= 0) .and. &
       & (sw%slowStart /= 0 .or. sw%wakeTipPlot /= 0)) &
-      & error stop 'ERROR: restart files are not supported with slowStart or wakeTipPlot'
-    if (sw%fdScheme < 0 .or. sw%fdScheme > 5) error stop 'ERROR: fdScheme must be 0..5'
-    if (sw%ntSub < 0 .or. sw%ntSubInit < 0) error stop 'ERROR: ntSub and ntSubInit must be >= 0'
+      & call host_fail('ERROR: restart files are not supported with slowStart or wakeTipPlot')
+    if (sw%fdScheme < 0 .or. sw%fdScheme > 5) call host_fail('ERROR: fdScheme must be 0..5')
+    if (sw%ntSub < 0 .or. sw%ntSubInit < 0) call host_fail('ERROR: ntSub and ntSubInit must be >= 0')
     if (sw%restartFromNt < 0 .or. sw%restartWriteNt < 0) &
-      & error stop 'ERROR: restartFromNt and restartWriteNt must be >= 0'
+      & call host_fail('ERROR: restartFromNt and restartWriteNt must be >= 0')
     if (sw%restartFromNt > 0 .and. predictor /= 0) &
-      & error stop 'ERROR: a restart does not carry the learned predictor; run without it'
-    if (nr > 16) error stop 'ERROR: at most 16 rotors'
+      & call host_fail('ERROR: a restart does not carry the learned predictor; run without it')
+    if (nr > 16) call host_fail('ERROR: at most 16 rotors')
     ! velocity probes (main.f90:83-93): read only for probe = 1, used at every
     ! step that is a multiple of probe; any other value would leave the
     ! reference's probe arrays unallocated
     nprobe = 0
     if (sw%probe /= 0) then
-      if (sw%probe /= 1) error stop 'ERROR: probe must be 0 or 1'
-      open (newunit=uprobe, file=trim(dir)//'/probes.in', status='old', action='read')
+      if (sw%probe /= 1) call host_fail('ERROR: probe must be 0 or 1')
+      call open_old(uprobe, trim(dir)//'/probes.in')
       read (uprobe, *) nprobe
       allocate (probe(3, nprobe), probeVel(3, nprobe), probeLoc(3, nprobe), probeOut(3, nprobe))
       do k = 1, nprobe
@@ -2370,6 +2422,7 @@ contains
     write_out = sw%rotorForcePlot /= 0 .and. verbose >= 0
 
     call vlm_check(c_null_ptr, vlm_create(ctx, 0_c_int), 'vlm_create')
+    active_ctx = ctx
     call vlm_check(ctx, vlm_set_precision(ctx, int(precision, c_int)), 'set_precision')
     if (predictor /= 0) call vlm_check(ctx, vlm_set_predictor(ctx, 1_c_int, pred_tol, 32_c_int, &
       & 8_c_int, 4_c_int, rotor(1)%chord), 'set_predictor')
@@ -2475,8 +2528,7 @@ contains
       block
         integer :: ur
         write (ts5, '(I0.5)') sw%restartFromNt
-        open (newunit=ur, file=trim(dir)//'/Restart/restart'//ts5//'.dat', status='old', &
-          & action='read', form='unformatted')
+        call open_old(ur, trim(dir)//'/Restart/restart'//ts5//'.dat', unformatted=.true.)
         call restart_io(ctx, ur, rotor, nr, t, .false.)
         close (ur)
       end block
@@ -2707,13 +2759,31 @@ contains
       enddo
     endif
     rn = 0; rf = 0
+    active_ctx = c_null_ptr
     call vlm_check(ctx, vlm_destroy(ctx), 'vlm_destroy')
   end subroutine run_case
 
-  integer(c_int) function volcanor_run(cdir, precision, nt_override, verbose, ct_hist, ct_cap, &
-      & hist_len, stats, tail_steps, predictor, pred_tol) bind(C, name='volcanor_run')
-    !! C entry of the whole solver: runs the case in directory cdir
-    !! (null-terminated) and returns the CL/CT history ct_hist(ct_cap, nt+1).
+  subroutine volcanor_host_cleanup() bind(C, name='volcanor_host_cleanup')
+    !! After a run stopped by an error (host_guard.c): destroys its libvlm
+    !! context, which frees every device buffer of the run.
+    integer(c_int) :: rc
+    integer :: k, ios
+    logical :: op
+    if (c_associated(active_ctx)) rc = vlm_destroy(active_ctx)
+    active_ctx = c_null_ptr
+    ! files the stopped run left open (newunit numbers are negative)
+    do k = -1, -4096, -1
+      op = .false.
+      inquire (unit=k, opened=op, iostat=ios)
+      if (ios == 0 .and. op) close (k, iostat=ios)
+    enddo
+  end subroutine volcanor_host_cleanup
+
+  subroutine volcanor_run_impl(cdir, precision, nt_override, verbose, ct_hist, ct_cap, &
+      & hist_len, stats, tail_steps, predictor, pred_tol) bind(C, name='volcanor_run_impl')
+    !! The whole solver for the C entry volcanor_run (host_guard.c): runs
+    !! the case in directory cdir (null-terminated) and returns the CL/CT
+    !! history ct_hist(ct_cap, nt+1).
     character(kind=c_char), intent(in) :: cdir(*)
     integer(c_int), value :: precision, nt_override, verbose, ct_cap, hist_len, tail_steps
     integer(c_int), value :: predictor
@@ -2729,7 +2799,6 @@ contains
     enddo
     call run_case(trim(dir), int(precision), int(nt_override), int(verbose), ct_hist, &
       & int(ct_cap), int(hist_len), stats, int(tail_steps), int(predictor), pred_tol)
-    volcanor_run = 0
-  end function volcanor_run
+  end subroutine volcanor_run_impl
 
 end module volcanor_host

@@ -41,13 +41,17 @@ def load():
         lib.volcanor_run.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                      C.c_int, C.POINTER(Stats), C.c_int, C.c_int, C.c_double]
         lib.volcanor_run.restype = C.c_int
+        lib.volcanor_last_error.argtypes = []
+        lib.volcanor_last_error.restype = C.c_char_p
         _host = lib
     return _host
 
 
 def run_case(casedir, precision="fp64", nt=0, verbose=0, nrotors=1, max_steps=20000,
              tail_steps=0, learned=False, tol=5e-4):
-    """Run the case; returns (ct_history[nrotors, nt+1], Stats)."""
+    """Run the case; returns (ct_history[nrotors, nt+1], Stats).  A case the
+    host rejects, or a failing library call, raises VLMError with the
+    host's message; the process carries on."""
     lib = load()
     hist = np.zeros((max_steps + 1, nrotors))  # Fortran (ct_cap, hist_len) order
     st = Stats()
@@ -56,5 +60,5 @@ def run_case(casedir, precision="fp64", nt=0, verbose=0, nrotors=1, max_steps=20
                           hist.ctypes.data_as(C.c_void_p), nrotors, max_steps + 1,
                           C.byref(st), tail_steps, 1 if learned else 0, tol)
     if rc != 0:
-        raise vlm.VLMError(f"volcanor_run failed ({rc})")
+        raise vlm.VLMError(f"volcanor_run failed: {lib.volcanor_last_error().decode()}")
     return hist[: st.nt + 1].T.copy(), st
