@@ -234,7 +234,7 @@ WORKLOADS = {
 }
 # steps of the case the reference solver runs for the CPU baseline (about
 # 20 s on 16 host threads)
-CPU_STEPS = {"rotor2": 100, "quad": 26, "rotorwing": 0}
+CPU_STEPS = {"rotor2": 100, "quad": 22, "rotorwing": 0}
 # The reference solver turns NaN from the first step on cases whose coupled
 # surfaces have different spanwise panel counts (cfg5: 30 and 40), so no
 # reference run is timed for them.
@@ -263,13 +263,14 @@ def bench_rotor(args):
     W = nt - K
 
     runs = {}
-    plan = (("fp64", "fp64", False), ("mixed", "mixed", False), ("learned_mixed", "mixed", True))
+    plan = (("fp64", "fp64", False), ("mixed", "mixed", False), ("learned_mixed", "mixed", True),
+            ("learned_fp64", "fp64", True))
     want = set(args.variants.split(",")) | ({"learned_mixed"} if args.learned else set())
     plan = tuple(v for v in plan if v[0] == "fp64" or v[0] in want)
     for name, prec, learned in plan:
         t0 = time.perf_counter()
         ct, st = host.run_case(case, precision=prec, nt=nt, tail_steps=K, learned=learned,
-                               tol=args.tol, nrotors=nrot)
+                               tol=args.tol, nrotors=nrot, fixed_ploop=args.fixed_ploop)
         runs[name] = dict(ct=ct, st=st, wall=time.perf_counter() - t0)
         print(f"# {name}: run {runs[name]['wall']:.2f} s, timed {st.wall_steps_tail:.3f} s for "
               f"{K} steps, BS kernel {st.bs_kernel_ms:.1f} ms over {st.bs_launches} launches, "
@@ -282,7 +283,8 @@ def bench_rotor(args):
         tdir = case_variant(case, "thetaC = 0.0", "thetaC = 1.0e-9")
         try:
             t0 = time.perf_counter()
-            ct, st = host.run_case(tdir, precision="fp64", nt=nt, tail_steps=K, nrotors=nrot)
+            ct, st = host.run_case(tdir, precision="fp64", nt=nt, tail_steps=K, nrotors=nrot,
+                                   fixed_ploop=args.fixed_ploop)
             runs["fp64_twin_thetaC_1e-9"] = dict(ct=ct, st=st, wall=time.perf_counter() - t0)
         finally:
             shutil.rmtree(tdir, ignore_errors=True)
@@ -315,6 +317,8 @@ def bench_rotor(args):
         "config": {"workload": desc, "case": f"cases/{case_name}", "timed_steps": K,
                    "timed_step_range": [nt - K + 1, nt], "total_steps": nt,
                    "precision": prim,
+                   "predicted_wake": ("every row (--fixed-ploop)" if args.fixed_ploop else
+                                      "row 1 only, as the reference (classdef.f90:1552-1560)"),
                    "warmup_note": (f"the {W} steps before the timed ones build the wake that "
                                    f"defines the workload; --warmup "
                                    f"{args.warmup if args.warmup is not None else '(unset)'} "
@@ -346,11 +350,13 @@ def bench_rotor(args):
                       "differences.")
     out["speedup_s_per_step_vs_fp64"] = {
         k: runs["fp64"]["st"].wall_steps_tail / r["st"].wall_steps_tail for k, r in runs.items()}
-    if "learned_mixed" in runs:
-        L = runs["learned_mixed"]
-        out["learned_vs_plain"] = {
+    for lname in ("learned_mixed", "learned_fp64"):
+        if lname not in runs:
+            continue
+        L = runs[lname]
+        out["learned_vs_plain" if lname == "learned_mixed" else "learned_fp64_vs_plain"] = {
             "s_per_step_plain_fp64": runs["fp64"]["st"].wall_steps_tail / K,
-            "s_per_step_learned_mixed": L["st"].wall_steps_tail / K,
+            "s_per_step_learned": L["st"].wall_steps_tail / K,
             "speedup": runs["fp64"]["st"].wall_steps_tail / L["st"].wall_steps_tail,
             "tol": args.tol,
             "ct_dev_per_rev": L["ct_dev_per_rev"],
@@ -449,8 +455,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", choices=["rotor2", "quad", "rotorwing", "microbench", "lu",
                                            "hbmcopy"], default="quad")
-    ap.add_argument("--variants", default="mixed,learned_mixed",
-                    help="comma list of variants run beside fp64 (mixed, learned_mixed)")
+    ap.add_argument("--variants", default="mixed,learned_mixed,learned_fp64",
+                    help="comma list of variants run beside fp64 (mixed, learned_mixed, "
+                         "learned_fp64)")
+    ap.add_argument("--fixed-ploop", action="store_true",
+                    help="convect every row of the predicted near wake in AM2 (the reference "
+                         "convects row 1 only; with it pass 2 sees pass 1's near wake)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--learned", action="store_true")
     ap.add_argument("--tol", type=float, default=5e-4,

@@ -42,6 +42,30 @@ def test_fp64_matches_oracle(nseg, m):
     np.testing.assert_allclose(v, w, rtol=1e-10, atol=1e-12 * np.max(np.abs(w)))
 
 
+@pytest.mark.parametrize("nseg,m", [(1, 1024), (300, 1500), (9000, 2049), (200_000, 1100)])
+def test_fp64_work_queue_matches_oracle(nseg, m):
+    """The launches of 1024 targets or more (stream-K over XCD rounds with
+    the fused fixed-order tile sums): one segment; fewer batches than
+    XCDs; partial tiles; and 200k segments, two rounds per XCD.  With
+    accumulate, the result is added to the given array."""
+    rng = np.random.default_rng(nseg + 7 * m)
+    a, b = rng.uniform(-2, 2, size=(2, nseg, 3))
+    g, rc = rng.normal(size=nseg), np.full(nseg, 0.01)
+    p = rng.uniform(-2, 2, size=(m, 3))
+    ctx = _ctx()
+    w = orc.filament_sum(a, b, g, rc, p, threads=8)
+    v = ctx.filament_vind(a, b, g, rc, p)
+    np.testing.assert_allclose(v, w, rtol=1e-10, atol=1e-12 * np.max(np.abs(w)))
+    v0 = rng.normal(size=(m, 3))
+    v = ctx.filament_vind(a, b, g, rc, p, v=v0)
+    np.testing.assert_allclose(v, v0 + w, rtol=1e-10, atol=1e-12 * np.max(np.abs(w)))
+    # an empty source list on the same path: zeros, or the given array
+    z = ctx.filament_vind(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0), np.zeros(0), p)
+    assert np.all(z == 0)
+    z = ctx.filament_vind(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0), np.zeros(0), p, v=v0)
+    assert np.array_equal(z, v0)
+
+
 def test_edge_cases():
     ctx = _ctx()
     A = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])

@@ -588,7 +588,8 @@ def test_error_after_device_setup_returns_and_next_run_works():
     try:
         src = os.path.join(CASES, "cfg1_wing.case")
         for f in os.listdir(src):
-            shutil.copy(os.path.join(src, f), d)
+            if os.path.isfile(os.path.join(src, f)):
+                shutil.copy(os.path.join(src, f), d)
         cfg = os.path.join(d, "config.nml")
         text = open(cfg).read()
         assert "restartFromNt = 0" in text

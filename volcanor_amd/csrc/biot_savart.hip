@@ -131,58 +131,231 @@ __device__ __forceinline__ bool pair32(float phx, float phy, float phz, float pl
     return nearp;
 }
 
-struct BsArgs {
-    const double *s64;   // 8 x stride SoA
-    const float *s32;    // 12 x stride SoA
+// ---------------------------------------------------------------------
+// Launches with 1024 targets or more: stream-K over XCD-resident rounds.
+//
+// Targets form tiles of SK_TILE (one per lane of a 4-wave workgroup) and
+// segments batches of SK_BATCH (one LDS staging).  The batch list is cut
+// into NX contiguous ranges, one per XCD: the dispatcher deals workgroups
+// round-robin over the 8 XCDs, so workgroup b runs on XCD b % NX.  Each
+// range is cut into R rounds of about RB_TARGET batches, few enough that
+// the records of a round stay in the XCD's 4 MB L2 while all of the XCD's
+// workgroups sweep it.  Within an XCD the work units (round, tile, batch)
+// are ordered round-major, then tile-major, and split into B = grid / NX
+// equal contiguous ranges, one per workgroup: every workgroup evaluates
+// the same number of pairs whatever the tile and batch counts (stream-K).
+//
+// A workgroup's range is a sequence of runs, one per (round, tile) it
+// touches; each run's partial sums go to a piece slot of their own.  The
+// workgroup that completes the last piece of a tile (per-tile counter,
+// device-scope release/acquire) adds the tile's pieces in a fixed order
+// (XCD, round, workgroup) and writes the result.  Results are bitwise
+// repeatable, there is no second launch, and each target's partial sums
+// are a few pieces instead of one per segment chunk.
+// ---------------------------------------------------------------------
+constexpr int NX = 8;
+constexpr int SK_TILE = 256;
+constexpr int SK_BATCH = 256;
+constexpr int RB_TARGET = 96;  // batches per round: 1.8 MB of FP64 records
+constexpr int SK_RMAX = 64;    // beyond it rounds grow instead of multiplying
+
+struct SkPlan {
+    int64_t nb;  // batches
+    int64_t ns;  // slices in all, NX * R; XCD x owns slices x*R .. x*R+R-1
+    int R;
+    __host__ __device__ int64_t sb(int64_t s) const { return s * nb / ns; }  // first batch
+};
+
+__host__ __device__ inline SkPlan sk_plan(int64_t nseg)
+{
+    SkPlan p;
+    p.nb = (nseg + SK_BATCH - 1) / SK_BATCH;
+    int64_t R = (p.nb + (int64_t)NX * RB_TARGET - 1) / ((int64_t)NX * RB_TARGET);
+    if (R < 1) R = 1;
+    if (R > SK_RMAX) R = SK_RMAX;
+    p.R = (int)R;
+    p.ns = (int64_t)NX * R;
+    return p;
+}
+
+// Piece slots per workgroup: a range of U/B units touches at most
+// 2 ntiles R / B + R + 2 (round, tile) pairs (every non-empty round holds
+// at least half the batches of the largest).
+__host__ __device__ inline int sk_kmax(int ntiles, int R, int B)
+{
+    return (int)((2LL * ntiles * R + B - 1) / B) + R + 2;
+}
+
+struct SkArgs {
+    const double *s64;  // SEG64_N x stride SoA
+    const float *s32;   // SEG32_N x stride SoA
     int64_t sstride;
     const long long *nseg;
-    int64_t chunk_len;
     const double *t;
     int64_t m;
     int64_t tstride;
-    double *part;
+    double *out;
+    int64_t ostride;
+    int acc;
+    double *piece;  // [grid][kmax][3][SK_TILE]
+    int kmax;
+    int *count;     // [ntiles], zero at launch
     int ntiles;
-    int nitems;
-    int *counter;
 };
 
-__device__ __forceinline__ int next_item(int *counter, int *sh)
+// The XCD-local unit space of XCD x.
+struct SkXcd {
+    SkPlan p;
+    int64_t s0;      // first slice
+    int64_t b0;      // first batch
+    int64_t ntiles;
+    int64_t U;       // units
+    int B;           // workgroups of the XCD
+    __device__ int64_t base(int r) const { return ntiles * (p.sb(s0 + r) - b0); }
+    __device__ int64_t nsb(int r) const { return p.sb(s0 + r + 1) - p.sb(s0 + r); }
+    __device__ int64_t lo(int64_t j) const { return j * U / B; }
+    // workgroup whose range holds unit u (ranges may be empty when U < B)
+    __device__ int64_t jof(int64_t u) const { return ((u + 1) * B + U - 1) / U - 1; }
+    __device__ int round_of(int64_t u, int r) const
+    {
+        while (r + 1 < p.R && base(r + 1) <= u) ++r;
+        return r;
+    }
+    // (round, tile) pair index of unit u
+    __device__ int64_t pair_of(int64_t u) const
+    {
+        const int r = round_of(u, 0);
+        return (int64_t)r * ntiles + (u - base(r)) / nsb(r);
+    }
+};
+
+__device__ inline SkXcd sk_xcd(const SkPlan &p, int x, int ntiles, int B)
 {
-    if (threadIdx.x == 0) *sh = atomicAdd(counter, 1);
+    SkXcd X;
+    X.p = p;
+    X.s0 = (int64_t)x * p.R;
+    X.b0 = p.sb(X.s0);
+    X.ntiles = ntiles;
+    X.U = (int64_t)ntiles * (p.sb(X.s0 + p.R) - X.b0);
+    X.B = B;
+    return X;
+}
+
+// Pieces of tile t over all XCDs and rounds; with sum != null also adds
+// them, in the fixed order, for target lane tid of the tile.
+__device__ int sk_tile_pieces(const SkArgs &a, const SkPlan &p, int B, int64_t t, int tid,
+                              double *sum)
+{
+    int n = 0;
+    for (int x = 0; x < NX; ++x) {
+        const SkXcd X = sk_xcd(p, x, a.ntiles, B);
+        if (X.U == 0) continue;
+        for (int r = 0; r < p.R; ++r) {
+            const int64_t nsb = X.nsb(r);
+            if (nsb == 0) continue;
+            const int64_t us = X.base(r) + t * nsb;
+            const int64_t j0 = X.jof(us), j1 = X.jof(us + nsb - 1);
+            n += (int)(j1 - j0 + 1);
+            if (!sum) continue;
+            for (int64_t j = j0; j <= j1; ++j) {
+                const int64_t k = (int64_t)r * a.ntiles + t - X.pair_of(X.lo(j));
+                const double *q =
+                    a.piece + ((j * NX + x) * (int64_t)a.kmax + k) * 3 * SK_TILE + tid;
+                sum[0] += q[0];
+                sum[1] += q[SK_TILE];
+                sum[2] += q[2 * SK_TILE];
+            }
+        }
+    }
+    return n;
+}
+
+// Stores this workgroup's piece of tile t and, when it completes the tile,
+// adds all of the tile's pieces into the output.
+__device__ void sk_finish_piece(const SkArgs &a, const SkPlan &p, int B, int64_t t, int64_t k,
+                                double vx, double vy, double vz, int *flag_sh)
+{
+    const int tid = threadIdx.x;
+    double *q = a.piece + (((int64_t)blockIdx.x * a.kmax + k) * 3) * SK_TILE + tid;
+    q[0] = vx;
+    q[SK_TILE] = vy;
+    q[2 * SK_TILE] = vz;
+    __threadfence();  // the piece is visible device-wide before the count moves
     __syncthreads();
-    const int it = *sh;
+    if (tid == 0) {
+        const int want = sk_tile_pieces(a, p, B, t, 0, nullptr);
+        const int old =
+            __hip_atomic_fetch_add(a.count + t, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        *flag_sh = (old + 1 == want);
+    }
     __syncthreads();
-    return it;
+    if (*flag_sh) {
+        __threadfence();  // acquire: the other workgroups' pieces
+        const int64_t e = t * SK_TILE + tid;
+        if (e < a.m) {
+            double s[3] = {0.0, 0.0, 0.0};
+            if (a.acc) {
+                s[0] = a.out[e];
+                s[1] = a.out[a.ostride + e];
+                s[2] = a.out[2 * a.ostride + e];
+            }
+            sk_tile_pieces(a, p, B, t, tid, s);
+            a.out[e] = s[0];
+            a.out[a.ostride + e] = s[1];
+            a.out[2 * a.ostride + e] = s[2];
+        }
+    }
+    __syncthreads();
 }
 
 template <int TILE>
-__global__ __launch_bounds__(TILE) void bs_fp64_kernel(BsArgs a)
+__global__ __launch_bounds__(TILE) void bs_fp64_kernel(SkArgs a)
 {
-    __shared__ double4 sa[TILE];
-    __shared__ double4 sb[TILE];
-    __shared__ double sn[TILE];
-    __shared__ int item_sh;
+    static_assert(TILE == SK_TILE && SK_BATCH == SK_TILE, "one record staged per lane");
+    __shared__ double4 sa[SK_BATCH];
+    __shared__ double4 sb[SK_BATCH];
+    __shared__ double sn[SK_BATCH];
+    __shared__ int flag_sh;
     const int tid = threadIdx.x;
     const int64_t nseg = *a.nseg;
     const int64_t S = a.sstride;
-    for (;;) {
-        const int item = next_item(a.counter, &item_sh);
-        if (item >= a.nitems) break;
-        const int chunk = item / a.ntiles;
-        const int tile = item - chunk * a.ntiles;
-        const int64_t s0 = (int64_t)chunk * a.chunk_len;
-        const int64_t s1 = min(nseg, s0 + a.chunk_len);
-        const int64_t t = (int64_t)tile * TILE + tid;
-        const bool live = t < a.m;
+    if (nseg <= 0) {  // every source was skipped: the sum is empty
+        if (!a.acc)
+            for (int64_t t0 = blockIdx.x; t0 < a.ntiles; t0 += gridDim.x) {
+                const int64_t e = t0 * SK_TILE + tid;
+                if (e < a.m) a.out[e] = a.out[a.ostride + e] = a.out[2 * a.ostride + e] = 0.0;
+            }
+        return;
+    }
+    const SkPlan p = sk_plan(nseg);
+    const int B = gridDim.x / NX;
+    const int x = blockIdx.x % NX;
+    const int64_t j = blockIdx.x / NX;
+    const SkXcd X = sk_xcd(p, x, a.ntiles, B);
+    if (X.U == 0) return;
+    int64_t u = X.lo(j);
+    const int64_t u1 = X.lo(j + 1);
+    if (u >= u1) return;
+    const int64_t pfirst = X.pair_of(u);
+    int r = 0;
+    while (u < u1) {
+        r = X.round_of(u, r);
+        const int64_t nsb = X.nsb(r), off = u - X.base(r);
+        const int64_t tile = off / nsb;
+        const int64_t ue = min(u1, X.base(r) + (tile + 1) * nsb);
+        const int64_t e = tile * SK_TILE + tid;
+        const bool live = e < a.m;
         double px = 0.0, py = 0.0, pz = 0.0;
         if (live) {
-            px = a.t[t];
-            py = a.t[a.tstride + t];
-            pz = a.t[2 * a.tstride + t];
+            px = a.t[e];
+            py = a.t[a.tstride + e];
+            pz = a.t[2 * a.tstride + e];
         }
         double vx = 0.0, vy = 0.0, vz = 0.0;
-        for (int64_t base = s0; base < s1; base += TILE) {
-            const int cnt = (int)min<int64_t>(TILE, s1 - base);
+        for (int64_t ub = u; ub < ue; ++ub) {
+            const int64_t g = p.sb(X.s0 + r) + (ub - X.base(r) - tile * nsb);
+            const int64_t base = g * SK_BATCH;
+            const int cnt = (int)min<int64_t>(SK_BATCH, nseg - base);
             __syncthreads();
             if (tid < cnt) {
                 const int64_t s = base + tid;
@@ -195,12 +368,9 @@ __global__ __launch_bounds__(TILE) void bs_fp64_kernel(BsArgs a)
 #pragma unroll 2
             for (int k = 0; k < cnt; ++k) pair64(px, py, pz, sa[k], sb[k], sn[k], vx, vy, vz);
         }
-        if (live) {
-            double *o = a.part + (size_t)chunk * 3 * a.m;
-            o[t] = vx;
-            o[a.m + t] = vy;
-            o[2 * a.m + t] = vz;
-        }
+        sk_finish_piece(a, p, B, tile, (int64_t)r * a.ntiles + tile - pfirst, vx, vy, vz,
+                        &flag_sh);
+        u = ue;
     }
 }
 
@@ -212,75 +382,155 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x)
     s = tt;
 }
 
-// Near pairs are flagged in a 32-bit mask per group of 32 staged segments
-// and recomputed in FP64 after the group, one wave-wide iteration per set
-// bit of the busiest lane (records read from the FP64 list in global
-// memory), so the FP32 loop stays free of the FP64 branch.  Staged groups
-// are padded with zero records, which contribute exactly zero and are
-// never near (r0 = 0).
-template <int TILE>
-__global__ __launch_bounds__(TILE) void bs_mixed_kernel(BsArgs a)
+// FP32 pair without the near-field handling, for a group of segments that
+// every target of the wave is far from (see the group test in
+// bs_mixed_kernel): no target lies within two segment lengths of a
+// segment, so no pair is near-singular, |r1| and |r2| are bounded away
+// from zero (no unitVec clamp) and every core term q is positive (no
+// zero-denominator guard).
+__device__ __forceinline__ void pair32_far(float phx, float phy, float phz, float plx,
+                                           float ply, float plz, float4 s0, float4 s1,
+                                           float4 s2, float &vx, float &vy, float &vz)
 {
-    static_assert(TILE % 32 == 0, "groups of 32 segments");
-    __shared__ float4 s0v[TILE];
-    __shared__ float4 s1v[TILE];
-    __shared__ float4 s2v[TILE];
-    __shared__ int item_sh;
+    const float r1x = (phx - s0.x) + (plx - s1.x);
+    const float r1y = (phy - s0.y) + (ply - s1.y);
+    const float r1z = (phz - s0.z) + (plz - s1.z);
+    const float r0x = s2.x, r0y = s2.y, r0z = s2.z;
+    const float r2x = r1x - r0x, r2y = r1y - r0y, r2z = r1z - r0z;
+    const float cx = r0y * r1z - r0z * r1y;
+    const float cy = r0z * r1x - r0x * r1z;
+    const float cz = r0x * r1y - r0y * r1x;
+    const float c2 = cx * cx + cy * cy + cz * cz;
+    const float n1 = r1x * r1x + r1y * r1y + r1z * r1z;
+    const float n2 = r2x * r2x + r2y * r2y + r2z * r2z;
+    const float d1 = r0x * r1x + r0y * r1y + r0z * r1z;
+    const float d2 = __builtin_fmaf(s2.w, -0.25f, d1);
+    const float den = __builtin_fmaf(c2, c2, s1.w * s1.w);
+    const float j1 = __builtin_amdgcn_rsqf(n1 * den);
+    const float j2 = __builtin_amdgcn_rsqf(n2 * den);
+    const float s = s0.w * (d1 * j1 - d2 * j2);
+    vx += cx * s;
+    vy += cy * s;
+    vz += cz * s;
+}
+
+// MIXED: the staged batch is cut into groups of 32 segments.  At staging,
+// each group gets a far radius around its first segment's start point:
+// the largest |a_i - c| + 2 |r0_i| over the group (plus a rounding
+// margin), or infinity for a short group or one holding a segment with
+// q = rc^2 |r0|^2 below 1e-18.  A wave whose targets all lie outside that
+// radius runs the group with pair32_far; otherwise it runs pair32, which
+// flags near-singular pairs in a 32-bit mask, and recomputes the flagged
+// ones in FP64 after the group.  Staged groups are padded with zero
+// records, which contribute exactly zero and are never near (r0 = 0).
+template <int TILE>
+__global__ __launch_bounds__(TILE) void bs_mixed_kernel(SkArgs a)
+{
+    static_assert(TILE == SK_TILE && SK_BATCH == SK_TILE && SK_BATCH % 32 == 0, "groups of 32");
+    __shared__ float4 s0v[SK_BATCH];
+    __shared__ float4 s1v[SK_BATCH];
+    __shared__ float4 s2v[SK_BATCH];
+    __shared__ float4 far_sh[SK_BATCH / 32];  // centre, squared far radius
+    __shared__ int flag_sh;
     const int tid = threadIdx.x;
     const int64_t nseg = *a.nseg;
     const int64_t S = a.sstride;
-    for (;;) {
-        const int item = next_item(a.counter, &item_sh);
-        if (item >= a.nitems) break;
-        const int chunk = item / a.ntiles;
-        const int tile = item - chunk * a.ntiles;
-        const int64_t sb0 = (int64_t)chunk * a.chunk_len;
-        const int64_t sb1 = min(nseg, sb0 + a.chunk_len);
-        const int64_t t = (int64_t)tile * TILE + tid;
-        const bool live = t < a.m;
+    if (nseg <= 0) {  // every source was skipped: the sum is empty
+        if (!a.acc)
+            for (int64_t t0 = blockIdx.x; t0 < a.ntiles; t0 += gridDim.x) {
+                const int64_t e = t0 * SK_TILE + tid;
+                if (e < a.m) a.out[e] = a.out[a.ostride + e] = a.out[2 * a.ostride + e] = 0.0;
+            }
+        return;
+    }
+    const SkPlan p = sk_plan(nseg);
+    const int B = gridDim.x / NX;
+    const int x = blockIdx.x % NX;
+    const int64_t j = blockIdx.x / NX;
+    const SkXcd X = sk_xcd(p, x, a.ntiles, B);
+    if (X.U == 0) return;
+    int64_t u = X.lo(j);
+    const int64_t u1 = X.lo(j + 1);
+    if (u >= u1) return;
+    const int64_t pfirst = X.pair_of(u);
+    int r = 0;
+    while (u < u1) {
+        r = X.round_of(u, r);
+        const int64_t nsb = X.nsb(r), off = u - X.base(r);
+        const int64_t tile = off / nsb;
+        const int64_t ue = min(u1, X.base(r) + (tile + 1) * nsb);
+        const int64_t e = tile * SK_TILE + tid;
+        const bool live = e < a.m;
         double px = 0.0, py = 0.0, pz = 0.0;
         if (live) {
-            px = a.t[t];
-            py = a.t[a.tstride + t];
-            pz = a.t[2 * a.tstride + t];
+            px = a.t[e];
+            py = a.t[a.tstride + e];
+            pz = a.t[2 * a.tstride + e];
         }
         const float phx = (float)px, phy = (float)py, phz = (float)pz;
         const float plx = (float)(px - (double)phx), ply = (float)(py - (double)phy),
                     plz = (float)(pz - (double)phz);
         float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
         double nx = 0.0, ny = 0.0, nz = 0.0;
-        for (int64_t base = sb0; base < sb1; base += TILE) {
-            const int cnt = (int)min<int64_t>(TILE, sb1 - base);
+        for (int64_t ub = u; ub < ue; ++ub) {
+            const int64_t g = p.sb(X.s0 + r) + (ub - X.base(r) - tile * nsb);
+            const int64_t base = g * SK_BATCH;
+            const int cnt = (int)min<int64_t>(SK_BATCH, nseg - base);
             const int cnt32 = (cnt + 31) & ~31;
             __syncthreads();
-            if (tid < cnt) {
-                const int64_t s = base + tid;
-                s0v[tid] = make_float4(a.s32[s], a.s32[S + s], a.s32[2 * S + s], a.s32[3 * S + s]);
-                s1v[tid] = make_float4(a.s32[4 * S + s], a.s32[5 * S + s], a.s32[6 * S + s],
-                                       a.s32[7 * S + s]);
-                s2v[tid] = make_float4(a.s32[8 * S + s], a.s32[9 * S + s], a.s32[10 * S + s],
-                                       a.s32[11 * S + s]);
-            } else if (tid < cnt32) {
-                const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-                s0v[tid] = z;
-                s1v[tid] = z;
-                s2v[tid] = z;
+            {
+                float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0, v2 = v0;
+                if (tid < cnt) {
+                    const int64_t s = base + tid;
+                    v0 = make_float4(a.s32[s], a.s32[S + s], a.s32[2 * S + s], a.s32[3 * S + s]);
+                    v1 = make_float4(a.s32[4 * S + s], a.s32[5 * S + s], a.s32[6 * S + s],
+                                     a.s32[7 * S + s]);
+                    v2 = make_float4(a.s32[8 * S + s], a.s32[9 * S + s], a.s32[10 * S + s],
+                                     a.s32[11 * S + s]);
+                }
+                if (tid < cnt32) {
+                    s0v[tid] = v0;
+                    s1v[tid] = v1;
+                    s2v[tid] = v2;
+                }
+                // far radius of this lane's group (lanes 32k .. 32k+31)
+                const float cx = __shfl(v0.x, 0, 32), cy = __shfl(v0.y, 0, 32),
+                            cz = __shfl(v0.z, 0, 32);
+                const float dx = v0.x - cx, dy = v0.y - cy, dz = v0.z - cz;
+                float ex = sqrtf(dx * dx + dy * dy + dz * dz) + sqrtf(v2.w);  // + 2 |r0|
+                if (tid >= cnt || !(v1.w > 1e-18f)) ex = __builtin_inff();
+                for (int o = 16; o > 0; o >>= 1) ex = fmaxf(ex, __shfl_xor(ex, o, 32));
+                if ((tid & 31) == 0 && tid < cnt32) {
+                    const float cm = fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)));
+                    const float rr = ex * 1.0001f + 1e-5f * cm + 1e-30f;
+                    far_sh[tid >> 5] = make_float4(cx, cy, cz, rr * rr);
+                }
             }
             __syncthreads();
             float tx = 0.f, ty = 0.f, tz = 0.f;
-            for (int g = 0; g < cnt32; g += 32) {
+            for (int g32 = 0; g32 < cnt32; g32 += 32) {
+                const float4 f = far_sh[g32 >> 5];
+                const float qx = phx - f.x, qy = phy - f.y, qz = phz - f.z;
+                const bool far = !live || (qx * qx + qy * qy + qz * qz > f.w);
+                if (__builtin_amdgcn_readfirstlane(__all(far))) {
+#pragma unroll 4
+                    for (int jj = 0; jj < 32; ++jj)
+                        pair32_far(phx, phy, phz, plx, ply, plz, s0v[g32 + jj], s1v[g32 + jj],
+                                   s2v[g32 + jj], tx, ty, tz);
+                    continue;
+                }
                 unsigned mask = 0u;
 #pragma unroll 4
-                for (int j = 0; j < 32; ++j) {
-                    const bool nearp = pair32(phx, phy, phz, plx, ply, plz, s0v[g + j],
-                                              s1v[g + j], s2v[g + j], tx, ty, tz);
-                    mask |= (nearp ? 1u : 0u) << j;
+                for (int jj = 0; jj < 32; ++jj) {
+                    const bool nearp = pair32(phx, phy, phz, plx, ply, plz, s0v[g32 + jj],
+                                              s1v[g32 + jj], s2v[g32 + jj], tx, ty, tz);
+                    mask |= (nearp ? 1u : 0u) << jj;
                 }
                 if (!live) mask = 0u;
                 while (mask) {
-                    const int j = __builtin_ctz(mask);
+                    const int jj = __builtin_ctz(mask);
                     mask &= mask - 1u;
-                    const int64_t s = base + g + j;
+                    const int64_t s = base + g32 + jj;
                     pair64(px, py, pz,
                            make_double4(a.s64[s], a.s64[S + s], a.s64[2 * S + s], a.s64[3 * S + s]),
                            make_double4(a.s64[4 * S + s], a.s64[5 * S + s], a.s64[6 * S + s],
@@ -292,12 +542,10 @@ __global__ __launch_bounds__(TILE) void bs_mixed_kernel(BsArgs a)
             kahan_add(Sy, Cy, ty);
             kahan_add(Sz, Cz, tz);
         }
-        if (live) {
-            double *o = a.part + (size_t)chunk * 3 * a.m;
-            o[t] = ((double)Sx - (double)Cx) + nx;
-            o[a.m + t] = ((double)Sy - (double)Cy) + ny;
-            o[2 * a.m + t] = ((double)Sz - (double)Cz) + nz;
-        }
+        sk_finish_piece(a, p, B, tile, (int64_t)r * a.ntiles + tile - pfirst,
+                        ((double)Sx - (double)Cx) + nx, ((double)Sy - (double)Cy) + ny,
+                        ((double)Sz - (double)Cz) + nz, &flag_sh);
+        u = ue;
     }
 }
 
@@ -393,7 +641,20 @@ __global__ __launch_bounds__(RED_BLOCK) void bs_reduce_kernel(const double *part
     out[d * ostride + t] = s;
 }
 
-__global__ void zero_int_kernel(int *p) { *p = 0; }
+__global__ void zero_ints_kernel(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void zero_soa3_kernel(double *o, int64_t m, int64_t stride)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    o[i] = 0.0;
+    o[stride + i] = 0.0;
+    o[2 * stride + i] = 0.0;
+}
 
 template <typename K>
 int resident_blocks(vlm_ctx *ctx, K kernel, int block, int *cache)
@@ -458,65 +719,61 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
         return VLM_OK;
     }
     const int blk = BS_BLOCK;
-    const int ntiles = (int)((m + blk - 1) / blk);
+    const int ntiles = (int)((m + SK_TILE - 1) / SK_TILE);
     const bool mixed = ctx->precision == VLM_MIXED;
     int cap = mixed ? resident_blocks(ctx, bs_mixed_kernel<BS_BLOCK>, blk, &ctx->occ[1])
                     : resident_blocks(ctx, bs_fp64_kernel<BS_BLOCK>, blk, &ctx->occ[3]);
     if (cap <= 0) return fail(ctx, VLM_ERR_HIP, "occupancy query failed");
-    int nchunks = 0;
-    int64_t chunk_len = 1;
-    if (nseg_max > 0) {
-        // about 16 work items per resident workgroup, chunks of at least
-        // four tiles
-        const int64_t min_len = 4 * BS_BLOCK;
-        int64_t nc = ((int64_t)16 * cap + ntiles - 1) / ntiles;
-        const int64_t maxc = (nseg_max + min_len - 1) / min_len;
-        if (nc > maxc) nc = maxc;
-        if (nc < 1) nc = 1;
-        chunk_len = (nseg_max + nc - 1) / nc;
-        nchunks = (int)((nseg_max + chunk_len - 1) / chunk_len);
-    }
-    if (nchunks > 0) {
-        VLM_HIP(ctx->part.reserve((size_t)nchunks * 3 * (size_t)m));
-        VLM_HIP(ctx->counter.reserve(1));
-        BsArgs a;
-        a.s64 = seg64;
-        a.s32 = seg32;
-        a.sstride = segstride;
-        a.nseg = nseg_dev;
-        a.chunk_len = chunk_len;
-        a.t = t;
-        a.m = m;
-        a.tstride = tstride;
-        a.part = ctx->part.p;
-        a.ntiles = ntiles;
-        a.nitems = ntiles * nchunks;
-        a.counter = ctx->counter.p;
-        const int grid = a.nitems < cap ? a.nitems : cap;
-        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->counter.p);
-        Timer tm{};
-        const bool timed = ctx->profile;
-        if (timed) {
-            VLM_HIP(hipEventCreate(&tm.a));
-            VLM_HIP(hipEventCreate(&tm.b));
-            VLM_HIP(hipEventRecord(tm.a, ctx->stream));
+    if (nseg_max <= 0) {
+        // no source segment can exist: the sum is empty
+        if (!accumulate) {
+            hipLaunchKernelGGL(zero_soa3_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
+                               ctx->stream, out, m, tstride);
+            VLM_HIP(hipGetLastError());
         }
-        if (mixed)
-            hipLaunchKernelGGL(bs_mixed_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0, ctx->stream,
-                               a);
-        else
-            hipLaunchKernelGGL(bs_fp64_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0, ctx->stream,
-                               a);
-        VLM_HIP(hipGetLastError());
-        if (timed) {
-            VLM_HIP(hipEventRecord(tm.b, ctx->stream));
-            VLM_TRY(profile_count(ctx, nseg_dev, m, 0, &tm));
-        }
+        return VLM_OK;
     }
-    hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + RED_BLOCK - 1) / RED_BLOCK), 3),
-                       dim3(RED_BLOCK), 0,
-                       ctx->stream, ctx->part.p, nchunks, m, out, tstride, accumulate);
+    // the resident grid, a multiple of the XCD count
+    const int grid = cap >= NX ? cap - cap % NX : NX;
+    const int B = grid / NX;
+    const SkPlan plan = sk_plan(nseg_max);   // bounds the device count's plan
+    const int kmax = sk_kmax(ntiles, plan.R, B);
+    VLM_HIP(ctx->part.reserve((size_t)grid * kmax * 3 * SK_TILE));
+    VLM_HIP(ctx->counter.reserve((size_t)ntiles));
+    SkArgs a;
+    a.s64 = seg64;
+    a.s32 = seg32;
+    a.sstride = segstride;
+    a.nseg = nseg_dev;
+    a.t = t;
+    a.m = m;
+    a.tstride = tstride;
+    a.out = out;
+    a.ostride = tstride;
+    a.acc = accumulate;
+    a.piece = ctx->part.p;
+    a.kmax = kmax;
+    a.count = ctx->counter.p;
+    a.ntiles = ntiles;
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0,
+                       ctx->stream, ctx->counter.p, ntiles);
     VLM_HIP(hipGetLastError());
+    Timer tm{};
+    const bool timed = ctx->profile;
+    if (timed) {
+        VLM_HIP(hipEventCreate(&tm.a));
+        VLM_HIP(hipEventCreate(&tm.b));
+        VLM_HIP(hipEventRecord(tm.a, ctx->stream));
+    }
+    if (mixed)
+        hipLaunchKernelGGL(bs_mixed_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0, ctx->stream, a);
+    else
+        hipLaunchKernelGGL(bs_fp64_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0, ctx->stream, a);
+    VLM_HIP(hipGetLastError());
+    if (timed) {
+        VLM_HIP(hipEventRecord(tm.b, ctx->stream));
+        VLM_TRY(profile_count(ctx, nseg_dev, m, 0, &tm));
+    }
     return VLM_OK;
 }
 

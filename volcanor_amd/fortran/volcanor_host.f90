@@ -2359,7 +2359,8 @@ contains
       & tail_steps, predictor, pred_tol)
     character(len=*), intent(in) :: dir
     integer, intent(in) :: precision, nt_override, verbose, ct_cap, hist_len, tail_steps
-    integer, intent(in) :: predictor
+    integer, intent(in) :: predictor  !! bit 0: learned pass 2; bit 1: predicted-wake
+                                      !! loop over every row (reference quirk off)
     real(dp), intent(in) :: pred_tol
     real(dp), intent(inout) :: ct_hist(ct_cap, hist_len)
     type(volcanor_stats), intent(out) :: stats
@@ -2393,7 +2394,7 @@ contains
     if (sw%ntSub < 0 .or. sw%ntSubInit < 0) call host_fail('ERROR: ntSub and ntSubInit must be >= 0')
     if (sw%restartFromNt < 0 .or. sw%restartWriteNt < 0) &
       & call host_fail('ERROR: restartFromNt and restartWriteNt must be >= 0')
-    if (sw%restartFromNt > 0 .and. predictor /= 0) &
+    if (sw%restartFromNt > 0 .and. iand(predictor, 1) /= 0) &
       & call host_fail('ERROR: a restart does not carry the learned predictor; run without it')
     if (nr > 16) call host_fail('ERROR: at most 16 rotors')
     ! velocity probes (main.f90:83-93): read only for probe = 1, used at every
@@ -2424,7 +2425,9 @@ contains
     call vlm_check(c_null_ptr, vlm_create(ctx, 0_c_int), 'vlm_create')
     active_ctx = ctx
     call vlm_check(ctx, vlm_set_precision(ctx, int(precision, c_int)), 'set_precision')
-    if (predictor /= 0) call vlm_check(ctx, vlm_set_predictor(ctx, 1_c_int, pred_tol, 32_c_int, &
+    if (iand(predictor, 2) /= 0) call vlm_check(ctx, vlm_set_bug_compat(ctx, 0_c_int), &
+      & 'set_bug_compat')
+    if (iand(predictor, 1) /= 0) call vlm_check(ctx, vlm_set_predictor(ctx, 1_c_int, pred_tol, 32_c_int, &
       & 8_c_int, 4_c_int, rotor(1)%chord), 'set_predictor')
 
     do ir = 1, nr

@@ -48,17 +48,20 @@ def load():
 
 
 def run_case(casedir, precision="fp64", nt=0, verbose=0, nrotors=1, max_steps=20000,
-             tail_steps=0, learned=False, tol=5e-4):
+             tail_steps=0, learned=False, tol=5e-4, fixed_ploop=False):
     """Run the case; returns (ct_history[nrotors, nt+1], Stats).  A case the
     host rejects, or a failing library call, raises VLMError with the
-    host's message; the process carries on."""
+    host's message; the process carries on.  fixed_ploop convects every
+    row of the predicted near wake (the reference convects row 1 only,
+    classdef.f90:1552-1560, kept by default)."""
     lib = load()
     hist = np.zeros((max_steps + 1, nrotors))  # Fortran (ct_cap, hist_len) order
     st = Stats()
     p = 1 if precision == "mixed" else 0
     rc = lib.volcanor_run(os.path.abspath(casedir).encode(), p, nt, verbose,
                           hist.ctypes.data_as(C.c_void_p), nrotors, max_steps + 1,
-                          C.byref(st), tail_steps, 1 if learned else 0, tol)
+                          C.byref(st), tail_steps,
+                          (1 if learned else 0) | (2 if fixed_ploop else 0), tol)
     if rc != 0:
         raise vlm.VLMError(f"volcanor_run failed: {lib.volcanor_last_error().decode()}")
     return hist[: st.nt + 1].T.copy(), st
