@@ -253,17 +253,21 @@ __device__ int sk_tile_pieces(const SkArgs &a, const SkPlan &p, int B, int64_t t
         for (int r = 0; r < p.R; ++r) {
             const int64_t nsb = X.nsb(r);
             if (nsb == 0) continue;
-            const int64_t us = X.base(r) + t * nsb;
-            const int64_t j0 = X.jof(us), j1 = X.jof(us + nsb - 1);
-            n += (int)(j1 - j0 + 1);
-            if (!sum) continue;
-            for (int64_t j = j0; j <= j1; ++j) {
-                const int64_t k = (int64_t)r * a.ntiles + t - X.pair_of(X.lo(j));
-                const double *q =
-                    a.piece + ((j * NX + x) * (int64_t)a.kmax + k) * 3 * SK_TILE + tid;
-                sum[0] += q[0];
-                sum[1] += q[SK_TILE];
-                sum[2] += q[2 * SK_TILE];
+            const int64_t us = X.base(r) + t * nsb, ue = us + nsb;
+            // the non-empty workgroups whose ranges meet [us, ue), in order
+            for (int64_t j = X.jof(us);;) {
+                ++n;
+                if (sum) {
+                    const int64_t k = (int64_t)r * a.ntiles + t - X.pair_of(X.lo(j));
+                    const double *q =
+                        a.piece + ((j * NX + x) * (int64_t)a.kmax + k) * 3 * SK_TILE + tid;
+                    sum[0] += q[0];
+                    sum[1] += q[SK_TILE];
+                    sum[2] += q[2 * SK_TILE];
+                }
+                const int64_t next = X.lo(j + 1);
+                if (next >= ue) break;
+                j = X.jof(next);
             }
         }
     }
