@@ -146,12 +146,13 @@ __device__ __forceinline__ bool pair32(float phx, float phy, float phz, float pl
 // the same number of pairs whatever the tile and batch counts (stream-K).
 //
 // A workgroup's range is a sequence of runs, one per (round, tile) it
-// touches; each run's partial sums go to a piece slot of their own.  The
-// workgroup that completes the last piece of a tile (per-tile counter,
-// device-scope release/acquire) adds the tile's pieces in a fixed order
-// (XCD, round, workgroup) and writes the result.  Results are bitwise
-// repeatable, there is no second launch, and each target's partial sums
-// are a few pieces instead of one per segment chunk.
+// touches; each run's partial sums go to a piece slot of their own.  A
+// planning launch writes every workgroup's run list (tile, first batch,
+// batch count), so the evaluation kernel's control is wave-uniform scalar
+// loads and its registers go to the pair arithmetic; a fix-up launch adds
+// each tile's pieces in a fixed order (XCD, round, workgroup) into the
+// output.  Results are bitwise repeatable, and each target has a few
+// pieces where a split over segment chunks gave one per chunk.
 // ---------------------------------------------------------------------
 constexpr int NX = 8;
 constexpr int SK_TILE = 256;
@@ -194,13 +195,10 @@ struct SkArgs {
     const double *t;
     int64_t m;
     int64_t tstride;
-    double *out;
-    int64_t ostride;
-    int acc;
-    double *piece;  // [grid][kmax][3][SK_TILE]
+    double *piece;      // [grid][kmax][3][SK_TILE]
+    const int4 *runs;   // [grid][kmax]: tile, first batch, batch count
+    const int *nruns;   // [grid]
     int kmax;
-    int *count;     // [ntiles], zero at launch
-    int ntiles;
 };
 
 // The XCD-local unit space of XCD x.
@@ -241,14 +239,59 @@ __device__ inline SkXcd sk_xcd(const SkPlan &p, int x, int ntiles, int B)
     return X;
 }
 
-// Pieces of tile t over all XCDs and rounds; with sum != null also adds
-// them, in the fixed order, for target lane tid of the tile.
-__device__ int sk_tile_pieces(const SkArgs &a, const SkPlan &p, int B, int64_t t, int tid,
-                              double *sum)
+// Run lists: one thread per evaluation workgroup b writes its runs
+// (tile, first batch, batch count) in range order; run k of b owns piece
+// slot b * kmax + k.
+__global__ void sk_plan_kernel(const long long *nseg_p, int ntiles, int grid, int kmax, int4 *runs,
+                               int *nruns)
 {
-    int n = 0;
-    for (int x = 0; x < NX; ++x) {
-        const SkXcd X = sk_xcd(p, x, a.ntiles, B);
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= grid) return;
+    const SkPlan p = sk_plan(*nseg_p);
+    const int B = grid / NX;
+    const SkXcd X = sk_xcd(p, b % NX, ntiles, B);
+    int k = 0;
+    if (p.nb > 0 && X.U > 0) {
+        const int64_t j = b / NX;
+        int64_t u = X.lo(j);
+        const int64_t u1 = X.lo(j + 1);
+        int r = 0;
+        while (u < u1 && k < kmax) {
+            r = X.round_of(u, r);
+            const int64_t nsb = X.nsb(r), off = u - X.base(r);
+            const int64_t tile = off / nsb;
+            const int64_t ue = min(u1, X.base(r) + (tile + 1) * nsb);
+            runs[(int64_t)b * kmax + k] =
+                make_int4((int)tile, (int)(p.sb(X.s0 + r) + (off - tile * nsb)), (int)(ue - u), 0);
+            ++k;
+            u = ue;
+        }
+    }
+    nruns[b] = k;
+}
+
+// Fix-up: one workgroup per tile, one target per lane; the tile's pieces
+// in the fixed order (XCD, round, workgroup), added after the given
+// values when accumulating.  With no segment at all the result is zero
+// (or the given values).
+__global__ __launch_bounds__(SK_TILE) void sk_fix_kernel(const long long *nseg_p, const double *piece,
+                                                         int kmax, int ntiles, int grid,
+                                                         int64_t m, double *out, int64_t ostride,
+                                                         int acc)
+{
+    const int64_t t = blockIdx.x;
+    const int64_t e = t * SK_TILE + threadIdx.x;
+    if (e >= m) return;
+    const SkPlan p = sk_plan(*nseg_p);
+    const int B = grid / NX;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (acc) {
+        s0 = out[e];
+        s1 = out[ostride + e];
+        s2 = out[2 * ostride + e];
+    }
+    for (int x = 0; x < NX && p.nb > 0; ++x) {
+        const SkXcd X = sk_xcd(p, x, ntiles, B);
         if (X.U == 0) continue;
         for (int r = 0; r < p.R; ++r) {
             const int64_t nsb = X.nsb(r);
@@ -256,98 +299,40 @@ __device__ int sk_tile_pieces(const SkArgs &a, const SkPlan &p, int B, int64_t t
             const int64_t us = X.base(r) + t * nsb, ue = us + nsb;
             // the non-empty workgroups whose ranges meet [us, ue), in order
             for (int64_t j = X.jof(us);;) {
-                ++n;
-                if (sum) {
-                    const int64_t k = (int64_t)r * a.ntiles + t - X.pair_of(X.lo(j));
-                    const double *q =
-                        a.piece + ((j * NX + x) * (int64_t)a.kmax + k) * 3 * SK_TILE + tid;
-                    sum[0] += q[0];
-                    sum[1] += q[SK_TILE];
-                    sum[2] += q[2 * SK_TILE];
-                }
+                const int64_t k = (int64_t)r * ntiles + t - X.pair_of(X.lo(j));
+                const double *q = piece + ((j * NX + x) * (int64_t)kmax + k) * 3 * SK_TILE +
+                                  threadIdx.x;
+                s0 += q[0];
+                s1 += q[SK_TILE];
+                s2 += q[2 * SK_TILE];
                 const int64_t next = X.lo(j + 1);
                 if (next >= ue) break;
                 j = X.jof(next);
             }
         }
     }
-    return n;
+    out[e] = s0;
+    out[ostride + e] = s1;
+    out[2 * ostride + e] = s2;
 }
 
-// Stores this workgroup's piece of tile t and, when it completes the tile,
-// adds all of the tile's pieces into the output.
-__device__ void sk_finish_piece(const SkArgs &a, const SkPlan &p, int B, int64_t t, int64_t k,
-                                double vx, double vy, double vz, int *flag_sh)
-{
-    const int tid = threadIdx.x;
-    double *q = a.piece + (((int64_t)blockIdx.x * a.kmax + k) * 3) * SK_TILE + tid;
-    q[0] = vx;
-    q[SK_TILE] = vy;
-    q[2 * SK_TILE] = vz;
-    __threadfence();  // the piece is visible device-wide before the count moves
-    __syncthreads();
-    if (tid == 0) {
-        const int want = sk_tile_pieces(a, p, B, t, 0, nullptr);
-        const int old =
-            __hip_atomic_fetch_add(a.count + t, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        *flag_sh = (old + 1 == want);
-    }
-    __syncthreads();
-    if (*flag_sh) {
-        __threadfence();  // acquire: the other workgroups' pieces
-        const int64_t e = t * SK_TILE + tid;
-        if (e < a.m) {
-            double s[3] = {0.0, 0.0, 0.0};
-            if (a.acc) {
-                s[0] = a.out[e];
-                s[1] = a.out[a.ostride + e];
-                s[2] = a.out[2 * a.ostride + e];
-            }
-            sk_tile_pieces(a, p, B, t, tid, s);
-            a.out[e] = s[0];
-            a.out[a.ostride + e] = s[1];
-            a.out[2 * a.ostride + e] = s[2];
-        }
-    }
-    __syncthreads();
-}
-
+// __launch_bounds__'s second argument asks for 8 waves per SIMD (64
+// VGPRs): the pair loop fits, and only loop-invariant addresses of the run
+// loop are spilled.
 template <int TILE>
-__global__ __launch_bounds__(TILE) void bs_fp64_kernel(SkArgs a)
+__global__ __launch_bounds__(TILE, 8) void bs_fp64_kernel(SkArgs a)
 {
     static_assert(TILE == SK_TILE && SK_BATCH == SK_TILE, "one record staged per lane");
     __shared__ double4 sa[SK_BATCH];
     __shared__ double4 sb[SK_BATCH];
     __shared__ double sn[SK_BATCH];
-    __shared__ int flag_sh;
     const int tid = threadIdx.x;
     const int64_t nseg = *a.nseg;
     const int64_t S = a.sstride;
-    if (nseg <= 0) {  // every source was skipped: the sum is empty
-        if (!a.acc)
-            for (int64_t t0 = blockIdx.x; t0 < a.ntiles; t0 += gridDim.x) {
-                const int64_t e = t0 * SK_TILE + tid;
-                if (e < a.m) a.out[e] = a.out[a.ostride + e] = a.out[2 * a.ostride + e] = 0.0;
-            }
-        return;
-    }
-    const SkPlan p = sk_plan(nseg);
-    const int B = gridDim.x / NX;
-    const int x = blockIdx.x % NX;
-    const int64_t j = blockIdx.x / NX;
-    const SkXcd X = sk_xcd(p, x, a.ntiles, B);
-    if (X.U == 0) return;
-    int64_t u = X.lo(j);
-    const int64_t u1 = X.lo(j + 1);
-    if (u >= u1) return;
-    const int64_t pfirst = X.pair_of(u);
-    int r = 0;
-    while (u < u1) {
-        r = X.round_of(u, r);
-        const int64_t nsb = X.nsb(r), off = u - X.base(r);
-        const int64_t tile = off / nsb;
-        const int64_t ue = min(u1, X.base(r) + (tile + 1) * nsb);
-        const int64_t e = tile * SK_TILE + tid;
+    const int nr = a.nruns[blockIdx.x];
+    for (int k = 0; k < nr; ++k) {
+        const int4 run = a.runs[(int64_t)blockIdx.x * a.kmax + k];
+        const int64_t e = (int64_t)run.x * SK_TILE + tid;
         const bool live = e < a.m;
         double px = 0.0, py = 0.0, pz = 0.0;
         if (live) {
@@ -356,9 +341,8 @@ __global__ __launch_bounds__(TILE) void bs_fp64_kernel(SkArgs a)
             pz = a.t[2 * a.tstride + e];
         }
         double vx = 0.0, vy = 0.0, vz = 0.0;
-        for (int64_t ub = u; ub < ue; ++ub) {
-            const int64_t g = p.sb(X.s0 + r) + (ub - X.base(r) - tile * nsb);
-            const int64_t base = g * SK_BATCH;
+        for (int ib = 0; ib < run.z; ++ib) {
+            const int64_t base = (int64_t)(run.y + ib) * SK_BATCH;
             const int cnt = (int)min<int64_t>(SK_BATCH, nseg - base);
             __syncthreads();
             if (tid < cnt) {
@@ -370,11 +354,12 @@ __global__ __launch_bounds__(TILE) void bs_fp64_kernel(SkArgs a)
             }
             __syncthreads();
 #pragma unroll 2
-            for (int k = 0; k < cnt; ++k) pair64(px, py, pz, sa[k], sb[k], sn[k], vx, vy, vz);
+            for (int q = 0; q < cnt; ++q) pair64(px, py, pz, sa[q], sb[q], sn[q], vx, vy, vz);
         }
-        sk_finish_piece(a, p, B, tile, (int64_t)r * a.ntiles + tile - pfirst, vx, vy, vz,
-                        &flag_sh);
-        u = ue;
+        double *o = a.piece + (((int64_t)blockIdx.x * a.kmax + k) * 3) * SK_TILE + tid;
+        o[0] = vx;
+        o[SK_TILE] = vy;
+        o[2 * SK_TILE] = vz;
     }
 }
 
@@ -427,43 +412,23 @@ __device__ __forceinline__ void pair32_far(float phx, float phy, float phz, floa
 // flags near-singular pairs in a 32-bit mask, and recomputes the flagged
 // ones in FP64 after the group.  Staged groups are padded with zero
 // records, which contribute exactly zero and are never near (r0 = 0).
+// 7 waves per SIMD (72 VGPRs): both pair loops fit; a few per-batch
+// addresses are spilled.
 template <int TILE>
-__global__ __launch_bounds__(TILE) void bs_mixed_kernel(SkArgs a)
+__global__ __launch_bounds__(TILE, 7) void bs_mixed_kernel(SkArgs a)
 {
     static_assert(TILE == SK_TILE && SK_BATCH == SK_TILE && SK_BATCH % 32 == 0, "groups of 32");
     __shared__ float4 s0v[SK_BATCH];
     __shared__ float4 s1v[SK_BATCH];
     __shared__ float4 s2v[SK_BATCH];
     __shared__ float4 far_sh[SK_BATCH / 32];  // centre, squared far radius
-    __shared__ int flag_sh;
     const int tid = threadIdx.x;
     const int64_t nseg = *a.nseg;
     const int64_t S = a.sstride;
-    if (nseg <= 0) {  // every source was skipped: the sum is empty
-        if (!a.acc)
-            for (int64_t t0 = blockIdx.x; t0 < a.ntiles; t0 += gridDim.x) {
-                const int64_t e = t0 * SK_TILE + tid;
-                if (e < a.m) a.out[e] = a.out[a.ostride + e] = a.out[2 * a.ostride + e] = 0.0;
-            }
-        return;
-    }
-    const SkPlan p = sk_plan(nseg);
-    const int B = gridDim.x / NX;
-    const int x = blockIdx.x % NX;
-    const int64_t j = blockIdx.x / NX;
-    const SkXcd X = sk_xcd(p, x, a.ntiles, B);
-    if (X.U == 0) return;
-    int64_t u = X.lo(j);
-    const int64_t u1 = X.lo(j + 1);
-    if (u >= u1) return;
-    const int64_t pfirst = X.pair_of(u);
-    int r = 0;
-    while (u < u1) {
-        r = X.round_of(u, r);
-        const int64_t nsb = X.nsb(r), off = u - X.base(r);
-        const int64_t tile = off / nsb;
-        const int64_t ue = min(u1, X.base(r) + (tile + 1) * nsb);
-        const int64_t e = tile * SK_TILE + tid;
+    const int nr = a.nruns[blockIdx.x];
+    for (int k = 0; k < nr; ++k) {
+        const int4 run = a.runs[(int64_t)blockIdx.x * a.kmax + k];
+        const int64_t e = (int64_t)run.x * SK_TILE + tid;
         const bool live = e < a.m;
         double px = 0.0, py = 0.0, pz = 0.0;
         if (live) {
@@ -476,9 +441,8 @@ __global__ __launch_bounds__(TILE) void bs_mixed_kernel(SkArgs a)
                     plz = (float)(pz - (double)phz);
         float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
         double nx = 0.0, ny = 0.0, nz = 0.0;
-        for (int64_t ub = u; ub < ue; ++ub) {
-            const int64_t g = p.sb(X.s0 + r) + (ub - X.base(r) - tile * nsb);
-            const int64_t base = g * SK_BATCH;
+        for (int ib = 0; ib < run.z; ++ib) {
+            const int64_t base = (int64_t)(run.y + ib) * SK_BATCH;
             const int cnt = (int)min<int64_t>(SK_BATCH, nseg - base);
             const int cnt32 = (cnt + 31) & ~31;
             __syncthreads();
@@ -546,10 +510,10 @@ __global__ __launch_bounds__(TILE) void bs_mixed_kernel(SkArgs a)
             kahan_add(Sy, Cy, ty);
             kahan_add(Sz, Cz, tz);
         }
-        sk_finish_piece(a, p, B, tile, (int64_t)r * a.ntiles + tile - pfirst,
-                        ((double)Sx - (double)Cx) + nx, ((double)Sy - (double)Cy) + ny,
-                        ((double)Sz - (double)Cz) + nz, &flag_sh);
-        u = ue;
+        double *o = a.piece + (((int64_t)blockIdx.x * a.kmax + k) * 3) * SK_TILE + tid;
+        o[0] = ((double)Sx - (double)Cx) + nx;
+        o[SK_TILE] = ((double)Sy - (double)Cy) + ny;
+        o[2 * SK_TILE] = ((double)Sz - (double)Cz) + nz;
     }
 }
 
@@ -645,12 +609,6 @@ __global__ __launch_bounds__(RED_BLOCK) void bs_reduce_kernel(const double *part
     out[d * ostride + t] = s;
 }
 
-__global__ void zero_ints_kernel(int *p, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
 __global__ void zero_soa3_kernel(double *o, int64_t m, int64_t stride)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -743,7 +701,12 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
     const SkPlan plan = sk_plan(nseg_max);   // bounds the device count's plan
     const int kmax = sk_kmax(ntiles, plan.R, B);
     VLM_HIP(ctx->part.reserve((size_t)grid * kmax * 3 * SK_TILE));
-    VLM_HIP(ctx->counter.reserve((size_t)ntiles));
+    VLM_HIP(ctx->skruns.reserve((size_t)grid * kmax));
+    VLM_HIP(ctx->counter.reserve((size_t)grid));
+    hipLaunchKernelGGL(sk_plan_kernel, dim3((unsigned)((grid + 255) / 256)), dim3(256), 0,
+                       ctx->stream, nseg_dev, ntiles, grid, kmax, ctx->skruns.p,
+                       ctx->counter.p);
+    VLM_HIP(hipGetLastError());
     SkArgs a;
     a.s64 = seg64;
     a.s32 = seg32;
@@ -752,16 +715,10 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
     a.t = t;
     a.m = m;
     a.tstride = tstride;
-    a.out = out;
-    a.ostride = tstride;
-    a.acc = accumulate;
     a.piece = ctx->part.p;
+    a.runs = ctx->skruns.p;
+    a.nruns = ctx->counter.p;
     a.kmax = kmax;
-    a.count = ctx->counter.p;
-    a.ntiles = ntiles;
-    hipLaunchKernelGGL(zero_ints_kernel, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0,
-                       ctx->stream, ctx->counter.p, ntiles);
-    VLM_HIP(hipGetLastError());
     Timer tm{};
     const bool timed = ctx->profile;
     if (timed) {
@@ -778,6 +735,9 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
         VLM_HIP(hipEventRecord(tm.b, ctx->stream));
         VLM_TRY(profile_count(ctx, nseg_dev, m, 0, &tm));
     }
+    hipLaunchKernelGGL(sk_fix_kernel, dim3((unsigned)ntiles), dim3(SK_TILE), 0, ctx->stream,
+                       nseg_dev, ctx->part.p, kmax, ntiles, grid, m, out, tstride, accumulate);
+    VLM_HIP(hipGetLastError());
     return VLM_OK;
 }
 

@@ -154,7 +154,8 @@ struct vlm_ctx {
     vlm::DBuf<unsigned char> scan_tmp;
     size_t scan_tmp_bytes = 0;
     vlm::DBuf<double> tgt, out, part;
-    vlm::DBuf<int> counter;  // work-item counter of the Biot-Savart launches
+    vlm::DBuf<int> counter;  // run counts per workgroup of the Biot-Savart launches
+    vlm::DBuf<int4> skruns;  // run lists (tile, first batch, batch count) per workgroup
     int occ[4] = {0, 0, 0, 0};  // resident workgroups per kernel variant
     vlm::DBuf<double> hostio;  // device staging for host point lists
     // pinned host arena for host<->device copies: truly asynchronous copies,
