@@ -131,219 +131,58 @@ __device__ __forceinline__ bool pair32(float phx, float phy, float phz, float pl
     return nearp;
 }
 
-// ---------------------------------------------------------------------
-// Launches with 1024 targets or more: stream-K over XCD-resident rounds.
-//
-// Targets form tiles of SK_TILE (one per lane of a 4-wave workgroup) and
-// segments batches of SK_BATCH (one LDS staging).  The batch list is cut
-// into NX contiguous ranges, one per XCD: the dispatcher deals workgroups
-// round-robin over the 8 XCDs, so workgroup b runs on XCD b % NX.  Each
-// range is cut into R rounds of about RB_TARGET batches, few enough that
-// the records of a round stay in the XCD's 4 MB L2 while all of the XCD's
-// workgroups sweep it.  Within an XCD the work units (round, tile, batch)
-// are ordered round-major, then tile-major, and split into B = grid / NX
-// equal contiguous ranges, one per workgroup: every workgroup evaluates
-// the same number of pairs whatever the tile and batch counts (stream-K).
-//
-// A workgroup's range is a sequence of runs, one per (round, tile) it
-// touches; each run's partial sums go to a piece slot of their own.  A
-// planning launch writes every workgroup's run list (tile, first batch,
-// batch count), so the evaluation kernel's control is wave-uniform scalar
-// loads and its registers go to the pair arithmetic; a fix-up launch adds
-// each tile's pieces in a fixed order (XCD, round, workgroup) into the
-// output.  Results are bitwise repeatable, and each target has a few
-// pieces where a split over segment chunks gave one per chunk.
-// ---------------------------------------------------------------------
-constexpr int NX = 8;
-constexpr int SK_TILE = 256;
-constexpr int SK_BATCH = 256;
-constexpr int RB_TARGET = 96;  // batches per round: 1.8 MB of FP64 records
-constexpr int SK_RMAX = 64;    // beyond it rounds grow instead of multiplying
-
-struct SkPlan {
-    int64_t nb;  // batches
-    int64_t ns;  // slices in all, NX * R; XCD x owns slices x*R .. x*R+R-1
-    int R;
-    __host__ __device__ int64_t sb(int64_t s) const { return s * nb / ns; }  // first batch
-};
-
-__host__ __device__ inline SkPlan sk_plan(int64_t nseg)
-{
-    SkPlan p;
-    p.nb = (nseg + SK_BATCH - 1) / SK_BATCH;
-    int64_t R = (p.nb + (int64_t)NX * RB_TARGET - 1) / ((int64_t)NX * RB_TARGET);
-    if (R < 1) R = 1;
-    if (R > SK_RMAX) R = SK_RMAX;
-    p.R = (int)R;
-    p.ns = (int64_t)NX * R;
-    return p;
-}
-
-// Piece slots per workgroup: a range of U/B units touches at most
-// 2 ntiles R / B + R + 2 (round, tile) pairs (every non-empty round holds
-// at least half the batches of the largest).
-__host__ __device__ inline int sk_kmax(int ntiles, int R, int B)
-{
-    return (int)((2LL * ntiles * R + B - 1) / B) + R + 2;
-}
-
-struct SkArgs {
-    const double *s64;  // SEG64_N x stride SoA
-    const float *s32;   // SEG32_N x stride SoA
+struct BsArgs {
+    const double *s64;   // 8 x stride SoA
+    const float *s32;    // 12 x stride SoA
     int64_t sstride;
     const long long *nseg;
+    int64_t chunk_len;
     const double *t;
     int64_t m;
     int64_t tstride;
-    double *piece;      // [grid][kmax][3][SK_TILE]
-    const int4 *runs;   // [grid][kmax]: tile, first batch, batch count
-    const int *nruns;   // [grid]
-    int kmax;
+    double *part;
+    int ntiles;
+    int nitems;
+    int *counter;
 };
 
-// The XCD-local unit space of XCD x.
-struct SkXcd {
-    SkPlan p;
-    int64_t s0;      // first slice
-    int64_t b0;      // first batch
-    int64_t ntiles;
-    int64_t U;       // units
-    int B;           // workgroups of the XCD
-    __device__ int64_t base(int r) const { return ntiles * (p.sb(s0 + r) - b0); }
-    __device__ int64_t nsb(int r) const { return p.sb(s0 + r + 1) - p.sb(s0 + r); }
-    __device__ int64_t lo(int64_t j) const { return j * U / B; }
-    // workgroup whose range holds unit u (ranges may be empty when U < B)
-    __device__ int64_t jof(int64_t u) const { return ((u + 1) * B + U - 1) / U - 1; }
-    __device__ int round_of(int64_t u, int r) const
-    {
-        while (r + 1 < p.R && base(r + 1) <= u) ++r;
-        return r;
-    }
-    // (round, tile) pair index of unit u
-    __device__ int64_t pair_of(int64_t u) const
-    {
-        const int r = round_of(u, 0);
-        return (int64_t)r * ntiles + (u - base(r)) / nsb(r);
-    }
-};
-
-__device__ inline SkXcd sk_xcd(const SkPlan &p, int x, int ntiles, int B)
+__device__ __forceinline__ int next_item(int *counter, int *sh)
 {
-    SkXcd X;
-    X.p = p;
-    X.s0 = (int64_t)x * p.R;
-    X.b0 = p.sb(X.s0);
-    X.ntiles = ntiles;
-    X.U = (int64_t)ntiles * (p.sb(X.s0 + p.R) - X.b0);
-    X.B = B;
-    return X;
+    if (threadIdx.x == 0) *sh = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = *sh;
+    __syncthreads();
+    return it;
 }
 
-// Run lists: one thread per evaluation workgroup b writes its runs
-// (tile, first batch, batch count) in range order; run k of b owns piece
-// slot b * kmax + k.
-__global__ void sk_plan_kernel(const long long *nseg_p, int ntiles, int grid, int kmax, int4 *runs,
-                               int *nruns)
-{
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= grid) return;
-    const SkPlan p = sk_plan(*nseg_p);
-    const int B = grid / NX;
-    const SkXcd X = sk_xcd(p, b % NX, ntiles, B);
-    int k = 0;
-    if (p.nb > 0 && X.U > 0) {
-        const int64_t j = b / NX;
-        int64_t u = X.lo(j);
-        const int64_t u1 = X.lo(j + 1);
-        int r = 0;
-        while (u < u1 && k < kmax) {
-            r = X.round_of(u, r);
-            const int64_t nsb = X.nsb(r), off = u - X.base(r);
-            const int64_t tile = off / nsb;
-            const int64_t ue = min(u1, X.base(r) + (tile + 1) * nsb);
-            runs[(int64_t)b * kmax + k] =
-                make_int4((int)tile, (int)(p.sb(X.s0 + r) + (off - tile * nsb)), (int)(ue - u), 0);
-            ++k;
-            u = ue;
-        }
-    }
-    nruns[b] = k;
-}
-
-// Fix-up: one workgroup per tile, one target per lane; the tile's pieces
-// in the fixed order (XCD, round, workgroup), added after the given
-// values when accumulating.  With no segment at all the result is zero
-// (or the given values).
-__global__ __launch_bounds__(SK_TILE) void sk_fix_kernel(const long long *nseg_p, const double *piece,
-                                                         int kmax, int ntiles, int grid,
-                                                         int64_t m, double *out, int64_t ostride,
-                                                         int acc)
-{
-    const int64_t t = blockIdx.x;
-    const int64_t e = t * SK_TILE + threadIdx.x;
-    if (e >= m) return;
-    const SkPlan p = sk_plan(*nseg_p);
-    const int B = grid / NX;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    if (acc) {
-        s0 = out[e];
-        s1 = out[ostride + e];
-        s2 = out[2 * ostride + e];
-    }
-    for (int x = 0; x < NX && p.nb > 0; ++x) {
-        const SkXcd X = sk_xcd(p, x, ntiles, B);
-        if (X.U == 0) continue;
-        for (int r = 0; r < p.R; ++r) {
-            const int64_t nsb = X.nsb(r);
-            if (nsb == 0) continue;
-            const int64_t us = X.base(r) + t * nsb, ue = us + nsb;
-            // the non-empty workgroups whose ranges meet [us, ue), in order
-            for (int64_t j = X.jof(us);;) {
-                const int64_t k = (int64_t)r * ntiles + t - X.pair_of(X.lo(j));
-                const double *q = piece + ((j * NX + x) * (int64_t)kmax + k) * 3 * SK_TILE +
-                                  threadIdx.x;
-                s0 += q[0];
-                s1 += q[SK_TILE];
-                s2 += q[2 * SK_TILE];
-                const int64_t next = X.lo(j + 1);
-                if (next >= ue) break;
-                j = X.jof(next);
-            }
-        }
-    }
-    out[e] = s0;
-    out[ostride + e] = s1;
-    out[2 * ostride + e] = s2;
-}
-
-// __launch_bounds__'s second argument asks for 8 waves per SIMD (64
-// VGPRs): the pair loop fits, and only loop-invariant addresses of the run
-// loop are spilled.
 template <int TILE>
-__global__ __launch_bounds__(TILE, 8) void bs_fp64_kernel(SkArgs a)
+__global__ __launch_bounds__(TILE) void bs_fp64_kernel(BsArgs a)
 {
-    static_assert(TILE == SK_TILE && SK_BATCH == SK_TILE, "one record staged per lane");
-    __shared__ double4 sa[SK_BATCH];
-    __shared__ double4 sb[SK_BATCH];
-    __shared__ double sn[SK_BATCH];
+    __shared__ double4 sa[TILE];
+    __shared__ double4 sb[TILE];
+    __shared__ double sn[TILE];
+    __shared__ int item_sh;
     const int tid = threadIdx.x;
     const int64_t nseg = *a.nseg;
     const int64_t S = a.sstride;
-    const int nr = a.nruns[blockIdx.x];
-    for (int k = 0; k < nr; ++k) {
-        const int4 run = a.runs[(int64_t)blockIdx.x * a.kmax + k];
-        const int64_t e = (int64_t)run.x * SK_TILE + tid;
-        const bool live = e < a.m;
+    for (;;) {
+        const int item = next_item(a.counter, &item_sh);
+        if (item >= a.nitems) break;
+        const int chunk = item / a.ntiles;
+        const int tile = item - chunk * a.ntiles;
+        const int64_t s0 = (int64_t)chunk * a.chunk_len;
+        const int64_t s1 = min(nseg, s0 + a.chunk_len);
+        const int64_t t = (int64_t)tile * TILE + tid;
+        const bool live = t < a.m;
         double px = 0.0, py = 0.0, pz = 0.0;
         if (live) {
-            px = a.t[e];
-            py = a.t[a.tstride + e];
-            pz = a.t[2 * a.tstride + e];
+            px = a.t[t];
+            py = a.t[a.tstride + t];
+            pz = a.t[2 * a.tstride + t];
         }
         double vx = 0.0, vy = 0.0, vz = 0.0;
-        for (int ib = 0; ib < run.z; ++ib) {
-            const int64_t base = (int64_t)(run.y + ib) * SK_BATCH;
-            const int cnt = (int)min<int64_t>(SK_BATCH, nseg - base);
+        for (int64_t base = s0; base < s1; base += TILE) {
+            const int cnt = (int)min<int64_t>(TILE, s1 - base);
             __syncthreads();
             if (tid < cnt) {
                 const int64_t s = base + tid;
@@ -354,12 +193,14 @@ __global__ __launch_bounds__(TILE, 8) void bs_fp64_kernel(SkArgs a)
             }
             __syncthreads();
 #pragma unroll 2
-            for (int q = 0; q < cnt; ++q) pair64(px, py, pz, sa[q], sb[q], sn[q], vx, vy, vz);
+            for (int k = 0; k < cnt; ++k) pair64(px, py, pz, sa[k], sb[k], sn[k], vx, vy, vz);
         }
-        double *o = a.piece + (((int64_t)blockIdx.x * a.kmax + k) * 3) * SK_TILE + tid;
-        o[0] = vx;
-        o[SK_TILE] = vy;
-        o[2 * SK_TILE] = vz;
+        if (live) {
+            double *o = a.part + (size_t)chunk * 3 * a.m;
+            o[t] = vx;
+            o[a.m + t] = vy;
+            o[2 * a.m + t] = vz;
+        }
     }
 }
 
@@ -410,40 +251,45 @@ __device__ __forceinline__ void pair32_far(float phx, float phy, float phz, floa
 // q = rc^2 |r0|^2 below 1e-18.  A wave whose targets all lie outside that
 // radius runs the group with pair32_far; otherwise it runs pair32, which
 // flags near-singular pairs in a 32-bit mask, and recomputes the flagged
-// ones in FP64 after the group.  Staged groups are padded with zero
-// records, which contribute exactly zero and are never near (r0 = 0).
-// 7 waves per SIMD (72 VGPRs): both pair loops fit; a few per-batch
-// addresses are spilled.
+// ones in FP64 after the group (records read from the FP64 list in global
+// memory), so the FP32 loops stay free of the FP64 branch.  Staged groups
+// are padded with zero records, which contribute exactly zero and are
+// never near (r0 = 0).  __launch_bounds__'s second argument asks for 7
+// waves per SIMD (72 VGPRs), which both pair loops fit.
 template <int TILE>
-__global__ __launch_bounds__(TILE, 7) void bs_mixed_kernel(SkArgs a)
+__global__ __launch_bounds__(TILE, 7) void bs_mixed_kernel(BsArgs a)
 {
-    static_assert(TILE == SK_TILE && SK_BATCH == SK_TILE && SK_BATCH % 32 == 0, "groups of 32");
-    __shared__ float4 s0v[SK_BATCH];
-    __shared__ float4 s1v[SK_BATCH];
-    __shared__ float4 s2v[SK_BATCH];
-    __shared__ float4 far_sh[SK_BATCH / 32];  // centre, squared far radius
+    static_assert(TILE % 32 == 0, "groups of 32 segments");
+    __shared__ float4 s0v[TILE];
+    __shared__ float4 s1v[TILE];
+    __shared__ float4 s2v[TILE];
+    __shared__ float4 far_sh[TILE / 32];  // centre, squared far radius
+    __shared__ int item_sh;
     const int tid = threadIdx.x;
     const int64_t nseg = *a.nseg;
     const int64_t S = a.sstride;
-    const int nr = a.nruns[blockIdx.x];
-    for (int k = 0; k < nr; ++k) {
-        const int4 run = a.runs[(int64_t)blockIdx.x * a.kmax + k];
-        const int64_t e = (int64_t)run.x * SK_TILE + tid;
-        const bool live = e < a.m;
+    for (;;) {
+        const int item = next_item(a.counter, &item_sh);
+        if (item >= a.nitems) break;
+        const int chunk = item / a.ntiles;
+        const int tile = item - chunk * a.ntiles;
+        const int64_t sb0 = (int64_t)chunk * a.chunk_len;
+        const int64_t sb1 = min(nseg, sb0 + a.chunk_len);
+        const int64_t t = (int64_t)tile * TILE + tid;
+        const bool live = t < a.m;
         double px = 0.0, py = 0.0, pz = 0.0;
         if (live) {
-            px = a.t[e];
-            py = a.t[a.tstride + e];
-            pz = a.t[2 * a.tstride + e];
+            px = a.t[t];
+            py = a.t[a.tstride + t];
+            pz = a.t[2 * a.tstride + t];
         }
         const float phx = (float)px, phy = (float)py, phz = (float)pz;
         const float plx = (float)(px - (double)phx), ply = (float)(py - (double)phy),
                     plz = (float)(pz - (double)phz);
         float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
         double nx = 0.0, ny = 0.0, nz = 0.0;
-        for (int ib = 0; ib < run.z; ++ib) {
-            const int64_t base = (int64_t)(run.y + ib) * SK_BATCH;
-            const int cnt = (int)min<int64_t>(SK_BATCH, nseg - base);
+        for (int64_t base = sb0; base < sb1; base += TILE) {
+            const int cnt = (int)min<int64_t>(TILE, sb1 - base);
             const int cnt32 = (cnt + 31) & ~31;
             __syncthreads();
             {
@@ -476,29 +322,29 @@ __global__ __launch_bounds__(TILE, 7) void bs_mixed_kernel(SkArgs a)
             }
             __syncthreads();
             float tx = 0.f, ty = 0.f, tz = 0.f;
-            for (int g32 = 0; g32 < cnt32; g32 += 32) {
-                const float4 f = far_sh[g32 >> 5];
+            for (int g = 0; g < cnt32; g += 32) {
+                const float4 f = far_sh[g >> 5];
                 const float qx = phx - f.x, qy = phy - f.y, qz = phz - f.z;
                 const bool far = !live || (qx * qx + qy * qy + qz * qz > f.w);
                 if (__builtin_amdgcn_readfirstlane(__all(far))) {
 #pragma unroll 4
-                    for (int jj = 0; jj < 32; ++jj)
-                        pair32_far(phx, phy, phz, plx, ply, plz, s0v[g32 + jj], s1v[g32 + jj],
-                                   s2v[g32 + jj], tx, ty, tz);
+                    for (int j = 0; j < 32; ++j)
+                        pair32_far(phx, phy, phz, plx, ply, plz, s0v[g + j], s1v[g + j],
+                                   s2v[g + j], tx, ty, tz);
                     continue;
                 }
                 unsigned mask = 0u;
 #pragma unroll 4
-                for (int jj = 0; jj < 32; ++jj) {
-                    const bool nearp = pair32(phx, phy, phz, plx, ply, plz, s0v[g32 + jj],
-                                              s1v[g32 + jj], s2v[g32 + jj], tx, ty, tz);
-                    mask |= (nearp ? 1u : 0u) << jj;
+                for (int j = 0; j < 32; ++j) {
+                    const bool nearp = pair32(phx, phy, phz, plx, ply, plz, s0v[g + j],
+                                              s1v[g + j], s2v[g + j], tx, ty, tz);
+                    mask |= (nearp ? 1u : 0u) << j;
                 }
                 if (!live) mask = 0u;
                 while (mask) {
-                    const int jj = __builtin_ctz(mask);
+                    const int j = __builtin_ctz(mask);
                     mask &= mask - 1u;
-                    const int64_t s = base + g32 + jj;
+                    const int64_t s = base + g + j;
                     pair64(px, py, pz,
                            make_double4(a.s64[s], a.s64[S + s], a.s64[2 * S + s], a.s64[3 * S + s]),
                            make_double4(a.s64[4 * S + s], a.s64[5 * S + s], a.s64[6 * S + s],
@@ -510,10 +356,12 @@ __global__ __launch_bounds__(TILE, 7) void bs_mixed_kernel(SkArgs a)
             kahan_add(Sy, Cy, ty);
             kahan_add(Sz, Cz, tz);
         }
-        double *o = a.piece + (((int64_t)blockIdx.x * a.kmax + k) * 3) * SK_TILE + tid;
-        o[0] = ((double)Sx - (double)Cx) + nx;
-        o[SK_TILE] = ((double)Sy - (double)Cy) + ny;
-        o[2 * SK_TILE] = ((double)Sz - (double)Cz) + nz;
+        if (live) {
+            double *o = a.part + (size_t)chunk * 3 * a.m;
+            o[t] = ((double)Sx - (double)Cx) + nx;
+            o[a.m + t] = ((double)Sy - (double)Cy) + ny;
+            o[2 * a.m + t] = ((double)Sz - (double)Cz) + nz;
+        }
     }
 }
 
@@ -609,14 +457,7 @@ __global__ __launch_bounds__(RED_BLOCK) void bs_reduce_kernel(const double *part
     out[d * ostride + t] = s;
 }
 
-__global__ void zero_soa3_kernel(double *o, int64_t m, int64_t stride)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    o[i] = 0.0;
-    o[stride + i] = 0.0;
-    o[2 * stride + i] = 0.0;
-}
+__global__ void zero_int_kernel(int *p) { *p = 0; }
 
 template <typename K>
 int resident_blocks(vlm_ctx *ctx, K kernel, int block, int *cache)
@@ -681,62 +522,64 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
         return VLM_OK;
     }
     const int blk = BS_BLOCK;
-    const int ntiles = (int)((m + SK_TILE - 1) / SK_TILE);
+    const int ntiles = (int)((m + blk - 1) / blk);
     const bool mixed = ctx->precision == VLM_MIXED;
     int cap = mixed ? resident_blocks(ctx, bs_mixed_kernel<BS_BLOCK>, blk, &ctx->occ[1])
                     : resident_blocks(ctx, bs_fp64_kernel<BS_BLOCK>, blk, &ctx->occ[3]);
     if (cap <= 0) return fail(ctx, VLM_ERR_HIP, "occupancy query failed");
-    if (nseg_max <= 0) {
-        // no source segment can exist: the sum is empty
-        if (!accumulate) {
-            hipLaunchKernelGGL(zero_soa3_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
-                               ctx->stream, out, m, tstride);
-            VLM_HIP(hipGetLastError());
+    int nchunks = 0;
+    int64_t chunk_len = 1;
+    if (nseg_max > 0) {
+        // about 16 work items per resident workgroup, chunks of at least
+        // four tiles
+        const int64_t min_len = 4 * BS_BLOCK;
+        int64_t nc = ((int64_t)16 * cap + ntiles - 1) / ntiles;
+        const int64_t maxc = (nseg_max + min_len - 1) / min_len;
+        if (nc > maxc) nc = maxc;
+        if (nc < 1) nc = 1;
+        chunk_len = (nseg_max + nc - 1) / nc;
+        nchunks = (int)((nseg_max + chunk_len - 1) / chunk_len);
+    }
+    if (nchunks > 0) {
+        VLM_HIP(ctx->part.reserve((size_t)nchunks * 3 * (size_t)m));
+        VLM_HIP(ctx->counter.reserve(1));
+        BsArgs a;
+        a.s64 = seg64;
+        a.s32 = seg32;
+        a.sstride = segstride;
+        a.nseg = nseg_dev;
+        a.chunk_len = chunk_len;
+        a.t = t;
+        a.m = m;
+        a.tstride = tstride;
+        a.part = ctx->part.p;
+        a.ntiles = ntiles;
+        a.nitems = ntiles * nchunks;
+        a.counter = ctx->counter.p;
+        const int grid = a.nitems < cap ? a.nitems : cap;
+        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->counter.p);
+        Timer tm{};
+        const bool timed = ctx->profile;
+        if (timed) {
+            VLM_HIP(hipEventCreate(&tm.a));
+            VLM_HIP(hipEventCreate(&tm.b));
+            VLM_HIP(hipEventRecord(tm.a, ctx->stream));
         }
-        return VLM_OK;
+        if (mixed)
+            hipLaunchKernelGGL(bs_mixed_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0, ctx->stream,
+                               a);
+        else
+            hipLaunchKernelGGL(bs_fp64_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0, ctx->stream,
+                               a);
+        VLM_HIP(hipGetLastError());
+        if (timed) {
+            VLM_HIP(hipEventRecord(tm.b, ctx->stream));
+            VLM_TRY(profile_count(ctx, nseg_dev, m, 0, &tm));
+        }
     }
-    // the resident grid, a multiple of the XCD count
-    const int grid = cap >= NX ? cap - cap % NX : NX;
-    const int B = grid / NX;
-    const SkPlan plan = sk_plan(nseg_max);   // bounds the device count's plan
-    const int kmax = sk_kmax(ntiles, plan.R, B);
-    VLM_HIP(ctx->part.reserve((size_t)grid * kmax * 3 * SK_TILE));
-    VLM_HIP(ctx->skruns.reserve((size_t)grid * kmax));
-    VLM_HIP(ctx->counter.reserve((size_t)grid));
-    hipLaunchKernelGGL(sk_plan_kernel, dim3((unsigned)((grid + 255) / 256)), dim3(256), 0,
-                       ctx->stream, nseg_dev, ntiles, grid, kmax, ctx->skruns.p,
-                       ctx->counter.p);
-    VLM_HIP(hipGetLastError());
-    SkArgs a;
-    a.s64 = seg64;
-    a.s32 = seg32;
-    a.sstride = segstride;
-    a.nseg = nseg_dev;
-    a.t = t;
-    a.m = m;
-    a.tstride = tstride;
-    a.piece = ctx->part.p;
-    a.runs = ctx->skruns.p;
-    a.nruns = ctx->counter.p;
-    a.kmax = kmax;
-    Timer tm{};
-    const bool timed = ctx->profile;
-    if (timed) {
-        VLM_HIP(hipEventCreate(&tm.a));
-        VLM_HIP(hipEventCreate(&tm.b));
-        VLM_HIP(hipEventRecord(tm.a, ctx->stream));
-    }
-    if (mixed)
-        hipLaunchKernelGGL(bs_mixed_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0, ctx->stream, a);
-    else
-        hipLaunchKernelGGL(bs_fp64_kernel<BS_BLOCK>, dim3(grid), dim3(blk), 0, ctx->stream, a);
-    VLM_HIP(hipGetLastError());
-    if (timed) {
-        VLM_HIP(hipEventRecord(tm.b, ctx->stream));
-        VLM_TRY(profile_count(ctx, nseg_dev, m, 0, &tm));
-    }
-    hipLaunchKernelGGL(sk_fix_kernel, dim3((unsigned)ntiles), dim3(SK_TILE), 0, ctx->stream,
-                       nseg_dev, ctx->part.p, kmax, ntiles, grid, m, out, tstride, accumulate);
+    hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + RED_BLOCK - 1) / RED_BLOCK), 3),
+                       dim3(RED_BLOCK), 0,
+                       ctx->stream, ctx->part.p, nchunks, m, out, tstride, accumulate);
     VLM_HIP(hipGetLastError());
     return VLM_OK;
 }
