@@ -496,28 +496,38 @@ int profile_count(vlm_ctx *ctx, const long long *nseg_dev, int64_t m, int kind, 
 
 int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segstride,
             const long long *nseg_dev, int64_t nseg_max, const double *t, int64_t m,
-            int64_t tstride, double *out, int accumulate)
+            int64_t tstride, double *out, int accumulate, const BsWork *ws)
 {
     if (m <= 0) return VLM_OK;
+    // chunk partial sums: the context's growing buffer, or a fixed
+    // workspace that caps the chunk count
+    const int64_t max_chunks = ws ? (int64_t)(ws->part_cap / (3 * (size_t)m)) : INT32_MAX;
+    if (ws && max_chunks < 1) return fail(ctx, VLM_ERR_ARG, "workspace too small for the targets");
+    const bool prof = ctx->profile && !ws;
     if (m < 1024) {
         // about 2048 blocks, at least 8 segments per lane
         const int64_t groups = (m + SM_T - 1) / SM_T;
         int64_t nc = (2048 + groups - 1) / groups;
         const int64_t maxc = nseg_max / (8 * SM_BLOCK);
         if (nc > maxc) nc = maxc;
+        if (nc > max_chunks) nc = max_chunks;
         if (nc < 1) nc = 1;
         const int nchunks = nseg_max > 0 ? (int)nc : 0;
+        double *part = ws ? ws->part : nullptr;
         if (nchunks > 0) {
-            VLM_HIP(ctx->part.reserve((size_t)nchunks * 3 * (size_t)m));
+            if (!ws) {
+                VLM_HIP(ctx->part.reserve((size_t)nchunks * 3 * (size_t)m));
+                part = ctx->part.p;
+            }
             hipLaunchKernelGGL(bs_small_kernel, dim3(nchunks, (unsigned)groups), dim3(SM_BLOCK),
                                0, ctx->stream, seg64, segstride, nseg_dev, nchunks, t, m, tstride,
-                               ctx->part.p);
+                               part);
             VLM_HIP(hipGetLastError());
-            if (ctx->profile) VLM_TRY(profile_count(ctx, nseg_dev, m, 1, nullptr));
+            if (prof) VLM_TRY(profile_count(ctx, nseg_dev, m, 1, nullptr));
         }
         hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + RED_BLOCK - 1) / RED_BLOCK), 3),
                            dim3(RED_BLOCK), 0,
-                           ctx->stream, ctx->part.p, nchunks, m, out, tstride, accumulate);
+                           ctx->stream, part, nchunks, m, out, tstride, accumulate);
         VLM_HIP(hipGetLastError());
         return VLM_OK;
     }
@@ -536,13 +546,20 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
         int64_t nc = ((int64_t)16 * cap + ntiles - 1) / ntiles;
         const int64_t maxc = (nseg_max + min_len - 1) / min_len;
         if (nc > maxc) nc = maxc;
+        if (nc > max_chunks) nc = max_chunks;
         if (nc < 1) nc = 1;
         chunk_len = (nseg_max + nc - 1) / nc;
         nchunks = (int)((nseg_max + chunk_len - 1) / chunk_len);
     }
+    double *part = ws ? ws->part : nullptr;
     if (nchunks > 0) {
-        VLM_HIP(ctx->part.reserve((size_t)nchunks * 3 * (size_t)m));
-        VLM_HIP(ctx->counter.reserve(1));
+        int *counter = ws ? ws->counter : nullptr;
+        if (!ws) {
+            VLM_HIP(ctx->part.reserve((size_t)nchunks * 3 * (size_t)m));
+            VLM_HIP(ctx->counter.reserve(1));
+            part = ctx->part.p;
+            counter = ctx->counter.p;
+        }
         BsArgs a;
         a.s64 = seg64;
         a.s32 = seg32;
@@ -552,14 +569,14 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
         a.t = t;
         a.m = m;
         a.tstride = tstride;
-        a.part = ctx->part.p;
+        a.part = part;
         a.ntiles = ntiles;
         a.nitems = ntiles * nchunks;
-        a.counter = ctx->counter.p;
+        a.counter = counter;
         const int grid = a.nitems < cap ? a.nitems : cap;
-        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, ctx->stream, ctx->counter.p);
+        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, ctx->stream, counter);
         Timer tm{};
-        const bool timed = ctx->profile;
+        const bool timed = prof;
         if (timed) {
             VLM_HIP(hipEventCreate(&tm.a));
             VLM_HIP(hipEventCreate(&tm.b));
@@ -579,7 +596,7 @@ int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32, int64_t segst
     }
     hipLaunchKernelGGL(bs_reduce_kernel, dim3((unsigned)((m + RED_BLOCK - 1) / RED_BLOCK), 3),
                        dim3(RED_BLOCK), 0,
-                       ctx->stream, ctx->part.p, nchunks, m, out, tstride, accumulate);
+                       ctx->stream, part, nchunks, m, out, tstride, accumulate);
     VLM_HIP(hipGetLastError());
     return VLM_OK;
 }

@@ -32,34 +32,7 @@ __global__ void compact_kernel(const Slot *slots, const int *flags, const int *i
     if (i >= n) return;
     if (i == n - 1) *count = (long long)idx[i] + (long long)flags[i];
     if (!flags[i]) return;
-    const int64_t k = idx[i];
-    const Slot s = slots[i];
-    const double g = s.gam * kInv4Pi;
-    const double r0x = s.bx - s.ax, r0y = s.by - s.ay, r0z = s.bz - s.az;
-    const double n0 = r0x * r0x + r0y * r0y + r0z * r0z;
-    const double q = s.rc * s.rc * n0;
-    d64[k] = s.ax;
-    d64[cap + k] = s.ay;
-    d64[2 * cap + k] = s.az;
-    d64[3 * cap + k] = g;
-    d64[4 * cap + k] = r0x;
-    d64[5 * cap + k] = r0y;
-    d64[6 * cap + k] = r0z;
-    d64[7 * cap + k] = q * q;
-    d64[8 * cap + k] = n0;
-    const float hx = (float)s.ax, hy = (float)s.ay, hz = (float)s.az;
-    d32[k] = hx;
-    d32[cap + k] = hy;
-    d32[2 * cap + k] = hz;
-    d32[3 * cap + k] = (float)g;
-    d32[4 * cap + k] = (float)(s.ax - (double)hx);
-    d32[5 * cap + k] = (float)(s.ay - (double)hy);
-    d32[6 * cap + k] = (float)(s.az - (double)hz);
-    d32[7 * cap + k] = (float)q;
-    d32[8 * cap + k] = (float)r0x;
-    d32[9 * cap + k] = (float)r0y;
-    d32[10 * cap + k] = (float)r0z;
-    d32[11 * cap + k] = (float)(4.0 * n0);
+    write_segment_record(slots[i], idx[i], d64, d32, cap);
 }
 
 __global__ void zero_count_kernel(long long *count) { *count = 0; }

@@ -62,6 +62,7 @@ struct Slot {
     double ax, ay, az, bx, by, bz, gam, rc;
 };
 
+
 // velocity arrays per rotor: the four of include/vlm_c.h, then two more
 // histories used only by AM3 and AM4
 constexpr int VEL_H2 = 4, VEL_H3 = 5, NVEL = 6;
@@ -215,14 +216,56 @@ constexpr double kPi = 3.141592653589793238462643383279502884;
 constexpr double kInv4Pi = 0.25 / kPi;
 constexpr double kEps = 2.220446049250313080847263336181640625e-16;  // epsilon(1._dp)
 
+// The FP64 and FP32 records of one segment at position k of a compacted
+// list of stride cap (layouts above).
+__device__ __forceinline__ void write_segment_record(const Slot &s, int64_t k, double *d64,
+                                                     float *d32, int64_t cap)
+{
+    const double g = s.gam * kInv4Pi;
+    const double r0x = s.bx - s.ax, r0y = s.by - s.ay, r0z = s.bz - s.az;
+    const double n0 = r0x * r0x + r0y * r0y + r0z * r0z;
+    const double q = s.rc * s.rc * n0;
+    d64[k] = s.ax;
+    d64[cap + k] = s.ay;
+    d64[2 * cap + k] = s.az;
+    d64[3 * cap + k] = g;
+    d64[4 * cap + k] = r0x;
+    d64[5 * cap + k] = r0y;
+    d64[6 * cap + k] = r0z;
+    d64[7 * cap + k] = q * q;
+    d64[8 * cap + k] = n0;
+    const float hx = (float)s.ax, hy = (float)s.ay, hz = (float)s.az;
+    d32[k] = hx;
+    d32[cap + k] = hy;
+    d32[2 * cap + k] = hz;
+    d32[3 * cap + k] = (float)g;
+    d32[4 * cap + k] = (float)(s.ax - (double)hx);
+    d32[5 * cap + k] = (float)(s.ay - (double)hy);
+    d32[6 * cap + k] = (float)(s.az - (double)hz);
+    d32[7 * cap + k] = (float)q;
+    d32[8 * cap + k] = (float)r0x;
+    d32[9 * cap + k] = (float)r0y;
+    d32[10 * cap + k] = (float)r0z;
+    d32[11 * cap + k] = (float)(4.0 * n0);
+}
+
 // Biot-Savart evaluation of a compacted segment list at n targets stored
 // as SoA (x at t[0..], y at t[tstride..], z at t[2*tstride..]).  Result
 // goes to out (SoA, same stride); accumulate adds into it.  nseg_max is an
 // upper bound of the device count used to size the launch.
+// A fixed workspace for bs_eval instead of the context's growing buffers
+// (no allocation): room for part_cap doubles of chunk partial sums (at
+// least 3 m, the chunk count is capped by it) and one int counter.  Such
+// launches are not profiled.
+struct BsWork {
+    double *part;
+    size_t part_cap;
+    int *counter;
+};
 int bs_eval(vlm_ctx *ctx, const double *seg64, const float *seg32,
             int64_t segstride, const long long *nseg_dev, int64_t nseg_max,
             const double *t, int64_t m, int64_t tstride, double *out,
-            int accumulate);
+            int accumulate, const BsWork *ws = nullptr);
 
 // Compact staged slots (flags != 0) into ctx->seg.  nslots on host.
 int compact_slots(vlm_ctx *ctx, int64_t nslots);
