@@ -96,6 +96,21 @@ int vlm_synchronize(vlm_ctx *ctx);
 int vlm_filament_vind(vlm_ctx *ctx, int64_t nfil, const double *a,
                       const double *b, const double *gam, const double *rc,
                       int64_t m, const double *p, double *v, int accumulate);
+/* The same sum on device-resident inputs and output, allocating nothing.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  a, b, p, v are device pointers to 3 x n arrays (as
+ * above), gam and rc device arrays of nfil.  The call only enqueues work on
+ * the context's stream; v holds the result once the stream reaches it
+ * (vlm_synchronize).  All intermediate data lives in a 64 MiB device array
+ * of the library, one per GPU, held until the process ends: about 120 B per
+ * filament plus 72 B per point plus 24 B per point for each chunk of
+ * partial sums must fit, else VLM_ERR_ARG with a message.  Calls that
+ * overlap on one GPU share that array and race: serialise them. */
+int vlm_filament_vind_noalloc(vlm_ctx *ctx, int64_t nfil, const double *a,
+                              const double *b, const double *gam,
+                              const double *rc, int64_t m, const double *p,
+                              double *v, int accumulate);
 
 /* Device-resident variant: upload a filament set and a point set once,
  * evaluate repeatedly, download the velocities. */

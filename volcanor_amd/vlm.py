@@ -49,6 +49,7 @@ _SIGS = {
     "vlm_set_bug_compat": [_P, _I],
     "vlm_synchronize": [_P],
     "vlm_filament_vind": [_P, _L, _P, _P, _P, _P, _L, _P, _P, _I],
+    "vlm_filament_vind_noalloc": [_P, _L, _P, _P, _P, _P, _L, _P, _P, _I],
     "vlm_fset_create": [_P, _L, _P, _P, _P, _P, C.POINTER(_I)],
     "vlm_pset_create": [_P, _L, _P, C.POINTER(_I)],
     "vlm_fset_vind": [_P, _I, _I],
