@@ -270,7 +270,8 @@ def bench_rotor(args):
     for name, prec, learned in plan:
         t0 = time.perf_counter()
         ct, st = host.run_case(case, precision=prec, nt=nt, tail_steps=K, learned=learned,
-                               tol=args.tol, nrotors=nrot, fixed_ploop=args.fixed_ploop)
+                               tol=args.tol, nrotors=nrot, fixed_ploop=args.fixed_ploop,
+                               verbose=args.progress)
         runs[name] = dict(ct=ct, st=st, wall=time.perf_counter() - t0)
         print(f"# {name}: run {runs[name]['wall']:.2f} s, timed {st.wall_steps_tail:.3f} s for "
               f"{K} steps, BS kernel {st.bs_kernel_ms:.1f} ms over {st.bs_launches} launches, "
@@ -467,6 +468,8 @@ def main():
                     help="judge tolerance of the learned pass: RMS error / RMS speed on the "
                          "judge sample above which the pass is recomputed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--progress", type=int, default=0,
+                    help="print a line to stdout every N steps of each run (0: none)")
     ap.add_argument("--no-twin", action="store_true",
                     help="skip the perturbed FP64 twin run (chaos floor of CT)")
     ap.add_argument("--cpu-threads", type=int, default=0)
