@@ -632,9 +632,6 @@ def test_cfg5_as_specified_one_revolution():
     assert np.max(np.abs(cm - ct) / scale) < 1e-3
 
 
-@pytest.mark.skip(reason="this run aborted in the host (SIGABRT, no GPU fault) with its message "
-                         "lost to output capture; the cause is not found, and the host now stops "
-                         "with an error for restarts combined with slowStart or wakeTipPlot")
 def test_restart_carries_tip_plot_ages():
     """A restart file carries the tip plot's azimuthal ages: tip_rotor (AM2,
     slow start, truncated far wake) run with restarts every 24 steps, and a
@@ -659,7 +656,9 @@ def test_restart_carries_tip_plot_ages():
         shutil.copy(os.path.join(dirs[0], "Restart", "restart00024.dat"),
                     os.path.join(dirs[1], "Restart"))
         cfg = os.path.join(dirs[1], "config.nml")
-        open(cfg, "w").write(open(cfg).read().replace("restartFromNt = 0", "restartFromNt = 24"))
+        text = open(cfg).read()
+        with open(cfg, "w") as fh:
+            fh.write(text.replace("restartFromNt = 0", "restartFromNt = 24"))
         resumed, _ = host.run_case(dirs[1])
         np.testing.assert_array_equal(resumed[:, 25:], full[:, 25:])
         for step in ("00036", "00048"):

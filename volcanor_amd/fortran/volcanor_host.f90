@@ -2385,11 +2385,6 @@ contains
     if (sw%slowStart < 0 .or. sw%slowStart > 3) &
       & call host_fail('ERROR: slowStart must be 0..3 (body dynamics are not supported)')
     if (sw%slowStart > 0 .and. sw%slowStartNt <= 0) call host_fail('ERROR: slowStartNt must be positive')
-    ! a restart run with both of these aborted in the host once, cause not
-    ! yet found; restarts are verified without them
-    if ((sw%restartWriteNt /= 0 .or. sw%restartFromNt /= 0) .and. &
-      & (sw%slowStart /= 0 .or. sw%wakeTipPlot /= 0)) &
-      & call host_fail('ERROR: restart files are not supported with slowStart or wakeTipPlot')
     if (sw%fdScheme < 0 .or. sw%fdScheme > 5) call host_fail('ERROR: fdScheme must be 0..5')
     if (sw%ntSub < 0 .or. sw%ntSubInit < 0) call host_fail('ERROR: ntSub and ntSubInit must be >= 0')
     if (sw%restartFromNt < 0 .or. sw%restartWriteNt < 0) &
