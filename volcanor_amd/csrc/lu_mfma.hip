@@ -6,16 +6,26 @@
 // the first row of largest magnitude.
 //
 // n <= LU_SMALL (the AIC of one rotor): one workgroup, unblocked.
-// Larger n: right-looking blocked LU with panels of NB columns.
-//   1. panel: one workgroup factorises A[k:n, k:k+NB] column by column,
-//      holding the panel in registers (LDS for the row exchange); the row
-//      swaps then reach the other columns through one composed permutation;
-//   2. U12 = L11^-1 A12 (unit lower triangular solve, one lane per column);
-//   3. A22 -= L21 U12 on the matrix cores: 64x64 output tiles, four waves
-//      of 2x2 v_mfma_f64_16x16x4f64 blocks, operands staged in LDS.
+// Larger n: right-looking blocked LU.  Panels are 32 columns wide where the
+// panel fits on chip (m <= 2048 rows), 16 columns above that.
+//   1. panel: one workgroup factorises A[k:n, k:k+jb] column by column,
+//      the panel in registers (and LDS rows for tall panels); rows never
+//      move during the panel, each lane tracks the logical position of its
+//      rows, and one barrier per column publishes every wave's pivot
+//      candidate with its row.  The panel's row swaps reach the other
+//      columns through one composed row-move list;
+//   2. U12 = L11^-1 A12 (unit lower triangular solve, jb lanes per column,
+//      after the panel's row moves);
+//   3. A22 -= L21 U12 on the matrix cores, K = jb: 64x64 output tiles, four
+//      waves of 2x2 v_mfma_f64_16x16x4f64 blocks, operands staged in LDS.
 //      Lane layout of the 16x16x4 FP64 MFMA (checked on gfx950): lane l
 //      supplies A[l%16][l/16] and B[l/16][l%16]; result register i of lane
-//      l is D[4i + l/16][l%16].
+//      l is D[4i + l/16][l%16].  The tile is computed transposed, so that
+//      each result register holds 16 consecutive rows of one column: the
+//      accumulators load A22 and store it back in 128-byte runs.
+#include <climits>
+#include <vector>
+
 #include "vlm_internal.h"
 
 namespace vlm {
@@ -24,13 +34,19 @@ namespace {
 constexpr int LU_THREADS = 1024;
 constexpr int LU_WAVES = LU_THREADS / 64;
 constexpr int LU_SMALL = 256;  // one-workgroup path up to this order
-constexpr int NB = 16;         // panel width of the blocked path
-constexpr int REG_THREADS = 512;  // workgroup of the register panel (256 VGPRs per lane)
-constexpr int REG_WAVES = REG_THREADS / 64;
-constexpr int RPT = 3;            // panel rows per lane in the register panel (m <= 1536)
-// Row-move list of one panel: [count, dst0, src0, dst1, src1, ...], at most
-// 2 NB moved rows (absolute row indices; the panel's swaps composed).
-constexpr int LSTRIDE = 1 + 4 * NB;
+constexpr int NB = 16;         // panel width of tall panels
+constexpr int NBMAX = 32;      // panel width where the panel fits on chip
+constexpr int PT = 512;        // workgroup of the on-chip panel
+constexpr int PW = PT / 64;
+constexpr int WIDE_ROWS = 2 * PT;  // 32-column panels up to this many rows (2 per lane)
+constexpr int MID_ROWS = 5 * PT;   // 16-column panels held in registers up to this many
+// Row-move list of one panel: [count, dst0, src0, dst1, src1, ..., k0, jb],
+// at most 2 jb moved rows (absolute row indices; the panel's swaps
+// composed), then the panel's first column and width.
+constexpr int LPAIRS = 2 * NBMAX;
+constexpr int LSTRIDE = 3 + 2 * LPAIRS;
+constexpr int LK0 = LSTRIDE - 2;
+constexpr int LJB = LSTRIDE - 1;
 
 // IDAMAX order: larger magnitude wins, ties go to the lower row.
 __device__ __forceinline__ void better(double &bv, int &bi, double v, int i)
@@ -97,176 +113,270 @@ __global__ __launch_bounds__(LU_THREADS) void lu_panel_kernel(int n, double *A, 
 }
 
 
-// Panel of the blocked path held on chip: lane t owns the physical panel
-// rows t + REG_THREADS*q, RPT of them in registers and RL more in LDS
-// (columns [k0, k0+jb), jb <= NB).  Rows never move during the panel:
-// pos is the logical row (after the swaps so far) of each owned row and
-// phys[] in LDS maps logical rows back, so a row swap is two index
-// updates.  Pivot search and rank-1 update touch no global memory; each row
-// is read once and written once, to its final logical position.  The
-// column loop is unrolled by template recursion so that every register
-// index is a constant.
-struct PanelCtx {
-    double *sv, *prow, *L;   // L: LDS rows, L[(q*NB + j)*REG_THREADS + tid]
-    int *si, *phys, *ipiv, *info;
-    int tid, lane, w, k0, m, jb;
-    bool dead;
-};
-
-template <int RL, int C>
-__device__ __forceinline__ void panel_column(double (&a)[RPT][NB], int (&pos)[RPT + RL],
-                                             PanelCtx &P)
+__device__ __forceinline__ double readlane_f64(double v, int lane)
 {
-    double *L = P.L + P.tid;
-    if (C < P.jb && !P.dead) {
-        // pivot: largest |a| among logical rows >= C, first logical row on ties
-        double bv = -1.0;
-        int bi = P.m;
-#pragma unroll
-        for (int q = 0; q < RPT; ++q)
-            if (pos[q] >= C && pos[q] < P.m) better(bv, bi, fabs(a[q][C]), pos[q]);
-#pragma unroll
-        for (int q = 0; q < RL; ++q)
-            if (pos[RPT + q] >= C && pos[RPT + q] < P.m)
-                better(bv, bi, fabs(L[(q * NB + C) * REG_THREADS]), pos[RPT + q]);
-        for (int off = 32; off > 0; off >>= 1)
-            better(bv, bi, __shfl_xor(bv, off), __shfl_xor(bi, off));
-        if (P.lane == 0) {
-            P.sv[P.w] = bv;
-            P.si[P.w] = bi;
-        }
-        __syncthreads();
-        bv = P.sv[0];
-        bi = P.si[0];
-        for (int q = 1; q < REG_WAVES; ++q) better(bv, bi, P.sv[q], P.si[q]);
-        const int lp = bi;                 // logical row of the pivot
-        const int pp = P.phys[lp], pc = P.phys[C];
-        if (P.tid == 0) P.ipiv[P.k0 + C] = P.k0 + lp;
-        if (bv == 0.0) {
-            P.dead = true;
-            if (P.tid == 0 && *P.info == 0) *P.info = P.k0 + C + 1;
-            for (int j = C + 1 + P.tid; j < P.jb; j += REG_THREADS) P.ipiv[P.k0 + j] = P.k0 + j;
-        } else {
-#pragma unroll
-            for (int q = 0; q < RPT + RL; ++q) {
-                const int r = P.tid + REG_THREADS * q;
-                if (r == pp) {
-#pragma unroll
-                    for (int j = 0; j < NB; ++j)
-                        P.prow[j] = q < RPT ? a[q < RPT ? q : 0][j]
-                                            : L[((q - RPT) * NB + j) * REG_THREADS];
-                    pos[q] = C;
-                } else if (r == pc) {
-                    pos[q] = lp;
-                }
-            }
-            __syncthreads();   // prow published; every lane has read phys[]
-            if (P.tid == 0) {
-                P.phys[C] = pp;
-                P.phys[lp] = pc;
-            }
-            const double inv = 1.0 / P.prow[C];
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-                if (pos[q] > C && pos[q] < P.m) {
-                    const double l = a[q][C] * inv;
-                    a[q][C] = l;
-#pragma unroll
-                    for (int j = C + 1; j < NB; ++j) a[q][j] -= l * P.prow[j];
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < RL; ++q) {
-                if (pos[RPT + q] > C && pos[RPT + q] < P.m) {
-                    double *row = L + (q * NB) * REG_THREADS;
-                    const double l = row[C * REG_THREADS] * inv;
-                    row[C * REG_THREADS] = l;
-#pragma unroll
-                    for (int j = C + 1; j < NB; ++j) row[j * REG_THREADS] -= l * P.prow[j];
-                }
-            }
-        }
-    }
-    if constexpr (C + 1 < NB) panel_column<RL, C + 1>(a, pos, P);
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
 }
 
-template <int RL>
-__global__ __launch_bounds__(REG_THREADS) void lu_panel_reg_kernel(int n, double *A, int *ipiv,
-                                                                   int *info, int k0, int jb,
-                                                                   int *list)
+// max(v, v of the lane that DPP control CTRL selects)
+template <int CTRL>
+__device__ __forceinline__ double dpp_max(double v)
 {
-    constexpr int ROWS = (RPT + RL) * REG_THREADS;
-    __shared__ double sv[REG_WAVES];
-    __shared__ int si[REG_WAVES];
-    __shared__ double prow[NB];
-    __shared__ int phys[ROWS];
-    __shared__ double Lrows[RL > 0 ? RL * NB * REG_THREADS : 1];
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return fmax(v, __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false),
+                                    __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false)));
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_min(int i)
+{
+    return min(i, __builtin_amdgcn_update_dpp(i, i, CTRL, 0xF, 0xF, false));
+}
+
+// Maximum over the wave (every lane active), returned to every lane.
+// Within each row of 16 lanes: xor 1, xor 2, half-row mirror, row mirror;
+// then the four rows through readlane.
+__device__ __forceinline__ double wave_max(double v)
+{
+    v = dpp_max<0xB1>(v);    // quad_perm [1,0,3,2]
+    v = dpp_max<0x4E>(v);    // quad_perm [2,3,0,1]
+    v = dpp_max<0x141>(v);   // row_half_mirror
+    v = dpp_max<0x140>(v);   // row_mirror
+    return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)),
+                fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+
+__device__ __forceinline__ int wave_min(int i)
+{
+    i = dpp_min<0xB1>(i);
+    i = dpp_min<0x4E>(i);
+    i = dpp_min<0x141>(i);
+    i = dpp_min<0x140>(i);
+    return min(min(__builtin_amdgcn_readlane(i, 0), __builtin_amdgcn_readlane(i, 16)),
+               min(__builtin_amdgcn_readlane(i, 32), __builtin_amdgcn_readlane(i, 48)));
+}
+
+// Panel held on chip: lane t owns the physical panel rows t + PT*q, RPT of
+// them in registers and RL more in LDS (columns [k0, k0+jb), jb <= JB).
+// Rows never move during the panel: pos[q] is the logical row (after the
+// swaps so far) of each owned row, so a row swap is two index updates.
+// Per column, each wave finds its candidate (wave argmax), the owning lane
+// copies the candidate row into the wave's LDS slot, and one barrier later
+// every lane picks the winning wave and reads the pivot row from its slot.
+// The slots alternate between two buffers by column parity, so a fast wave
+// writing the next column's slot never overwrites one still being read.
+// The column loop is unrolled by template recursion so that every register
+// index is a constant.
+template <int JB>
+struct FastCtx {
+    double *cand;   // [2][PW][JB]
+    double *ckv;    // [2][PW] candidate |value|
+    double *cinv;   // [2][PW] 1 / candidate value
+    int *cki;       // [2][PW] candidate logical row
+    double *L;      // LDS rows: L[(q*JB + j)*PT + tid]
+    int *ipiv, *info, *list;
+    int tid, lane, w, k0, m;
+};
+
+template <int JB, int RPT, int RL, int C>
+__device__ __forceinline__ void fast_column(double (&a)[RPT][JB], int (&pos)[RPT + RL],
+                                            FastCtx<JB> &P)
+{
+    constexpr int par = C & 1;
+    double *L = P.L + P.tid;
+    // this lane's candidate: largest |a| among its rows at logical >= C,
+    // the lowest logical row on ties (branch-free)
+    double bv = -1.0;
+    int bi = INT_MAX;
+#pragma unroll
+    for (int q = 0; q < RPT + RL; ++q) {
+        const int p = pos[q];
+        const double x = fabs(q < RPT ? a[q < RPT ? q : 0][C] : L[((q - RPT) * JB + C) * PT]);
+        const bool t = (p >= C) & (p < P.m) & ((x > bv) | ((x == bv) & (p < bi)));
+        bv = t ? x : bv;
+        bi = t ? p : bi;
+    }
+    // the wave's candidate: its maximum, and the lowest row holding it
+    const double M = wave_max(bv);
+    const unsigned long long hit = __ballot(bv == M);
+    const int wi = __popcll(hit) == 1 ? __builtin_amdgcn_readlane(bi, __ffsll((long long)hit) - 1)
+                                      : wave_min(bv == M ? bi : INT_MAX);
+    // its owner publishes the row (columns C..JB-1) and the reciprocal
+    if (bi == wi) {
+        double *crow = P.cand + (par * PW + P.w) * JB;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+            if (pos[q] == wi) {
+                P.cinv[par * PW + P.w] = 1.0 / a[q][C];
+#pragma unroll
+                for (int j = C; j < JB; ++j) crow[j] = a[q][j];
+            }
+#pragma unroll
+        for (int q = 0; q < RL; ++q)
+            if (pos[RPT + q] == wi) {
+                P.cinv[par * PW + P.w] = 1.0 / L[(q * JB + C) * PT];
+#pragma unroll
+                for (int j = C; j < JB; ++j) crow[j] = L[(q * JB + j) * PT];
+            }
+    }
+    if (P.lane == 0) {
+        P.ckv[par * PW + P.w] = M;
+        P.cki[par * PW + P.w] = wi;
+    }
+    __syncthreads();
+    // the block's pivot: largest wave maximum, lowest row among the waves
+    // holding it
+    double kv[PW];
+    int ki[PW];
+#pragma unroll
+    for (int w = 0; w < PW; ++w) {
+        kv[w] = P.ckv[par * PW + w];
+        ki[w] = P.cki[par * PW + w];
+    }
+    double gv = kv[0];
+#pragma unroll
+    for (int w = 1; w < PW; ++w) gv = fmax(gv, kv[w]);
+    int gi = INT_MAX, gw = 0;
+#pragma unroll
+    for (int w = 0; w < PW; ++w) {
+        const bool t = (kv[w] == gv) & (ki[w] < gi);
+        gi = t ? ki[w] : gi;
+        gw = t ? w : gw;
+    }
+    const int lp = gi;   // logical row of the pivot
+    if (P.tid == 0) {
+        P.ipiv[P.k0 + C] = P.k0 + lp;
+        // a zero column (every candidate 0: lp = C, no swap) is recorded
+        // and left as it is, as DGETF2 does; the factorisation reports it
+        if (gv == 0.0 && *P.info == 0) *P.info = P.k0 + C + 1;
+    }
+    const double *prow = P.cand + (par * PW + gw) * JB;
+    const double inv = gv == 0.0 ? 0.0 : P.cinv[par * PW + gw];
+#pragma unroll
+    for (int q = 0; q < RPT + RL; ++q) {
+        if (pos[q] == lp) pos[q] = C;
+        else if (pos[q] == C) pos[q] = lp;
+    }
+    double l[RPT + RL];
+#pragma unroll
+    for (int q = 0; q < RPT + RL; ++q) {
+        const bool upd = pos[q] > C && pos[q] < P.m;
+        if (q < RPT) {
+            const double x = a[q < RPT ? q : 0][C];
+            l[q] = upd ? x * inv : 0.0;
+            a[q < RPT ? q : 0][C] = upd ? l[q] : x;
+        } else {
+            double *e = L + ((q - RPT) * JB + C) * PT;
+            const double x = *e;
+            l[q] = upd ? x * inv : 0.0;
+            if (upd) *e = l[q];
+        }
+    }
+    // rank-1 update, CH pivot-row entries at a time (the scheduling barrier
+    // keeps later entries' loads from piling up in registers when the
+    // panel fills them)
+    constexpr int CH = RPT >= 3 ? 4 : 16;
+#pragma unroll
+    for (int j0 = C + 1; j0 < JB; j0 += CH) {
+        double u[CH];
+#pragma unroll
+        for (int t = 0; t < CH; ++t) u[t] = j0 + t < JB ? prow[j0 + t < JB ? j0 + t : 0] : 0.0;
+#pragma unroll
+        for (int t = 0; t < CH; ++t) {
+            if (j0 + t < JB) {
+                const int j = j0 + t < JB ? j0 + t : 0;
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) a[q][j] -= l[q] * u[t];
+#pragma unroll
+                for (int q = 0; q < RL; ++q)
+                    if (l[RPT + q] != 0.0) L[(q * JB + j) * PT] -= l[RPT + q] * u[t];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (C + 1 < JB) fast_column<JB, RPT, RL, C + 1>(a, pos, P);
+}
+
+// Exactly JB columns (the host gives ragged last panels to the tall kernel).
+template <int JB, int RPT, int RL>
+__global__ __launch_bounds__(PT) void lu_panel_fast_kernel(int n, double *A, int *ipiv, int *info,
+                                                           int k0, int *list)
+{
+    constexpr int S = RPT + RL;
+    constexpr int EMPTY = 1 << 30;   // logical position of a slot past the panel
+    __shared__ double cand[2 * PW * JB];
+    __shared__ double ckv[2 * PW];
+    __shared__ double cinv[2 * PW];
+    __shared__ int cki[2 * PW];
+    __shared__ double Lrows[RL > 0 ? RL * JB * PT : 1];
     __shared__ int moved;
     const int tid = threadIdx.x;
     const size_t ld = (size_t)n;
     const int m = n - k0;
+    if (tid == 0) {
+        list[LK0] = k0;
+        list[LJB] = JB;
+    }
     if (*info != 0) {
-        for (int k = tid; k < jb; k += REG_THREADS) ipiv[k0 + k] = k0 + k;
+        for (int k = tid; k < JB; k += PT) ipiv[k0 + k] = k0 + k;
         if (tid == 0) list[0] = 0;
         return;
     }
     if (tid == 0) moved = 0;
-    double a[RPT][NB];
-    int pos[RPT + RL];
+    double a[RPT][JB];
+    int pos[S];
 #pragma unroll
-    for (int q = 0; q < RPT + RL; ++q) {
-        const int r = tid + REG_THREADS * q;
-        pos[q] = r < m ? r : m;   // m marks an empty slot
-        phys[r] = r;
+    for (int q = 0; q < S; ++q) {
+        const int r = tid + PT * q;
+        pos[q] = r < m ? r : EMPTY;
     }
+    const double *col = A + ld * (size_t)k0 + k0;
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        const double *col = A + ld * (size_t)(k0 + (c < jb ? c : 0));  // uniform base
+    for (int c = 0; c < JB; ++c) {
 #pragma unroll
-        for (int q = 0; q < RPT + RL; ++q) {
-            const int r = tid + REG_THREADS * q;
-            const double v = (r < m && c < jb) ? col[k0 + r] : 0.0;
+        for (int q = 0; q < S; ++q) {
+            // unconditional load from a clamped row, then the select: no
+            // branch, so every load of the panel is in flight at once
+            const int r = tid + PT * q;
+            const double x = col[ld * c + (r < m ? r : m - 1)];
+            const double v = r < m ? x : 0.0;
             if (q < RPT) a[q < RPT ? q : 0][c] = v;
-            else Lrows[((q - RPT) * NB + c) * REG_THREADS + tid] = v;
+            else Lrows[((q - RPT) * JB + c) * PT + tid] = v;
         }
+        __builtin_amdgcn_sched_barrier(0);   // one column's addresses live at a time
     }
-    __syncthreads();
-    PanelCtx P{sv, prow, Lrows, si, phys, ipiv, info, tid, tid & 63, tid >> 6, k0, m, jb, false};
-    panel_column<RL, 0>(a, pos, P);
-    if (P.dead) {
-        if (tid == 0) list[0] = 0;
-        return;
-    }
-    __syncthreads();   // final phys[] visible
+    __syncthreads();   // moved = 0 visible
+    FastCtx<JB> P{cand, ckv, cinv, cki, Lrows, ipiv, info, list, tid, tid & 63, tid >> 6, k0, m};
+    fast_column<JB, RPT, RL, 0>(a, pos, P);
 #pragma unroll
-    for (int q = 0; q < RPT + RL; ++q) {
-        const int r = tid + REG_THREADS * q;
-        if (r < m && phys[r] != r) {
+    for (int q = 0; q < S; ++q) {
+        const int r = tid + PT * q;
+        if (r < m && pos[q] != r) {
             const int e = atomicAdd(&moved, 1);
-            if (e < 2 * NB) {
-                list[1 + 2 * e] = k0 + r;
-                list[2 + 2 * e] = k0 + phys[r];
+            if (e < LPAIRS) {
+                list[1 + 2 * e] = k0 + pos[q];   // destination (logical) row
+                list[2 + 2 * e] = k0 + r;        // source (physical) row
             }
         }
     }
     __syncthreads();
-    if (tid == 0) list[0] = moved < 2 * NB ? moved : 2 * NB;
+    if (tid == 0) list[0] = moved < LPAIRS ? moved : LPAIRS;
+    double *out = A + ld * (size_t)k0 + k0;
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        if (c < jb) {
-            double *col = A + ld * (size_t)(k0 + c);
+    for (int c = 0; c < JB; ++c) {
 #pragma unroll
-            for (int q = 0; q < RPT + RL; ++q)
-                if (pos[q] < m)
-                    col[k0 + pos[q]] = q < RPT ? a[q < RPT ? q : 0][c]
-                                               : Lrows[((q - RPT) * NB + c) * REG_THREADS + tid];
-        }
+        for (int q = 0; q < S; ++q)
+            if (pos[q] < m)
+                out[ld * c + pos[q]] = q < RPT ? a[q < RPT ? q : 0][c]
+                                               : Lrows[((q - RPT) * JB + c) * PT + tid];
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
-// Panel of the blocked path too tall for the register panel: one lane per
-// row (strided), the row's NB panel entries loaded together, scaled and
-// updated in one pass; the pivot row is read by broadcast loads.
+// Panel of the blocked path too tall to be held on chip (jb <= NB): one
+// lane per row (strided), the row's NB panel entries loaded together,
+// scaled and updated in one pass; the pivot row is read by broadcast loads.
 __global__ __launch_bounds__(LU_THREADS) void lu_panel_tall_kernel(int n, double *A, int *ipiv,
                                                                    int *info, int k0, int jb,
                                                                    int *list)
@@ -275,6 +385,10 @@ __global__ __launch_bounds__(LU_THREADS) void lu_panel_tall_kernel(int n, double
     __shared__ int si[LU_WAVES];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const size_t ld = (size_t)n;
+    if (tid == 0) {
+        list[LK0] = k0;
+        list[LJB] = jb;
+    }
     if (*info != 0) {
         for (int k = tid; k < jb; k += LU_THREADS) ipiv[k0 + k] = k0 + k;
         if (tid == 0) list[0] = 0;
@@ -361,12 +475,11 @@ __global__ __launch_bounds__(LU_THREADS) void lu_panel_tall_kernel(int n, double
     }
 }
 
-// Applies row-move lists, in panel order, to one column per 64-lane block:
-// the moves of every panel right of the column's own (its swaps reach the
-// columns left of it only at the end, as in a deferred DLASWP).
+// Applies one row-move list to one column (64-lane block): every moved
+// row is read before any is written.
 __device__ __forceinline__ void apply_moves(int n, double *col, const int *L)
 {
-    const int na = min(L[0], 2 * NB);
+    const int na = min(L[0], LPAIRS);
     const int lane = threadIdx.x;
     int dst = -1;
     double v = 0.0;
@@ -381,54 +494,61 @@ __device__ __forceinline__ void apply_moves(int n, double *col, const int *L)
     __syncthreads();
 }
 
+// The deferred DLASWP: column c receives, in panel order, the row moves of
+// every panel that starts right of it.
 __global__ __launch_bounds__(64) void lu_swap_left_kernel(int n, double *A, const int *lists,
                                                           int npanels)
 {
     const int c = blockIdx.x;
     double *col = A + (size_t)n * c;
-    for (int p = c / NB + 1; p < npanels; ++p) apply_moves(n, col, lists + (size_t)p * LSTRIDE);
+    for (int p = 0; p < npanels; ++p) {
+        const int *L = lists + (size_t)p * LSTRIDE;
+        if (L[LK0] > c) apply_moves(n, col, L);
+    }
 }
 
 __global__ void zero_info_kernel(int *info) { *info = 0; }
 
-constexpr int GT = 64;       // output tile of the trailing update
-constexpr int AS_LD = NB + 1;
-constexpr int BS_LD = GT + 1;
-
 // U12 <- L11^-1 U12 for the columns [k+jb, n): rows k..k+jb of those columns
-// hold A12 on entry; L11 is the unit lower triangle of the panel.  Each
-// group of NB lanes (one per row) solves one column by forward
-// substitution, the solved entry broadcast by a lane shuffle.
-constexpr int TRSM_COLS = 256 / NB;  // columns per 256-thread block
+// hold A12 on entry, after this panel's row moves (applied here to the
+// block's columns); L11 is the unit lower triangle of the panel.  Each
+// group of JB lanes (one per row) solves one column by forward
+// substitution, the solved entry broadcast from its lane by readlane.
+template <int JB>
 __global__ __launch_bounds__(256) void lu_trsm_kernel(int n, double *A, int k, int jb,
                                                      const int *list)
 {
-    __shared__ double L[NB][NB + 1];
-    __shared__ int mv[LSTRIDE];
+    constexpr int CPB = 256 / JB;   // columns per block
+    __shared__ double L[JB][JB + 1];
+    __shared__ int mv[1 + 2 * LPAIRS];
     const size_t ld = (size_t)n;
-    const int tid = threadIdx.x;
-    // this panel's row moves, applied to the block's columns
-    if (tid == 0) mv[0] = min(list[0], 2 * NB);
+    const int tid = threadIdx.x, lane = tid & 63;
+    if (tid == 0) mv[0] = min(list[0], 2 * JB);
     __syncthreads();
     const int na = mv[0];
     if (tid < 2 * na) mv[1 + tid] = list[1 + tid];
+    for (int e = tid; e < JB * JB; e += 256) {
+        const int i = e % JB, c = e / JB;
+        L[i][c] = (i < jb && c < jb) ? A[(size_t)(k + i) + ld * (k + c)] : 0.0;
+    }
     __syncthreads();
+    const int row = tid % JB;
+    const int c = k + jb + blockIdx.x * CPB + tid / JB;
     {
-        const int g = tid / NB, e0 = tid % NB;   // NB lanes per column
-        const int c = k + jb + blockIdx.x * TRSM_COLS + g;
+        // the moves of this panel, at most 2 per lane of the column's group
         double v0 = 0.0, v1 = 0.0;
         int d0 = -1, d1 = -1;
         if (c < n) {
-            double *col = A + ld * c;
-            if (e0 < na) {
-                d0 = mv[1 + 2 * e0];
-                const int s0 = mv[2 + 2 * e0];
+            const double *col = A + ld * c;
+            if (row < na) {
+                d0 = mv[1 + 2 * row];
+                const int s0 = mv[2 + 2 * row];
                 if (d0 >= 0 && d0 < n && s0 >= 0 && s0 < n) v0 = col[s0];
                 else d0 = -1;
             }
-            if (e0 + NB < na) {
-                d1 = mv[1 + 2 * (e0 + NB)];
-                const int s1 = mv[2 + 2 * (e0 + NB)];
+            if (row + JB < na) {
+                d1 = mv[1 + 2 * (row + JB)];
+                const int s1 = mv[2 + 2 * (row + JB)];
                 if (d1 >= 0 && d1 < n && s1 >= 0 && s1 < n) v1 = col[s1];
                 else d1 = -1;
             }
@@ -441,76 +561,83 @@ __global__ __launch_bounds__(256) void lu_trsm_kernel(int n, double *A, int k, i
         }
         __syncthreads();
     }
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int i = e % NB, m = e / NB;
-        L[i][m] = (i < jb && m < jb) ? A[(size_t)(k + i) + ld * (k + m)] : 0.0;
-    }
-    __syncthreads();
-    const int row = tid % NB;
-    const int c = k + jb + blockIdx.x * TRSM_COLS + tid / NB;
     const bool live = c < n && row < jb;
     double x = live ? A[(size_t)(k + row) + ld * c] : 0.0;
-    const int base = (tid & 63) & ~(NB - 1);  // first lane of this group
+    const int grp = lane / JB;
     for (int i = 0; i < jb - 1; ++i) {
-        const double xi = __shfl(x, base + i);
+        double xi = 0.0;
+#pragma unroll
+        for (int g = 0; g < 64 / JB; ++g) {
+            const double t = readlane_f64(x, g * JB + i);
+            xi = grp == g ? t : xi;
+        }
         if (row > i) x -= L[row][i] * xi;
     }
     if (live) A[(size_t)(k + row) + ld * c] = x;
 }
 
 typedef double dx4 __attribute__((ext_vector_type(4)));
+constexpr int GT = 64;            // output tile of the trailing update
+constexpr int GS_LD = GT + 1;
 
-// A22 -= L21 U12 for one 64x64 tile of the trailing matrix.
+// A22 -= L21 U12 for one 64x64 tile of the trailing matrix, K = jb <= 32.
+// Computed transposed, D' = U12^T (-L21)^T + A22^T, so that result register
+// i of lane l is A22[row l%16][column 4i + l/16]: the accumulators start
+// from A22 and go back to it in runs of 16 consecutive rows.
 __global__ __launch_bounds__(256) void lu_gemm_mfma_kernel(int n, double *A, int k, int jb)
 {
-    __shared__ double As[GT * AS_LD];   // L21 tile: As[r*AS_LD + kk]
-    __shared__ double Bs[NB * BS_LD];   // U12 tile: Bs[kk*BS_LD + c]
+    __shared__ double Ls[NBMAX * GS_LD];   // -L21^T: Ls[kk*GS_LD + r]
+    __shared__ double Us[NBMAX * GS_LD];   // U12:    Us[kk*GS_LD + c]
     const size_t ld = (size_t)n;
     const int tid = threadIdx.x;
     const int t0 = k + jb;
     const int R0 = t0 + blockIdx.y * GT, C0 = t0 + blockIdx.x * GT;
-    for (int e = tid; e < GT * NB; e += 256) {
+    const int kend = (jb + 3) & ~3;
+    for (int e = tid; e < GT * kend; e += 256) {
         const int r = e % GT, kk = e / GT;
         const int row = R0 + r;
-        As[r * AS_LD + kk] = (row < n && kk < jb) ? A[(size_t)row + ld * (k + kk)] : 0.0;
+        Ls[kk * GS_LD + r] = (row < n && kk < jb) ? -A[(size_t)row + ld * (k + kk)] : 0.0;
     }
-    for (int e = tid; e < GT * NB; e += 256) {
-        const int kk = e % NB, c = e / NB;
+    for (int e = tid; e < GT * kend; e += 256) {
+        const int kk = e % kend, c = e / kend;
         const int col = C0 + c;
-        Bs[kk * BS_LD + c] = (col < n && kk < jb) ? A[(size_t)(k + kk) + ld * col] : 0.0;
+        Us[kk * GS_LD + c] = (col < n && kk < jb) ? A[(size_t)(k + kk) + ld * col] : 0.0;
     }
-    __syncthreads();
     const int lane = tid & 63, w = tid >> 6;
     const int rw = (w & 1) * 32, cw = (w >> 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
-    dx4 acc[2][2];
+    dx4 acc[2][2];   // [column block][row block]
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = dx4{0.0, 0.0, 0.0, 0.0};
-    const int kend = (jb + 3) & ~3;
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = R0 + rw + 16 * b + li;
+                const int col = C0 + cw + 16 * a + 4 * i + lk;
+                acc[a][b][i] = (row < n && col < n) ? A[(size_t)row + ld * col] : 0.0;
+            }
+    __syncthreads();
     for (int kk = 0; kk < kend; kk += 4) {
-        const double a0 = As[(rw + li) * AS_LD + kk + lk];
-        const double a1 = As[(rw + 16 + li) * AS_LD + kk + lk];
-        const double b0 = Bs[(kk + lk) * BS_LD + cw + li];
-        const double b1 = Bs[(kk + lk) * BS_LD + cw + 16 + li];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        const double u0 = Us[(kk + lk) * GS_LD + cw + li];
+        const double u1 = Us[(kk + lk) * GS_LD + cw + 16 + li];
+        const double l0 = Ls[(kk + lk) * GS_LD + rw + li];
+        const double l1 = Ls[(kk + lk) * GS_LD + rw + 16 + li];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(u0, l0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(u0, l1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(u1, l0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(u1, l1, acc[1][1], 0, 0, 0);
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int col = C0 + cw + 16 * b + li;
-            if (col >= n) continue;
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int row = R0 + rw + 16 * a + 4 * i + lk;
-                if (row < n) A[(size_t)row + ld * col] -= acc[a][b][i];
+                const int row = R0 + rw + 16 * b + li;
+                const int col = C0 + cw + 16 * a + 4 * i + lk;
+                if (row < n && col < n) A[(size_t)row + ld * col] = acc[a][b][i];
             }
-        }
 }
 
 }  // namespace
@@ -519,9 +646,6 @@ namespace {
 
 int lu_launch(vlm_ctx *ctx, int n, double *A, int *ipiv, int *info_dev)
 {
-    const int npanels = (n + NB - 1) / NB;
-    if (n > LU_SMALL) VLM_HIP(ctx->lu_work.reserve((size_t)npanels * LSTRIDE));
-    int *lists = ctx->lu_work.p;
     hipLaunchKernelGGL(zero_info_kernel, dim3(1), dim3(1), 0, ctx->stream, info_dev);
     if (n <= LU_SMALL) {
         hipLaunchKernelGGL(lu_panel_kernel, dim3(1), dim3(LU_THREADS), 0, ctx->stream, n, A,
@@ -529,22 +653,50 @@ int lu_launch(vlm_ctx *ctx, int n, double *A, int *ipiv, int *info_dev)
         VLM_HIP(hipGetLastError());
         return VLM_OK;
     }
-    for (int k = 0; k < n; k += NB) {
-        const int jb = n - k < NB ? n - k : NB;
-        int *list = lists + (size_t)(k / NB) * LSTRIDE;
-        if (n - k <= RPT * REG_THREADS)
-            hipLaunchKernelGGL(lu_panel_reg_kernel<0>, dim3(1), dim3(REG_THREADS), 0, ctx->stream,
-                               n, A, ipiv, info_dev, k, jb, list);
-        else if (n - k <= (RPT + 2) * REG_THREADS)
-            hipLaunchKernelGGL(lu_panel_reg_kernel<2>, dim3(1), dim3(REG_THREADS), 0, ctx->stream,
-                               n, A, ipiv, info_dev, k, jb, list);
+    // panel schedule: 16 columns while the panel is taller than the
+    // 32-column on-chip panel, 32 after; a last panel of fewer than 32
+    // rows goes 16 columns at a time through the tall kernel
+    std::vector<int> k0s;
+    for (int k = 0; k < n;) {
+        k0s.push_back(k);
+        const int m = n - k;
+        k += m > WIDE_ROWS ? NB : (m >= NBMAX ? NBMAX : (m < NB ? m : NB));
+    }
+    const int npanels = (int)k0s.size();
+    VLM_HIP(ctx->lu_work.reserve((size_t)npanels * LSTRIDE));
+    int *lists = ctx->lu_work.p;
+    for (int p = 0; p < npanels; ++p) {
+        const int k = k0s[p], m = n - k;
+        const int jb = (p + 1 < npanels ? k0s[p + 1] : n) - k;
+        int *list = lists + (size_t)p * LSTRIDE;
+        const dim3 g1(1), bp(PT);
+        if (m > MID_ROWS || jb < NB || (m <= WIDE_ROWS && jb < NBMAX))
+            hipLaunchKernelGGL(lu_panel_tall_kernel, g1, dim3(LU_THREADS), 0, ctx->stream, n, A,
+                               ipiv, info_dev, k, jb, list);
+        else if (m > 4 * PT)
+            hipLaunchKernelGGL((lu_panel_fast_kernel<NB, 5, 0>), g1, bp, 0, ctx->stream, n, A,
+                               ipiv, info_dev, k, list);
+        else if (m > 3 * PT)
+            hipLaunchKernelGGL((lu_panel_fast_kernel<NB, 4, 0>), g1, bp, 0, ctx->stream, n, A,
+                               ipiv, info_dev, k, list);
+        else if (m > WIDE_ROWS)
+            hipLaunchKernelGGL((lu_panel_fast_kernel<NB, 3, 0>), g1, bp, 0, ctx->stream, n, A,
+                               ipiv, info_dev, k, list);
+        else if (m > PT)
+            hipLaunchKernelGGL((lu_panel_fast_kernel<NBMAX, 2, 0>), g1, bp, 0, ctx->stream, n,
+                               A, ipiv, info_dev, k, list);
         else
-            hipLaunchKernelGGL(lu_panel_tall_kernel, dim3(1), dim3(LU_THREADS), 0, ctx->stream, n,
-                               A, ipiv, info_dev, k, jb, list);
+            hipLaunchKernelGGL((lu_panel_fast_kernel<NBMAX, 1, 0>), g1, bp, 0, ctx->stream, n,
+                               A, ipiv, info_dev, k, list);
         const int rest = n - k - jb;
         if (rest > 0) {
-            hipLaunchKernelGGL(lu_trsm_kernel, dim3((rest + TRSM_COLS - 1) / TRSM_COLS),
-                               dim3(256), 0, ctx->stream, n, A, k, jb, list);
+            if (jb <= NB)
+                hipLaunchKernelGGL(lu_trsm_kernel<NB>, dim3((rest + 256 / NB - 1) / (256 / NB)),
+                                   dim3(256), 0, ctx->stream, n, A, k, jb, list);
+            else
+                hipLaunchKernelGGL(lu_trsm_kernel<NBMAX>,
+                                   dim3((rest + 256 / NBMAX - 1) / (256 / NBMAX)), dim3(256), 0,
+                                   ctx->stream, n, A, k, jb, list);
             const int tiles = (rest + GT - 1) / GT;
             hipLaunchKernelGGL(lu_gemm_mfma_kernel, dim3(tiles, tiles), dim3(256), 0,
                                ctx->stream, n, A, k, jb);
@@ -552,8 +704,8 @@ int lu_launch(vlm_ctx *ctx, int n, double *A, int *ipiv, int *info_dev)
         VLM_HIP(hipGetLastError());
     }
     if (npanels > 1) {
-        hipLaunchKernelGGL(lu_swap_left_kernel, dim3((npanels - 1) * NB), dim3(64),
-                           0, ctx->stream, n, A, lists, npanels);
+        hipLaunchKernelGGL(lu_swap_left_kernel, dim3(k0s.back()), dim3(64), 0, ctx->stream, n,
+                           A, lists, npanels);
         VLM_HIP(hipGetLastError());
     }
     return VLM_OK;
