@@ -22,6 +22,9 @@ PLOT3D blade file) used by the tests and the benchmark.
   traj_wing          the AR-4 wing on a custom trajectory (trajectory01.in)
   camber_wing        the AR-4 wing with a camber line
   skew_rotor         near-wake skew plots of a forward-flight rotor
+  presc_rotor, presc_gen_rotor  the prescribed far wake (helix) of a climbing
+                     rotor, from every far filament and, with axisymmetric
+                     blades, from the last six
   restart_rotor      restart files of a forward-flight rotor
   restart_rotor_wing restart files of the rotor above the wing (AM2, sub-iterations)
   tip_rotor(_cw)     wake tip plots of a forward-flight rotor in slow start, turning
@@ -294,6 +297,22 @@ def make_all(out):
                span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25, velBody=[-13.6, 0.0, -2.0],
                apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
                streamwiseCoreVec=ROTOR_CORE)
+
+    # prescribed far wake (pFwake_update, classdef.f90:998-1066, 5172-5217):
+    # far wake from step 12, truncated at 24, the helix rebuilt after every
+    # convection from step 31 (prescWakeAfterTruncNt 6), wake plots at
+    # steps 24 and 48.  presc_rotor: two blades convected each, the helix
+    # from every active far filament; presc_gen_rotor: three axisymmetric
+    # blades, the helix from the last 6 far filaments (prescWakeGenNt 5)
+    for name, nb_p, axi, gen in (("presc_rotor", 2, 0, 0), ("presc_gen_rotor", 3, 1, 5)):
+        d = case(name)
+        write_config(os.path.join(d, "config.nml"), nt=48, dt=math.radians(5.0) / om, nr=1,
+                     velSound=340.0, wakeDissipation=1, wakeStrain=1, wakePlot=24)
+        write_geom(os.path.join(d, "geom01.nml"), nb=nb_p, nc=2, ns=8, nNwake=12,
+                   wakeTruncateNt=24, prescWakeAfterTruncNt=6, prescWakeGenNt=gen,
+                   axisymmetrySwitch=axi, span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25,
+                   velBody=[0.0, 0.0, -2.0], apparentViscCoeff=ROTOR_AVC,
+                   spanwiseCore=ROTOR_CORE, streamwiseCoreVec=ROTOR_CORE)
 
     # custom trajectory (classdef.f90:3064-3080, main.f90:112-117 and
     # 419-426): the AR-4 wing's body velocity and rate follow

@@ -209,6 +209,22 @@ int vlm_wake_velocity(vlm_ctx *ctx, int wakeType);
 int vlm_rotor_set_init_wake_vel(vlm_ctx *ctx, int ir, double initWakeVel,
                                 int initWakeVelNt);
 
+/* Prescribed far wake (pFwake_class, classdef.f90:222-236; pFwake_update
+ * 998-1066; rotor_updatePrescribedWake 5172-5217): per blade and wake type
+ * a 10-revolution helix of 240 filaments, rebuilt after every convection of
+ * that wake type (inside vlm_convect) once the step given to
+ * vlm_rotor_begin_step exceeds prescWakeNt, from the far filaments at rowFar
+ * and after (prescWakeGenNt = 0) or the last prescWakeGenNt + 1 of them;
+ * deltaPsi = omegaSlow * dt of the step (set again whenever it changes).
+ * While a far wake exists its filaments join the wake source sets.
+ * prescWakeNt = 0 switches it off.  The shaft must lie along z. */
+int vlm_rotor_set_prescribed_wake(vlm_ctx *ctx, int ir, int prescWakeNt, int prescWakeGenNt,
+                                  double deltaPsi);
+/* fc: 6 x 240 x nb end points (TE end, LE end), gam and rc: 240 x nb;
+ * *built = 1 once the wake type has been built (geom2file's test). */
+int vlm_rotor_get_prescribed_wake(vlm_ctx *ctx, int ir, int wakeType, double *fc, double *gam,
+                                  double *rc, int *built);
+
 /* One whole convection block, main.f90:800-1406: pass 1, the scheme
  * algebra of fdScheme (FT1, PC2, AB2, AM2, AM3 or AM4), pass 2 where the
  * scheme has one, and the final convection of every rotor.  The learned

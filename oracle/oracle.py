@@ -41,6 +41,8 @@ _SIGS = {
     "or_rotor_set_wake": ([_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _I], None),
     "or_rotor_get_velocity": ([_P, _I, _I, _P, _P], None),
     "or_rotor_calc_aic": ([_P, _I, _P, _P, _P], None),
+    "or_rotor_set_prescribed": ([_P, _I, _I, _I, _D], None),
+    "or_rotor_get_prescribed": ([_P, _I, _I, _P, _P, _P], _I),
     "or_lu_factor": ([_I, _P, _P], _I),
     "or_lu_solve": ([_I, _P, _P, _P], None),
     "or_matvec": ([_I, _P, _P, _P], None),
@@ -230,6 +232,18 @@ class System:
         vf = np.zeros((nb, max(nF, 0), 3))
         self.lib.or_rotor_get_velocity(self.h, ir, which, _ptr(vn), _ptr(vf))
         return vn, vf
+
+    def rotor_set_prescribed_wake(self, ir, presc_nt, presc_gen_nt, dpsi):
+        self.lib.or_rotor_set_prescribed(self.h, ir, presc_nt, presc_gen_nt, dpsi)
+
+    def rotor_get_prescribed_wake(self, ir, nb, wake_type=0):
+        """(fc (nb, 240, 2, 3), gam (nb, 240), rc (nb, 240), built)"""
+        fc = np.zeros((nb, 240, 2, 3))
+        g = np.zeros((nb, 240))
+        rc = np.zeros((nb, 240))
+        built = self.lib.or_rotor_get_prescribed(self.h, ir, wake_type, _ptr(fc), _ptr(g),
+                                                 _ptr(rc))
+        return fc, g, rc, bool(built)
 
     def rotor_calc_aic(self, ir, cp, ncap, n):
         A = np.zeros((n, n), order="F")

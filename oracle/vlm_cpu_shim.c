@@ -100,6 +100,22 @@ int vlm_rotor_set_init_wake_vel(vlm_ctx *c, int ir, double v, int nt)
     return v == 0.0 ? VLM_OK : shim_fail(c, "no initWakeVel on the CPU");
 }
 
+int vlm_rotor_set_prescribed_wake(vlm_ctx *c, int ir, int nt, int gen_nt, double dpsi)
+{
+    ROTOR_OK(c, ir);
+    or_rotor_set_prescribed(c->s, ir, nt, gen_nt, dpsi);
+    return VLM_OK;
+}
+
+int vlm_rotor_get_prescribed_wake(vlm_ctx *c, int ir, int wakeType, double *fc, double *gam,
+                                  double *rc, int *built)
+{
+    ROTOR_OK(c, ir);
+    const int b = or_rotor_get_prescribed(c->s, ir, wakeType, fc, gam, rc);
+    if (built) *built = b;
+    return VLM_OK;
+}
+
 int vlm_rotor_init_cursors(vlm_ctx *c, int ir) { ROTOR_OK(c, ir); or_rotor_init_cursors(c->s, ir); return VLM_OK; }
 int vlm_rotor_begin_step(vlm_ctx *c, int ir, int it) { ROTOR_OK(c, ir); or_rotor_begin_step(c->s, ir, it); return VLM_OK; }
 
@@ -210,13 +226,15 @@ int vlm_rotor_get_wake(vlm_ctx *c, int ir, int wakeType, double *nodes, double *
     return VLM_OK;
 }
 
-/* Restart state: the rotor's cursors and every wake and velocity array. */
+/* Restart state: the rotor's cursors, every wake and velocity array and
+ * the prescribed far wake. */
 static size_t state_size(const rotor_t *r)
 {
     const size_t nw = (size_t)r->nNwake * r->ns, nf = (size_t)(r->nFwake > 0 ? r->nFwake : 1);
     const size_t nvn = 3u * (size_t)r->nNwake * (r->ns + 1), nvf = 3u * nf;
-    return 2 * sizeof(int) + (size_t)r->nb * (2 * nw * sizeof(ring_t) + 2 * nf * sizeof(fil_t) +
-                                              4 * (nvn + nvf) * sizeof(double));
+    return 4 * sizeof(int) + (size_t)r->nb * (2 * nw * sizeof(ring_t) + 2 * nf * sizeof(fil_t) +
+                                              4 * (nvn + nvf) * sizeof(double) +
+                                              2 * OR_NPF * sizeof(fil_t) + 4 * sizeof(double));
 }
 
 static void state_io(rotor_t *r, char *buf, int out)
@@ -231,6 +249,13 @@ static void state_io(rotor_t *r, char *buf, int out)
     } while (0)
     IO(&r->rowNear, sizeof(int));
     IO(&r->rowFar, sizeof(int));
+    IO(&r->presc_present[0], sizeof(int));
+    IO(&r->presc_present[1], sizeof(int));
+    for (int t = 0; t < 2; ++t) {
+        IO(r->pf[t], (size_t)r->nb * OR_NPF * sizeof(fil_t));
+        IO(r->presc_pitch[t], (size_t)r->nb * sizeof(double));
+        IO(r->presc_radius[t], (size_t)r->nb * sizeof(double));
+    }
     for (int b = 0; b < r->nb; ++b) {
         blade_t *bl = &r->blade[b];
         IO(bl->waN, nw * sizeof(ring_t));

@@ -138,12 +138,23 @@ typedef struct {
     double *velF, *velF1, *velFP, *velFS;   /* 3*nFwake */
 } blade_t;
 
+#define OR_NPF 240  /* prescribed far-wake filaments per blade, classdef.f90:225 */
+
 typedef struct {
     int nb, nc, ns, nNwake, nFwake, nbConvect;
     int rollupStart, rollupEnd, rollupMin, suppressFwake;
     int rowNear, rowFar;
     double shaft[3], hub[3];
     blade_t *blade;
+    /* prescribed far wake (pFwake_class, classdef.f90:222-236): per wake
+     * type (0 'C', 1 'P') the blades' OR_NPF filaments and their relaxed
+     * helix pitch and radius; the step number of the last begin_step */
+    int presc_nt, presc_gen_nt, iter;
+    double presc_dpsi;
+    int presc_present[2];
+    fil_t *pf[2];            /* nb * OR_NPF */
+    double *presc_pitch[2];  /* nb */
+    double *presc_radius[2]; /* nb */
 } rotor_t;
 
 #define OR_MAX_ROTORS 16
@@ -174,6 +185,11 @@ void or_sys_destroy(or_sys *s)
     for (int ir = 0; ir < s->nr; ++ir) {
         for (int b = 0; b < s->rotor[ir].nb; ++b) free_blade(&s->rotor[ir].blade[b]);
         free(s->rotor[ir].blade);
+        for (int t = 0; t < 2; ++t) {
+            free(s->rotor[ir].pf[t]);
+            free(s->rotor[ir].presc_pitch[t]);
+            free(s->rotor[ir].presc_radius[t]);
+        }
     }
     free(s);
 }
@@ -208,6 +224,11 @@ int or_rotor_create(or_sys *s, int nb, int nc, int ns, int nNwake, int nFwake,
         bl->velF1 = (double *)calloc(nvf, sizeof(double));
         bl->velFP = (double *)calloc(nvf, sizeof(double));
         bl->velFS = (double *)calloc(nvf, sizeof(double));
+    }
+    for (int t = 0; t < 2; ++t) {
+        r->pf[t] = (fil_t *)calloc((size_t)nb * OR_NPF, sizeof(fil_t));
+        r->presc_pitch[t] = (double *)calloc((size_t)nb, sizeof(double));
+        r->presc_radius[t] = (double *)calloc((size_t)nb, sizeof(double));
     }
     return s->nr++;
 }
@@ -269,6 +290,7 @@ void or_rotor_init_cursors(or_sys *s, int ir)
 void or_rotor_begin_step(or_sys *s, int ir, int iter)
 {
     rotor_t *r = &s->rotor[ir];
+    r->iter = iter;
     r->rowNear = r->rowNear - 1 > 1 ? r->rowNear - 1 : 1;
     if (iter > r->nNwake) r->rowFar = r->rowFar - 1 > 1 ? r->rowFar - 1 : 1;
 }
@@ -358,8 +380,13 @@ static void blade_vind_bywake(const rotor_t *r, const blade_t *bl, int pred,
                 v[0] += t[0] * f->gam; v[1] += t[1] * f->gam; v[2] += t[2] * f->gam;
             }
         }
-        /* prescribed far wake: every filament has zero circulation in the
-         * supported configurations and is skipped by the |gam| > eps rule */
+        /* prescribed far wake, classdef.f90:1471-1476 and 1501-1507 */
+        const fil_t *W = r->pf[pred] + (size_t)(bl - r->blade) * OR_NPF;
+        for (int i = 0; i < OR_NPF; ++i)
+            if (fabs(W[i].gam) > OR_EPS) {
+                or_vf_vind(P, W[i].fc[0], W[i].fc[1], W[i].rc, t);
+                v[0] += t[0] * W[i].gam; v[1] += t[1] * W[i].gam; v[2] += t[2] * W[i].gam;
+            }
     }
 }
 
@@ -641,8 +668,102 @@ static void rot_point(const double T[9], const double o[3], double p[3])
         p[i] = T[i] * q[0] + T[i + 3] * q[1] + T[i + 6] * q[2] + o[i];
 }
 
+/* rotor_updatePrescribedWake (classdef.f90:5172-5217) with pFwake_update
+ * (998-1066): per convected blade, a 10-revolution helix of OR_NPF
+ * filaments anchored at the TE end of the oldest far filament, its radius
+ * the mean distance from the z axis of the far filaments' TE ends (from
+ * rowFar, or the last presc_gen_nt + 1 of them) and its pitch their mean
+ * z step per revolution, each relaxed by half against the previous update
+ * of the same wake type.  The helix winds clockwise whatever the rotor
+ * (isClockwiseRotor is never set, classdef.f90:231, 1048).  Every filament
+ * takes the oldest far filament's circulation and core radius. */
+static void presc_update(rotor_t *r, int pred)
+{
+    const int nF = r->nFwake;
+    const int start = r->presc_gen_nt == 0 ? r->rowFar - 1 : nF - r->presc_gen_nt - 1;
+    const int n = nF - start;
+    if (start < 0 || n < 1) return;  /* no far filament to anchor to */
+    const double twoPi = 2.0 * OR_PI;
+    const double dx = (twoPi * 10.0 - 0.0) / (OR_NPF + 1 - 1);
+    for (int b = 0; b < r->nbConvect; ++b) {
+        const fil_t *F = (pred ? r->blade[b].waFP : r->blade[b].waF) + start;
+        const double *anchor = F[n - 1].fc[0];
+        double pitch = 0.0, radius = 0.0;
+        for (int i = 0; i < n; ++i) {
+            radius += sqrt(F[i].fc[0][1] * F[i].fc[0][1] + F[i].fc[0][0] * F[i].fc[0][0]);
+            if (i < n - 1) pitch += F[i].fc[0][2] - F[i + 1].fc[0][2];
+        }
+        pitch = fabs(pitch) * (-twoPi / r->presc_dpsi) / (n - 1);
+        radius = radius / n;
+        double *hp = &r->presc_pitch[pred][b], *hr = &r->presc_radius[pred][b];
+        *hp = 0.5 * pitch + (1 - 0.5) * *hp;
+        *hr = 0.5 * radius + (1 - 0.5) * *hr;
+        const double dth = atan2(anchor[1], anchor[0]);
+        const double dz = anchor[2] - r->hub[2];
+        double c[OR_NPF + 1][3];
+        for (int k = 0; k <= OR_NPF; ++k) {
+            const double th = -(k * dx + 0.0);
+            c[k][0] = *hr * cos(th + dth);
+            c[k][1] = *hr * sin(th + dth);
+            c[k][2] = *hp * fabs(th) / twoPi + dz;
+        }
+        fil_t *W = r->pf[pred] + (size_t)b * OR_NPF;
+        for (int i = 0; i < OR_NPF; ++i)
+            for (int d = 0; d < 3; ++d) {
+                W[i].fc[1][d] = r->hub[d] + c[i][d];
+                W[i].fc[0][d] = r->hub[d] + c[i + 1][d];
+            }
+        memcpy(W[0].fc[1], anchor, 3 * sizeof(double));
+        for (int i = 0; i < OR_NPF; ++i) {
+            W[i].gam = F[n - 1].gam;
+            W[i].rc = F[n - 1].rc;
+        }
+    }
+    r->presc_present[pred] = 1;
+    if (r->nbConvect == 1 && r->nb > 1)
+        for (int b = 1; b < r->nb; ++b) {
+            fil_t *W = r->pf[pred] + (size_t)b * OR_NPF;
+            memcpy(W, r->pf[pred], OR_NPF * sizeof(fil_t));
+            r->presc_pitch[pred][b] = r->presc_pitch[pred][0];
+            r->presc_radius[pred][b] = r->presc_radius[pred][0];
+            const double theta = 2.0 * OR_PI / r->nb * b;
+            if (fabs(theta) > OR_EPS) {
+                double T[9];
+                transform_axis(theta, r->shaft, T);
+                for (int i = 0; i < OR_NPF; ++i) {
+                    rot_point(T, r->hub, W[i].fc[0]);
+                    rot_point(T, r->hub, W[i].fc[1]);
+                }
+            }
+        }
+}
+
+void or_rotor_set_prescribed(or_sys *s, int ir, int presc_nt, int presc_gen_nt, double dpsi)
+{
+    rotor_t *r = &s->rotor[ir];
+    r->presc_nt = presc_nt;
+    r->presc_gen_nt = presc_gen_nt;
+    r->presc_dpsi = dpsi;
+}
+
+int or_rotor_get_prescribed(const or_sys *s, int ir, int pred, double *fc, double *gam,
+                            double *rc)
+{
+    const rotor_t *r = &s->rotor[ir];
+    for (int b = 0; b < r->nb; ++b)
+        for (int i = 0; i < OR_NPF; ++i) {
+            const fil_t *W = r->pf[pred] + (size_t)b * OR_NPF + i;
+            const size_t q = (size_t)b * OR_NPF + i;
+            if (fc) memcpy(fc + 6 * q, W->fc, 6 * sizeof(double));
+            if (gam) gam[q] = W->gam;
+            if (rc) rc[q] = W->rc;
+        }
+    return r->presc_present[pred];
+}
+
 /* blade_convectwake + blade_wake_continuity, classdef.f90:1515-1702;
- * rotor_convectwake with the axisymmetric copy, 4786-4830 */
+ * rotor_convectwake with the axisymmetric copy, 4786-4830, and the
+ * prescribed far wake after it (4826-4829) */
 void or_rotor_convectwake(or_sys *s, int ir, double dt, int pred, int bug_compat)
 {
     rotor_t *r = &s->rotor[ir];
@@ -702,6 +823,7 @@ void or_rotor_convectwake(or_sys *s, int ir, double dt, int pred, int bug_compat
             }
         }
     }
+    if (r->presc_nt > 0 && r->iter > r->presc_nt) presc_update(r, pred);
 }
 
 /* ------------------------------------------------------------------ */

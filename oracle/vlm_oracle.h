@@ -81,6 +81,14 @@ void or_wake_velocity(or_sys *s, int predicted);
 void or_convect(or_sys *s, int scheme, int iter, double dt, int bug_compat);
 
 /* Accessors in the lattice form used by the library (see vlm_c.h). */
+/* Prescribed far wake (pFwake_update, classdef.f90:998-1066, 5172-5217):
+ * rebuilt after every convection of the wake type once the step number
+ * of or_rotor_begin_step exceeds presc_nt; dpsi = omegaSlow * dt.
+ * or_rotor_get_prescribed: 6 x 240 x nb end points (TE end, LE end), the
+ * circulation and core radius of each filament; returns 1 once built. */
+void or_rotor_set_prescribed(or_sys *s, int ir, int presc_nt, int presc_gen_nt, double dpsi);
+int or_rotor_get_prescribed(const or_sys *s, int ir, int predicted, double *fc, double *gam,
+                            double *rc);
 void or_rotor_get_wake(const or_sys *s, int ir, int predicted, double *nodes,
                        double *le, double *gam, double *rc4, double *far_fc,
                        double *far_gam, double *far_rc);

@@ -141,6 +141,19 @@ for c in tip_rotor tip_rotor_cw; do
   done
   rm -rf "$tmp"
 done
+# 5i. prescribed far wake: CL/CT history and wing and wake plots at steps 24 and 48
+for c in presc_rotor presc_gen_rotor; do
+  tmp=$(mktemp -d)
+  cp "$ROOT/cases/$c.case/"* "$tmp/"
+  mkdir -p "$tmp/Results" "$tmp/Restart"
+  (cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 OMP_SCHEDULE=static \
+     "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
+  cp "$tmp/Results/r01ForceNonDim.csv" "$HERE/${c}_ForceNonDim.csv"
+  for st in 00024 00048; do
+    cp "$tmp/Results/r01wingNwake$st.tec" "$HERE/${c}_wingNwake$st.tec"
+  done
+  rm -rf "$tmp"
+done
 # 6. velocity probes: the reference's probes2file output of a short wing run
 tmp=$(mktemp -d)
 cp "$ROOT/cases/probes_wing.case/"* "$tmp/"

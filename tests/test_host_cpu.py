@@ -2,9 +2,10 @@
 oracle/vlm_cpu_shim.c, the libvlm C ABI restated over the oracle
 (make -C oracle hostcpu -> oracle/_cpu/libvolcanor_host_cpu.so).  This
 checks host-side logic (namelists, cursors, slow start, tip plots, restart
-records) with no GPU: the CL/CT history and tip plots of tip_rotor against
-the reference solver's own outputs, and a restart at step 24 of the same
-case against the uninterrupted run."""
+records) and the oracle's restatement of whole cases with no GPU: the CL/CT
+history and tip plots of tip_rotor against the reference solver's own
+outputs, a restart at step 24 of the same case against the uninterrupted
+run, and the prescribed far wake against the reference solver."""
 import ctypes as C
 import os
 import shutil
@@ -134,5 +135,37 @@ def test_cpu_emptied_config_returns_status_and_next_run_works(cpu_host):
         shutil.copy(os.path.join(CASES, "tip_rotor.case", "config.nml"), d)
         rc, ct, msg = run_cpu(cpu_host, d)
         assert rc == 0 and len(ct) == 49, msg
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+@pytest.mark.parametrize("name", ["presc_rotor", "presc_gen_rotor"])
+def test_cpu_prescribed_far_wake_matches_reference_solver(cpu_host, name):
+    """The prescribed far wake (pFwake_update, classdef.f90:998-1066,
+    5172-5217) on the CPU build: a climbing rotor whose 240-filament helix
+    is rebuilt after every convection from step 31, from every far
+    filament (presc_rotor) or, with three axisymmetric blades, from the
+    last six (presc_gen_rotor).  The CL/CT history against the reference
+    solver's, and every zone of the wing and wake plots at steps 24 and 48,
+    the helix (PrescFarWake) included, to 1e-8 relative or 1e-10 absolute:
+    the helix switches on with a jump in CT (0.0117 to -0.018 at step 32
+    for presc_rotor), after which the wake amplifies round-off differences
+    to about 4e-9 relative by step 48."""
+    from tests.test_host_gpu import read_tec
+    d = case_copy(name)
+    try:
+        rc, ct, msg = run_cpu(cpu_host, d)
+        assert rc == 0, msg
+        ref = read_force(os.path.join(GOLD, f"{name}_ForceNonDim.csv"))[:, 0]
+        np.testing.assert_allclose(ct, ref, rtol=2e-6, atol=1e-9)
+        for step in ("00024", "00048"):
+            got = read_tec(os.path.join(d, "Results", f"r01wingNwake{step}.tec"))
+            exp = read_tec(os.path.join(GOLD, f"{name}_wingNwake{step}.tec"))
+            assert [z[:3] for z in got] == [z[:3] for z in exp]
+            for zg, ze in zip(got, exp):
+                for k in range(3, 8):
+                    np.testing.assert_allclose(zg[k], ze[k], rtol=1e-8, atol=1e-10,
+                                               err_msg=f"{step} {zg[0]} column {k - 2}")
+        assert any(z[0] == "PrescFarWake" and z[1] == 241 for z in got)
     finally:
         shutil.rmtree(d, ignore_errors=True)

@@ -669,3 +669,30 @@ def test_restart_carries_tip_plot_ages():
     finally:
         for d in dirs:
             shutil.rmtree(d, ignore_errors=True)
+
+
+@pytest.mark.parametrize("name", ["presc_rotor", "presc_gen_rotor"])
+def test_prescribed_far_wake_matches_reference_solver(name):
+    """The prescribed far wake (pFwake_update, classdef.f90:998-1066,
+    rotor_updatePrescribedWake 5172-5217) on the device: a climbing rotor
+    whose 240-filament helix is rebuilt after both AM2 convections from
+    step 31 and joins every wake source set, from every far filament
+    (presc_rotor) or, with three axisymmetric blades, from the last six
+    (presc_gen_rotor).  The CL/CT history against the reference solver's
+    and every zone of the wing and wake plots at steps 24 and 48, the helix
+    (PrescFarWake) included, to 1e-8 relative or 1e-10 absolute (the bound
+    of the CPU build of the same case: the helix switches on with a jump in
+    CT, after which the wake amplifies round-off to about 4e-9 by step 48)."""
+    ref = read_force(os.path.join(GOLD, f"{name}_ForceNonDim.csv"))[:, 0]
+    ct, _ = run(name)
+    np.testing.assert_allclose(ct[0], ref, rtol=2e-6, atol=1e-9)
+    res = os.path.join(CASES, f"{name}.case", "Results")
+    for step in ("00024", "00048"):
+        got = read_tec(os.path.join(res, f"r01wingNwake{step}.tec"))
+        exp = read_tec(os.path.join(GOLD, f"{name}_wingNwake{step}.tec"))
+        assert [z[:3] for z in got] == [z[:3] for z in exp]
+        for zg, ze in zip(got, exp):
+            for k in range(3, 8):
+                np.testing.assert_allclose(zg[k], ze[k], rtol=1e-8, atol=1e-10,
+                                           err_msg=f"{step} {zg[0]} column {k - 2}")
+    assert any(z[0] == "PrescFarWake" and z[1] == 241 for z in got)

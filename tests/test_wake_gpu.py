@@ -41,6 +41,14 @@ def compare(sa, sb, case, tol=1e-11):
         for k in ("vn", "vn1", "vf", "vf1"):
             sv = np.max(np.abs(rb[k])) + 1e-30 if rb[k].size else 1.0
             _cmp(ra[k], rb[k], sv, tol, k)
+        for t in (0, 1):
+            if f"presc{t}" in rb:
+                fa, ga, ca, ba = ra[f"presc{t}"]
+                fb, gb, cb, bb = rb[f"presc{t}"]
+                assert ba == bb, f"prescribed wake {t} built"
+                _cmp(fa, fb, np.max(np.abs(fb)) + 1.0, tol, f"prescribed wake {t} ends")
+                _cmp(ga, gb, np.max(np.abs(gb)) + 1e-30, tol, f"prescribed wake {t} gam")
+                _cmp(ca, cb, 1.0, tol, f"prescribed wake {t} cores")
 
 
 CASES = {
@@ -62,6 +70,14 @@ CASES = {
     "nonuniform_cores": dict(nb=1, nc=2, ns=4, nNwake=4, nFwake=2, nt=8, scheme=3,
                              scv=np.array([0.01, 0.02, 0.03, 0.02, 0.01]), dissipation=True),
     "truncated_no_far": dict(nb=1, nc=1, ns=3, nNwake=3, nFwake=0, nt=7, scheme=3, rollup=(2, 3)),
+    # prescribed far wake rebuilt after both AM2 convections from step 9:
+    # from every active far filament, and with axisymmetric blades from the
+    # last four (prescWakeGenNt 3)
+    "rotor_am2_presc": dict(nb=2, nc=1, ns=4, nNwake=4, nFwake=5, nt=14, scheme=3, omega=30.0,
+                            vel=(0, 0, 1.0), presc=(8, 0)),
+    "rotor_am2_axisym_presc": dict(nb=3, nc=1, ns=4, nNwake=4, nFwake=6, nt=14, scheme=3,
+                                   omega=25.0, vel=(0, 0, 2.0), axisym=True, dissipation=True,
+                                   presc=(8, 3)),
 }
 
 

@@ -13,13 +13,14 @@ from tests.geometry import blade_grid, rings_from_grids, rot_z
 class Case:
     """surfaces: optional list of per-rotor overrides (nb, nc, ns, nNwake,
     nFwake, omega, span, chord, pitch, rollup, hub, ...); rotor ir then
-    uses the case's values with surfaces[ir] applied (nr = len(surfaces))."""
+    uses the case's values with surfaces[ir] applied (nr = len(surfaces)).
+    presc: (prescWakeNt, prescWakeGenNt) of a prescribed far wake."""
 
     def __init__(self, nb=2, nc=2, ns=4, nNwake=6, nFwake=4, nt=14, scheme=3, axisym=False,
                  dissipation=False, strain=False, omega=0.0, vel=(10.0, 0.0, 0.0), dt=0.02,
                  span=1.0, root=0.2, chord=0.25, pitch=6.0, rollup=(3, 4), rollupMin=1,
                  spanwiseCore=0.02, scv=None, bug_compat=True, nr=1, hub_dx=0.0, hub=None,
-                 surfaces=None):
+                 surfaces=None, presc=None):
         self.__dict__.update(locals())
         del self.__dict__["self"]
         if scv is None:
@@ -81,6 +82,8 @@ def setup(sys_, case):
         rc = np.full(corners.shape[:3] + (4,), 0.01)
         sys_.rotor_set_wing(k, corners, rc, wing_gamma(case, 0, ir))
         sys_.rotor_set_axis(k, [0.0, 0.0, 1.0], hub)
+        if c.presc is not None:
+            sys_.rotor_set_prescribed_wake(k, c.presc[0], c.presc[1], c.omega * c.dt)
         sys_.rotor_init_cursors(k)
         sys_.rotor_assignshed(k, 1)  # main.f90:227-233: shed as TE
     sys_.set_bug_compat(case.bug_compat)
@@ -119,5 +122,10 @@ def snapshot(sys_, case, irs):
         w = sys_.rotor_get_wake(ir, dims)
         vn, vf = sys_.rotor_get_velocity(ir, 0, dims)
         vn1, vf1 = sys_.rotor_get_velocity(ir, 1, dims)
-        out.append(dict(cursors=sys_.rotor_cursors(ir), vn=vn, vf=vf, vn1=vn1, vf1=vf1, **w))
+        extra = {}
+        if case.surface(k).presc is not None:
+            for t in (0, 1):
+                extra[f"presc{t}"] = sys_.rotor_get_prescribed_wake(ir, dims[0], t)
+        out.append(dict(cursors=sys_.rotor_cursors(ir), vn=vn, vf=vf, vn1=vn1, vf1=vf1, **w,
+                        **extra))
     return out

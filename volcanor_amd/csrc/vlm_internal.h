@@ -67,6 +67,8 @@ struct Slot {
 // histories used only by AM3 and AM4
 constexpr int VEL_H2 = 4, VEL_H3 = 5, NVEL = 6;
 
+constexpr int PF_N = 240;  // prescribed far-wake filaments per blade (classdef.f90:225)
+
 struct Rotor {
     vlm_rotor_desc d{};
     int rowNear = 0, rowFar = 0;  // 1-based, as in the reference
@@ -88,6 +90,14 @@ struct Rotor {
     // velocities, logical rows: [VLM_VEL, PREV, PRED, STEP] and the older
     // histories of AM3/AM4 (velNwake2, velNwake3; PREV is velNwake1)
     DBuf<double> velN[NVEL], velF[NVEL];
+    // prescribed far wake, per wake type (0 current, 1 predicted): PF_N
+    // filaments per blade, end points pf[6*(b*PF_N + i) + (0..2 TE, 3..5 LE)],
+    // and per blade ps[4*b + (0 gam, 1 core radius, 2 helix pitch, 3 radius)]
+    DBuf<double> pf[2], ps[2];
+    int presc_nt = 0, presc_gen_nt = 0;  // 0: no prescribed far wake
+    double presc_dpsi = 0.0;             // omegaSlow * dt
+    int presc_built[2] = {0, 0};
+    int iter = 0;                        // step of the last vlm_rotor_begin_step
     size_t nvn() const { return 3u * d.nb * (size_t)d.nNwake * (d.ns + 1); }
     size_t nvf() const { return 3u * d.nb * (size_t)(d.nFwake > 0 ? d.nFwake : 0); }
 };

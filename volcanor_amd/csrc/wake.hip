@@ -231,6 +231,80 @@ __global__ void stage_wake_kernel(RV v, int pred, int far_exists, int64_t per_bl
     }
 }
 
+// Prescribed far-wake filaments as sources (classdef.f90:1471-1476,
+// 1501-1507), one slot each, while a far wake exists.
+__global__ void stage_presc_kernel(const double *pf, const double *ps, int nb, Slot *slots,
+                                   int *flags)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb * PF_N) return;
+    const int b = e / PF_N;
+    put(slots + e, flags + e, pf + 6 * (size_t)e, pf + 6 * (size_t)e + 3, skipg(ps[4 * b]),
+        ps[4 * b + 1]);
+}
+
+// pFwake_update (classdef.f90:998-1066) for blade blockIdx.x of wake type
+// pred, from the far filaments start .. start+n-1 (0-based):
+//   radius = mean over them of |(y, x)| of the TE end, pitch = |sum of the
+//   z steps between consecutive TE ends| (-2 pi / dpsi) / (n - 1), both
+//   relaxed by half against the blade's previous values;
+//   a 10-revolution helix of PF_N filaments, clockwise whatever the rotor
+//   (isClockwiseRotor is never set, 231 and 1048), turned to the TE end of
+//   the last filament (the anchor) and lifted to its height over the hub;
+//   filament i joins helix points i (LE end) and i+1 (TE end), filament 0
+//   starts at the anchor; every filament takes the last far filament's
+//   circulation and core radius.
+// The sums run in one lane in the reference's order.
+__global__ __launch_bounds__(256) void presc_update_kernel(RV v, int pred, int start, int n,
+                                                           double hx, double hy, double hz,
+                                                           double dpsi, double *pf, double *ps)
+{
+    __shared__ double sh[4];
+    const int b = blockIdx.x;
+    const double *FC = (pred ? v.pffc : v.ffc) + 6 * ((size_t)b * v.nF + start);
+    const double *anc = FC + 6 * (size_t)(n - 1);
+    constexpr double twoPi = 2.0 * kPi;
+    if (threadIdx.x == 0) {
+        double pitch = 0.0, radius = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double *a = FC + 6 * (size_t)i;
+            radius += sqrt(a[1] * a[1] + a[0] * a[0]);
+            if (i < n - 1) pitch += a[2] - a[6 + 2];
+        }
+        pitch = fabs(pitch) * (-twoPi / dpsi) / (n - 1);
+        radius = radius / n;
+        double *S = ps + 4 * b;
+        const size_t q = (size_t)b * v.nF + start + n - 1;
+        S[0] = v.fgam[q];
+        S[1] = v.frc[q];
+        S[2] = 0.5 * pitch + (1 - 0.5) * S[2];
+        S[3] = 0.5 * radius + (1 - 0.5) * S[3];
+        sh[0] = S[2];
+        sh[1] = S[3];
+        sh[2] = atan2(anc[1], anc[0]);
+        sh[3] = anc[2] - hz;
+    }
+    __syncthreads();
+    const double pitch = sh[0], radius = sh[1], dth = sh[2], dz = sh[3];
+    const double dx = (twoPi * 10.0 - 0.0) / (PF_N + 1 - 1);
+    double *W = pf + 6 * (size_t)b * PF_N;
+    for (int k = threadIdx.x; k <= PF_N; k += blockDim.x) {
+        const double th = -(k * dx + 0.0);
+        const double c0 = radius * cos(th + dth), c1 = radius * sin(th + dth);
+        const double c2 = pitch * fabs(th) / twoPi + dz;
+        if (k < PF_N) {   // LE end of filament k (filament 0: the anchor)
+            W[6 * k + 3] = k == 0 ? anc[0] : hx + c0;
+            W[6 * k + 4] = k == 0 ? anc[1] : hy + c1;
+            W[6 * k + 5] = k == 0 ? anc[2] : hz + c2;
+        }
+        if (k > 0) {      // TE end of filament k-1
+            W[6 * (k - 1) + 0] = hx + c0;
+            W[6 * (k - 1) + 1] = hy + c1;
+            W[6 * (k - 1) + 2] = hz + c2;
+        }
+    }
+}
+
 // Targets: near-wake nodes of rows rn..nN-1 then far trailing ends, for
 // blades < nbConvect, written as SoA at t[off..].
 __global__ void gather_targets_kernel(RV v, int pred, double *t, int64_t stride, int64_t off)
@@ -382,6 +456,23 @@ __global__ void axisym_kernel(RV v, int pred, RotMats M, double ox, double oy, d
         v.age[rb] = v.age[r0];
         for (int q = 0; q < 4; ++q) v.rc4[4 * rb + q] = v.rc4[4 * r0 + q];
     }
+}
+
+// Axisymmetric blades take blade 1's prescribed wake turned by
+// 2 pi (b-1)/nb about the shaft (rotor_updatePrescribedWake,
+// classdef.f90:5203-5215, pFwake_rot_wake_axis 1068-1086).
+__global__ void presc_axisym_kernel(int nb, RotMats M, double ox, double oy, double oz,
+                                    double *pf, double *ps)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (nb - 1) * PF_N) return;
+    const int b = 1 + e / PF_N, i = e % PF_N;
+    const double o[3] = {ox, oy, oz};
+    const double *src = pf + 6 * (size_t)i;
+    double *dst = pf + 6 * ((size_t)b * PF_N + i);
+    rotp(M.T[b], o, src, dst);
+    rotp(M.T[b], o, src + 3, dst + 3);
+    if (i < 4) ps[4 * b + i] = ps[i];
 }
 
 // rotor_assignshed (classdef.f90:4297-4325).  LE: shed line of row rowNear
@@ -821,7 +912,13 @@ static int64_t wake_edges_per_blade(const Rotor &r)
     return R * r.d.ns + r.d.ns + R * (r.d.ns + 1) + (F > 0 ? F : 0);
 }
 
-int64_t wake_slots(const Rotor &r) { return 2 * wake_edges_per_blade(r) * r.d.nb; }
+// prescribed far-wake sources: one slot per filament while a far wake exists
+static int64_t presc_slots(const Rotor &r)
+{
+    return (r.presc_nt > 0 && r.rowFar <= r.d.nFwake) ? (int64_t)PF_N * r.d.nb : 0;
+}
+
+int64_t wake_slots(const Rotor &r) { return 2 * wake_edges_per_blade(r) * r.d.nb + presc_slots(r); }
 
 int stage_wing(vlm_ctx *ctx, Rotor &r, int set, int64_t off, int64_t *used)
 {
@@ -846,6 +943,14 @@ int stage_wake(vlm_ctx *ctx, Rotor &r, int pred, int64_t off, int64_t *used)
     const int far_exists = r.rowFar <= r.d.nFwake ? 1 : 0;
     hipLaunchKernelGGL(stage_wake_kernel, dim3(nblk(per * r.d.nb)), dim3(256), 0, ctx->stream,
                        v, pred, far_exists, per, ctx->slots.p + off, ctx->flags.p + off);
+    const int64_t np = presc_slots(r);
+    if (np > 0) {
+        const int t = pred ? 1 : 0;
+        hipLaunchKernelGGL(stage_presc_kernel, dim3(nblk(np)), dim3(256), 0, ctx->stream,
+                           r.pf[t].p, r.ps[t].p, r.d.nb, ctx->slots.p + off + *used,
+                           ctx->flags.p + off + *used);
+        *used += np;
+    }
     VLM_HIP(hipGetLastError());
     return VLM_OK;
 }
@@ -1004,6 +1109,35 @@ int copy_predicted(vlm_ctx *ctx, Rotor &r)
 
 }  // namespace
 
+// rotor_updatePrescribedWake, classdef.f90:5172-5217: the far filaments
+// from rowFar (prescWakeGenNt = 0) or the last prescWakeGenNt + 1.
+static int presc_update(vlm_ctx *ctx, Rotor &r, int pred)
+{
+    const int nF = r.d.nFwake;
+    const int start = r.presc_gen_nt == 0 ? r.rowFar - 1 : nF - r.presc_gen_nt - 1;
+    const int n = nF - start;
+    if (start < 0 || n < 1) return VLM_OK;  // no far filament to anchor to
+    if (std::fabs(r.shaft[0]) > kEps || std::fabs(r.shaft[1]) > kEps)
+        return fail(ctx, VLM_ERR_ARG, "Prescribed far wake only implemented for shaft along Z-axis");
+    const int t = pred ? 1 : 0;
+    hipLaunchKernelGGL(presc_update_kernel, dim3(r.d.nbConvect), dim3(256), 0, ctx->stream,
+                       view(r), pred, start, n, r.hub[0], r.hub[1], r.hub[2], r.presc_dpsi,
+                       r.pf[t].p, r.ps[t].p);
+    VLM_HIP(hipGetLastError());
+    if (r.d.nbConvect == 1 && r.d.nb > 1) {
+        if (r.d.nb > 16) return fail(ctx, VLM_ERR_ARG, "axisymmetric copy supports <= 16 blades");
+        RotMats M;
+        std::memset(&M, 0, sizeof(M));
+        for (int b = 1; b < r.d.nb; ++b) transform_axis(2.0 * kPi / r.d.nb * b, r.shaft, M.T[b]);
+        hipLaunchKernelGGL(presc_axisym_kernel, dim3(nblk((int64_t)(r.d.nb - 1) * PF_N)),
+                           dim3(256), 0, ctx->stream, r.d.nb, M, r.hub[0], r.hub[1], r.hub[2],
+                           r.pf[t].p, r.ps[t].p);
+        VLM_HIP(hipGetLastError());
+    }
+    r.presc_built[t] = 1;
+    return VLM_OK;
+}
+
 int rotor_convect(vlm_ctx *ctx, Rotor &r, double dt, int pred)
 {
     if (r.d.nNwake <= 0) return VLM_OK;
@@ -1041,6 +1175,8 @@ int rotor_convect(vlm_ctx *ctx, Rotor &r, double dt, int pred)
                            ctx->stream, v, pred, M, r.hub[0], r.hub[1], r.hub[2]);
         VLM_HIP(hipGetLastError());
     }
+    // the prescribed far wake after the convection (classdef.f90:4826-4829)
+    if (r.presc_nt > 0 && r.iter > r.presc_nt) VLM_TRY(presc_update(ctx, r, pred));
     return VLM_OK;
 }
 
@@ -1507,6 +1643,10 @@ int vlm_rotor_create(vlm_ctx *ctx, const vlm_rotor_desc *d, double spanwiseCore,
         R(r->velN[k], r->nvn());
         R(r->velF[k], r->nvf());
     }
+    for (int t = 0; t < 2; ++t) {
+        R(r->pf[t], 6 * nb * PF_N);
+        R(r->ps[t], 4 * nb);
+    }
     if (e != hipSuccess) {
         delete r;
         return hip_fail(ctx, e, "rotor allocation");
@@ -1520,6 +1660,10 @@ int vlm_rotor_create(vlm_ctx *ctx, const vlm_rotor_desc *d, double spanwiseCore,
     for (int k = 0; k < NVEL; ++k) {
         all.push_back(&r->velN[k]);
         all.push_back(&r->velF[k]);
+    }
+    for (int t = 0; t < 2; ++t) {
+        all.push_back(&r->pf[t]);
+        all.push_back(&r->ps[t]);
     }
     for (auto *b : all)
         hipLaunchKernelGGL(fill_kernel, dim3(nblk((int64_t)b->n)), dim3(256), 0, ctx->stream,
@@ -1602,6 +1746,7 @@ int vlm_rotor_begin_step(vlm_ctx *ctx, int ir, int iter)
 {
     Rotor *r = get_rotor(ctx, ir);
     if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    r->iter = iter;
     r->rowNear = r->rowNear - 1 > 1 ? r->rowNear - 1 : 1;
     if (iter > r->d.nNwake) r->rowFar = r->rowFar - 1 > 1 ? r->rowFar - 1 : 1;
     return VLM_OK;
@@ -1696,6 +1841,10 @@ static std::vector<DBuf<double> *> state_bufs(Rotor &r)
         v.push_back(&r.velN[k]);
         v.push_back(&r.velF[k]);
     }
+    for (int t = 0; t < 2; ++t) {
+        v.push_back(&r.pf[t]);
+        v.push_back(&r.ps[t]);
+    }
     return v;
 }
 
@@ -1727,7 +1876,7 @@ int vlm_rotor_get_state(vlm_ctx *ctx, int ir, void *buf, int64_t bytes)
     h[0] = r->rowNear;
     h[1] = r->rowFar;
     h[2] = r->head;
-    h[3] = 0;
+    h[3] = r->presc_built[0] | (r->presc_built[1] << 1);
     char *p = static_cast<char *>(buf) + 4 * sizeof(int64_t);
     for (auto *b : state_bufs(*r)) {
         if (b->n) VLM_HIP(hipMemcpy(p, b->p, b->n * sizeof(double), hipMemcpyDeviceToHost));
@@ -1756,6 +1905,8 @@ int vlm_rotor_set_state(vlm_ctx *ctx, int ir, const void *buf, int64_t bytes)
     r->rowNear = (int)h[0];
     r->rowFar = (int)h[1];
     r->head = (int)h[2];
+    r->presc_built[0] = (int)(h[3] & 1);
+    r->presc_built[1] = (int)((h[3] >> 1) & 1);
     return VLM_OK;
 }
 
@@ -2014,6 +2165,48 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
             VLM_TRY(rotor_convect(ctx, *r, dt, 0));
             VLM_TRY(launch_scheme(ctx, *r, 3));
         }
+    return VLM_OK;
+}
+
+int vlm_rotor_set_prescribed_wake(vlm_ctx *ctx, int ir, int prescWakeNt, int prescWakeGenNt,
+                                  double deltaPsi)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (prescWakeNt < 0 || prescWakeGenNt < 0)
+        return fail(ctx, VLM_ERR_ARG, "prescWakeNt and prescWakeGenNt must be >= 0");
+    if (prescWakeNt > 0 && (r->d.nFwake < 1 || prescWakeGenNt >= r->d.nFwake))
+        return fail(ctx, VLM_ERR_ARG,
+                    "the prescribed far wake needs a far wake longer than prescWakeGenNt");
+    r->presc_nt = prescWakeNt;
+    r->presc_gen_nt = prescWakeGenNt;
+    r->presc_dpsi = deltaPsi;
+    return VLM_OK;
+}
+
+int vlm_rotor_get_prescribed_wake(vlm_ctx *ctx, int ir, int wakeType, double *fc, double *gam,
+                                  double *rc, int *built)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (wakeType != 0 && wakeType != 1) return fail(ctx, VLM_ERR_ARG, "wakeType must be 0 or 1");
+    VLM_DEVICE(ctx);
+    VLM_TRY(sync_stream(ctx));
+    const size_t nb = (size_t)r->d.nb;
+    if (fc)
+        VLM_HIP(hipMemcpy(fc, r->pf[wakeType].p, 6 * nb * PF_N * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    if (gam || rc) {
+        std::vector<double> S(4 * nb);
+        VLM_HIP(hipMemcpy(S.data(), r->ps[wakeType].p, 4 * nb * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        for (size_t b = 0; b < nb; ++b)
+            for (int i = 0; i < PF_N; ++i) {
+                if (gam) gam[b * PF_N + i] = S[4 * b];
+                if (rc) rc[b * PF_N + i] = S[4 * b + 1];
+            }
+    }
+    if (built) *built = r->presc_built[wakeType];
     return VLM_OK;
 }
 

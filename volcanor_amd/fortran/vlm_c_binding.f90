@@ -105,6 +105,23 @@ module vlm_c
       real(c_double), value :: initWakeVel
     end function
 
+    integer(c_int) function vlm_rotor_set_prescribed_wake(ctx, ir, prescWakeNt, prescWakeGenNt, &
+        & deltaPsi) bind(C, name='vlm_rotor_set_prescribed_wake')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir, prescWakeNt, prescWakeGenNt
+      real(c_double), value :: deltaPsi
+    end function
+
+    integer(c_int) function vlm_rotor_get_prescribed_wake(ctx, ir, wakeType, fc, gam, rc, built) &
+        bind(C, name='vlm_rotor_get_prescribed_wake')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir, wakeType
+      type(c_ptr), value :: fc, gam, rc
+      integer(c_int), intent(out) :: built
+    end function
+
     integer(c_int) function vlm_rotor_init_cursors(ctx, ir) bind(C, name='vlm_rotor_init_cursors')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

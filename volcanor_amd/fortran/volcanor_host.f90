@@ -84,6 +84,7 @@ module volcanor_host
     integer :: surfaceType = 0, propConvention = 0, spanSpacing = 1, chordSpacing = 1
     integer :: axisymmetrySwitch = 0, symmetricTau = 0, forceCalcSwitch = 0
     integer :: wakeTruncateNt = 0, rollupStart = 1, rollupEnd = 1
+    integer :: prescWakeAfterTruncNt = 0, prescWakeGenNt = 0, prescWakeNt = 0
     integer :: spanwiseLiftSwitch = 0, suppressFwakeSwitch = 0
     integer :: rowNear = 0, rowFar = 0
     character(len=30) :: geometryFile = '0'
@@ -620,7 +621,7 @@ contains
     if (customTrajectorySwitch /= 0 .and. customTrajectorySwitch /= 1) &
       & call host_fail('ERROR: customTrajectorySwitch must be 0 or 1')
     r%customTrajectorySwitch = customTrajectorySwitch
-    if (prescWakeAfterTruncNt /= 0) call host_fail('ERROR: the prescribed far wake is not supported')
+    r%prescWakeAfterTruncNt = prescWakeAfterTruncNt; r%prescWakeGenNt = prescWakeGenNt
 
     r%surfaceType = surfaceType
     r%nb = nb; r%propConvention = propConvention; r%spanSpacing = spanSpacing
@@ -875,10 +876,20 @@ contains
     call to_chords_revs(r, sw%wakePlot, dt)
     call to_chords_revs(r, sw%gridPlot, dt)
     call to_chords_revs(r, r%wakeTruncateNt, dt)
+    call to_chords_revs(r, r%prescWakeAfterTruncNt, dt)
+    call to_chords_revs(r, r%prescWakeGenNt, dt)
     call to_chords_revs(r, r%nNwake, dt)
     call to_chords_revs(r, r%inflowPlotSwitch, dt)
     call to_chords_revs(r, r%gammaPlotSwitch, dt)
     call to_chords_revs(r, r%skewPlotSwitch, dt)
+    ! the prescribed far wake starts prescWakeAfterTruncNt steps after the
+    ! truncation, counted before wakeTruncateNt is raised to nNwake + 1
+    ! (classdef.f90:3013-3021)
+    if (r%wakeTruncateNt > 0 .and. r%prescWakeAfterTruncNt > 0) then
+      r%prescWakeNt = r%wakeTruncateNt + r%prescWakeAfterTruncNt
+    else
+      r%prescWakeNt = 0
+    endif
     if (r%wakeTruncateNt > 0) r%wakeTruncateNt = max(r%wakeTruncateNt, r%nNwake + 1)
     if (r%surfaceType == 0) r%surfaceType = 1
     if (r%nNwake > 0 .and. r%nNwake < 2) call host_fail('ERROR: Atleast 2 near wake rows mandatory')
@@ -1878,10 +1889,11 @@ contains
   end subroutine tip_file
 
   subroutine geom_file(ctx, dir, timestamp, r, wakeSuppress)
-    !! geom2file, libPostprocess.f90:143-331, for lifting surfaces without a
-    !! prescribed far wake: per blade the wing panel corners and -Γ, the
-    !! active near-wake lattice and -Γ, and the far-wake tip line and -Γ, as
-    !! Tecplot blocks in the reference's list-directed format.  The wake
+    !! geom2file, libPostprocess.f90:143-331, for lifting surfaces: per
+    !! blade the wing panel corners and -Γ, the active near-wake lattice and
+    !! -Γ, the far-wake tip line and -Γ and, once built, the prescribed far
+    !! wake with -Γ and the core radius, as Tecplot blocks in the
+    !! reference's list-directed format.  The wake
     !! comes from the device (vlm_rotor_get_wake).  The skew column is 0:
     !! the reference fills the wake rings' skew only for skew plots.
     type(c_ptr), intent(in) :: ctx
@@ -1894,17 +1906,24 @@ contains
     real(dp) :: wakeTip(3, max(r%nFwake, 1) + 1), tip1(3)
     character(len=5) :: nxChar, nyChar
     character(len=2) :: bid
+    integer, parameter :: PF_N = 240
+    real(c_double), allocatable, target :: pfc(:, :, :, :), pgam(:, :), prc(:, :)
+    integer(c_int) :: pbuilt
     integer :: u, ib, i, j, nx, ny, nN, nF
     logical :: wake
 
     nN = r%nNwake; nF = r%nFwake
     wake = wakeSuppress == 0 .and. nN > 0
+    pbuilt = 0
     if (wake) then
       allocate (nodes(3, r%ns + 1, nN, r%nb), le(3, r%ns + 1, r%nb), gam(r%ns, nN, r%nb), &
         & rc4(4, r%ns, nN, r%nb), ffc(3, 2, max(nF, 1), r%nb), fgam(max(nF, 1), r%nb), &
         & frc(max(nF, 1), r%nb))
       call vlm_check(ctx, vlm_rotor_get_wake(ctx, r%dev, VLM_WAKE_C, c_loc(nodes), c_loc(le), &
         & c_loc(gam), c_loc(rc4), c_loc(ffc), c_loc(fgam), c_loc(frc)), 'get_wake')
+      allocate (pfc(3, 2, PF_N, r%nb), pgam(PF_N, r%nb), prc(PF_N, r%nb))
+      call vlm_check(ctx, vlm_rotor_get_prescribed_wake(ctx, r%dev, VLM_WAKE_C, c_loc(pfc), &
+        & c_loc(pgam), c_loc(prc), pbuilt), 'get_prescribed_wake')
     endif
     ! the reference appends (position='append'); within one run each file is
     ! written once, so replacing gives its output also when a case is rerun
@@ -1986,16 +2005,30 @@ contains
         write (u, *) 0._dp
         write (u, *) 0._dp
       endif
-      ! no prescribed far wake: the same redundant point
-      write (nxChar, '(I5)') 2
-      write (u, *) 'Zone I='//trim(nxChar)//' J=1   K=1   T="PrescFarWake"'
-      write (u, *) 'DATAPACKING=BLOCK'
-      write (u, *) 'VARLOCATION=([4]=CELLCENTERED, [5]=CELLCENTERED)'
-      write (u, *) tip1(1), tip1(1)
-      write (u, *) tip1(2), tip1(2)
-      write (u, *) tip1(3), tip1(3)
-      write (u, *) 0._dp
-      write (u, *) 0._dp
+      if (pbuilt /= 0) then
+        ! prescribed far wake: the LE end of every filament and the TE end
+        ! of the last, -Γ and the core radius
+        write (nxChar, '(I5)') PF_N + 1
+        write (u, *) 'Zone I='//trim(nxChar)//' J=1   K=1   T="PrescFarWake"'
+        write (u, *) 'DATAPACKING=BLOCK'
+        write (u, *) 'VARLOCATION=([4]=CELLCENTERED ,[5]=CELLCENTERED)'
+        write (u, *) (pfc(1, 2, i, ib), i=1, PF_N), pfc(1, 1, PF_N, ib)
+        write (u, *) (pfc(2, 2, i, ib), i=1, PF_N), pfc(2, 1, PF_N, ib)
+        write (u, *) (pfc(3, 2, i, ib), i=1, PF_N), pfc(3, 1, PF_N, ib)
+        write (u, *) (-1._dp*pgam(i, ib), i=1, PF_N)
+        write (u, *) (prc(i, ib), i=1, PF_N)
+      else
+        ! no prescribed far wake: the same redundant point
+        write (nxChar, '(I5)') 2
+        write (u, *) 'Zone I='//trim(nxChar)//' J=1   K=1   T="PrescFarWake"'
+        write (u, *) 'DATAPACKING=BLOCK'
+        write (u, *) 'VARLOCATION=([4]=CELLCENTERED, [5]=CELLCENTERED)'
+        write (u, *) tip1(1), tip1(1)
+        write (u, *) tip1(2), tip1(2)
+        write (u, *) tip1(3), tip1(3)
+        write (u, *) 0._dp
+        write (u, *) 0._dp
+      endif
     enddo
     close (u)
   end subroutine geom_file
@@ -2328,7 +2361,7 @@ contains
     call jr('thetaS', r%controlPitch(3)*radToDeg); call jr('thetaTwist', r%thetaTwist*radToDeg)
     call jr('u', r%velBody(1)); call jr('v', r%velBody(2)); call jr('w', r%velBody(3))
     call jr('nonDimForceDenom', r%nonDimforceDenominator); call jr('alpha0', 0._dp)
-    call ji('wakeTruncateNt', r%wakeTruncateNt); call ji('prescWakeNt', 0)
+    call ji('wakeTruncateNt', r%wakeTruncateNt); call ji('prescWakeNt', r%prescWakeNt)
     call jr('initWakeVel', r%initWakeVel); call jr('psiStart', r%psiStart*radToDeg)
     call jr('forceCalcSwitch', r%forceCalcSwitch*radToDeg)
     call jr('apparentViscCoeff', r%apparentViscCoeff); call jr('decayCoeff', r%decayCoeff)
@@ -2653,6 +2686,12 @@ contains
       ! circulation and the load velocities; the loads (main.f90:632-719)
       ! depend only on the wing and its circulation, which the convection
       ! does not change, so they are computed while the passes run.
+      ! the prescribed far wake's azimuth step is omegaSlow*dt of this step
+      do ir = 1, nr
+        if (rotor(ir)%prescWakeNt > 0) call vlm_check(ctx, vlm_rotor_set_prescribed_wake(ctx, &
+          & rotor(ir)%dev, int(rotor(ir)%prescWakeNt, c_int), &
+          & int(rotor(ir)%prescWakeGenNt, c_int), rotor(ir)%omegaSlow*dt), 'set_prescribed_wake')
+      enddo
       if (sw%wakeSuppress == 0) &
         & call vlm_check(ctx, vlm_convect(ctx, int(sw%fdScheme, c_int), int(iter, c_int), dt), &
         & 'convect')

@@ -58,6 +58,8 @@ _SIGS = {
     "vlm_rotor_set_wing": [_P, _I, _P, _P, _P],
     "vlm_rotor_set_axis": [_P, _I, _P, _P],
     "vlm_rotor_set_init_wake_vel": [_P, _I, _D, _I],
+    "vlm_rotor_set_prescribed_wake": [_P, _I, _I, _I, _D],
+    "vlm_rotor_get_prescribed_wake": [_P, _I, _I, _P, _P, _P, _P],
     "vlm_rotor_init_cursors": [_P, _I],
     "vlm_rotor_begin_step": [_P, _I, _I],
     "vlm_rotor_cursors": [_P, _I, C.POINTER(_I), C.POINTER(_I)],
@@ -237,6 +239,20 @@ class Context:
 
     def rotor_init_cursors(self, ir):
         self._chk(self.lib.vlm_rotor_init_cursors(self.h, ir))
+
+    def rotor_set_prescribed_wake(self, ir, presc_nt, presc_gen_nt, dpsi):
+        self._chk(self.lib.vlm_rotor_set_prescribed_wake(self.h, ir, presc_nt, presc_gen_nt,
+                                                         dpsi))
+
+    def rotor_get_prescribed_wake(self, ir, nb, wake_type=WAKE_C):
+        """(fc (nb, 240, 2, 3), gam (nb, 240), rc (nb, 240), built)"""
+        fc = np.zeros((nb, 240, 2, 3))
+        g = np.zeros((nb, 240))
+        rc = np.zeros((nb, 240))
+        built = C.c_int(0)
+        self._chk(self.lib.vlm_rotor_get_prescribed_wake(self.h, ir, wake_type, _ptr(fc),
+                                                         _ptr(g), _ptr(rc), C.byref(built)))
+        return fc, g, rc, bool(built.value)
 
     def rotor_begin_step(self, ir, it):
         self._chk(self.lib.vlm_rotor_begin_step(self.h, ir, it))
