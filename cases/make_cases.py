@@ -25,6 +25,9 @@ PLOT3D blade file) used by the tests and the benchmark.
   presc_rotor, presc_gen_rotor  the prescribed far wake (helix) of a climbing
                      rotor, from every far filament and, with axisymmetric
                      blades, from the last six
+  image_rotor(_am4), image_rotor_xy  image rotors: an axisymmetric rotor in
+                     ground effect and its image in z = 0 (AM2, AM4); a rotor
+                     with cyclic pitch and its images in x = 0 and y = 0
   restart_rotor      restart files of a forward-flight rotor
   restart_rotor_wing restart files of the rotor above the wing (AM2, sub-iterations)
   tip_rotor(_cw)     wake tip plots of a forward-flight rotor in slow start, turning
@@ -313,6 +316,43 @@ def make_all(out):
                    axisymmetrySwitch=axi, span=R, chord=c, Omega=om, theta0=8.0, pivotLE=0.25,
                    velBody=[0.0, 0.0, -2.0], apparentViscCoeff=ROTOR_AVC,
                    spanwiseCore=ROTOR_CORE, streamwiseCoreVec=ROTOR_CORE)
+
+    # image rotors (surfaceType -1; rotor_init classdef.f90:2798-2960,
+    # rotor_mirrorGamma / rotor_mirrorWake 4990-5131): image_rotor, a
+    # two-bladed axisymmetric rotor 0.6 R above the ground plane z = 0,
+    # descending, and its image (AM2, far wake from step 12 truncated at 24,
+    # wake plots at steps 24 and 48); image_rotor_xy, a three-bladed rotor
+    # with cyclic pitch and its images in the planes x = 0 and y = 0 (AB2,
+    # wake and tip plots at steps 24 and 48; PC2 on this rotor stops the
+    # reference solver with "free(): invalid pointer", images or not).  The
+    # image files hold only the entries an image reads (surface and panel
+    # counts).
+    # image_rotor_am4: the same pair under AM4 (start-up steps, predictor
+    # and corrector mirrors)
+    for name, scheme in (("image_rotor", 3), ("image_rotor_am4", 5)):
+        d = case(name)
+        write_config(os.path.join(d, "config.nml"), nt=48, dt=math.radians(5.0) / om, nr=2,
+                     velSound=340.0, wakeDissipation=1, wakeStrain=1, wakePlot=24,
+                     fdScheme=scheme)
+        write_geom(os.path.join(d, "geom01.nml"), nb=2, nc=2, ns=8, nNwake=12,
+                   wakeTruncateNt=24, axisymmetrySwitch=1, span=R, chord=c, Omega=om,
+                   theta0=8.0, pivotLE=0.25, hubCoords=[0.0, 0.0, 0.6],
+                   cgCoords=[0.0, 0.0, 0.6], velBody=[0.0, 0.0, -2.0],
+                   apparentViscCoeff=ROTOR_AVC, spanwiseCore=ROTOR_CORE,
+                   streamwiseCoreVec=ROTOR_CORE)
+        write_geom(os.path.join(d, "geom02.nml"), surfaceType=-1, imagePlane=3,
+                   imageRotorNum=1, nb=2, nc=2, ns=8, nNwake=12)
+    d = case("image_rotor_xy")
+    write_config(os.path.join(d, "config.nml"), nt=48, dt=math.radians(5.0) / om, nr=3,
+                 velSound=340.0, wakeDissipation=1, fdScheme=2, wakePlot=24, wakeTipPlot=24)
+    write_geom(os.path.join(d, "geom01.nml"), nb=3, nc=2, ns=8, nNwake=12, wakeTruncateNt=24,
+               span=R, chord=c, Omega=om, theta0=8.0, thetaC=2.0,
+               pivotLE=0.25, hubCoords=[1.3, 1.6, 0.0], cgCoords=[1.3, 1.6, 0.0],
+               velBody=[0.0, 0.0, -2.0], apparentViscCoeff=ROTOR_AVC,
+               spanwiseCore=ROTOR_CORE, streamwiseCoreVec=ROTOR_CORE)
+    for k, plane in ((2, 1), (3, 2)):
+        write_geom(os.path.join(d, f"geom0{k}.nml"), surfaceType=-1, imagePlane=plane,
+                   imageRotorNum=1, nb=3, nc=2, ns=8, nNwake=12)
 
     # custom trajectory (classdef.f90:3064-3080, main.f90:112-117 and
     # 419-426): the AR-4 wing's body velocity and rate follow

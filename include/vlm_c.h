@@ -69,6 +69,7 @@ enum {
 };
 
 enum { VLM_WAKE_C = 0, VLM_WAKE_P = 1 };     /* current / predicted wake */
+enum { VLM_WAKE_A = 2 };                     /* every row of the current wake (mirror) */
 enum { VLM_EDGE_LE = 0, VLM_EDGE_TE = 1 };   /* assignshed edge */
 enum {                                        /* fdScheme, main.f90:844 */
     VLM_SCHEME_FT1 = 0, VLM_SCHEME_PC2 = 1, VLM_SCHEME_AB2 = 2,
@@ -225,10 +226,41 @@ int vlm_rotor_set_prescribed_wake(vlm_ctx *ctx, int ir, int prescWakeNt, int pre
 int vlm_rotor_get_prescribed_wake(vlm_ctx *ctx, int ir, int wakeType, double *fc, double *gam,
                                   double *rc, int *built);
 
+/* Image rotors (surfaceType = -1; rotor_init 2798-2960, rotor_mirrorGamma
+ * 4990-4995, rotor_mirrorWake 5024-5131; main.f90:181-192, 595-599,
+ * 844-1406, 1418-1437): rotor ir becomes the mirror image of rotor source
+ * in the plane where coordinate imagePlane (1 x, 2 y, 3 z) is zero.  Both
+ * rotors must have the same nb, nc, ns, nNwake and nFwake, and the source
+ * must not itself be an image.  An image takes no part as a target of the
+ * wake passes (its velocities there would go unused); inside vlm_convect,
+ * at its place in the rotor order, every convection of a lifting rotor
+ * (wake type C or P) is a vlm_rotor_mirror_wake of that type for an image.
+ * Its wing, wake and circulation remain sources like any rotor's. */
+int vlm_rotor_set_image(vlm_ctx *ctx, int ir, int source, int imagePlane);
+
+/* rotor_mirrorGamma followed by rotor_map_gam: the device wing circulation
+ * of image ir becomes the negated wing circulation of its source, enqueued
+ * in stream order (after a vlm_rotor_solve_deferred of the source that came
+ * before it).  The caller's gamVec is the source's negated (the device
+ * holds none for an image). */
+int vlm_rotor_mirror_gamma(vlm_ctx *ctx, int ir);
+
+/* rotor_mirrorWake: the image's wake becomes its source's with coordinate
+ * imagePlane and the circulation negated, cores, ages and lengths copied.
+ * VLM_WAKE_C: current near rows from rowNear and far filaments from rowFar;
+ * VLM_WAKE_P: the predicted wake's positions over the same rows, with the
+ * circulation, cores and ages (which the predicted wake shares with the
+ * current one here, and which the reference copies inside its predicted
+ * rings); VLM_WAKE_A:
+ * every near row and far filament of the current wake (end of a step, in
+ * place of the rollup and the trailing-edge shed). */
+int vlm_rotor_mirror_wake(vlm_ctx *ctx, int ir, int wakeType);
+
 /* One whole convection block, main.f90:800-1406: pass 1, the scheme
  * algebra of fdScheme (FT1, PC2, AB2, AM2, AM3 or AM4), pass 2 where the
- * scheme has one, and the final convection of every rotor.  The learned
- * pass 2 (vlm_set_predictor) replaces pass 2 of AM2 only. */
+ * scheme has one, and the final convection of every rotor (the wake
+ * mirror for an image rotor).  The learned pass 2 (vlm_set_predictor)
+ * replaces pass 2 of AM2 only. */
 int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt);
 
 /* Restart snapshot of a rotor's device state: every wing, near-wake,

@@ -32,6 +32,9 @@ _SIGS = {
     "or_rotor_age_wake": ([_P, _I, _D, _D], None),
     "or_rotor_dissipate_wake": ([_P, _I, _D, _D, _D, _D], None),
     "or_rotor_strain_wake": ([_P, _I], None),
+    "or_rotor_set_image": ([_P, _I, _I, _I], None),
+    "or_rotor_mirror_gamma": ([_P, _I], None),
+    "or_rotor_mirror_wake": ([_P, _I, _I], None),
     "or_rotor_rollup": ([_P, _I], None),
     "or_rotor_convectwake": ([_P, _I, _D, _I, _I], None),
     "or_rotor_vind": ([_P, _I, _I, _L, _P, _P], None),
@@ -189,6 +192,15 @@ class System:
 
     def rotor_strain_wake(self, ir):
         self.lib.or_rotor_strain_wake(self.h, ir)
+
+    def rotor_set_image(self, ir, source, plane):
+        self.lib.or_rotor_set_image(self.h, ir, source, plane)
+
+    def rotor_mirror_gamma(self, ir):
+        self.lib.or_rotor_mirror_gamma(self.h, ir)
+
+    def rotor_mirror_wake(self, ir, mode):
+        self.lib.or_rotor_mirror_wake(self.h, ir, mode)
 
     def rotor_rollup(self, ir):
         self.lib.or_rotor_rollup(self.h, ir)

@@ -145,6 +145,23 @@ int vlm_rotor_burst_wake(vlm_ctx *c, int ir, double skew, double chord)
 int vlm_rotor_strain_wake(vlm_ctx *c, int ir) { ROTOR_OK(c, ir); or_rotor_strain_wake(c->s, ir); return VLM_OK; }
 int vlm_rotor_rollup(vlm_ctx *c, int ir) { ROTOR_OK(c, ir); or_rotor_rollup(c->s, ir); return VLM_OK; }
 
+int vlm_rotor_set_image(vlm_ctx *c, int ir, int source, int imagePlane)
+{
+    ROTOR_OK(c, ir);
+    ROTOR_OK(c, source);
+    if (source == ir || imagePlane < 1 || imagePlane > 3) return shim_fail(c, "bad image rotor");
+    or_rotor_set_image(c->s, ir, source, imagePlane);
+    return VLM_OK;
+}
+int vlm_rotor_mirror_gamma(vlm_ctx *c, int ir) { ROTOR_OK(c, ir); or_rotor_mirror_gamma(c->s, ir); return VLM_OK; }
+int vlm_rotor_mirror_wake(vlm_ctx *c, int ir, int wakeType)
+{
+    ROTOR_OK(c, ir);
+    if (wakeType < 0 || wakeType > 2) return shim_fail(c, "wakeType must be 0, 1 or 2");
+    or_rotor_mirror_wake(c->s, ir, wakeType == VLM_WAKE_A ? 2 : wakeType);
+    return VLM_OK;
+}
+
 int vlm_convect(vlm_ctx *c, int scheme, int iter, double dt)
 {
     if (scheme < 0 || scheme > 3) return shim_fail(c, "schemes FT1, PC2, AB2 and AM2 only");

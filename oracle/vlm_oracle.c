@@ -155,6 +155,9 @@ typedef struct {
     fil_t *pf[2];            /* nb * OR_NPF */
     double *presc_pitch[2];  /* nb */
     double *presc_radius[2]; /* nb */
+    /* image rotor (surfaceType -1): the rotor it mirrors (-1: none) and
+     * the negated coordinate (0 x, 1 y, 2 z) */
+    int image_of, image_axis;
 } rotor_t;
 
 #define OR_MAX_ROTORS 16
@@ -206,6 +209,7 @@ int or_rotor_create(or_sys *s, int nb, int nc, int ns, int nNwake, int nFwake,
     r->rollupStart = rollupStart; r->rollupEnd = rollupEnd;
     r->rollupMin = rollup_min; r->suppressFwake = suppressFwake;
     r->shaft[2] = 1.0;
+    r->image_of = -1;
     r->blade = (blade_t *)calloc((size_t)nb, sizeof(blade_t));
     size_t nvn = 3u * (size_t)nNwake * (size_t)(ns + 1);
     size_t nvf = 3u * (size_t)(nFwake > 0 ? nFwake : 1);
@@ -415,7 +419,9 @@ void or_wake_velocity(or_sys *s, int pred)
 {
     for (int ir = 0; ir < s->nr; ++ir) {
         rotor_t *r = &s->rotor[ir];
-        if (r->nNwake <= 0) continue;
+        /* an image rotor's wake velocities are never used: it mirrors its
+         * source instead of convecting (main.f90:851-856, 884-915) */
+        if (r->nNwake <= 0 || r->image_of >= 0) continue;
         for (int b = 0; b < r->nbConvect; ++b) {
             blade_t *bl = &r->blade[b];
             double *VN_ = pred ? bl->velNP : bl->velN;
@@ -827,6 +833,58 @@ void or_rotor_convectwake(or_sys *s, int ir, double dt, int pred, int bug_compat
 }
 
 /* ------------------------------------------------------------------ */
+/* Image rotors, classdef.f90:722-735, 982-993, 4990-5131              */
+/* ------------------------------------------------------------------ */
+void or_rotor_set_image(or_sys *s, int ir, int src, int plane)
+{
+    s->rotor[ir].image_of = src;
+    s->rotor[ir].image_axis = plane - 1;
+}
+
+/* rotor_mirrorGamma + map_gam: every wing ring's circulation is the
+ * source's negated. */
+void or_rotor_mirror_gamma(or_sys *s, int ir)
+{
+    rotor_t *r = &s->rotor[ir];
+    const rotor_t *q = &s->rotor[r->image_of];
+    for (int b = 0; b < r->nb; ++b)
+        for (int k = 0; k < r->nc * r->ns; ++k)
+            r->blade[b].wiP[k].gam = -q->blade[b].wiP[k].gam;
+}
+
+/* rotor_mirrorWake: mode 0 'C' (waN from rowNear, waF from rowFar), 1 'P'
+ * (waNPredicted, waFPredicted over the same rows), 2 'A' (every waN and
+ * waF); rings and filaments are copied whole, then vr_mirror /
+ * Fwake_mirror negate coordinate image_axis of every end and gam. */
+void or_rotor_mirror_wake(or_sys *s, int ir, int mode)
+{
+    rotor_t *r = &s->rotor[ir];
+    const rotor_t *q = &s->rotor[r->image_of];
+    const int ax = r->image_axis;
+    const int i0 = mode == 2 ? 0 : r->rowNear - 1, f0 = mode == 2 ? 0 : r->rowFar - 1;
+    for (int b = 0; b < r->nb; ++b) {
+        ring_t *N = mode == 1 ? r->blade[b].waNP : r->blade[b].waN;
+        fil_t *F = mode == 1 ? r->blade[b].waFP : r->blade[b].waF;
+        const ring_t *NQ = mode == 1 ? q->blade[b].waNP : q->blade[b].waN;
+        const fil_t *FQ = mode == 1 ? q->blade[b].waFP : q->blade[b].waF;
+        for (int j = 0; j < r->ns; ++j)
+            for (int i = i0; i < r->nNwake; ++i) {
+                ring_t *w = &N[WN(r, i, j)];
+                *w = NQ[WN(r, i, j)];
+                for (int c = 0; c < 4; ++c) w->c[c][ax] = -w->c[c][ax];
+                w->gam = -w->gam;
+            }
+        for (int i = f0; i < r->nFwake; ++i) {
+            fil_t *f = &F[i];
+            *f = FQ[i];
+            f->fc[0][ax] = -f->fc[0][ax];
+            f->fc[1][ax] = -f->fc[1][ax];
+            f->gam = -f->gam;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* Scheme algebra, main.f90:844-1115; libCommon.f90:213-258            */
 /* ------------------------------------------------------------------ */
 static void copy_predicted(rotor_t *r)
@@ -870,14 +928,24 @@ void or_convect(or_sys *s, int scheme, int iter, double dt, int bug_compat)
 {
     size_t nr = (size_t)s->nr;
     or_wake_velocity(s, 0);
+    /* an image rotor (main.f90:851-856 and every later scheme block)
+     * mirrors its source's wake where a lifting rotor convects */
+#define OR_IMAGE(r, ir, mode)                                \
+    if ((r)->image_of >= 0) {                                \
+        if ((r)->nNwake > 0) or_rotor_mirror_wake(s, (int)(ir), (mode)); \
+        continue;                                            \
+    }
     if (scheme == 0) {
-        for (size_t ir = 0; ir < nr; ++ir)
+        for (size_t ir = 0; ir < nr; ++ir) {
+            OR_IMAGE(&s->rotor[ir], ir, 0)
             if (s->rotor[ir].nNwake > 0) or_rotor_convectwake(s, (int)ir, dt, 0, bug_compat);
+        }
         return;
     }
     if (scheme == 1) {  /* PC2, main.f90:861-949 */
         for (size_t ir = 0; ir < nr; ++ir) {
             rotor_t *r = &s->rotor[ir];
+            OR_IMAGE(r, ir, 1)
             if (r->nNwake <= 0) continue;
             copy_predicted(r);
             or_rotor_convectwake(s, (int)ir, dt, 1, bug_compat);
@@ -885,6 +953,7 @@ void or_convect(or_sys *s, int scheme, int iter, double dt, int bug_compat)
         or_wake_velocity(s, 1);
         for (size_t ir = 0; ir < nr; ++ir) {
             rotor_t *r = &s->rotor[ir];
+            OR_IMAGE(r, ir, 0)
             if (r->nNwake <= 0) continue;
             for (int b = 0; b < r->nbConvect; ++b) {
                 blade_t *bl = &r->blade[b];
@@ -901,6 +970,7 @@ void or_convect(or_sys *s, int scheme, int iter, double dt, int bug_compat)
     if (iter == 1) {  /* AB2 / AM2 first step */
         for (size_t ir = 0; ir < nr; ++ir) {
             rotor_t *r = &s->rotor[ir];
+            OR_IMAGE(r, ir, 0)
             if (r->nNwake <= 0) continue;
             or_rotor_convectwake(s, (int)ir, dt, 0, bug_compat);
             nvn = 3u * (size_t)r->nNwake * (size_t)(r->ns + 1);
@@ -915,6 +985,7 @@ void or_convect(or_sys *s, int scheme, int iter, double dt, int bug_compat)
     if (scheme == 2) {  /* AB2, main.f90:968-1000 */
         for (size_t ir = 0; ir < nr; ++ir) {
             rotor_t *r = &s->rotor[ir];
+            OR_IMAGE(r, ir, 0)
             if (r->nNwake <= 0) continue;
             nvn = 3u * (size_t)r->nNwake * (size_t)(r->ns + 1);
             nvf = 3u * (size_t)r->nFwake;
@@ -938,6 +1009,7 @@ void or_convect(or_sys *s, int scheme, int iter, double dt, int bug_compat)
     /* AM2, main.f90:1019-1115 */
     for (size_t ir = 0; ir < nr; ++ir) {
         rotor_t *r = &s->rotor[ir];
+        OR_IMAGE(r, ir, 1)
         if (r->nNwake <= 0) continue;
         copy_predicted(r);
         nvn = 3u * (size_t)r->nNwake * (size_t)(r->ns + 1);
@@ -958,6 +1030,7 @@ void or_convect(or_sys *s, int scheme, int iter, double dt, int bug_compat)
     or_wake_velocity(s, 1);
     for (size_t ir = 0; ir < nr; ++ir) {
         rotor_t *r = &s->rotor[ir];
+        OR_IMAGE(r, ir, 0)
         if (r->nNwake <= 0) continue;
         nvn = 3u * (size_t)r->nNwake * (size_t)(r->ns + 1);
         nvf = 3u * (size_t)r->nFwake;

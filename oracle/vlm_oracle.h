@@ -80,6 +80,14 @@ void or_wake_velocity(or_sys *s, int predicted);
 /* Whole convection block of main.f90:844-1406 for fdScheme 0..3. */
 void or_convect(or_sys *s, int scheme, int iter, double dt, int bug_compat);
 
+/* Image rotors (surfaceType -1): rotor ir mirrors rotor src in the plane
+ * where coordinate `plane` (1..3) is zero; or_convect then mirrors instead
+ * of convecting it.  mirror_gamma: wing circulation = -source's;
+ * mirror_wake: mode 0 'C', 1 'P', 2 'A' (rotor_mirrorWake). */
+void or_rotor_set_image(or_sys *s, int ir, int src, int plane);
+void or_rotor_mirror_gamma(or_sys *s, int ir);
+void or_rotor_mirror_wake(or_sys *s, int ir, int mode);
+
 /* Accessors in the lattice form used by the library (see vlm_c.h). */
 /* Prescribed far wake (pFwake_update, classdef.f90:998-1066, 5172-5217):
  * rebuilt after every convection of the wake type once the step number

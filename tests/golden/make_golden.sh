@@ -154,6 +154,24 @@ for c in presc_rotor presc_gen_rotor; do
   done
   rm -rf "$tmp"
 done
+# 5j. image rotors: CL/CT histories of every rotor, wing and wake plots at
+# steps 24 and 48, and (image_rotor_xy) tip plots at steps 24 and 48
+for c in image_rotor image_rotor_am4 image_rotor_xy; do
+  tmp=$(mktemp -d)
+  cp "$ROOT/cases/$c.case/"* "$tmp/"
+  mkdir -p "$tmp/Results" "$tmp/Restart"
+  (cd "$tmp" && ulimit -s unlimited && OMP_NUM_THREADS=8 OMP_SCHEDULE=static \
+     "$ROOT/oracle/_ref/volcanor" > run.log 2>&1)
+  for r in r01 r02 r03; do
+    [ -f "$tmp/Results/${r}ForceNonDim.csv" ] || continue
+    cp "$tmp/Results/${r}ForceNonDim.csv" "$HERE/${c}_${r}ForceNonDim.csv"
+    for st in 00024 00048; do
+      cp "$tmp/Results/${r}wingNwake$st.tec" "$HERE/${c}_${r}wingNwake$st.tec"
+      [ -f "$tmp/Results/${r}tip$st.tec" ] && cp "$tmp/Results/${r}tip$st.tec" "$HERE/${c}_${r}tip$st.tec"
+    done
+  done
+  rm -rf "$tmp"
+done
 # 6. velocity probes: the reference's probes2file output of a short wing run
 tmp=$(mktemp -d)
 cp "$ROOT/cases/probes_wing.case/"* "$tmp/"

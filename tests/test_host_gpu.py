@@ -696,3 +696,38 @@ def test_prescribed_far_wake_matches_reference_solver(name):
                 np.testing.assert_allclose(zg[k], ze[k], rtol=1e-8, atol=1e-10,
                                            err_msg=f"{step} {zg[0]} column {k - 2}")
     assert any(z[0] == "PrescFarWake" and z[1] == 241 for z in got)
+
+
+IMAGE_CASES = [("image_rotor", 2, False), ("image_rotor_am4", 2, False),
+               ("image_rotor_xy", 3, True)]
+
+
+@pytest.mark.parametrize("name,nr,tips", IMAGE_CASES)
+def test_image_rotors_match_reference_solver(name, nr, tips):
+    """Image rotors (surfaceType -1: rotor_init classdef.f90:2798-2960,
+    rotor_mirrorVelCP / rotor_mirrorGamma / rotor_mirrorWake 4990-5131,
+    main.f90:181-193, 595-599, 844-1437) on the device: image_rotor, a
+    two-bladed axisymmetric rotor in ground effect (0.6 R above z = 0,
+    descending, AM2, far wake truncated at 24 steps) and its image in z = 0,
+    and the same pair under AM4 (image_rotor_am4); image_rotor_xy, a three-bladed rotor with cyclic pitch and its images in
+    x = 0 (rotation reversed) and y = 0 (rotation reversed, start azimuth +
+    pi), AB2.  Every column of every rotor's force history against the
+    reference solver's, and every zone of every rotor's wing and wake plots
+    (and tip plots) at steps 24 and 48 to 1e-8 relative or 1e-10 absolute."""
+    ct, _ = run(name, nrotors=nr)
+    res = os.path.join(CASES, f"{name}.case", "Results")
+    for k in range(1, nr + 1):
+        ref = read_force(os.path.join(GOLD, f"{name}_r{k:02d}ForceNonDim.csv"))
+        got = read_force(os.path.join(res, f"r{k:02d}ForceNonDim.csv"))
+        np.testing.assert_allclose(ct[k - 1], ref[:, 0], rtol=2e-6, atol=1e-9, err_msg=f"r{k:02d}")
+        np.testing.assert_allclose(got, ref, rtol=2e-6, atol=1e-9, err_msg=f"r{k:02d} columns")
+        plots = ["wingNwake"] + (["tip"] if tips else [])
+        for step in ("00024", "00048"):
+            for p in plots:
+                g = read_tec(os.path.join(res, f"r{k:02d}{p}{step}.tec"))
+                e = read_tec(os.path.join(GOLD, f"{name}_r{k:02d}{p}{step}.tec"))
+                assert [z[:3] for z in g] == [z[:3] for z in e]
+                for zg, ze in zip(g, e):
+                    for c in range(3, 8):
+                        np.testing.assert_allclose(zg[c], ze[c], rtol=1e-8, atol=1e-10,
+                                                   err_msg=f"r{k:02d} {p}{step} {zg[0]} column {c - 2}")

@@ -78,6 +78,21 @@ CASES = {
     "rotor_am2_axisym_presc": dict(nb=3, nc=1, ns=4, nNwake=4, nFwake=6, nt=14, scheme=3,
                                    omega=25.0, vel=(0, 0, 2.0), axisym=True, dissipation=True,
                                    presc=(8, 3)),
+    # image rotors (rotor_mirrorGamma / rotor_mirrorWake, classdef.f90:4990-5131):
+    # the mirror after every convection stage of the scheme and of the whole
+    # wake at the end of the step, beside the lifting source
+    "image_ft1_z": dict(nb=2, nc=1, ns=4, nNwake=4, nFwake=5, nt=14, scheme=0, omega=30.0,
+                        vel=(0, 0, 1.0), hub=[0.0, 0.0, 0.5],
+                        surfaces=[{}, {"image": (0, 3)}]),
+    "image_pc2_z": dict(nb=2, nc=1, ns=4, nNwake=4, nFwake=5, nt=14, scheme=1, omega=30.0,
+                        vel=(0, 0, 1.0), hub=[0.0, 0.0, 0.5],
+                        surfaces=[{}, {"image": (0, 3)}]),
+    "image_am2_z": dict(nb=2, nc=1, ns=4, nNwake=4, nFwake=5, nt=14, scheme=3, omega=30.0,
+                        vel=(0, 0, 1.0), hub=[0.0, 0.0, 0.5], dissipation=True, strain=True,
+                        surfaces=[{}, {"image": (0, 3)}]),
+    "image_ab2_xy_axisym": dict(nb=3, nc=1, ns=4, nNwake=4, nFwake=5, nt=14, scheme=2,
+                                omega=25.0, vel=(0, 0, 2.0), hub=[1.5, 1.8, 0.0], axisym=True,
+                                surfaces=[{}, {"image": (0, 1)}, {"image": (0, 2)}]),
 }
 
 

@@ -14,13 +14,15 @@ class Case:
     """surfaces: optional list of per-rotor overrides (nb, nc, ns, nNwake,
     nFwake, omega, span, chord, pitch, rollup, hub, ...); rotor ir then
     uses the case's values with surfaces[ir] applied (nr = len(surfaces)).
-    presc: (prescWakeNt, prescWakeGenNt) of a prescribed far wake."""
+    presc: (prescWakeNt, prescWakeGenNt) of a prescribed far wake.
+    image: (source rotor, imagePlane) of an image rotor, whose wing is the
+    source's mirrored and whose circulation and wake are mirrored copies."""
 
     def __init__(self, nb=2, nc=2, ns=4, nNwake=6, nFwake=4, nt=14, scheme=3, axisym=False,
                  dissipation=False, strain=False, omega=0.0, vel=(10.0, 0.0, 0.0), dt=0.02,
                  span=1.0, root=0.2, chord=0.25, pitch=6.0, rollup=(3, 4), rollupMin=1,
                  spanwiseCore=0.02, scv=None, bug_compat=True, nr=1, hub_dx=0.0, hub=None,
-                 surfaces=None, presc=None):
+                 surfaces=None, presc=None, image=None):
         self.__dict__.update(locals())
         del self.__dict__["self"]
         if scv is None:
@@ -46,6 +48,14 @@ class Case:
 def blade_grids(case, t, ir=0):
     """Rigid motion at time t: blades rotate about z through the hub and
     the whole rotor translates with -vel."""
+    if case.surface(ir).image is not None:
+        src, plane = case.surface(ir).image
+        grids, hub = blade_grids(case, t, src)
+        for g in grids:
+            g[..., plane - 1] *= -1.0
+        hub = hub.copy()
+        hub[plane - 1] *= -1.0
+        return grids, hub
     case = case.surface(ir)
     hub = np.array(case.hub if case.hub is not None else [case.hub_dx * ir, 0.0, 0.0], float)
     g0 = blade_grid(case.nc, case.ns, chord=case.chord, span=case.span,
@@ -84,6 +94,8 @@ def setup(sys_, case):
         sys_.rotor_set_axis(k, [0.0, 0.0, 1.0], hub)
         if c.presc is not None:
             sys_.rotor_set_prescribed_wake(k, c.presc[0], c.presc[1], c.omega * c.dt)
+        if c.image is not None:
+            sys_.rotor_set_image(k, irs[c.image[0]], c.image[1])
         sys_.rotor_init_cursors(k)
         sys_.rotor_assignshed(k, 1)  # main.f90:227-233: shed as TE
     sys_.set_bug_compat(case.bug_compat)
@@ -102,13 +114,19 @@ def step(sys_, case, irs, it):
         sys_.rotor_age_wake(ir, case.dt)
         if case.dissipation:
             sys_.rotor_dissipate_wake(ir, case.dt, 1.8e-5, 50.0, 0.0)
-    for ir in irs:  # stands in for the circulation solve
-        sys_.rotor_set_wing(ir, gam=wing_gamma(case, it, ir))
+    for k, ir in enumerate(irs):  # stands in for the circulation solve
+        if case.surface(k).image is not None:
+            sys_.rotor_mirror_gamma(ir)
+        else:
+            sys_.rotor_set_wing(ir, gam=wing_gamma(case, it, k))
     sys_.convect(case.scheme, it, case.dt)
     if case.strain:
         for ir in irs:
             sys_.rotor_strain_wake(ir)
-    for ir in irs:
+    for k, ir in enumerate(irs):
+        if case.surface(k).image is not None:  # main.f90:1433-1436
+            sys_.rotor_mirror_wake(ir, 2)
+            continue
         rn, _ = sys_.rotor_cursors(ir)
         if rn == 1:
             sys_.rotor_rollup(ir)

@@ -98,6 +98,9 @@ struct Rotor {
     double presc_dpsi = 0.0;             // omegaSlow * dt
     int presc_built[2] = {0, 0};
     int iter = 0;                        // step of the last vlm_rotor_begin_step
+    // image rotor: index of the rotor it mirrors (-1: a lifting rotor of its
+    // own) and the coordinate negated by the mirror (0 x, 1 y, 2 z)
+    int image_of = -1, image_axis = 0;
     size_t nvn() const { return 3u * d.nb * (size_t)d.nNwake * (d.ns + 1); }
     size_t nvf() const { return 3u * d.nb * (size_t)(d.nFwake > 0 ? d.nFwake : 0); }
 };

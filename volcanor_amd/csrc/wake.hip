@@ -894,6 +894,74 @@ __global__ void fill_kernel(double *p, int64_t n, double val)
     if (k < n) p[k] = val;
 }
 
+// rotor_mirrorWake (classdef.f90:5024-5131) with vr_mirror / Fwake_mirror
+// (722-735, 982-993): image d takes source s's wake on every blade, with
+// coordinate ax and the circulation negated.  mode 0 (C): current near
+// rows from i0 and far filaments from f0; 1 (P): predicted positions over
+// the same rows, with the circulation, cores and ages, which the predicted
+// wake shares with the current one (the reference copies them inside its
+// predicted rings; the current rows get the same values again at the C
+// mirror that follows); 2 (A): every row, i0 = f0 = 0.  Rows are logical
+// on both sides (each rotor has its own ring head).  One thread per near node, leading-edge
+// node, near ring or far filament of a blade.
+__global__ void mirror_wake_kernel(RV d, RV s, int mode, int ax, int i0, int f0)
+{
+    const int ns = d.ns, R = d.nN - i0, F = d.nF - f0;
+    const int64_t per = (int64_t)R * (ns + 1) + (ns + 1) + (int64_t)R * ns + F;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per * d.nb) return;
+    const int b = (int)(e / per);
+    int64_t k = e - b * per;
+    const bool pred = mode == 1;
+    auto copy3 = [ax](const double *src, double *dst) {
+        dst[0] = src[0];
+        dst[1] = src[1];
+        dst[2] = src[2];
+        dst[ax] = -src[ax];
+    };
+    if (k < (int64_t)R * (ns + 1)) {
+        const int i = i0 + (int)(k / (ns + 1)), j = (int)(k % (ns + 1));
+        copy3((pred ? s.pnode : s.node) + 3 * nidx(s, b, phys(s, i), j),
+              (pred ? d.pnode : d.node) + 3 * nidx(d, b, phys(d, i), j));
+        return;
+    }
+    k -= (int64_t)R * (ns + 1);
+    if (k < ns + 1) {
+        const size_t q = 3 * ((size_t)b * (ns + 1) + k);
+        copy3((pred ? s.ple : s.le) + q, (pred ? d.ple : d.le) + q);
+        return;
+    }
+    k -= ns + 1;
+    if (k < (int64_t)R * ns) {
+        const int i = i0 + (int)(k / ns), j = (int)(k % ns);
+        const size_t rs = ridx(s, b, phys(s, i), j), rd = ridx(d, b, phys(d, i), j);
+        d.gam[rd] = -s.gam[rs];
+        d.age[rd] = s.age[rs];
+        for (int q = 0; q < 4; ++q) d.rc4[4 * rd + q] = s.rc4[4 * rs + q];
+        return;
+    }
+    k -= (int64_t)R * ns;
+    const size_t q = (size_t)b * d.nF + f0 + k;
+    const double *FS = pred ? s.pffc : s.ffc;
+    double *FD = pred ? d.pffc : d.ffc;
+    copy3(FS + 6 * q, FD + 6 * q);
+    copy3(FS + 6 * q + 3, FD + 6 * q + 3);
+    d.fgam[q] = -s.fgam[q];
+    d.frc[q] = s.frc[q];
+    d.frc0[q] = s.frc0[q];
+    d.fl0[q] = s.fl0[q];
+    d.flc[q] = s.flc[q];
+    d.fage[q] = s.fage[q];
+}
+
+// rotor_mirrorGamma + rotor_map_gam: the image's wing circulation is the
+// negated source's, ring for ring.
+__global__ void mirror_gamma_kernel(const double *src, double *dst, int n)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) dst[e] = -src[e];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------
@@ -959,7 +1027,7 @@ namespace {
 
 int64_t target_count(const Rotor &r)
 {
-    if (r.d.nNwake <= 0) return 0;
+    if (r.d.nNwake <= 0 || r.image_of >= 0) return 0;
     const int64_t R = r.d.nNwake - (r.rowNear - 1);
     const int64_t F = r.d.nFwake - (r.rowFar - 1);
     if (R <= 0) return 0;
@@ -1138,8 +1206,27 @@ static int presc_update(vlm_ctx *ctx, Rotor &r, int pred)
     return VLM_OK;
 }
 
+// rotor_mirrorWake of image r from its source (mode 0 C, 1 P, 2 A).
+int mirror_wake(vlm_ctx *ctx, Rotor &r, int mode)
+{
+    if (r.image_of < 0 || r.image_of >= (int)ctx->rotors.size())
+        return fail(ctx, VLM_ERR_STATE, "rotor is not an image (vlm_rotor_set_image)");
+    Rotor &s = *ctx->rotors[r.image_of];
+    if (r.d.nNwake <= 0) return VLM_OK;
+    const int i0 = mode == 2 ? 0 : r.rowNear - 1, f0 = mode == 2 ? 0 : r.rowFar - 1;
+    const int R = r.d.nNwake - i0, F = r.d.nFwake - f0;
+    if (R < 0 || F < 0) return fail(ctx, VLM_ERR_STATE, "wake cursors out of range");
+    const int64_t per = (int64_t)R * (r.d.ns + 1) + (r.d.ns + 1) + (int64_t)R * r.d.ns + F;
+    hipLaunchKernelGGL(mirror_wake_kernel, dim3(nblk(per * r.d.nb)), dim3(256), 0, ctx->stream,
+                       view(r), view(s), mode, r.image_axis, i0, f0);
+    VLM_HIP(hipGetLastError());
+    return VLM_OK;
+}
+
 int rotor_convect(vlm_ctx *ctx, Rotor &r, double dt, int pred)
 {
+    // an image's convection is the mirror of its source's wake
+    if (r.image_of >= 0) return mirror_wake(ctx, r, pred ? 1 : 0);
     if (r.d.nNwake <= 0) return VLM_OK;
     RV v = view(r);
     int i0 = r.rowNear - 1, i1 = r.d.nNwake;
@@ -2050,7 +2137,11 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
     if (scheme < VLM_SCHEME_FT1 || scheme > VLM_SCHEME_AM4)
         return fail(ctx, VLM_ERR_ARG, "fdScheme must be 0..5");
     VLM_TRY(wake_pass(ctx, 0));
-    auto active = [](Rotor *r) { return r->d.nNwake > 0; };
+    // lifting rotors convect; an image mirrors its source's wake at the
+    // same place in the rotor order (rotor_convect does the mirror), and
+    // takes no part in the velocity algebra
+    auto active = [](Rotor *r) { return r->d.nNwake > 0 && r->image_of < 0; };
+    auto image = [](Rotor *r) { return r->d.nNwake > 0 && r->image_of >= 0; };
     auto init_vel = [&](bool pass2) -> int {
         for (auto *r : ctx->rotors)
             if (active(r)) VLM_TRY(add_init_wake_vel(ctx, *r, iter, pass2));
@@ -2071,6 +2162,8 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
                 if (active(r)) {
                     VLM_TRY(rotor_convect(ctx, *r, dt, 0));
                     VLM_TRY(launch_am(ctx, *r, start));
+                } else if (image(r)) {
+                    VLM_TRY(rotor_convect(ctx, *r, dt, 0));
                 }
             return VLM_OK;
         }
@@ -2078,6 +2171,8 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
             if (active(r)) {
                 VLM_TRY(copy_predicted(ctx, *r));
                 VLM_TRY(launch_am(ctx, *r, am3 ? AM3_PRED : AM4_PRED));
+                VLM_TRY(rotor_convect(ctx, *r, dt, 1));
+            } else if (image(r)) {
                 VLM_TRY(rotor_convect(ctx, *r, dt, 1));
             }
         VLM_TRY(wake_pass(ctx, 1));
@@ -2087,12 +2182,14 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
                 VLM_TRY(launch_am(ctx, *r, am3 ? AM3_CORR : AM4_CORR));
                 VLM_TRY(rotor_convect(ctx, *r, dt, 0));
                 VLM_TRY(launch_am(ctx, *r, am3 ? AM3_ROTATE : AM4_ROTATE));
+            } else if (image(r)) {
+                VLM_TRY(rotor_convect(ctx, *r, dt, 0));
             }
         return VLM_OK;
     }
     if (scheme == VLM_SCHEME_FT1) {
         for (auto *r : ctx->rotors)
-            if (active(r)) VLM_TRY(rotor_convect(ctx, *r, dt, 0));
+            if (active(r) || image(r)) VLM_TRY(rotor_convect(ctx, *r, dt, 0));
         return VLM_OK;
     }
     if (scheme == VLM_SCHEME_PC2) {
@@ -2100,10 +2197,13 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
             if (active(r)) {
                 VLM_TRY(copy_predicted(ctx, *r));
                 VLM_TRY(rotor_convect(ctx, *r, dt, 1));
+            } else if (image(r)) {
+                VLM_TRY(rotor_convect(ctx, *r, dt, 1));
             }
         VLM_TRY(wake_pass(ctx, 1));
         VLM_TRY(init_vel(true));
         for (auto *r : ctx->rotors) {
+            if (image(r)) VLM_TRY(rotor_convect(ctx, *r, dt, 0));
             if (!active(r)) continue;
             const int rows = r->d.nNwake - (r->rowNear - 1);
             const int frows = r->d.nFwake - (r->rowFar - 1);
@@ -2138,6 +2238,8 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
             if (active(r)) {
                 VLM_TRY(rotor_convect(ctx, *r, dt, 0));
                 VLM_TRY(launch_scheme(ctx, *r, 4));
+            } else if (image(r)) {
+                VLM_TRY(rotor_convect(ctx, *r, dt, 0));
             }
         return VLM_OK;
     }
@@ -2145,6 +2247,8 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
         for (auto *r : ctx->rotors)
             if (active(r)) {
                 VLM_TRY(launch_scheme(ctx, *r, 0));
+                VLM_TRY(rotor_convect(ctx, *r, dt, 0));
+            } else if (image(r)) {
                 VLM_TRY(rotor_convect(ctx, *r, dt, 0));
             }
         return VLM_OK;
@@ -2155,6 +2259,8 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
             VLM_TRY(copy_predicted(ctx, *r));
             VLM_TRY(launch_scheme(ctx, *r, 1));
             VLM_TRY(rotor_convect(ctx, *r, dt, 1));
+        } else if (image(r)) {
+            VLM_TRY(rotor_convect(ctx, *r, dt, 1));
         }
     if (ctx->pred.mode == 1) VLM_TRY(learned_pass(ctx));
     else VLM_TRY(wake_pass(ctx, 1));
@@ -2164,8 +2270,53 @@ int vlm_convect(vlm_ctx *ctx, int scheme, int iter, double dt)
             VLM_TRY(launch_scheme(ctx, *r, 2));
             VLM_TRY(rotor_convect(ctx, *r, dt, 0));
             VLM_TRY(launch_scheme(ctx, *r, 3));
+        } else if (image(r)) {
+            VLM_TRY(rotor_convect(ctx, *r, dt, 0));
         }
     return VLM_OK;
+}
+
+int vlm_rotor_set_image(vlm_ctx *ctx, int ir, int source, int imagePlane)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    Rotor *s = get_rotor(ctx, source);
+    if (!s || source == ir) return fail(ctx, VLM_ERR_ARG, "bad source rotor of the image");
+    if (imagePlane < 1 || imagePlane > 3) return fail(ctx, VLM_ERR_ARG, "imagePlane must be 1..3");
+    if (s->image_of >= 0) return fail(ctx, VLM_ERR_ARG, "the source of an image is an image");
+    for (auto *q : ctx->rotors)
+        if (q->image_of == ir) return fail(ctx, VLM_ERR_ARG, "rotor is the source of an image");
+    if (r->d.nb != s->d.nb || r->d.nc != s->d.nc || r->d.ns != s->d.ns ||
+        r->d.nNwake != s->d.nNwake || r->d.nFwake != s->d.nFwake)
+        return fail(ctx, VLM_ERR_ARG, "an image needs its source's nb, nc, ns, nNwake and nFwake");
+    r->image_of = source;
+    r->image_axis = imagePlane - 1;
+    return VLM_OK;
+}
+
+int vlm_rotor_mirror_gamma(vlm_ctx *ctx, int ir)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (r->image_of < 0) return fail(ctx, VLM_ERR_STATE, "rotor is not an image (vlm_rotor_set_image)");
+    VLM_DEVICE(ctx);
+    const Rotor &s = *ctx->rotors[r->image_of];
+    if (r->nring > 0) {
+        hipLaunchKernelGGL(mirror_gamma_kernel, dim3(nblk(r->nring)), dim3(256), 0, ctx->stream,
+                           s.wg.p, r->wg.p, r->nring);
+        VLM_HIP(hipGetLastError());
+    }
+    return VLM_OK;
+}
+
+int vlm_rotor_mirror_wake(vlm_ctx *ctx, int ir, int wakeType)
+{
+    Rotor *r = get_rotor(ctx, ir);
+    if (!r) return ctx ? fail(ctx, VLM_ERR_ARG, "unknown rotor") : VLM_ERR_ARG;
+    if (wakeType != VLM_WAKE_C && wakeType != VLM_WAKE_P && wakeType != VLM_WAKE_A)
+        return fail(ctx, VLM_ERR_ARG, "wakeType must be VLM_WAKE_C, VLM_WAKE_P or VLM_WAKE_A");
+    VLM_DEVICE(ctx);
+    return mirror_wake(ctx, *r, wakeType == VLM_WAKE_A ? 2 : wakeType == VLM_WAKE_P ? 1 : 0);
 }
 
 int vlm_rotor_set_prescribed_wake(vlm_ctx *ctx, int ir, int prescWakeNt, int prescWakeGenNt,

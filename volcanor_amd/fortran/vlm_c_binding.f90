@@ -12,7 +12,7 @@ module vlm_c
   integer(c_int), parameter :: VLM_FP64 = 0, VLM_MIXED = 1
   integer(c_int), parameter :: VLM_SET_WING = 0, VLM_SET_WING_BOUND = 1
   integer(c_int), parameter :: VLM_SET_WAKE = 2, VLM_SET_WAKE_PRED = 3
-  integer(c_int), parameter :: VLM_WAKE_C = 0, VLM_WAKE_P = 1
+  integer(c_int), parameter :: VLM_WAKE_C = 0, VLM_WAKE_P = 1, VLM_WAKE_A = 2
   integer(c_int), parameter :: VLM_EDGE_LE = 0, VLM_EDGE_TE = 1
   integer(c_int), parameter :: VLM_TERM_NEGATE = 1
   integer(c_int), parameter :: VLM_VEL = 0, VLM_VEL_PREV = 1
@@ -103,6 +103,26 @@ module vlm_c
       type(c_ptr), value :: ctx
       integer(c_int), value :: ir, initWakeVelNt
       real(c_double), value :: initWakeVel
+    end function
+
+    integer(c_int) function vlm_rotor_set_image(ctx, ir, source, imagePlane) &
+        bind(C, name='vlm_rotor_set_image')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir, source, imagePlane
+    end function
+
+    integer(c_int) function vlm_rotor_mirror_gamma(ctx, ir) bind(C, name='vlm_rotor_mirror_gamma')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir
+    end function
+
+    integer(c_int) function vlm_rotor_mirror_wake(ctx, ir, wakeType) &
+        bind(C, name='vlm_rotor_mirror_wake')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir, wakeType
     end function
 
     integer(c_int) function vlm_rotor_set_prescribed_wake(ctx, ir, prescWakeNt, prescWakeGenNt, &

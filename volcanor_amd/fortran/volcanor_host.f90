@@ -609,8 +609,18 @@ contains
     call io_check(ios, emsg, trim(fname)//' &PLOTS')
     close (u)
 
-    if (surfaceType < 0) call host_fail('ERROR: image surfaces are not supported')
-    if (surfaceType == 2) call host_fail('ERROR: non-lifting surfaces are not supported')
+    if (abs(surfaceType) == 2) call host_fail('ERROR: non-lifting surfaces are not supported')
+    r%surfaceType = surfaceType
+    r%imagePlane = imagePlane; r%imageRotorNum = imageRotorNum
+    r%nb = nb; r%propConvention = propConvention; r%spanSpacing = spanSpacing
+    r%chordSpacing = chordSpacing; r%geometryFile = geometryFile
+    r%nc = nc; r%ns = ns; r%nNwake = nNwake
+    if (surfaceType < 0) then
+      ! an image keeps only the entries above; rotor_init takes the rest
+      ! from its source (classdef.f90:2657-2675, 2798-2955)
+      if (imagePlane < 1 .or. imagePlane > 3) call host_fail('ERROR: imagePlane must be 1, 2 or 3')
+      return
+    endif
     if (nCamberFiles > 8) call host_fail('ERROR: at most 8 camber files')
     r%nCamberFiles = nCamberFiles
     r%camberSectionLimit = camberSectionLimit; r%camberFile = camberFile
@@ -623,10 +633,6 @@ contains
     r%customTrajectorySwitch = customTrajectorySwitch
     r%prescWakeAfterTruncNt = prescWakeAfterTruncNt; r%prescWakeGenNt = prescWakeGenNt
 
-    r%surfaceType = surfaceType
-    r%nb = nb; r%propConvention = propConvention; r%spanSpacing = spanSpacing
-    r%chordSpacing = chordSpacing; r%geometryFile = geometryFile
-    r%nc = nc; r%ns = ns; r%nNwake = nNwake
     r%hubCoords = hubCoords; r%cgCoords = cgCoords; r%fromCoords = fromCoords
     r%pts = phiThetaPsi
     r%radius = span; r%root_cut = rootcut; r%chord = chord; r%preconeAngle = preconeAngle
@@ -644,7 +650,6 @@ contains
     endif
     r%rollupStartRadius = rollupStartRadius; r%rollupEndRadius = rollupEndRadius
     r%psiStart = psiStart
-    r%imagePlane = imagePlane; r%imageRotorNum = imageRotorNum
     r%thetaTwist = thetaTwist; r%initWakeVel = initWakeVel
     r%inflowPlotSwitch = inflowPlotSwitch; r%gammaPlotSwitch = gammaPlotSwitch
     r%skewLimit = skewLimit
@@ -838,7 +843,80 @@ contains
   ! ------------------------------------------------------------------
   ! Geometry initialisation, classdef.f90:2769-3891
   ! ------------------------------------------------------------------
-  subroutine rotor_init(r, ir, density, dt, nt, sw, dir)
+  subroutine image_params(r, src, nt)
+    !! The parameters an image rotor takes from its (initialised) source,
+    !! mirrored in the plane where coordinate imagePlane is zero
+    !! (rotor_init, classdef.f90:2798-2955).  As there, only the hub and cg
+    !! are mirrored for every plane; the rotation sense, start azimuth,
+    !! pitch and body motion follow the per-plane rules of 2922-2933, and
+    !! the body axes, mirrored there, are reset to x, y, z later (3034-3036),
+    !! which leaves them as they are here.  prescWakeAfterTruncNt is not
+    !! copied (the image never builds a prescribed wake: it does not convect).
+    type(rotor_class), intent(inout) :: r
+    type(rotor_class), intent(in) :: src
+    integer, intent(in) :: nt
+    integer :: p
+    p = r%imagePlane
+    r%nb = src%nb; r%propConvention = src%propConvention
+    r%spanSpacing = src%spanSpacing; r%chordSpacing = src%chordSpacing
+    r%nCamberFiles = src%nCamberFiles
+    r%camberSectionLimit = src%camberSectionLimit; r%camberFile = src%camberFile
+    r%nc = src%nc; r%ns = src%ns; r%nNwake = src%nNwake
+    r%hubCoords = src%hubCoords; r%cgCoords = src%cgCoords; r%fromCoords = src%fromCoords
+    r%hubCoords(p) = -1._dp*r%hubCoords(p)
+    r%cgCoords(p) = -1._dp*r%cgCoords(p)
+    r%pts = src%pts; r%radius = src%radius; r%root_cut = src%root_cut; r%chord = src%chord
+    r%preconeAngle = src%preconeAngle
+    r%Omega = src%Omega; r%shaftAxis = src%shaftAxis
+    r%controlPitch = src%controlPitch; r%thetaTwist = src%thetaTwist
+    r%customTrajectorySwitch = src%customTrajectorySwitch
+    r%axisymmetrySwitch = src%axisymmetrySwitch
+    r%velBody = src%velBody; r%omegaBody = src%omegaBody
+    r%pivotLE = src%pivotLE; r%flapHinge = src%flapHinge
+    r%spanwiseLiftSwitch = src%spanwiseLiftSwitch; r%symmetricTau = src%symmetricTau
+    r%apparentViscCoeff = src%apparentViscCoeff; r%decayCoeff = src%decayCoeff
+    r%wakeTruncateNt = src%wakeTruncateNt
+    r%prescWakeNt = src%prescWakeNt; r%prescWakeGenNt = src%prescWakeGenNt
+    r%spanwiseCore = src%spanwiseCore
+    if (allocated(r%streamwiseCoreVec)) deallocate (r%streamwiseCoreVec)
+    allocate (r%streamwiseCoreVec(r%ns + 1))
+    r%streamwiseCoreVec = src%streamwiseCoreVec
+    r%rollupStartRadius = src%rollupStartRadius; r%rollupEndRadius = src%rollupEndRadius
+    r%rollupStart = src%rollupStart; r%rollupEnd = src%rollupEnd
+    r%initWakeVel = src%initWakeVel; r%psiStart = src%psiStart; r%skewLimit = src%skewLimit
+    r%flapInitial = src%flapInitial; r%dflapInitial = src%dflapInitial; r%Iflap = src%Iflap
+    r%cflap = src%cflap; r%kflap = src%kflap; r%MflapConstant = src%MflapConstant
+    r%dpitch = src%dpitch
+    r%inflowPlotSwitch = src%inflowPlotSwitch; r%gammaPlotSwitch = src%gammaPlotSwitch
+    r%skewPlotSwitch = src%skewPlotSwitch
+    r%forceCalcSwitch = src%forceCalcSwitch
+    select case (p)
+    case (1)
+      r%Omega = -1._dp*r%Omega
+    case (2)
+      r%Omega = -1._dp*r%Omega
+      r%psiStart = r%psiStart + pi
+    case (3)
+      r%controlPitch = -1._dp*r%controlPitch
+      r%velBody(3) = -1._dp*r%velBody(3)
+      r%omegaBody(1) = -1._dp*r%omegaBody(1)
+      r%omegaBody(2) = -1._dp*r%omegaBody(2)
+    end select
+    if (r%customTrajectorySwitch == 1) then
+      allocate (r%velBodyHistory(3, nt), r%omegaBodyHistory(3, nt))
+      r%velBodyHistory = src%velBodyHistory
+      r%omegaBodyHistory = src%omegaBodyHistory
+      r%velBodyHistory(p, :) = -1._dp*r%velBodyHistory(p, :)
+      if (p == 3) then
+        r%omegaBodyHistory(1, :) = -1._dp*r%omegaBodyHistory(1, :)
+        r%omegaBodyHistory(2, :) = -1._dp*r%omegaBodyHistory(2, :)
+      else
+        r%omegaBodyHistory(3, :) = -1._dp*r%omegaBodyHistory(3, :)
+      endif
+    endif
+  end subroutine image_params
+
+  subroutine rotor_init(r, ir, density, dt, nt, sw, dir, src)
     type(rotor_class), intent(inout) :: r
     integer, intent(in) :: ir
     real(dp), intent(in) :: density
@@ -846,11 +924,17 @@ contains
     integer, intent(inout) :: nt
     type(switches_class), intent(inout) :: sw
     character(len=*), intent(in) :: dir
-    real(dp) :: xVec(r%nc + 1), yVec(r%ns + 1), grid(3, r%nc + 1, r%ns + 1)
+    type(rotor_class), intent(in), optional :: src  !! the source of an image rotor
+    real(dp), allocatable :: xVec(:), yVec(:), grid(:, :, :)
     real(dp) :: dxdymin, velShed, xs(4), dummy
-    integer :: ib, i, j, nx, ny, nz, u, ic, is
+    integer :: ib, i, j, nx, ny, nz, u, ic, is, kb
 
     write (r%id, '(I0.2)') ir
+    if (r%surfaceType < 0) then
+      if (.not. present(src)) call host_fail('ERROR: image rotor without its source rotor')
+      call image_params(r, src, nt)
+    endif
+    allocate (xVec(r%nc + 1), yVec(r%ns + 1), grid(3, r%nc + 1, r%ns + 1))
     if (dt < 0._dp) then
       if (abs(r%Omega) < eps) then
         dt = abs(dt)*r%chord/norm2(r%velBody)
@@ -903,7 +987,7 @@ contains
     r%ageAzN = 0._dp; r%ageAzF = 0._dp
     ! custom trajectory: body velocity and rate of every step from
     ! trajectoryXX.in, nt lines of six values (classdef.f90:3064-3080)
-    if (r%customTrajectorySwitch == 1) then
+    if (r%customTrajectorySwitch == 1 .and. r%surfaceType >= 0) then
       block
         integer :: u, i
         allocate (r%velBodyHistory(3, nt), r%omegaBodyHistory(3, nt))
@@ -941,15 +1025,18 @@ contains
       end associate
     enddo
 
-    r%controlPitch = r%controlPitch*degToRad
-    r%pts = r%pts*degToRad
-    r%preconeAngle = r%preconeAngle*degToRad
-    r%psiStart = r%psiStart*degToRad
-    r%thetaTwist = r%thetaTwist*degToRad
-    r%spanwiseCore = r%spanwiseCore*r%chord
-    r%streamwiseCoreVec = r%streamwiseCoreVec*r%chord
-    r%rollupStart = ceiling(r%rollupStartRadius*r%ns)
-    r%rollupEnd = floor(r%rollupEndRadius*r%ns)
+    ! an image's copied parameters are converted already (classdef.f90:3126-3137)
+    if (r%surfaceType >= 0) then
+      r%controlPitch = r%controlPitch*degToRad
+      r%pts = r%pts*degToRad
+      r%preconeAngle = r%preconeAngle*degToRad
+      r%psiStart = r%psiStart*degToRad
+      r%thetaTwist = r%thetaTwist*degToRad
+      r%spanwiseCore = r%spanwiseCore*r%chord
+      r%streamwiseCoreVec = r%streamwiseCoreVec*r%chord
+      r%rollupStart = ceiling(r%rollupStartRadius*r%ns)
+      r%rollupEnd = floor(r%rollupEndRadius*r%ns)
+    endif
 
     ! Panel corners
     if (r%geometryFile(1:1) == '0') then
@@ -974,6 +1061,9 @@ contains
         & ((grid(3, i, j), i=1, nx), j=1, ny)
       close (u)
     endif
+    ! an image in the plane z = 0 starts from the blade mirrored in z
+    ! (classdef.f90:3228-3230, 3990-3997)
+    if (r%surfaceType < 0 .and. r%imagePlane == 3) grid(3, :, :) = -1._dp*grid(3, :, :)
     do ib = 1, r%nb
       do j = 1, r%ns
         do i = 1, r%nc
@@ -1084,6 +1174,23 @@ contains
       end associate
     enddo
 
+    if (r%surfaceType < 0) then
+      ! the image's normals are its source's mirrored in the image plane,
+      ! taken before the image's own hub, precone and azimuth rotations
+      ! (classdef.f90:3434-3445); axisymmetric blades copy blade 1's
+      ! (3566-3572)
+      do ib = 1, r%nb
+        kb = ib
+        if (r%axisymmetrySwitch == 1) kb = 1
+        do j = 1, r%ns
+          do i = 1, r%nc
+            r%blade(ib)%wiP(i, j)%nCap = src%blade(kb)%wiP(i, j)%nCap
+            r%blade(ib)%wiP(i, j)%nCap(r%imagePlane) = &
+              & -1._dp*src%blade(kb)%wiP(i, j)%nCap(r%imagePlane)
+          enddo
+        enddo
+      enddo
+    endif
     do ib = 1, r%nb
       call blade_move(r%blade(ib), r%hubCoords - r%fromCoords)
     enddo
@@ -1206,6 +1313,10 @@ contains
     real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :), v(:, :, :, :)
 
     do ir = 1, nr
+      if (rotor(ir)%surfaceType < 0) then
+        call mirror_velcp(rotor(ir), rotor(rotor(ir)%imageRotorNum))
+        cycle
+      endif
       associate (r => rotor(ir))
         allocate (cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb), v(3, r%nc, r%ns, r%nb))
         call rotor_cp_arrays(r, cp, ncap)
@@ -1258,11 +1369,39 @@ contains
     enddo
     do ir = 1, nr
       rotor(ir)%gamVecPrev = rotor(ir)%gamVec
+      if (rotor(ir)%surfaceType < 0) then
+        ! rotor_mirrorGamma: the device wing circulation now, gamVec in
+        ! finish_solve (main.f90:595-599)
+        call vlm_check(ctx, vlm_rotor_mirror_gamma(ctx, rotor(ir)%dev), 'mirror_gamma')
+        cycle
+      endif
       call vlm_check(ctx, vlm_rotor_solve_deferred(ctx, rotor(ir)%dev, rotor(ir)%RHS, &
         & c_loc(rotor(ir)%gamVec), 3_c_int, 1.e-15_c_double, c_loc(rotor(ir)%solve_iters)), &
         & 'rotor_solve')
     enddo
   end subroutine compute_rhs_and_solve
+
+  subroutine mirror_velcp(r, src)
+    !! rotor_mirrorVelCP (classdef.f90:4997-5022): every panel's velCP and
+    !! velCPm are the source's with the image-plane component negated; the
+    !! RHS stays zero (main.f90:181-186)
+    type(rotor_class), intent(inout) :: r
+    type(rotor_class), intent(in) :: src
+    integer :: ib, i, j
+    do ib = 1, r%nb
+      do j = 1, r%ns
+        do i = 1, r%nc
+          associate (w => r%blade(ib)%wiP(i, j))
+            w%velCP = src%blade(ib)%wiP(i, j)%velCP
+            w%velCPm = src%blade(ib)%wiP(i, j)%velCPm
+            w%velCP(r%imagePlane) = -1._dp*w%velCP(r%imagePlane)
+            w%velCPm(r%imagePlane) = -1._dp*w%velCPm(r%imagePlane)
+          end associate
+        enddo
+      enddo
+    enddo
+    r%RHS = 0._dp
+  end subroutine mirror_velcp
 
   subroutine finish_solve(rotor, nr)
     !! After vlm_wait: rotor_map_gam on the host panels
@@ -1270,6 +1409,8 @@ contains
     integer, intent(in) :: nr
     integer :: ir
     do ir = 1, nr
+      ! an image's gamVec is its source's negated (rotor_mirrorGamma)
+      if (rotor(ir)%surfaceType < 0) rotor(ir)%gamVec = -1._dp*rotor(rotor(ir)%imageRotorNum)%gamVec
       call rotor_map_gam(rotor(ir))
     enddo
   end subroutine finish_solve
@@ -2445,7 +2586,17 @@ contains
       call rotor_readGeom(rotor(ir), dir, 'geom'//rc2//'.nml')
     enddo
     do ir = 1, nr
-      call rotor_init(rotor(ir), ir, density, dt, nt, sw, dir)
+      if (rotor(ir)%surfaceType < 0) then
+        ! the reference initialises an image from its source's initialised
+        ! state (main.f90:32-38), so the source comes first and is lifting
+        if (rotor(ir)%imageRotorNum < 1 .or. rotor(ir)%imageRotorNum >= ir) &
+          & call host_fail('ERROR: imageRotorNum must name a rotor before the image')
+        if (rotor(rotor(ir)%imageRotorNum)%surfaceType < 0) &
+          & call host_fail('ERROR: the source of an image rotor must not be an image')
+        call rotor_init(rotor(ir), ir, density, dt, nt, sw, dir, rotor(rotor(ir)%imageRotorNum))
+      else
+        call rotor_init(rotor(ir), ir, density, dt, nt, sw, dir)
+      endif
     enddo
     if (nt_override > 0) nt = nt_override
     write_out = sw%rotorForcePlot /= 0 .and. verbose >= 0
@@ -2470,21 +2621,34 @@ contains
         r%dev = dev
         call vlm_check(ctx, vlm_rotor_set_init_wake_vel(ctx, r%dev, r%initWakeVel, &
           & int(sw%initWakeVelNt, c_int)), 'set_init_wake_vel')
-        ! initial pitch, main.f90:43-58
+        ! initial pitch, main.f90:43-58; an image takes its source's
+        ! (negated for the plane z = 0)
         block
           integer :: ib
           do ib = 1, r%nb
-            r%blade(ib)%theta = rotor_gettheta(r, r%psiStart, ib)
+            if (r%surfaceType < 0) then
+              r%blade(ib)%theta = rotor(r%imageRotorNum)%blade(ib)%theta
+              if (r%imagePlane == 3) r%blade(ib)%theta = -1._dp*r%blade(ib)%theta
+            else
+              r%blade(ib)%theta = rotor_gettheta(r, r%psiStart, ib)
+            endif
             call blade_rot_pitch(r%blade(ib), sign(1._dp, r%Omega)*r%blade(ib)%theta)
           enddo
         end block
         call rotor_upload_wing(ctx, r)
-        block
-          real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :)
-          allocate (cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb))
-          call rotor_cp_arrays(r, cp, ncap)
-          call vlm_check(ctx, vlm_rotor_calc_aic(ctx, r%dev, cp, ncap, c_null_ptr), 'calcAIC')
-        end block
+        if (r%surfaceType < 0) then
+          ! no influence matrix: the image's circulation is its source's
+          ! negated (main.f90:66-81, 189-193)
+          call vlm_check(ctx, vlm_rotor_set_image(ctx, r%dev, rotor(r%imageRotorNum)%dev, &
+            & int(r%imagePlane, c_int)), 'set_image')
+        else
+          block
+            real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :)
+            allocate (cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb))
+            call rotor_cp_arrays(r, cp, ncap)
+            call vlm_check(ctx, vlm_rotor_calc_aic(ctx, r%dev, cp, ncap, c_null_ptr), 'calcAIC')
+          end block
+        endif
       end associate
     enddo
 
@@ -2737,6 +2901,14 @@ contains
         endif
         do ir = 1, nr
           if (rotor(ir)%nNwake <= 0) cycle
+          if (rotor(ir)%surfaceType < 0) then
+            ! an image copies its source's whole wake, mirrored, in place of
+            ! the rollup and the shed (main.f90:1433-1436)
+            call vlm_check(ctx, vlm_rotor_mirror_wake(ctx, rotor(ir)%dev, VLM_WAKE_A), 'mirror_wake')
+            rotor(ir)%ageAzN = rotor(rotor(ir)%imageRotorNum)%ageAzN
+            rotor(ir)%ageAzF = rotor(rotor(ir)%imageRotorNum)%ageAzF
+            cycle
+          endif
           call vlm_check(ctx, vlm_rotor_cursors(ctx, rotor(ir)%dev, rowNear, rowFar), 'cursors')
           if (rowNear == 1) then
             call vlm_check(ctx, vlm_rotor_rollup(ctx, rotor(ir)%dev), 'rollup')

@@ -67,6 +67,9 @@ _SIGS = {
     "vlm_rotor_age_wake": [_P, _I, _D],
     "vlm_rotor_dissipate_wake": [_P, _I, _D, _D, _D, _D],
     "vlm_rotor_strain_wake": [_P, _I],
+    "vlm_rotor_set_image": [_P, _I, _I, _I],
+    "vlm_rotor_mirror_gamma": [_P, _I],
+    "vlm_rotor_mirror_wake": [_P, _I, _I],
     "vlm_rotor_burst_wake": [_P, _I, _D, _D],
     "vlm_rotor_state_bytes": [_P, _I, C.POINTER(_L)],
     "vlm_rotor_get_state": [_P, _I, _P, _L],
@@ -273,6 +276,18 @@ class Context:
 
     def rotor_strain_wake(self, ir):
         self._chk(self.lib.vlm_rotor_strain_wake(self.h, ir))
+
+    def rotor_set_image(self, ir, source, plane):
+        """Rotor ir mirrors rotor source in the plane where coordinate
+        plane (1..3) is zero."""
+        self._chk(self.lib.vlm_rotor_set_image(self.h, ir, source, plane))
+
+    def rotor_mirror_gamma(self, ir):
+        self._chk(self.lib.vlm_rotor_mirror_gamma(self.h, ir))
+
+    def rotor_mirror_wake(self, ir, mode):
+        """mode 0 current rows, 1 predicted rows, 2 every row (VLM_WAKE_A)."""
+        self._chk(self.lib.vlm_rotor_mirror_wake(self.h, ir, mode))
 
     def rotor_burst_wake(self, ir, skew_limit, large_core):
         self._chk(self.lib.vlm_rotor_burst_wake(self.h, ir, skew_limit, large_core))
