@@ -318,6 +318,57 @@ int vlm_induced_velocity_terms_deferred(vlm_ctx *ctx, int nterms, const int *ir,
                                         const double *p, double *v, int accumulate);
 int vlm_wait(vlm_ctx *ctx);
 
+/* ---- the step on the device: right-hand side, solve and loads ----------
+ * With these a step runs from the wake sums through the circulation solve
+ * to the loads without a host round trip (main.f90:521-764): the host
+ * uploads the panel geometry after each rigid motion and reads the loads
+ * at vlm_wait.  vlm_rotor_solve_deferred with rhs = NULL solves the
+ * right-hand side vlm_rotor_rhs left on the device.
+ *
+ * vlm_rotor_set_panels: geometry after the last rigid motion, in ring
+ * order (blade, section, chordwise panel): per panel 15 doubles, CP(3),
+ * nCap(3), tauCapChord(3), tauCapSpan(3), panelArea, meanChord, meanSpan;
+ * per blade and section 16, secCP(3), corner 1 of the section's leading
+ * panel (3), secTauCapChord(3), secTauCapSpan(3), secNormalVec(3), secArea;
+ * per blade 9, xAxisAziFlap, yAxisAziFlap, zAxisAziFlap. */
+int vlm_rotor_set_panels(vlm_ctx *ctx, int ir, const double *panel, const double *section,
+                         const double *blade);
+
+/* Right-hand side of the convected blades (main.f90:121-186 initial,
+ * 521-591 every step): velCPm = -velBody - omegaBody x (CP - cgCoords)
+ * - omegaSlow shaft x (CP - hub), velCP = velCPm + the ordered term sum at
+ * the collocation points (as vlm_induced_velocity_terms), RHS = -(velCP .
+ * nCap), copied from blade 1 under axisymmetry.  kin: velBody(3),
+ * omegaBody(3), cgCoords(3), omegaSlow; shaft and hub from
+ * vlm_rotor_set_axis.  velCP, velCPm and the RHS stay on the device; rhs,
+ * when not NULL, receives a host copy at vlm_wait. */
+int vlm_rotor_rhs(vlm_ctx *ctx, int ir, int nterms, const int *irs, const int *sets,
+                  const int *flags, const double *kin, double *rhs);
+
+/* rotor_mirrorVelCP (classdef.f90:4997-5022): an image's velCP and velCPm
+ * are its source's (as left by vlm_rotor_rhs) with coordinate imagePlane
+ * negated, on every blade. */
+int vlm_rotor_mirror_velcp(vlm_ctx *ctx, int ir);
+
+/* Loads of forceCalcSwitch = 0 (main.f90:247-290 and 632-668;
+ * blade_calc_secChordwiseResVel, blade_calc_secAlpha, rotor_dirLiftDrag,
+ * blade_calc_force and rotor_calc_force, classdef.f90:1704-1896,
+ * 2197-2265, 4607-4671, 4766-4784): velCPTotal = velCP + the ordered term
+ * sum at the collocation points (the caller lists every rotor's bound
+ * vortices, then the wing's own rings with VLM_TERM_NEGATE; the result is
+ * the negated sum, as vlm_induced_velocity_terms), then per convected
+ * blade and section the chordwise resultant velocity (quadratic fit over
+ * three or more panels), angles, lift direction and the panel pressure
+ * loop with gamPrev kept on the device (restart state included), and the
+ * blade and rotor sums.  out (host, written at vlm_wait), per convected
+ * blade and section 19 doubles: secChordwiseResVel(3), secAlpha, secPhi,
+ * secViz, secVix, secTheta, secLift(3), secLiftInPlane(3),
+ * secLiftOutPlane(3), secCL, secCLu; then the rotor's forceInertial(3),
+ * lift(3), liftUnsteady(3) (drag is zero in this load model). */
+int vlm_rotor_loads(vlm_ctx *ctx, int ir, int nterms, const int *irs, const int *sets,
+                    const int *flags, double density, double dt, double omegaSign,
+                    int spanwiseLiftSwitch, double *out);
+
 /* Dense kernels on explicit matrices (the LU microbenchmark and tests):
  * factorise an n x n column-major matrix in place (ipiv 0-based), and
  * solve with refinement against the unfactorised A. */

@@ -248,7 +248,12 @@ int solve_refine(vlm_ctx *ctx, int n, const double *A, const double *LU, const i
     VLM_HIP(ctx->solve_tmp.reserve((size_t)3 * n + 1));
     double *b = ctx->solve_tmp.p, *x = b + n, *r = x + n;
     int *st = reinterpret_cast<int *>(r + n);
-    VLM_TRY(h2d(ctx, b, rhs_h, sizeof(double) * n));
+    if (rhs_h) {
+        VLM_TRY(h2d(ctx, b, rhs_h, sizeof(double) * n));
+    } else {  // the right-hand side left on the device by vlm_rotor_rhs
+        VLM_HIP(hipMemcpyAsync(b, rotor->rhs.p, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    }
     if (n <= SOLVE_WAVE_N) {
         hipLaunchKernelGGL(solve_wave_kernel, dim3(1), dim3(64), 0, ctx->stream, n, A, LU, ipiv,
                            b, x, max_iter, tol, st);
@@ -356,8 +361,10 @@ int vlm_rotor_solve_deferred(vlm_ctx *ctx, int ir, const double *rhs, double *ga
 {
     if (!ctx || ir < 0 || ir >= (int)ctx->rotors.size()) return VLM_ERR_ARG;
     VLM_DEVICE(ctx);
-    if (!rhs || !gam) return fail(ctx, VLM_ERR_ARG, "null buffer");
+    if (!gam) return fail(ctx, VLM_ERR_ARG, "null buffer");
     if (!ctx->lu[ir]) return fail(ctx, VLM_ERR_STATE, "vlm_rotor_calc_aic not called");
+    if (!rhs && !ctx->rotors[ir]->panels_set)
+        return fail(ctx, VLM_ERR_STATE, "no right-hand side on the device (vlm_rotor_rhs)");
     const int n = ctx->aic_n[ir];
     return solve_refine(ctx, n, ctx->aic[ir], ctx->lu[ir], ctx->piv[ir], rhs, gam, max_iter, tol,
                         iters, ctx->rotors[ir], true);

@@ -69,6 +69,9 @@ constexpr int VEL_H2 = 4, VEL_H3 = 5, NVEL = 6;
 
 constexpr int PF_N = 240;  // prescribed far-wake filaments per blade (classdef.f90:225)
 
+// Records of vlm_rotor_set_panels and vlm_rotor_loads (include/vlm_c.h).
+constexpr int PANEL_N = 15, SECTION_N = 16, BLADE_N = 9, LOADS_SEC_N = 19, LOADS_ROTOR_N = 9;
+
 struct Rotor {
     vlm_rotor_desc d{};
     int rowNear = 0, rowFar = 0;  // 1-based, as in the reference
@@ -101,6 +104,14 @@ struct Rotor {
     // image rotor: index of the rotor it mirrors (-1: a lifting rotor of its
     // own) and the coordinate negated by the mirror (0 x, 1 y, 2 z)
     int image_of = -1, image_axis = 0;
+    // right-hand side and loads on the device (vlm_rotor_set_panels,
+    // vlm_rotor_rhs, vlm_rotor_loads): panel, section and blade geometry
+    // after the last rigid motion (PANEL_N, SECTION_N, BLADE_N doubles
+    // each), collocation points SoA, velCP and velCPm (6 per panel), the
+    // RHS, the load velocities (SoA), gamPrev (restart state) and the
+    // packed load results
+    DBuf<double> pan, sec, bla, cps, vcp, rhs, vld, gprev, lout;
+    int panels_set = 0;
     size_t nvn() const { return 3u * d.nb * (size_t)d.nNwake * (d.ns + 1); }
     size_t nvf() const { return 3u * d.nb * (size_t)(d.nFwake > 0 ? d.nFwake : 0); }
 };
@@ -294,6 +305,13 @@ int64_t wing_slots(const Rotor &r, int set);
 int64_t wake_slots(const Rotor &r);
 
 int ensure_slots(vlm_ctx *ctx, int64_t n);
+
+// Rotor ir of the context (nullptr when out of range).
+Rotor *rotor_of(vlm_ctx *ctx, int ir);
+// The ordered sum of source-set terms (vlm_induced_velocity_terms) at m
+// device targets, SoA with stride m, into the device SoA buffer out.
+int induced_terms_soa(vlm_ctx *ctx, int nterms, const int *ir, const int *set,
+                      const int *flags, int64_t m, const double *tgt, double *out);
 void mlp_release(vlm_ctx *ctx);
 
 // Host<->device copies through the pinned arena (ctx->pin): h2d copies the

@@ -1734,6 +1734,15 @@ int vlm_rotor_create(vlm_ctx *ctx, const vlm_rotor_desc *d, double spanwiseCore,
         R(r->pf[t], 6 * nb * PF_N);
         R(r->ps[t], 4 * nb);
     }
+    R(r->pan, (size_t)PANEL_N * r->nring);
+    R(r->sec, (size_t)SECTION_N * nb * ns);
+    R(r->bla, (size_t)BLADE_N * nb);
+    R(r->cps, 3 * (size_t)r->nring);
+    R(r->vcp, 6 * (size_t)r->nring);
+    R(r->rhs, r->nring);
+    R(r->vld, 3 * (size_t)r->nring);
+    R(r->gprev, r->nring);
+    R(r->lout, (size_t)LOADS_SEC_N * nb * ns + LOADS_ROTOR_N);
     if (e != hipSuccess) {
         delete r;
         return hip_fail(ctx, e, "rotor allocation");
@@ -1752,6 +1761,9 @@ int vlm_rotor_create(vlm_ctx *ctx, const vlm_rotor_desc *d, double spanwiseCore,
         all.push_back(&r->pf[t]);
         all.push_back(&r->ps[t]);
     }
+    for (auto *b : {&r->pan, &r->sec, &r->bla, &r->cps, &r->vcp, &r->rhs, &r->vld, &r->gprev,
+                    &r->lout})
+        all.push_back(b);
     for (auto *b : all)
         hipLaunchKernelGGL(fill_kernel, dim3(nblk((int64_t)b->n)), dim3(256), 0, ctx->stream,
                            b->p, (int64_t)b->n, 0.0);
@@ -1932,6 +1944,7 @@ static std::vector<DBuf<double> *> state_bufs(Rotor &r)
         v.push_back(&r.pf[t]);
         v.push_back(&r.ps[t]);
     }
+    v.push_back(&r.gprev);  // gamPrev of the loads (blade_calc_force)
     return v;
 }
 
@@ -2106,6 +2119,41 @@ static int induced_terms(vlm_ctx *ctx, int nterms, const int *ir, const int *set
     }
     return points_to_host(ctx, m, ctx->out.p, m, v, deferred);
 }
+
+}  // extern "C"
+
+// The ordered term sum of induced_terms at m device targets (SoA, stride
+// m) into a device SoA buffer out (overwritten): the right-hand side and
+// the loads keep their collocation velocities on the device.
+int vlm::induced_terms_soa(vlm_ctx *ctx, int nterms, const int *ir, const int *set,
+                           const int *flags, int64_t m, const double *tgt, double *out)
+{
+    for (int k = 0; k < nterms; ++k) {
+        if (!get_rotor(ctx, ir[k])) return fail(ctx, VLM_ERR_ARG, "unknown rotor");
+        if (set[k] < VLM_SET_WING || set[k] > VLM_SET_WAKE_PRED)
+            return fail(ctx, VLM_ERR_ARG, "bad set");
+    }
+    if (m <= 0) return VLM_OK;
+    hipLaunchKernelGGL(fill_kernel, dim3(nblk(3 * m)), dim3(256), 0, ctx->stream, out, 3 * m,
+                       0.0);
+    VLM_HIP(hipGetLastError());
+    for (int k = 0; k < nterms; ++k) {
+        if (flags[k] & VLM_TERM_NEGATE) {
+            hipLaunchKernelGGL(negate_kernel, dim3(nblk(3 * m)), dim3(256), 0, ctx->stream, out,
+                               3 * m);
+            VLM_HIP(hipGetLastError());
+        }
+        std::vector<std::pair<int, int>> src = {{ir[k], set[k]}};
+        const int64_t nmax = staged_upper_bound(ctx, src);
+        VLM_TRY(build_sources(ctx, src));
+        VLM_TRY(eval_sources(ctx, tgt, m, m, out, 1, nmax));
+    }
+    return VLM_OK;
+}
+
+vlm::Rotor *vlm::rotor_of(vlm_ctx *ctx, int ir) { return get_rotor(ctx, ir); }
+
+extern "C" {
 
 int vlm_induced_velocity_terms(vlm_ctx *ctx, int nterms, const int *ir, const int *set,
                                const int *flags, int64_t m, const double *p, double *v,

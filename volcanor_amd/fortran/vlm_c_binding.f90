@@ -274,11 +274,46 @@ module vlm_c
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       integer(c_int), value :: ir
-      real(c_double), intent(in) :: rhs(*)
+      type(c_ptr), value :: rhs             ! C_NULL_PTR: the RHS of vlm_rotor_rhs
       type(c_ptr), value :: gam
       integer(c_int), value :: max_iter
       real(c_double), value :: tol
       type(c_ptr), value :: iters
+    end function
+
+    integer(c_int) function vlm_rotor_set_panels(ctx, ir, panel, section, blade) &
+        bind(C, name='vlm_rotor_set_panels')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir
+      real(c_double), intent(in) :: panel(*), section(*), blade(*)
+    end function
+
+    integer(c_int) function vlm_rotor_rhs(ctx, ir, nterms, irs, sets, flags, kin, rhs) &
+        bind(C, name='vlm_rotor_rhs')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir, nterms
+      integer(c_int), intent(in) :: irs(*), sets(*), flags(*)
+      real(c_double), intent(in) :: kin(*)
+      type(c_ptr), value :: rhs
+    end function
+
+    integer(c_int) function vlm_rotor_mirror_velcp(ctx, ir) bind(C, name='vlm_rotor_mirror_velcp')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir
+    end function
+
+    integer(c_int) function vlm_rotor_loads(ctx, ir, nterms, irs, sets, flags, density, dt, &
+        & omegaSign, spanwiseLiftSwitch, out) bind(C, name='vlm_rotor_loads')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ir, nterms
+      integer(c_int), intent(in) :: irs(*), sets(*), flags(*)
+      real(c_double), value :: density, dt, omegaSign
+      integer(c_int), value :: spanwiseLiftSwitch
+      type(c_ptr), value :: out
     end function
 
     integer(c_int) function vlm_induced_velocity_terms_deferred(ctx, nterms, ir, set, flags, &

@@ -101,7 +101,7 @@ module volcanor_host
     real(dp) :: rollupStartRadius = 0._dp, rollupEndRadius = 1._dp
     real(dp), allocatable :: streamwiseCoreVec(:)
     real(dp), allocatable :: gamVec(:), gamVecPrev(:), RHS(:)
-    real(dp), allocatable :: vload(:, :, :, :)  !! load velocities at the CPs (deferred download)
+    real(dp), allocatable :: lout(:)  !! packed loads of vlm_rotor_loads (deferred download)
     ! inputs kept only for rXXParams.json (libPostprocess.f90:52-141)
     integer :: imagePlane = 0, imageRotorNum = 0
     real(dp) :: thetaTwist = 0._dp, initWakeVel = 0._dp
@@ -187,17 +187,6 @@ contains
       p = 0._dp
     endif
   end function proj3
-
-  pure function noproj3(a, d) result(p)
-    real(dp), intent(in) :: a(3), d(3)
-    real(dp) :: p(3), n2
-    n2 = sum(d**2)
-    if (n2 > eps) then
-      p = a - dot_product(a, d)*d/n2
-    else
-      p = a
-    endif
-  end function noproj3
 
   function spaced(kind, x0, x1, n) result(x)
     !! linspace / cosspace / halfsinspace / tanspace, libMath.f90:138-200
@@ -364,52 +353,6 @@ contains
       w = Omega
     end select
   end function slow_omega
-
-  pure function angle_tan(a, b) result(ang)
-    !! Angle between two vectors by the tangent formula, libMath.f90:224-236
-    real(dp), intent(in) :: a(3), b(3)
-    real(dp) :: ang, d
-    d = dot_product(a, b)
-    ang = atan2(sqrt(sum(a**2)*sum(b**2) - d*d), d)
-  end function angle_tan
-
-  function lsq2_eval(xq, xd, yd) result(y)
-    !! Quadratic least-squares fit evaluated at xq (libMath.f90:577-605),
-    !! normal equations solved by Gaussian elimination with pivoting.
-    real(dp), intent(in) :: xq, xd(:), yd(:)
-    real(dp) :: y, A(3, 3), b(3), c(3), f, t(4)
-    integer :: i, j, k, p
-    A(1, 1) = size(xd)
-    A(1, 2) = sum(xd)
-    A(1, 3) = sum(xd**2)
-    A(2, 1) = A(1, 2)
-    A(2, 2) = A(1, 3)
-    A(2, 3) = sum(xd**3)
-    A(3, 1) = A(1, 3)
-    A(3, 2) = A(2, 3)
-    A(3, 3) = sum(xd**4)
-    b = [sum(yd), sum(yd*xd), sum(yd*xd**2)]
-    do k = 1, 3
-      p = k - 1 + maxloc(abs(A(k:3, k)), 1)
-      if (p /= k) then
-        t(1:3) = A(k, :); A(k, :) = A(p, :); A(p, :) = t(1:3)
-        t(4) = b(k); b(k) = b(p); b(p) = t(4)
-      endif
-      do i = k + 1, 3
-        f = A(i, k)/A(k, k)
-        A(i, k:3) = A(i, k:3) - f*A(k, k:3)
-        b(i) = b(i) - f*b(k)
-      enddo
-    enddo
-    do i = 3, 1, -1
-      c(i) = b(i)
-      do j = i + 1, 3
-        c(i) = c(i) - A(i, j)*c(j)
-      enddo
-      c(i) = c(i)/A(i, i)
-    enddo
-    y = c(1) + c(2)*xq + c(3)*xq*xq
-  end function lsq2_eval
 
   function pwl(x, y, q) result(v)
     !! Piecewise-linear interpolation, libMath.f90:476-517
@@ -1263,7 +1206,44 @@ contains
     call vlm_check(ctx, vlm_rotor_set_wing(ctx, r%dev, wc, wr, wg), 'rotor_upload_wing')
     call vlm_check(ctx, vlm_rotor_set_axis(ctx, r%dev, r%shaftAxis, r%hubCoords), &
       & 'rotor_set_axis')
+    call rotor_upload_panels(ctx, r)
   end subroutine rotor_upload_wing
+
+  subroutine rotor_upload_panels(ctx, r)
+    !! The panel, section and blade geometry of the right-hand side and the
+    !! loads on the device (vlm_rotor_set_panels records, ring order)
+    type(c_ptr), intent(in) :: ctx
+    type(rotor_class), intent(in) :: r
+    real(c_double) :: pn(15, r%nc, r%ns, r%nb), sc(16, r%ns, r%nb), bl(9, r%nb)
+    integer :: ib, i, j
+    do ib = 1, r%nb
+      associate (b => r%blade(ib))
+        do j = 1, r%ns
+          do i = 1, r%nc
+            associate (w => b%wiP(i, j))
+              pn(1:3, i, j, ib) = w%CP
+              pn(4:6, i, j, ib) = w%nCap
+              pn(7:9, i, j, ib) = w%tauCapChord
+              pn(10:12, i, j, ib) = w%tauCapSpan
+              pn(13, i, j, ib) = w%panelArea
+              pn(14, i, j, ib) = w%meanChord
+              pn(15, i, j, ib) = w%meanSpan
+            end associate
+          enddo
+          sc(1:3, j, ib) = b%secCP(:, j)
+          sc(4:6, j, ib) = b%wiP(1, j)%PC(:, 1)
+          sc(7:9, j, ib) = b%secTauCapChord(:, j)
+          sc(10:12, j, ib) = b%secTauCapSpan(:, j)
+          sc(13:15, j, ib) = b%secNormalVec(:, j)
+          sc(16, j, ib) = b%secArea(j)
+        enddo
+        bl(1:3, ib) = b%xAxisAziFlap
+        bl(4:6, ib) = b%yAxisAziFlap
+        bl(7:9, ib) = b%zAxisAziFlap
+      end associate
+    enddo
+    call vlm_check(ctx, vlm_rotor_set_panels(ctx, r%dev, pn, sc, bl), 'set_panels')
+  end subroutine rotor_upload_panels
 
   subroutine rotor_cp_arrays(r, cp, ncap)
     type(rotor_class), intent(in) :: r
@@ -1301,40 +1281,29 @@ contains
   ! RHS, solve and loads
   ! ------------------------------------------------------------------
   subroutine compute_rhs_and_solve(ctx, rotor, nr, with_wake)
-    !! main.f90:124-202 (initial) and 521-603 (every step).  The solve is
-    !! deferred: gamVec arrives at vlm_wait, after which finish_solve maps it
-    !! onto the panels (the library has set the device wing circulation).
+    !! main.f90:124-202 (initial) and 521-603 (every step), enqueued on the
+    !! device: per rotor the right-hand side from the rigid motion and the
+    !! wake and (other) wing velocities at the collocation points, summed
+    !! in the reference's order (main.f90:549-561), then the solve of that
+    !! right-hand side; an image mirrors its source's collocation velocities
+    !! and circulation instead (main.f90:181-193, 595-599).  gamVec and RHS
+    !! arrive at vlm_wait, after which finish_solve maps gamVec onto the
+    !! panels (the device wing circulation is already set).
     type(c_ptr), intent(in) :: ctx
     type(rotor_class), intent(inout), target :: rotor(:)
     integer, intent(in) :: nr
     logical, intent(in) :: with_wake
-    integer :: ir, jr, ib, i, j, n, row, nt
+    integer :: ir, jr, nt
     integer(c_int) :: tir(2*nr), tset(2*nr), tfl(2*nr)
-    real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :), v(:, :, :, :)
+    real(c_double) :: kin(10)
 
     do ir = 1, nr
-      if (rotor(ir)%surfaceType < 0) then
-        call mirror_velcp(rotor(ir), rotor(rotor(ir)%imageRotorNum))
-        cycle
-      endif
       associate (r => rotor(ir))
-        allocate (cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb), v(3, r%nc, r%ns, r%nb))
-        call rotor_cp_arrays(r, cp, ncap)
-        n = r%nc*r%ns*r%nb
-        v = 0._dp
-        do ib = 1, r%nbConvect
-          do j = 1, r%ns
-            do i = 1, r%nc
-              associate (w => r%blade(ib)%wiP(i, j))
-                w%velCP = -1._dp*r%velBody - cross3(r%omegaBody, w%CP - r%cgCoords) &
-                  & - cross3(r%omegaSlow*r%shaftAxis, w%CP - r%hubCoords)
-                w%velCPm = w%velCP
-              end associate
-            enddo
-          enddo
-        enddo
-        ! sum over rotors of wake and (other) wing velocities, main.f90:549-561,
-        ! in the reference's order, as one device-side sequence
+        if (r%surfaceType < 0) then
+          call vlm_check(ctx, vlm_rotor_mirror_velcp(ctx, r%dev), 'mirror_velcp')
+          r%RHS = 0._dp
+          cycle
+        endif
         nt = 0
         do jr = 1, nr
           if (with_wake .and. rotor(jr)%nNwake > 0) then
@@ -1344,27 +1313,12 @@ contains
             nt = nt + 1; tir(nt) = rotor(jr)%dev; tset(nt) = VLM_SET_WING; tfl(nt) = 0
           endif
         enddo
-        call vlm_check(ctx, vlm_induced_velocity_terms(ctx, int(nt, c_int), tir, tset, tfl, &
-          & int(n, c_int64_t), cp, v, 0_c_int), 'RHS velocity')
-        r%RHS = 0._dp
-        do ib = 1, r%nbConvect
-          do j = 1, r%ns
-            do i = 1, r%nc
-              row = i + r%nc*(j - 1) + r%ns*r%nc*(ib - 1)
-              associate (w => r%blade(ib)%wiP(i, j))
-                w%velCP = w%velCP + v(:, i, j, ib)
-                r%RHS(row) = dot_product(w%velCP, w%nCap)
-              end associate
-            enddo
-          enddo
-        enddo
-        if (r%axisymmetrySwitch == 1) then
-          do ib = 2, r%nb
-            r%RHS(r%nc*r%ns*(ib - 1) + 1:r%nc*r%ns*ib) = r%RHS(1:r%nc*r%ns)
-          enddo
-        endif
-        r%RHS = -1._dp*r%RHS
-        deallocate (cp, ncap, v)
+        kin(1:3) = r%velBody
+        kin(4:6) = r%omegaBody
+        kin(7:9) = r%cgCoords
+        kin(10) = r%omegaSlow
+        call vlm_check(ctx, vlm_rotor_rhs(ctx, r%dev, int(nt, c_int), tir, tset, tfl, kin, &
+          & c_loc(r%RHS)), 'RHS')
       end associate
     enddo
     do ir = 1, nr
@@ -1375,33 +1329,11 @@ contains
         call vlm_check(ctx, vlm_rotor_mirror_gamma(ctx, rotor(ir)%dev), 'mirror_gamma')
         cycle
       endif
-      call vlm_check(ctx, vlm_rotor_solve_deferred(ctx, rotor(ir)%dev, rotor(ir)%RHS, &
+      call vlm_check(ctx, vlm_rotor_solve_deferred(ctx, rotor(ir)%dev, c_null_ptr, &
         & c_loc(rotor(ir)%gamVec), 3_c_int, 1.e-15_c_double, c_loc(rotor(ir)%solve_iters)), &
         & 'rotor_solve')
     enddo
   end subroutine compute_rhs_and_solve
-
-  subroutine mirror_velcp(r, src)
-    !! rotor_mirrorVelCP (classdef.f90:4997-5022): every panel's velCP and
-    !! velCPm are the source's with the image-plane component negated; the
-    !! RHS stays zero (main.f90:181-186)
-    type(rotor_class), intent(inout) :: r
-    type(rotor_class), intent(in) :: src
-    integer :: ib, i, j
-    do ib = 1, r%nb
-      do j = 1, r%ns
-        do i = 1, r%nc
-          associate (w => r%blade(ib)%wiP(i, j))
-            w%velCP = src%blade(ib)%wiP(i, j)%velCP
-            w%velCPm = src%blade(ib)%wiP(i, j)%velCPm
-            w%velCP(r%imagePlane) = -1._dp*w%velCP(r%imagePlane)
-            w%velCPm(r%imagePlane) = -1._dp*w%velCPm(r%imagePlane)
-          end associate
-        enddo
-      enddo
-    enddo
-    r%RHS = 0._dp
-  end subroutine mirror_velcp
 
   subroutine finish_solve(rotor, nr)
     !! After vlm_wait: rotor_map_gam on the host panels
@@ -2201,205 +2133,62 @@ contains
     enddo
   end subroutine sub_iterate
 
-  subroutine blade_sec_resvel(b)
-    !! Section resultant velocity, classdef.f90:2197-2232
-    type(blade_class), intent(inout) :: b
-    integer :: is, ic, i, nc
-    real(dp) :: xd(size(b%wiP, 1)), yd(size(b%wiP, 1))
-    nc = size(b%wiP, 1)
-    do is = 1, size(b%wiP, 2)
-      do ic = 1, nc
-        b%wiP(ic, is)%chordwiseResVel = noproj3(b%wiP(ic, is)%velCPTotal, b%wiP(ic, is)%tauCapSpan)
-        xd(ic) = dot_product(b%wiP(ic, is)%CP - b%wiP(1, is)%PC(:, 1), b%secTauCapChord(:, is))
-      enddo
-      do i = 1, 3
-        yd = b%wiP(:, is)%chordwiseResVel(i)
-        if (nc >= 3) then
-          b%secChordwiseResVel(i, is) = lsq2_eval(dot_product(b%secCP(:, is) - &
-            & b%wiP(1, is)%PC(:, 1), b%secTauCapChord(:, is)), xd, yd)
-        else
-          b%secChordwiseResVel(i, is) = sum(yd)/nc
-        endif
-      enddo
-    enddo
-  end subroutine blade_sec_resvel
-
-  subroutine forces_launch(ctx, rotor, nr, ir)
-    !! Enqueues the load velocities at the collocation points of rotor ir:
-    !! the wing's own rings less every rotor's bound vortices
-    !! (classdef.f90:1376-1418, 1734), as one device-side sequence whose
-    !! result reaches rotor(ir)%vload at vlm_wait.
+  subroutine forces_launch(ctx, rotor, nr, ir, density, dt)
+    !! Enqueues the loads of rotor ir on the device (vlm_rotor_loads): the
+    !! load velocities at the collocation points, the wing's own rings less
+    !! every rotor's bound vortices (classdef.f90:1376-1418, 1734), then
+    !! the sectional and rotor loads of forceCalcSwitch = 0; the results
+    !! reach rotor(ir)%lout at vlm_wait and compute_forces unpacks them.
     type(c_ptr), intent(in) :: ctx
     type(rotor_class), intent(inout), target :: rotor(:)
     integer, intent(in) :: nr, ir
-    integer :: jr, n
+    real(dp), intent(in) :: density, dt
+    integer :: jr
     integer(c_int) :: tir(nr + 1), tset(nr + 1), tfl(nr + 1)
-    real(c_double), allocatable :: cp(:, :, :, :), ncap(:, :, :, :)
 
     associate (r => rotor(ir))
-      n = r%nc*r%ns*r%nb
-      allocate (cp(3, r%nc, r%ns, r%nb), ncap(3, r%nc, r%ns, r%nb))
-      if (.not. allocated(r%vload)) allocate (r%vload(3, r%nc, r%ns, r%nb))
-      call rotor_cp_arrays(r, cp, ncap)
+      if (.not. allocated(r%lout)) allocate (r%lout(19*r%nbConvect*r%ns + 9))
       do jr = 1, nr
         tir(jr) = rotor(jr)%dev; tset(jr) = VLM_SET_WING_BOUND; tfl(jr) = 0
       enddo
       tir(nr + 1) = r%dev; tset(nr + 1) = VLM_SET_WING; tfl(nr + 1) = VLM_TERM_NEGATE
-      call vlm_check(ctx, vlm_induced_velocity_terms_deferred(ctx, int(nr + 1, c_int), tir, &
-        & tset, tfl, int(n, c_int64_t), cp, c_loc(r%vload), 0_c_int), 'load velocity')
-      deallocate (cp, ncap)
+      call vlm_check(ctx, vlm_rotor_loads(ctx, r%dev, int(nr + 1, c_int), tir, tset, tfl, &
+        & density, dt, r%Omega, int(r%spanwiseLiftSwitch, c_int), c_loc(r%lout)), 'loads')
     end associate
   end subroutine forces_launch
 
-  subroutine compute_forces(ctx, rotor, nr, ir, density, dt)
-    !! forceCalcSwitch = 0 branch of main.f90:247-290 / 632-668 and
-    !! rotor_calc_force + blade_calc_force (classdef.f90:1704-1896, 4607-4671),
-    !! from the load velocities of forces_launch (after vlm_wait)
-    type(c_ptr), intent(in) :: ctx
+  subroutine compute_forces(rotor, ir)
+    !! After vlm_wait: the device loads of forces_launch into the blade
+    !! sections and the rotor totals that the force files and CT read
+    !! (records of vlm_rotor_loads)
     type(rotor_class), intent(inout) :: rotor(:)
-    integer, intent(in) :: nr, ir
-    real(dp), intent(in) :: density, dt
-    integer :: ib, i, j, is, ic
-    real(dp) :: vtc(rotor(ir)%nc, rotor(ir)%ns), gec(rotor(ir)%nc, rotor(ir)%ns)
-    real(dp) :: ges(rotor(ir)%nc, rotor(ir)%ns), q(rotor(ir)%ns)
-    real(dp) :: inv, sgn, vts, vi(3)
+    integer, intent(in) :: ir
+    integer :: ib, is, k
 
     associate (r => rotor(ir))
-      do ib = 1, r%nbConvect
-        do j = 1, r%ns
-          do i = 1, r%nc
-            r%blade(ib)%wiP(i, j)%velCPTotal = r%blade(ib)%wiP(i, j)%velCP + r%vload(:, i, j, ib)
-          enddo
-        enddo
-      enddo
-      if (r%axisymmetrySwitch == 1) then
-        do ib = 2, r%nb
-          r%blade(ib)%wiP%velCPTotal(1) = r%blade(1)%wiP%velCPTotal(1)
-          r%blade(ib)%wiP%velCPTotal(2) = r%blade(1)%wiP%velCPTotal(2)
-          r%blade(ib)%wiP%velCPTotal(3) = r%blade(1)%wiP%velCPTotal(3)
-        enddo
-      endif
-
-      ! rotor_calc_secAlpha / blade_calc_secAlpha (classdef.f90:4766-4784,
-      ! 2234-2265); the vertical axis is the blade's zAxisAziFlap
+      k = 0
       do ib = 1, r%nbConvect
         associate (b => r%blade(ib))
-          call blade_sec_resvel(b)
           do is = 1, r%ns
-            b%secAlpha(is) = atan2(dot_product(b%secChordwiseResVel(:, is), &
-              & b%secNormalVec(:, is)), dot_product(b%secChordwiseResVel(:, is), &
-              & b%secTauCapChord(:, is)))
-            b%secPhi(is) = angle_tan(b%secChordwiseResVel(:, is), &
-              & noproj3(b%wiP(1, is)%velCPm, b%secTauCapSpan(:, is)))
-            vi = b%secChordwiseResVel(:, is) - b%wiP(1, is)%velCPm
-            b%secViz(is) = vi(3)
-            b%secVix(is) = norm2(noproj3(vi, b%secTauCapSpan(:, is)) - [0._dp, 0._dp, vi(3)])
-            b%secTheta(is) = pi*0.5_dp - angle_tan(-1._dp*b%secTauCapChord(:, is), b%zAxisAziFlap)
+            b%secChordwiseResVel(:, is) = r%lout(k + 1:k + 3)
+            b%secAlpha(is) = r%lout(k + 4)
+            b%secPhi(is) = r%lout(k + 5)
+            b%secViz(is) = r%lout(k + 6)
+            b%secVix(is) = r%lout(k + 7)
+            b%secTheta(is) = r%lout(k + 8)
+            b%secLift(:, is) = r%lout(k + 9:k + 11)
+            b%secLiftInPlane(:, is) = r%lout(k + 12:k + 14)
+            b%secLiftOutPlane(:, is) = r%lout(k + 15:k + 17)
+            b%secCL(is) = r%lout(k + 18)
+            b%secCLu(is) = r%lout(k + 19)
+            k = k + 19
           enddo
         end associate
       enddo
-
-      r%forceInertial = 0._dp
-      ! dirLiftDrag
-      do ib = 1, r%nbConvect
-        call blade_sec_resvel(r%blade(ib))
-        do is = 1, r%ns
-          r%blade(ib)%secDragDir(:, is) = unit3(r%blade(ib)%secChordwiseResVel(:, is))
-          r%blade(ib)%secLiftDir(:, is) = sign(1._dp, r%Omega)*unit3(cross3( &
-            & r%blade(ib)%secDragDir(:, is), r%blade(ib)%yAxisAziFlap))
-        enddo
-      enddo
-
-      inv = -1._dp*sign(1._dp, r%Omega)
-      do ib = 1, r%nbConvect
-        associate (b => r%blade(ib))
-          b%secForceInertial = 0._dp; b%secLift = 0._dp; b%secLiftUnsteady = 0._dp
-          do is = 1, r%ns
-            do ic = 1, r%nc
-              vtc(ic, is) = dot_product(b%wiP(ic, is)%velCP, b%wiP(ic, is)%tauCapChord)
-            enddo
-          enddo
-          do is = 1, r%ns
-            gec(1, is) = b%wiP(1, is)%vr%gam
-          enddo
-          do ic = 2, r%nc
-            gec(ic, 1) = b%wiP(ic, 1)%vr%gam - b%wiP(ic - 1, 1)%vr%gam
-          enddo
-          do ic = 1, r%nc
-            ges(ic, 1) = b%wiP(ic, 1)%vr%gam
-          enddo
-          do is = 2, r%ns
-            ges(1, is) = b%wiP(1, is)%vr%gam - b%wiP(1, is - 1)%vr%gam
-          enddo
-          do is = 2, r%ns
-            do ic = 2, r%nc
-              gec(ic, is) = b%wiP(ic, is)%vr%gam - b%wiP(ic - 1, is)%vr%gam
-              ges(ic, is) = b%wiP(ic, is)%vr%gam - b%wiP(ic, is - 1)%vr%gam
-            enddo
-          enddo
-          gec = inv*gec
-          ges = inv*ges
-          do is = 1, r%ns
-            do ic = 1, r%nc
-              associate (w => b%wiP(ic, is))
-                if (ic > 1) then
-                  w%gamTrapz = inv*0.5_dp*(w%vr%gam + b%wiP(ic - 1, is)%vr%gam)
-                else
-                  w%gamTrapz = inv*0.5_dp*w%vr%gam
-                endif
-                w%delPUnsteady = density*(w%gamTrapz - w%gamPrev)/dt
-                w%delP = w%delPUnsteady + density*vtc(ic, is)*gec(ic, is)/w%meanChord
-                if (r%spanwiseLiftSwitch /= 0) then
-                  vts = dot_product(w%velCP, w%tauCapSpan)
-                  w%delP = w%delP + density*vts*ges(ic, is)/w%meanSpan
-                endif
-                w%gamPrev = w%gamTrapz
-                w%normalForce = w%delP*w%panelArea*w%nCap
-                w%normalForceUnsteady = w%delPUnsteady*w%panelArea*w%nCap
-                b%secForceInertial(:, is) = b%secForceInertial(:, is) + w%normalForce
-                b%secLift(:, is) = b%secLift(:, is) + proj3(w%normalForce, b%secLiftDir(:, is))
-                b%secLiftUnsteady(:, is) = b%secLiftUnsteady(:, is) + &
-                  & proj3(w%normalForceUnsteady, b%secLiftDir(:, is))
-              end associate
-            enddo
-            b%secLiftInPlane(:, is) = inv*proj3(b%secLift(:, is), b%xAxisAziFlap)
-            b%secLiftOutPlane(:, is) = proj3(b%secLift(:, is), b%zAxisAziFlap)
-          enddo
-          do is = 1, r%ns
-            q(is) = 0.5_dp*density*norm2(b%secChordwiseResVel(:, is))**2
-          enddo
-          do is = 1, r%ns
-            if (abs(q(is)) > eps) then
-              sgn = sign(1._dp, dot_product(b%secLift(:, is), b%zAxisAziFlap))
-              b%secCL(is) = norm2(b%secLift(:, is))*sgn/(q(is)*b%secArea(is))
-              b%secCLu(is) = norm2(b%secLiftUnsteady(:, is))*sgn/(q(is)*b%secArea(is))
-            else
-              b%secCL(is) = 0._dp
-              b%secCLu(is) = 0._dp
-            endif
-          enddo
-          b%forceInertial = sum(b%secForceInertial, 2)
-          b%lift = sum(b%secLift, 2)
-          b%drag = 0._dp
-          b%liftUnsteady = sum(b%secLiftUnsteady, 2)
-        end associate
-      enddo
-      if (r%axisymmetrySwitch == 1) then
-        do ib = 2, r%nb
-          r%blade(ib)%wiP%gamPrev = r%blade(1)%wiP%gamPrev
-        enddo
-        r%forceInertial = r%nb*r%blade(1)%forceInertial
-        r%lift = r%nb*r%blade(1)%lift
-        r%drag = 0._dp
-        r%liftUnsteady = r%nb*r%blade(1)%liftUnsteady
-      else
-        r%forceInertial = 0._dp; r%lift = 0._dp; r%drag = 0._dp; r%liftUnsteady = 0._dp
-        do ib = 1, r%nbConvect
-          r%forceInertial = r%forceInertial + r%blade(ib)%forceInertial
-          r%lift = r%lift + r%blade(ib)%lift
-          r%liftUnsteady = r%liftUnsteady + r%blade(ib)%liftUnsteady
-        enddo
-      endif
+      r%forceInertial = r%lout(k + 1:k + 3)
+      r%lift = r%lout(k + 4:k + 6)
+      r%liftUnsteady = r%lout(k + 7:k + 9)
+      r%drag = 0._dp
     end associate
   end subroutine compute_forces
 
@@ -2704,11 +2493,11 @@ contains
     endif
     if (sw%rotorForcePlot /= 0) then
       do ir = 1, nr
-        call forces_launch(ctx, rotor, nr, ir)
+        call forces_launch(ctx, rotor, nr, ir, density, dt)
       enddo
       call vlm_check(ctx, vlm_wait(ctx), 'wait')
       do ir = 1, nr
-        call compute_forces(ctx, rotor, nr, ir, density, dt)
+        call compute_forces(rotor, ir)
         if (ir <= ct_cap .and. hist_len >= 1) ct_hist(ir, 1) = rotor_CT(rotor(ir))
         if (write_out) call force_files(uforce(ir), udim(ir), dir, rotor(ir), 0)
       enddo
@@ -2806,7 +2595,7 @@ contains
       if (sw%rotorForcePlot /= 0) loads = mod(iter, sw%rotorForcePlot) == 0
       if (loads) then
         do ir = 1, nr
-          call forces_launch(ctx, rotor, nr, ir)
+          call forces_launch(ctx, rotor, nr, ir, density, dt)
         enddo
       endif
       ! velocity probes (probes2file, libPostprocess.f90:333-361): the probe's
@@ -2887,7 +2676,7 @@ contains
       endif
       if (loads) then
         do ir = 1, nr
-          call compute_forces(ctx, rotor, nr, ir, density, dt)
+          call compute_forces(rotor, ir)
           if (ir <= ct_cap .and. iter + 1 <= hist_len) ct_hist(ir, iter + 1) = rotor_CT(rotor(ir))
           if (write_out) call force_files(uforce(ir), udim(ir), dir, rotor(ir), iter)
         enddo

@@ -80,6 +80,10 @@ _SIGS = {
     "vlm_induced_velocity_terms": [_P, _I, _P, _P, _P, _L, _P, _P, _I],
     "vlm_induced_velocity_terms_deferred": [_P, _I, _P, _P, _P, _L, _P, _P, _I],
     "vlm_rotor_solve_deferred": [_P, _I, _P, _P, _I, _D, _P],
+    "vlm_rotor_set_panels": [_P, _I, _P, _P, _P],
+    "vlm_rotor_rhs": [_P, _I, _I, _P, _P, _P, _P, _P],
+    "vlm_rotor_mirror_velcp": [_P, _I],
+    "vlm_rotor_loads": [_P, _I, _I, _P, _P, _P, _D, _D, _D, _I, _P],
     "vlm_wait": [_P],
     "vlm_wake_velocity": [_P, _I],
     "vlm_convect": [_P, _I, _I, _D],
@@ -389,6 +393,42 @@ class Context:
                                                     tol, _ptr(it)))
         self._pending.extend([g, it])
         return g, it
+
+    def rotor_set_panels(self, ir, panel, section, blade):
+        """Panel (15), section (16) and blade (9) records after a rigid motion."""
+        pn, sc, bl = _f64(panel, (-1, 15)), _f64(section, (-1, 16)), _f64(blade, (-1, 9))
+        self._chk(self.lib.vlm_rotor_set_panels(self.h, ir, _ptr(pn), _ptr(sc), _ptr(bl)))
+
+    @staticmethod
+    def _terms(terms):
+        ir = np.array([t[0] for t in terms], dtype=np.int32)
+        st = np.array([t[1] for t in terms], dtype=np.int32)
+        fl = np.array([TERM_NEGATE if t[2] else 0 for t in terms], dtype=np.int32)
+        return ir, st, fl
+
+    def rotor_rhs(self, ir, terms, kin, n):
+        """Right-hand side of rotor ir (n panels) on the device; returns its
+        host copy (waits).  kin: velBody, omegaBody, cgCoords, omegaSlow."""
+        a, b, c = self._terms(terms)
+        k = _f64(kin, (10,))
+        out = np.zeros(n)
+        self._chk(self.lib.vlm_rotor_rhs(self.h, ir, len(terms), _ptr(a), _ptr(b), _ptr(c),
+                                         _ptr(k), _ptr(out)))
+        self.wait()
+        return out
+
+    def rotor_mirror_velcp(self, ir):
+        self._chk(self.lib.vlm_rotor_mirror_velcp(self.h, ir))
+
+    def rotor_loads(self, ir, terms, density, dt, omega_sign, span_lift, nbc, ns):
+        """Loads of rotor ir (vlm_rotor_loads records; waits): returns the
+        (nbc*ns, 19) section records and the 9 rotor totals."""
+        a, b, c = self._terms(terms)
+        out = np.zeros(19 * nbc * ns + 9)
+        self._chk(self.lib.vlm_rotor_loads(self.h, ir, len(terms), _ptr(a), _ptr(b), _ptr(c),
+                                           density, dt, omega_sign, int(span_lift), _ptr(out)))
+        self.wait()
+        return out[:19 * nbc * ns].reshape(nbc * ns, 19), out[19 * nbc * ns:]
 
     def induced_velocity_terms_deferred(self, terms, p, v=None):
         p = _f64(p, (-1, 3))
