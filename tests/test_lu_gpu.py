@@ -47,9 +47,11 @@ def test_lu_factor_large_matches_lapack(n):
     assert err < 1e-10, err
 
 
-@pytest.mark.parametrize("n", [1, 7, 40, 64, 65, 100, 600])
+@pytest.mark.parametrize("n", [1, 7, 40, 64, 65, 90, 96, 97, 100, 600])
 def test_lu_solve_refine_reaches_fp64_residual(n):
-    """n <= 64 runs the one-wave solve, larger n the multi-launch path."""
+    """n <= 64 runs the one-wave solve with a row per lane, 65 <= n <= 96
+    the one-wave solve with two rows per lane (90: a three-bladed rotor of
+    30 panels per blade), larger n the multi-launch path."""
     A = _matrix(n, 11 + n)
     b = np.random.default_rng(3).standard_normal(n)
     ctx = _ctx()
@@ -62,6 +64,22 @@ def test_lu_solve_refine_reaches_fp64_residual(n):
     x0, it0 = ctx.lu_solve_refine(A, LU, piv, b, max_iter=0, tol=1e-15)
     assert it0 == 0
     np.testing.assert_allclose(x0, x, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [33, 64, 65, 90, 96, 97, 130])
+def test_triangular_solve_matches_oracle(n):
+    """Without refinement the solve is the pivoted forward and back
+    substitution of or_lu_solve, row by row in the same order, on every path
+    (one and two rows per lane, multi-launch); the device contracts a*b + c
+    into one fused operation, so the agreement is to rounding, not bitwise."""
+    A = _matrix(n, 23 + n)
+    b = np.random.default_rng(n).standard_normal(n)
+    ctx = _ctx()
+    LU, piv = ctx.lu_factor(A)
+    x0, it0 = ctx.lu_solve_refine(A, LU, piv, b, max_iter=0, tol=1e-15)
+    xo = orc.lu_solve(np.asfortranarray(LU), piv, b)
+    assert it0 == 0
+    np.testing.assert_allclose(x0, xo, rtol=1e-10, atol=1e-12 * np.max(np.abs(xo)))
 
 
 @pytest.mark.parametrize("n", [50, 300])

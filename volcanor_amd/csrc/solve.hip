@@ -162,12 +162,14 @@ __global__ void refine_check_kernel(int n, const double *r, const double *b, dou
     }
 }
 
-// The whole solve with refinement for n <= 64 in one wave: lane l holds
-// row l of x, b and r; LU and A are staged in LDS; values move between
-// lanes by shuffles.  Same operations in the same order as lu_solve_kernel,
-// residual_kernel, refine_check_kernel and axpy_kernel, so the result is
-// bitwise that of the multi-launch path.
-constexpr int SOLVE_WAVE_N = 64;
+// The whole solve with refinement in one wave for n <= 64 R: lane l holds
+// rows l + 64 r (r < R) of x, b and r; LU and A are staged in LDS (both fit
+// the 160 KB of a gfx950 CU up to n = 96); values move between lanes by
+// shuffles.  Each row takes the same operations in the same order as in
+// lu_solve_kernel, residual_kernel, refine_check_kernel and axpy_kernel, so
+// the result is bitwise that of the multi-launch path.
+constexpr int SOLVE_WAVE_N = 64;   // one row per lane
+constexpr int SOLVE_WAVE_N2 = 96;  // two rows per lane
 
 __device__ __forceinline__ double wave_max(double v)
 {
@@ -175,61 +177,108 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-__device__ double wave_lu_solve(int n, const double *sLU, const int *spiv, double v)
+// element i (uniform over the wave) of a vector held as v[r] at row l + 64 r
+template <int R>
+__device__ __forceinline__ double wave_get(const double (&v)[R], int i)
+{
+    const int r = i >> 6;
+    double x = v[0];
+#pragma unroll
+    for (int q = 1; q < R; ++q)
+        if (r == q) x = v[q];
+    return __shfl(x, i & 63);
+}
+
+template <int R>
+__device__ void wave_lu_solve(int n, const double *sLU, const int *spiv, double (&v)[R])
 {
     const int l = threadIdx.x;
     for (int k = 0; k < n; ++k) {
         const int p = spiv[k];
         if (p != k) {
-            const double vk = __shfl(v, k), vp = __shfl(v, p);
-            if (l == k) v = vp;
-            else if (l == p) v = vk;
+            const double vk = wave_get<R>(v, k), vp = wave_get<R>(v, p);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int row = l + 64 * q;
+                if (row == k) v[q] = vp;
+                else if (row == p) v[q] = vk;
+            }
         }
     }
     for (int j = 0; j < n; ++j) {
-        const double xj = __shfl(v, j);
-        if (l > j && l < n) v -= sLU[l + n * j] * xj;
+        const double xj = wave_get<R>(v, j);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int row = l + 64 * q;
+            if (row > j && row < n) v[q] -= sLU[row + n * j] * xj;
+        }
     }
     for (int j = n - 1; j >= 0; --j) {
-        if (l == j) v /= sLU[j + n * j];
-        const double xj = __shfl(v, j);
-        if (l < j) v -= sLU[l + n * j] * xj;
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            if (l + 64 * q == j) v[q] /= sLU[j + n * j];
+        const double xj = wave_get<R>(v, j);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int row = l + 64 * q;
+            if (row < j) v[q] -= sLU[row + n * j] * xj;
+        }
     }
-    return v;
 }
 
+template <int R, int NMAX>
 __global__ __launch_bounds__(64) void solve_wave_kernel(int n, const double *A, const double *LU,
                                                         const int *ipiv, const double *b,
                                                         double *x, int max_iter, double tol,
                                                         int *st)
 {
-    __shared__ double sLU[SOLVE_WAVE_N * SOLVE_WAVE_N], sA[SOLVE_WAVE_N * SOLVE_WAVE_N];
-    __shared__ int spiv[SOLVE_WAVE_N];
+    __shared__ double sLU[NMAX * NMAX], sA[NMAX * NMAX];
+    __shared__ int spiv[NMAX];
     const int l = threadIdx.x;
     for (int k = l; k < n * n; k += 64) {
         sLU[k] = LU[k];
         sA[k] = A[k];
     }
-    if (l < n) spiv[l] = ipiv[l];
+    for (int k = l; k < n; k += 64) spiv[k] = ipiv[k];
     __syncthreads();
-    const double bl = l < n ? b[l] : 0.0;
-    const double bmax = wave_max(fabs(bl));
-    double xv = wave_lu_solve(n, sLU, spiv, bl);
+    double bl[R], xv[R], rl[R];
+    double bm = 0.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int row = l + 64 * q;
+        bl[q] = row < n ? b[row] : 0.0;
+        bm = fmax(bm, fabs(bl[q]));
+        xv[q] = bl[q];
+    }
+    const double bmax = wave_max(bm);
+    wave_lu_solve<R>(n, sLU, spiv, xv);
     int conv = 0, corr = 0;
     for (int it = 0; it < max_iter; ++it) {
-        double rl = 0.0;
-        if (l < n) {
-            rl = bl;
-            for (int j = 0; j < n; ++j) rl -= sA[l + n * j] * __shfl(xv, j);
+#pragma unroll
+        for (int q = 0; q < R; ++q) rl[q] = l + 64 * q < n ? bl[q] : 0.0;
+        for (int j = 0; j < n; ++j) {
+            const double xj = wave_get<R>(xv, j);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int row = l + 64 * q;
+                if (row < n) rl[q] -= sA[row + n * j] * xj;
+            }
         }
-        if (wave_max(fabs(rl)) <= tol * bmax) {
+        double rm = 0.0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) rm = fmax(rm, fabs(rl[q]));
+        if (wave_max(rm) <= tol * bmax) {
             conv = 1;
             break;
         }
         ++corr;
-        xv += wave_lu_solve(n, sLU, spiv, rl);
+        wave_lu_solve<R>(n, sLU, spiv, rl);
+#pragma unroll
+        for (int q = 0; q < R; ++q) xv[q] += rl[q];
     }
-    if (l < n) x[l] = xv;
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+        if (l + 64 * q < n) x[l + 64 * q] = xv[q];
     if (l == 0) {
         st[0] = conv;
         st[1] = corr;
@@ -255,8 +304,12 @@ int solve_refine(vlm_ctx *ctx, int n, const double *A, const double *LU, const i
                                ctx->stream));
     }
     if (n <= SOLVE_WAVE_N) {
-        hipLaunchKernelGGL(solve_wave_kernel, dim3(1), dim3(64), 0, ctx->stream, n, A, LU, ipiv,
-                           b, x, max_iter, tol, st);
+        hipLaunchKernelGGL((solve_wave_kernel<1, SOLVE_WAVE_N>), dim3(1), dim3(64), 0,
+                           ctx->stream, n, A, LU, ipiv, b, x, max_iter, tol, st);
+        max_iter = 0;
+    } else if (n <= SOLVE_WAVE_N2) {
+        hipLaunchKernelGGL((solve_wave_kernel<2, SOLVE_WAVE_N2>), dim3(1), dim3(64), 0,
+                           ctx->stream, n, A, LU, ipiv, b, x, max_iter, tol, st);
         max_iter = 0;
     } else {
         VLM_HIP(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
