@@ -392,12 +392,64 @@ def bench_rotor(args):
     print(json.dumps(out), flush=True)
 
 
+LU_N, LU_SEED = 2176, 20261202
+
+
+def lu_matrix():
+    return np.asfortranarray(np.random.default_rng(LU_SEED).standard_normal((LU_N, LU_N)))
+
+
+def lu_library_baseline():
+    """The same factorisation by the vendor libraries that torch.linalg
+    reaches on this device (hipSOLVER/rocSOLVER and MAGMA): a comparison,
+    not part of the measured path.  torch carries its own HIP runtime, which
+    cannot start in a process where libvlm's has, so this runs in a child
+    process (bench.py --workload lulib)."""
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--workload", "lulib"],
+                           capture_output=True, text=True, timeout=300)
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except (subprocess.SubprocessError, ValueError, IndexError) as ex:
+        return {"error": str(ex)[:200]}
+
+
+def bench_lulib(args, reps=5):
+    """Child of lu_library_baseline: events on torch's stream around reps
+    factorisations per library, after two untimed ones."""
+    import torch
+    A = lu_matrix()
+    n = A.shape[0]
+    At = torch.from_numpy(np.ascontiguousarray(A)).cuda()
+    res = {}
+    prev = torch.backends.cuda.preferred_linalg_library()
+    for name, backend in (("rocsolver", "cusolver"), ("magma", "magma")):
+        try:
+            torch.backends.cuda.preferred_linalg_library(backend)
+            for _ in range(2):
+                torch.linalg.lu_factor(At)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                torch.linalg.lu_factor(At)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            res[name] = {"ms": ms, "tflops": 2 * n**3 / 3 / (ms * 1e-3) / 1e12}
+        except RuntimeError as ex:
+            res[name] = {"error": str(ex)[:200]}
+    torch.backends.cuda.preferred_linalg_library(prev)
+    res["via"] = "torch.linalg.lu_factor on the same matrix (FP64)"
+    print(json.dumps(res), flush=True)
+
+
 def bench_lu(args):
     """Dense LU of an N = 2176 FP64 matrix (the size of a rotor-wing AIC
     rebuilt every step): device time of the factorisation only."""
     from volcanor_amd import vlm
-    n = 2176
-    A = np.asfortranarray(np.random.default_rng(20261202).standard_normal((n, n)))
+    A = lu_matrix()
+    n = A.shape[0]
+    lib = None if args.no_library else lu_library_baseline()
     ctx = vlm.Context(0)
     for _ in range(args.warmup):
         ctx.lu_factor(A)
@@ -414,6 +466,8 @@ def bench_lu(args):
            "config": {"workload": f"LU with partial pivoting, N={n}, normal random matrix"},
            "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": 78.6,
                         "unit": "TFLOP/s", "frac": achieved / 78.6e12, "traffic": None}}
+    if lib is not None:
+        out["library_baseline"] = lib
     if not args.no_cpu:
         from scipy.linalg import lu_factor
         t0 = time.perf_counter()
@@ -454,7 +508,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", choices=["rotor2", "quad", "rotorwing", "microbench", "lu",
+    ap.add_argument("--workload", choices=["rotor2", "quad", "rotorwing", "microbench", "lu", "lulib",
                                            "hbmcopy"], default="quad")
     ap.add_argument("--variants", default="mixed,learned_mixed,learned_fp64",
                     help="comma list of variants run beside fp64 (mixed, learned_mixed, "
@@ -468,6 +522,8 @@ def main():
                     help="judge tolerance of the learned pass: RMS error / RMS speed on the "
                          "judge sample above which the pass is recomputed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-library", action="store_true",
+                    help="lu: skip the rocSOLVER/MAGMA comparison")
     ap.add_argument("--progress", type=int, default=0,
                     help="print a line to stdout every N steps of each run (0: none)")
     ap.add_argument("--no-twin", action="store_true",
@@ -480,6 +536,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); a multi-GPU "
               "sharding of the wake is not built.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "lulib":
+        bench_lulib(args)
+        return
     if args.workload == "lu":
         args.steps = args.steps or 10
         args.warmup = 2 if args.warmup is None else args.warmup

@@ -9,11 +9,14 @@
 // Larger n: right-looking blocked LU.  Panels are 32 columns wide where the
 // panel fits on chip (m <= 2048 rows), 16 columns above that.
 //   1. panel: one workgroup factorises A[k:n, k:k+jb] column by column,
-//      the panel in registers (and LDS rows for tall panels); rows never
-//      move during the panel, each lane tracks the logical position of its
-//      rows, and one barrier per column publishes every wave's pivot
-//      candidate with its row.  The panel's row swaps reach the other
-//      columns through one composed row-move list;
+//      the panel in registers; rows never move during the panel, each lane
+//      tracks the logical position of its rows, and one barrier per column
+//      publishes every wave's pivot candidate with its row.  16-column
+//      panels retire each pivot row from the registers, 32-column panels
+//      keep it and mask it (the faster of the two at each width, measured);
+//      panels too tall for registers run from global memory.  The panel's
+//      row swaps reach the other columns through one composed row-move
+//      list;
 //   2. U12 = L11^-1 A12 (unit lower triangular solve, jb lanes per column,
 //      after the panel's row moves);
 //   3. A22 -= L21 U12 on the matrix cores, K = jb: 64x64 output tiles, four
@@ -119,6 +122,19 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
                             __builtin_amdgcn_readlane(__double2loint(v), lane));
 }
 
+// max over the lanes that DPP control CTRL pairs (unsigned 32-bit)
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_umax(unsigned v)
+{
+    return max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_min(int i)
+{
+    return min(i, __builtin_amdgcn_update_dpp(INT_MAX, i, CTRL, 0xF, 0xF, false));
+}
+
 // max(v, v of the lane that DPP control CTRL selects)
 template <int CTRL>
 __device__ __forceinline__ double dpp_max(double v)
@@ -126,12 +142,6 @@ __device__ __forceinline__ double dpp_max(double v)
     const int lo = __double2loint(v), hi = __double2hiint(v);
     return fmax(v, __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false),
                                     __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false)));
-}
-
-template <int CTRL>
-__device__ __forceinline__ int dpp_min(int i)
-{
-    return min(i, __builtin_amdgcn_update_dpp(i, i, CTRL, 0xF, 0xF, false));
 }
 
 // Maximum over the wave (every lane active), returned to every lane.
@@ -147,6 +157,21 @@ __device__ __forceinline__ double wave_max(double v)
                 fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
 }
 
+// Maximum / minimum over the wave (every lane active), uniform result.
+// Within each row of 16 lanes: xor 1, xor 2, half-row mirror, row mirror;
+// then the four rows through readlane.
+__device__ __forceinline__ unsigned wave_umax(unsigned v)
+{
+    v = dpp_umax<0xB1>(v);    // quad_perm [1,0,3,2]
+    v = dpp_umax<0x4E>(v);    // quad_perm [2,3,0,1]
+    v = dpp_umax<0x141>(v);   // row_half_mirror
+    v = dpp_umax<0x140>(v);   // row_mirror
+    return max(max((unsigned)__builtin_amdgcn_readlane((int)v, 0),
+                   (unsigned)__builtin_amdgcn_readlane((int)v, 16)),
+               max((unsigned)__builtin_amdgcn_readlane((int)v, 32),
+                   (unsigned)__builtin_amdgcn_readlane((int)v, 48)));
+}
+
 __device__ __forceinline__ int wave_min(int i)
 {
     i = dpp_min<0xB1>(i);
@@ -157,31 +182,297 @@ __device__ __forceinline__ int wave_min(int i)
                min(__builtin_amdgcn_readlane(i, 32), __builtin_amdgcn_readlane(i, 48)));
 }
 
-// Panel held on chip: lane t owns the physical panel rows t + PT*q, RPT of
+// A wave's pivot candidate as the other waves read it: the bit pattern of
+// its |value| (lo, hi: for non-negative doubles the unsigned 64-bit order
+// is the numeric order), its logical row times 8 plus the wave, and the
+// owning thread times 8 plus the register slot of the row.
+struct PivRec {
+    unsigned lo, hi;
+    int key, owner;
+};
+
+// Keep the better of this lane's record and the one DPP control CTRL
+// pairs it with: larger |value|, then the lower logical row (IDAMAX).
+template <int CTRL>
+__device__ __forceinline__ void rec_best(PivRec &r)
+{
+    const unsigned lo = __builtin_amdgcn_update_dpp((int)r.lo, (int)r.lo, CTRL, 0xF, 0xF, false);
+    const unsigned hi = __builtin_amdgcn_update_dpp((int)r.hi, (int)r.hi, CTRL, 0xF, 0xF, false);
+    const int key = __builtin_amdgcn_update_dpp(r.key, r.key, CTRL, 0xF, 0xF, false);
+    const int own = __builtin_amdgcn_update_dpp(r.owner, r.owner, CTRL, 0xF, 0xF, false);
+    const unsigned long long a = ((unsigned long long)r.hi << 32) | r.lo;
+    const unsigned long long b = ((unsigned long long)hi << 32) | lo;
+    const bool t = (b > a) | ((b == a) & (key < r.key));
+    r.lo = t ? lo : r.lo;
+    r.hi = t ? hi : r.hi;
+    r.key = t ? key : r.key;
+    r.owner = t ? own : r.owner;
+}
+
+__device__ __forceinline__ bool slot_is(int q, int s)
+{
+    asm("" : "+s"(q));
+    return q == s;
+}
+
+constexpr int NOROW = 0x0FFFFFFF;   // logical position of a retired or empty slot
+
+// Panel held in registers, retiring design (the 16-column panels, up to
+// 2560 rows): lane t owns the physical panel rows t + PT*q,
+// RPT of them (columns [k0, k0+JB)).  Rows never move during the panel:
+// pos[q] is the logical row (after the swaps so far) of each owned row.
+// Per column, each wave finds its candidate (a DPP maximum of the high
+// words of |value|, the low words and the row index only on ties), its
+// owner lane publishes the candidate row and record, and one barrier later
+// every wave takes the best of the PW records by a DPP butterfly.  The
+// owner of the winning row then retires it: the row goes to LDS (Ur, at
+// its final position C) and its registers are zeroed, so every later
+// column treats all slots alike (a zero row scales to l = 0 and is never
+// updated) and only the row at logical C, which takes the pivot's old
+// position, changes its pos.  The slots of the candidate rows alternate
+// by column parity, so one barrier per column suffices.  The column loop
+// is unrolled by template recursion so that every register index is a
+// constant; the arithmetic per element (the reciprocal of the pivot, the
+// scaling, the fused rank-1 updates in column order) is DGETF2's, so both
+// designs give the same factors bit for bit.  Against the masking design
+// below this needs fewer registers (no per-row bookkeeping at 5 rows per
+// lane: no spills), but the owner's copy and zeroing of the row grows
+// with the panel width, and at 32 columns it costs more than it saves.
+template <int JB>
+struct PanelCtx {
+    double *cand;   // [2][PW][JB] candidate rows
+    PivRec *rec;    // [2][PW]
+    double *Ur;     // [JB][JB] the row retired at column C
+    int *Uphys;     // [JB] its physical row in the panel
+    int *ipiv, *info;
+    int tid, lane, w, k0;
+};
+
+template <int JB, int RPT, int C>
+__device__ __forceinline__ void panel_column(double (&a)[RPT][JB], int (&pos)[RPT],
+                                             PanelCtx<JB> &P)
+{
+    constexpr int par = C & 1;
+    // this lane's candidate: largest |a| over its rows, the lowest logical
+    // row on ties (retired and empty slots hold 0 at logical NOROW)
+    double bv = -1.0;
+    int bi = NOROW, bq = 0;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const double x = fabs(a[q][C]);
+        const int p = pos[q];
+        const bool t = (x > bv) | ((x == bv) & (p < bi));
+        bv = t ? x : bv;
+        bi = t ? p : bi;
+        bq = t ? q : bq;
+    }
+    // the wave's candidate lane: the high words decide unless tied
+    const unsigned hi = (unsigned)__double2hiint(bv);
+    const unsigned M = wave_umax(hi);
+    unsigned long long hit = __ballot(hi == M);
+    if (__popcll(hit) != 1) {
+        const unsigned lo = hi == M ? (unsigned)__double2loint(bv) : 0u;
+        const unsigned Ml = wave_umax(lo);
+        const bool top = (hi == M) & (lo == Ml);
+        hit = __ballot(top);
+        if (__popcll(hit) != 1) {
+            const int r = wave_min(top ? bi : INT_MAX);
+            hit = __ballot(top & (bi == r));
+        }
+    }
+    const int wl = __ffsll((long long)hit) - 1;
+    const int uq = __builtin_amdgcn_readlane(bq, wl);
+    if (P.lane == wl) {
+        double *crow = P.cand + (par * PW + P.w) * JB;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+            if (slot_is(q, uq)) {
+#pragma unroll
+                for (int j = C; j < JB; ++j) crow[j] = a[q][j];
+            }
+        P.rec[par * PW + P.w] = PivRec{(unsigned)__double2loint(bv), hi, bi * 8 + P.w,
+                                       P.tid * 8 + uq};
+    }
+    __syncthreads();
+    // the block's pivot: butterfly over the PW records in each group of
+    // 8 lanes (half-row mirror, then xor 2 and xor 1)
+    static_assert(PW == 8, "butterfly over 8 waves");
+    PivRec r = P.rec[par * PW + (P.lane & 7)];
+    rec_best<0x141>(r);
+    rec_best<0x4E>(r);
+    rec_best<0xB1>(r);
+    const int gkey = __builtin_amdgcn_readfirstlane(r.key);
+    const int gown = __builtin_amdgcn_readfirstlane(r.owner);
+    const bool zero = (__builtin_amdgcn_readfirstlane((int)r.hi) |
+                       __builtin_amdgcn_readfirstlane((int)r.lo)) == 0;
+    const int gi = gkey >> 3, gw = gkey & 7;
+    const double *prow = P.cand + (par * PW + gw) * JB;
+    if (P.tid == 0) {
+        P.ipiv[P.k0 + C] = P.k0 + gi;
+        // a zero column (every candidate 0: gi = C, no swap) is recorded
+        // and left as it is, as DGETF2 does; the factorisation reports it
+        if (zero && *P.info == 0) *P.info = P.k0 + C + 1;
+    }
+    const double inv = zero ? 0.0 : 1.0 / prow[C];
+    // the row at logical C takes the pivot's logical position
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) pos[q] = pos[q] == C ? gi : pos[q];
+    // the owner retires the pivot row
+    const int otid = gown >> 3, oq = gown & 7;
+    if (P.tid == otid) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+            if (slot_is(q, oq)) {
+#pragma unroll
+                for (int j = 0; j < JB; ++j) P.Ur[C * JB + j] = a[q][j];
+#pragma unroll
+                for (int j = C; j < JB; ++j) a[q][j] = 0.0;
+                pos[q] = NOROW;
+            }
+        P.Uphys[C] = otid + PT * oq;
+    }
+    // scale and rank-1 update, CH pivot-row entries at a time (the
+    // scheduling barrier keeps later entries' loads from piling up in
+    // registers)
+    double l[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        l[q] = a[q][C] * inv;
+        a[q][C] = l[q];
+    }
+    constexpr int CH = RPT >= 3 ? 4 : 16;
+#pragma unroll
+    for (int j0 = C + 1; j0 < JB; j0 += CH) {
+        double u[CH];
+#pragma unroll
+        for (int t = 0; t < CH; ++t) u[t] = j0 + t < JB ? prow[j0 + t < JB ? j0 + t : 0] : 0.0;
+#pragma unroll
+        for (int t = 0; t < CH; ++t) {
+            if (j0 + t < JB) {
+                const int j = j0 + t < JB ? j0 + t : 0;
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) a[q][j] -= l[q] * u[t];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (C + 1 < JB) panel_column<JB, RPT, C + 1>(a, pos, P);
+}
+
+// Exactly JB columns of at most PT*RPT rows (the host gives ragged last
+// panels to the tall kernel).
+template <int JB, int RPT>
+__global__ __launch_bounds__(PT) void lu_panel_fast_kernel(int n, double *A, int *ipiv, int *info,
+                                                           int k0, int *list)
+{
+    __shared__ double cand[2 * PW * JB];
+    __shared__ PivRec rec[2 * PW];
+    __shared__ double Ur[JB * JB];
+    __shared__ int Uphys[JB];
+    __shared__ int moved;
+    const int tid = threadIdx.x;
+    const size_t ld = (size_t)n;
+    const int m = n - k0;
+    if (tid == 0) {
+        list[LK0] = k0;
+        list[LJB] = JB;
+    }
+    if (*info != 0) {
+        for (int k = tid; k < JB; k += PT) ipiv[k0 + k] = k0 + k;
+        if (tid == 0) list[0] = 0;
+        return;
+    }
+    if (tid == 0) moved = 0;
+    double a[RPT][JB];
+    int pos[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = tid + PT * q;
+        pos[q] = r < m ? r : NOROW;
+    }
+    const double *col = A + ld * (size_t)k0 + k0;
+#pragma unroll
+    for (int c = 0; c < JB; ++c) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            // unconditional load from a clamped row, then the select: no
+            // branch, so every load of the panel is in flight at once
+            const int r = tid + PT * q;
+            const double x = col[ld * c + (r < m ? r : m - 1)];
+            a[q][c] = r < m ? x : 0.0;
+        }
+        __builtin_amdgcn_sched_barrier(0);   // one column's addresses live at a time
+    }
+    __syncthreads();   // moved = 0 visible
+    PanelCtx<JB> P{cand, rec, Ur, Uphys, ipiv, info, tid, tid & 63, tid >> 6, k0};
+    panel_column<JB, RPT, 0>(a, pos, P);
+    // rows still in registers go to their logical rows, the retired ones
+    // (from LDS) to rows 0..JB-1 of the panel; every row that changed
+    // place enters the row-move list
+    double *out = A + ld * (size_t)k0 + k0;
+#pragma unroll
+    for (int c = 0; c < JB; ++c) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+            if (pos[q] != NOROW) out[ld * c + pos[q]] = a[q][c];
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = tid + PT * q;
+        if (pos[q] != NOROW && pos[q] != r) {
+            const int e = atomicAdd(&moved, 1);
+            if (e < LPAIRS) {
+                list[1 + 2 * e] = k0 + pos[q];   // destination (logical) row
+                list[2 + 2 * e] = k0 + r;        // source (physical) row
+            }
+        }
+    }
+    __syncthreads();   // Ur, Uphys complete
+    if (tid < JB && Uphys[tid] != tid) {
+        const int e = atomicAdd(&moved, 1);
+        if (e < LPAIRS) {
+            list[1 + 2 * e] = k0 + tid;
+            list[2 + 2 * e] = k0 + Uphys[tid];
+        }
+    }
+    for (int e = tid; e < JB * JB; e += PT) {
+        const int rr = e % JB, c = e / JB;
+        out[ld * c + rr] = Ur[rr * JB + c];
+    }
+    __syncthreads();
+    if (tid == 0) list[0] = moved < LPAIRS ? moved : LPAIRS;
+}
+
+
+// Panel held in registers, masking design (the 32-column panels, up to
+// 1024 rows): lane t owns the physical panel rows t + PT*q, RPT of
 // them in registers and RL more in LDS (columns [k0, k0+jb), jb <= JB).
+// A pivot row stays in its lane and is masked out of later columns by its
+// logical position.
 // Rows never move during the panel: pos[q] is the logical row (after the
 // swaps so far) of each owned row, so a row swap is two index updates.
 // Per column, each wave finds its candidate (wave argmax), the owning lane
 // copies the candidate row into the wave's LDS slot, and one barrier later
-// every lane picks the winning wave and reads the pivot row from its slot.
+// every wave picks the winning record (a DPP butterfly over the waves'
+// records) and reads the pivot row from its slot.
 // The slots alternate between two buffers by column parity, so a fast wave
 // writing the next column's slot never overwrites one still being read.
 // The column loop is unrolled by template recursion so that every register
 // index is a constant.
 template <int JB>
-struct FastCtx {
+struct WideCtx {
     double *cand;   // [2][PW][JB]
-    double *ckv;    // [2][PW] candidate |value|
+    PivRec *rec;    // [2][PW] candidate records
     double *cinv;   // [2][PW] 1 / candidate value
-    int *cki;       // [2][PW] candidate logical row
     double *L;      // LDS rows: L[(q*JB + j)*PT + tid]
     int *ipiv, *info, *list;
     int tid, lane, w, k0, m;
 };
 
 template <int JB, int RPT, int RL, int C>
-__device__ __forceinline__ void fast_column(double (&a)[RPT][JB], int (&pos)[RPT + RL],
-                                            FastCtx<JB> &P)
+__device__ __forceinline__ void wide_column(double (&a)[RPT][JB], int (&pos)[RPT + RL],
+                                            WideCtx<JB> &P)
 {
     constexpr int par = C & 1;
     double *L = P.L + P.tid;
@@ -221,30 +512,26 @@ __device__ __forceinline__ void fast_column(double (&a)[RPT][JB], int (&pos)[RPT
             }
     }
     if (P.lane == 0) {
-        P.ckv[par * PW + P.w] = M;
-        P.cki[par * PW + P.w] = wi;
+        // a wave without candidate rows (M = -1) loses to every other
+        const bool none = M < 0.0;
+        P.rec[par * PW + P.w] = PivRec{none ? 0u : (unsigned)__double2loint(M),
+                                       none ? 0u : (unsigned)__double2hiint(M),
+                                       none ? INT_MAX : wi * 8 + P.w, 0};
     }
     __syncthreads();
     // the block's pivot: largest wave maximum, lowest row among the waves
-    // holding it
-    double kv[PW];
-    int ki[PW];
-#pragma unroll
-    for (int w = 0; w < PW; ++w) {
-        kv[w] = P.ckv[par * PW + w];
-        ki[w] = P.cki[par * PW + w];
-    }
-    double gv = kv[0];
-#pragma unroll
-    for (int w = 1; w < PW; ++w) gv = fmax(gv, kv[w]);
-    int gi = INT_MAX, gw = 0;
-#pragma unroll
-    for (int w = 0; w < PW; ++w) {
-        const bool t = (kv[w] == gv) & (ki[w] < gi);
-        gi = t ? ki[w] : gi;
-        gw = t ? w : gw;
-    }
-    const int lp = gi;   // logical row of the pivot
+    // holding it, by a butterfly over the PW records in each group of 8
+    // lanes (half-row mirror, then xor 2 and xor 1)
+    static_assert(PW == 8, "butterfly over 8 waves");
+    PivRec r = P.rec[par * PW + (P.lane & 7)];
+    rec_best<0x141>(r);
+    rec_best<0x4E>(r);
+    rec_best<0xB1>(r);
+    const double gv = __hiloint2double(__builtin_amdgcn_readfirstlane((int)r.hi),
+                                       __builtin_amdgcn_readfirstlane((int)r.lo));
+    const int gkey = __builtin_amdgcn_readfirstlane(r.key);
+    const int gw = gkey & 7;
+    const int lp = gkey >> 3;   // logical row of the pivot
     if (P.tid == 0) {
         P.ipiv[P.k0 + C] = P.k0 + lp;
         // a zero column (every candidate 0: lp = C, no swap) is recorded
@@ -295,20 +582,19 @@ __device__ __forceinline__ void fast_column(double (&a)[RPT][JB], int (&pos)[RPT
         }
         __builtin_amdgcn_sched_barrier(0);
     }
-    if constexpr (C + 1 < JB) fast_column<JB, RPT, RL, C + 1>(a, pos, P);
+    if constexpr (C + 1 < JB) wide_column<JB, RPT, RL, C + 1>(a, pos, P);
 }
 
 // Exactly JB columns (the host gives ragged last panels to the tall kernel).
 template <int JB, int RPT, int RL>
-__global__ __launch_bounds__(PT) void lu_panel_fast_kernel(int n, double *A, int *ipiv, int *info,
+__global__ __launch_bounds__(PT) void lu_panel_wide_kernel(int n, double *A, int *ipiv, int *info,
                                                            int k0, int *list)
 {
     constexpr int S = RPT + RL;
     constexpr int EMPTY = 1 << 30;   // logical position of a slot past the panel
     __shared__ double cand[2 * PW * JB];
-    __shared__ double ckv[2 * PW];
+    __shared__ PivRec rec[2 * PW];
     __shared__ double cinv[2 * PW];
-    __shared__ int cki[2 * PW];
     __shared__ double Lrows[RL > 0 ? RL * JB * PT : 1];
     __shared__ int moved;
     const int tid = threadIdx.x;
@@ -347,8 +633,8 @@ __global__ __launch_bounds__(PT) void lu_panel_fast_kernel(int n, double *A, int
         __builtin_amdgcn_sched_barrier(0);   // one column's addresses live at a time
     }
     __syncthreads();   // moved = 0 visible
-    FastCtx<JB> P{cand, ckv, cinv, cki, Lrows, ipiv, info, list, tid, tid & 63, tid >> 6, k0, m};
-    fast_column<JB, RPT, RL, 0>(a, pos, P);
+    WideCtx<JB> P{cand, rec, cinv, Lrows, ipiv, info, list, tid, tid & 63, tid >> 6, k0, m};
+    wide_column<JB, RPT, RL, 0>(a, pos, P);
 #pragma unroll
     for (int q = 0; q < S; ++q) {
         const int r = tid + PT * q;
@@ -674,19 +960,19 @@ int lu_launch(vlm_ctx *ctx, int n, double *A, int *ipiv, int *info_dev)
             hipLaunchKernelGGL(lu_panel_tall_kernel, g1, dim3(LU_THREADS), 0, ctx->stream, n, A,
                                ipiv, info_dev, k, jb, list);
         else if (m > 4 * PT)
-            hipLaunchKernelGGL((lu_panel_fast_kernel<NB, 5, 0>), g1, bp, 0, ctx->stream, n, A,
+            hipLaunchKernelGGL((lu_panel_fast_kernel<NB, 5>), g1, bp, 0, ctx->stream, n, A,
                                ipiv, info_dev, k, list);
         else if (m > 3 * PT)
-            hipLaunchKernelGGL((lu_panel_fast_kernel<NB, 4, 0>), g1, bp, 0, ctx->stream, n, A,
+            hipLaunchKernelGGL((lu_panel_fast_kernel<NB, 4>), g1, bp, 0, ctx->stream, n, A,
                                ipiv, info_dev, k, list);
         else if (m > WIDE_ROWS)
-            hipLaunchKernelGGL((lu_panel_fast_kernel<NB, 3, 0>), g1, bp, 0, ctx->stream, n, A,
+            hipLaunchKernelGGL((lu_panel_fast_kernel<NB, 3>), g1, bp, 0, ctx->stream, n, A,
                                ipiv, info_dev, k, list);
         else if (m > PT)
-            hipLaunchKernelGGL((lu_panel_fast_kernel<NBMAX, 2, 0>), g1, bp, 0, ctx->stream, n,
+            hipLaunchKernelGGL((lu_panel_wide_kernel<NBMAX, 2, 0>), g1, bp, 0, ctx->stream, n,
                                A, ipiv, info_dev, k, list);
         else
-            hipLaunchKernelGGL((lu_panel_fast_kernel<NBMAX, 1, 0>), g1, bp, 0, ctx->stream, n,
+            hipLaunchKernelGGL((lu_panel_wide_kernel<NBMAX, 1, 0>), g1, bp, 0, ctx->stream, n,
                                A, ipiv, info_dev, k, list);
         const int rest = n - k - jb;
         if (rest > 0) {
