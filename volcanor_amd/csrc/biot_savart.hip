@@ -212,19 +212,19 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x)
     s = tt;
 }
 
-// FP32 pair without the near-field handling, for a group of segments that
-// every target of the wave is far from (see the group test in
-// bs_mixed_kernel): no target lies within two segment lengths of a
-// segment, so no pair is near-singular, |r1| and |r2| are bounded away
-// from zero (no unitVec clamp) and every core term q is positive (no
-// zero-denominator guard).
-__device__ __forceinline__ void pair32_far(float phx, float phy, float phz, float plx,
-                                           float ply, float plz, float4 s0, float4 s1,
-                                           float4 s2, float &vx, float &vy, float &vz)
+// FP32 pair for a group of segments that every target of the wave is far
+// from (see the group test in bs_mixed_kernel), without the near-field
+// handling: no target lies within two segment lengths of a segment, so no
+// pair is near-singular, |r1| and |r2| are bounded away from zero (no
+// unitVec clamp) and every core term q is positive (no zero-denominator
+// guard).  Coordinates are relative to the group's centre c: q = p - c
+// (formed once per group), al = (a - c, (rc^2 |r0|^2)^2), g the
+// circulation factor, s2 = (r0, 4 |r0|^2).  The far test keeps |r1| above
+// |p - c| - |a - c|, so r1 = q - al needs no hi/lo split.
+__device__ __forceinline__ void pair32_local(float qx, float qy, float qz, float4 al, float g,
+                                             float4 s2, float &vx, float &vy, float &vz)
 {
-    const float r1x = (phx - s0.x) + (plx - s1.x);
-    const float r1y = (phy - s0.y) + (ply - s1.y);
-    const float r1z = (phz - s0.z) + (plz - s1.z);
+    const float r1x = qx - al.x, r1y = qy - al.y, r1z = qz - al.z;
     const float r0x = s2.x, r0y = s2.y, r0z = s2.z;
     const float r2x = r1x - r0x, r2y = r1y - r0y, r2z = r1z - r0z;
     const float cx = r0y * r1z - r0z * r1y;
@@ -235,10 +235,10 @@ __device__ __forceinline__ void pair32_far(float phx, float phy, float phz, floa
     const float n2 = r2x * r2x + r2y * r2y + r2z * r2z;
     const float d1 = r0x * r1x + r0y * r1y + r0z * r1z;
     const float d2 = __builtin_fmaf(s2.w, -0.25f, d1);
-    const float den = __builtin_fmaf(c2, c2, s1.w * s1.w);
+    const float den = __builtin_fmaf(c2, c2, al.w);
     const float j1 = __builtin_amdgcn_rsqf(n1 * den);
     const float j2 = __builtin_amdgcn_rsqf(n2 * den);
-    const float s = s0.w * (d1 * j1 - d2 * j2);
+    const float s = g * (d1 * j1 - d2 * j2);
     vx += cx * s;
     vy += cy * s;
     vz += cz * s;
@@ -249,7 +249,8 @@ __device__ __forceinline__ void pair32_far(float phx, float phy, float phz, floa
 // the largest |a_i - c| + 2 |r0_i| over the group (plus a rounding
 // margin), or infinity for a short group or one holding a segment with
 // q = rc^2 |r0|^2 below 1e-18.  A wave whose targets all lie outside that
-// radius runs the group with pair32_far; otherwise it runs pair32, which
+// radius runs the group with pair32_local (coordinates relative to the
+// group's centre, staged beside the split ones); otherwise it runs pair32, which
 // flags near-singular pairs in a 32-bit mask, and recomputes the flagged
 // ones in FP64 after the group (records read from the FP64 list in global
 // memory), so the FP32 loops stay free of the FP64 branch.  Staged groups
@@ -263,6 +264,7 @@ __global__ __launch_bounds__(TILE, 7) void bs_mixed_kernel(BsArgs a)
     __shared__ float4 s0v[TILE];
     __shared__ float4 s1v[TILE];
     __shared__ float4 s2v[TILE];
+    __shared__ float4 s3v[TILE];          // a - centre of the group, q^2
     __shared__ float4 far_sh[TILE / 32];  // centre, squared far radius
     __shared__ int item_sh;
     const int tid = threadIdx.x;
@@ -302,14 +304,16 @@ __global__ __launch_bounds__(TILE, 7) void bs_mixed_kernel(BsArgs a)
                     v2 = make_float4(a.s32[8 * S + s], a.s32[9 * S + s], a.s32[10 * S + s],
                                      a.s32[11 * S + s]);
                 }
+                // far radius of this lane's group (lanes 32k .. 32k+31)
+                const float cx = __shfl(v0.x, 0, 32), cy = __shfl(v0.y, 0, 32),
+                            cz = __shfl(v0.z, 0, 32);
                 if (tid < cnt32) {
                     s0v[tid] = v0;
                     s1v[tid] = v1;
                     s2v[tid] = v2;
+                    s3v[tid] = make_float4((v0.x - cx) + v1.x, (v0.y - cy) + v1.y,
+                                           (v0.z - cz) + v1.z, v1.w * v1.w);
                 }
-                // far radius of this lane's group (lanes 32k .. 32k+31)
-                const float cx = __shfl(v0.x, 0, 32), cy = __shfl(v0.y, 0, 32),
-                            cz = __shfl(v0.z, 0, 32);
                 const float dx = v0.x - cx, dy = v0.y - cy, dz = v0.z - cz;
                 float ex = sqrtf(dx * dx + dy * dy + dz * dz) + sqrtf(v2.w);  // + 2 |r0|
                 if (tid >= cnt || !(v1.w > 1e-18f)) ex = __builtin_inff();
@@ -327,10 +331,11 @@ __global__ __launch_bounds__(TILE, 7) void bs_mixed_kernel(BsArgs a)
                 const float qx = phx - f.x, qy = phy - f.y, qz = phz - f.z;
                 const bool far = !live || (qx * qx + qy * qy + qz * qz > f.w);
                 if (__builtin_amdgcn_readfirstlane(__all(far))) {
+                    const float lx = qx + plx, ly = qy + ply, lz = qz + plz;
 #pragma unroll 4
                     for (int j = 0; j < 32; ++j)
-                        pair32_far(phx, phy, phz, plx, ply, plz, s0v[g + j], s1v[g + j],
-                                   s2v[g + j], tx, ty, tz);
+                        pair32_local(lx, ly, lz, s3v[g + j], s0v[g + j].w, s2v[g + j], tx, ty,
+                                     tz);
                     continue;
                 }
                 unsigned mask = 0u;
