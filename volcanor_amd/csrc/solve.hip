@@ -165,9 +165,13 @@ __global__ void refine_check_kernel(int n, const double *r, const double *b, dou
 // The whole solve with refinement in one wave for n <= 64 R: lane l holds
 // rows l + 64 r (r < R) of x, b and r; LU and A are staged in LDS (both fit
 // the 160 KB of a gfx950 CU up to n = 96); values move between lanes by
-// shuffles.  Each row takes the same operations in the same order as in
+// readlane (the index is uniform).  Each row takes the same operations in the same order as in
 // lu_solve_kernel, residual_kernel, refine_check_kernel and axpy_kernel, so
-// the result is bitwise that of the multi-launch path.
+// the result is bitwise that of the multi-launch path.  The workgroup has
+// SOLVE_STAGE threads so that the staging copy keeps many loads in flight
+// (one wave alone, waiting on each load, took 136 us for n = 90); after it
+// every wave but the first leaves.
+constexpr int SOLVE_STAGE = 512;
 constexpr int SOLVE_WAVE_N = 64;   // one row per lane
 constexpr int SOLVE_WAVE_N2 = 96;  // two rows per lane
 
@@ -177,7 +181,8 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-// element i (uniform over the wave) of a vector held as v[r] at row l + 64 r
+// element i (uniform over the wave) of a vector held as v[r] at row l + 64 r,
+// by readlane (no LDS round trip on the substitution's dependency chain)
 template <int R>
 __device__ __forceinline__ double wave_get(const double (&v)[R], int i)
 {
@@ -186,7 +191,9 @@ __device__ __forceinline__ double wave_get(const double (&v)[R], int i)
 #pragma unroll
     for (int q = 1; q < R; ++q)
         if (r == q) x = v[q];
-    return __shfl(x, i & 63);
+    const int lane = __builtin_amdgcn_readfirstlane(i & 63);
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), lane),
+                            __builtin_amdgcn_readlane(__double2loint(x), lane));
 }
 
 template <int R>
@@ -205,15 +212,61 @@ __device__ void wave_lu_solve(int n, const double *sLU, const int *spiv, double 
             }
         }
     }
-    for (int j = 0; j < n; ++j) {
+    // Substitutions in blocks of SU steps whose LDS reads issue first, off
+    // the dependency chain through v (a loop holding readlane is not
+    // unrolled by the compiler); reads are unconditional, from clamped rows.
+    constexpr int SU = 4;
+    int j = 0;
+    for (; j + SU <= n; j += SU) {
+        double L[SU][R];
+#pragma unroll
+        for (int u = 0; u < SU; ++u)
+#pragma unroll
+            for (int q = 0; q < R; ++q) L[u][q] = sLU[min(l + 64 * q, n - 1) + n * (j + u)];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const double xj = wave_get<R>(v, j + u);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int row = l + 64 * q;
+                const double t = v[q] - L[u][q] * xj;
+                v[q] = (row > j + u && row < n) ? t : v[q];
+            }
+        }
+    }
+    for (; j < n; ++j) {
         const double xj = wave_get<R>(v, j);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int row = l + 64 * q;
-            if (row > j && row < n) v[q] -= sLU[row + n * j] * xj;
+            const double t = v[q] - sLU[min(row, n - 1) + n * j] * xj;
+            v[q] = (row > j && row < n) ? t : v[q];
         }
     }
-    for (int j = n - 1; j >= 0; --j) {
+    j = n - 1;
+    for (; j - SU + 1 >= 0; j -= SU) {
+        double U[SU][R], D[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            D[u] = sLU[(j - u) + n * (j - u)];
+#pragma unroll
+            for (int q = 0; q < R; ++q) U[u][q] = sLU[min(l + 64 * q, n - 1) + n * (j - u)];
+        }
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+                if (l + 64 * q == j - u) v[q] /= D[u];
+            const double xj = wave_get<R>(v, j - u);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int row = l + 64 * q;
+                const double t = v[q] - U[u][q] * xj;
+                v[q] = row < j - u ? t : v[q];
+            }
+        }
+    }
+    for (; j >= 0; --j) {
 #pragma unroll
         for (int q = 0; q < R; ++q)
             if (l + 64 * q == j) v[q] /= sLU[j + n * j];
@@ -221,13 +274,15 @@ __device__ void wave_lu_solve(int n, const double *sLU, const int *spiv, double 
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int row = l + 64 * q;
-            if (row < j) v[q] -= sLU[row + n * j] * xj;
+            const double t = v[q] - sLU[min(row, n - 1) + n * j] * xj;
+            v[q] = row < j ? t : v[q];
         }
     }
 }
 
 template <int R, int NMAX>
-__global__ __launch_bounds__(64) void solve_wave_kernel(int n, const double *A, const double *LU,
+__global__ __launch_bounds__(SOLVE_STAGE) void solve_wave_kernel(int n, const double *A,
+                                                                 const double *LU,
                                                         const int *ipiv, const double *b,
                                                         double *x, int max_iter, double tol,
                                                         int *st)
@@ -235,12 +290,14 @@ __global__ __launch_bounds__(64) void solve_wave_kernel(int n, const double *A, 
     __shared__ double sLU[NMAX * NMAX], sA[NMAX * NMAX];
     __shared__ int spiv[NMAX];
     const int l = threadIdx.x;
-    for (int k = l; k < n * n; k += 64) {
+#pragma unroll 4
+    for (int k = l; k < n * n; k += SOLVE_STAGE) {
         sLU[k] = LU[k];
         sA[k] = A[k];
     }
-    for (int k = l; k < n; k += 64) spiv[k] = ipiv[k];
+    for (int k = l; k < n; k += SOLVE_STAGE) spiv[k] = ipiv[k];
     __syncthreads();
+    if (l >= 64) return;
     double bl[R], xv[R], rl[R];
     double bm = 0.0;
 #pragma unroll
@@ -256,12 +313,29 @@ __global__ __launch_bounds__(64) void solve_wave_kernel(int n, const double *A, 
     for (int it = 0; it < max_iter; ++it) {
 #pragma unroll
         for (int q = 0; q < R; ++q) rl[q] = l + 64 * q < n ? bl[q] : 0.0;
-        for (int j = 0; j < n; ++j) {
+        int j = 0;
+        for (; j + 4 <= n; j += 4) {
+            double a[4][R];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int q = 0; q < R; ++q) a[u][q] = sA[min(l + 64 * q, n - 1) + n * (j + u)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double xj = wave_get<R>(xv, j + u);
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const double t = rl[q] - a[u][q] * xj;
+                    rl[q] = l + 64 * q < n ? t : rl[q];
+                }
+            }
+        }
+        for (; j < n; ++j) {
             const double xj = wave_get<R>(xv, j);
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                const int row = l + 64 * q;
-                if (row < n) rl[q] -= sA[row + n * j] * xj;
+                const double t = rl[q] - sA[min(l + 64 * q, n - 1) + n * j] * xj;
+                rl[q] = l + 64 * q < n ? t : rl[q];
             }
         }
         double rm = 0.0;
@@ -304,11 +378,11 @@ int solve_refine(vlm_ctx *ctx, int n, const double *A, const double *LU, const i
                                ctx->stream));
     }
     if (n <= SOLVE_WAVE_N) {
-        hipLaunchKernelGGL((solve_wave_kernel<1, SOLVE_WAVE_N>), dim3(1), dim3(64), 0,
+        hipLaunchKernelGGL((solve_wave_kernel<1, SOLVE_WAVE_N>), dim3(1), dim3(SOLVE_STAGE), 0,
                            ctx->stream, n, A, LU, ipiv, b, x, max_iter, tol, st);
         max_iter = 0;
     } else if (n <= SOLVE_WAVE_N2) {
-        hipLaunchKernelGGL((solve_wave_kernel<2, SOLVE_WAVE_N2>), dim3(1), dim3(64), 0,
+        hipLaunchKernelGGL((solve_wave_kernel<2, SOLVE_WAVE_N2>), dim3(1), dim3(SOLVE_STAGE), 0,
                            ctx->stream, n, A, LU, ipiv, b, x, max_iter, tol, st);
         max_iter = 0;
     } else {
