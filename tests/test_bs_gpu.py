@@ -119,6 +119,35 @@ def test_mixed_precision_helix_error_bound():
     assert np.max(err) / rms > 1e-10   # FP32 pair arithmetic did run
 
 
+def test_mixed_precision_scattered_group_far_targets():
+    """One group of 32 consecutive segments whose last segment lies 1000
+    away from the other 31 (records jumping between distant wake rows), and
+    2048 targets beside that segment (it runs across the radial direction),
+    outside the group's far radius but within 3.5-6 segment lengths of it.  Evaluated in coordinates
+    centred on the group, r1 would carry an error of eps32 * 2000 against
+    |r1| < 0.6 there; such a group must take the split-coordinate path, and
+    the result stays within the helix bound of the FP64 oracle."""
+    rng = np.random.default_rng(11)
+    a = np.zeros((32, 3))
+    a[:31, 0] = 0.1 * np.arange(31)
+    a[:31] += 1e-3 * rng.normal(size=(31, 3))
+    a[31] = (1000.0, 0.0, 0.0)
+    r0 = np.tile([0.1, 0.0, 0.0], (32, 1)) + 1e-3 * rng.normal(size=(32, 3))
+    r0[31] = (0.0, 0.1, 0.0)   # across the radial direction: targets beside it
+    b = a + r0
+    g = rng.normal(size=32)
+    rc = np.full(32, 0.01)
+    m = 2048
+    p = np.stack([1000.0 + rng.uniform(0.35, 0.6, m), rng.uniform(-0.05, 0.15, m),
+                  rng.uniform(-0.1, 0.1, m)], axis=1)
+    w = orc.filament_sum(a, b, g, rc, p, threads=8)
+    v = _ctx(1).filament_vind(a, b, g, rc, p)
+    rms = np.sqrt(np.mean(np.sum(w**2, axis=1)))
+    err = np.linalg.norm(v - w, axis=1)
+    assert np.max(err) / rms < 2e-5
+    assert np.max(err) / rms > 1e-12   # FP32 pair arithmetic did run
+
+
 def test_mixed_precision_near_core_targets():
     """Targets around isolated segments, radial distance h from 0 to 4 core
     radii, along-segment position from -0.2 to 1.2 of the segment, plus

@@ -219,8 +219,13 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x)
 // unitVec clamp) and every core term q is positive (no zero-denominator
 // guard).  Coordinates are relative to the group's centre c: q = p - c
 // (formed once per group), al = (a - c, (rc^2 |r0|^2)^2), g the
-// circulation factor, s2 = (r0, 4 |r0|^2).  The far test keeps |r1| above
-// |p - c| - |a - c|, so r1 = q - al needs no hi/lo split.
+// circulation factor, s2 = (r0, 4 |r0|^2).  The rounding of q and al is
+// about eps (|p - c| + |a - c|) <= eps (|r1| + 2 R) for a group of radius
+// R; the far test keeps |r1| >= 2 |r0| for every segment, and groups wider
+// than 64 times their longest segment are never far.  A segment's error
+// and its contribution both scale with its |r0|, so no pair's error
+// exceeds about 64 eps of the contribution of the group's longest segment
+// at the same distance, without the hi/lo split.
 __device__ __forceinline__ void pair32_local(float qx, float qy, float qz, float4 al, float g,
                                              float4 s2, float &vx, float &vy, float &vz)
 {
@@ -247,8 +252,8 @@ __device__ __forceinline__ void pair32_local(float qx, float qy, float qz, float
 // MIXED: the staged batch is cut into groups of 32 segments.  At staging,
 // each group gets a far radius around its first segment's start point:
 // the largest |a_i - c| + 2 |r0_i| over the group (plus a rounding
-// margin), or infinity for a short group or one holding a segment with
-// q = rc^2 |r0|^2 below 1e-18.  A wave whose targets all lie outside that
+// margin), or infinity for a short group, one holding a segment with
+// q = rc^2 |r0|^2 below 1e-18, or one wider than 64 of its longest segments.  A wave whose targets all lie outside that
 // radius runs the group with pair32_local (coordinates relative to the
 // group's centre, staged beside the split ones); otherwise it runs pair32, which
 // flags near-singular pairs in a 32-bit mask, and recomputes the flagged
@@ -316,8 +321,17 @@ __global__ __launch_bounds__(TILE, 7) void bs_mixed_kernel(BsArgs a)
                 }
                 const float dx = v0.x - cx, dy = v0.y - cy, dz = v0.z - cz;
                 float ex = sqrtf(dx * dx + dy * dy + dz * dz) + sqrtf(v2.w);  // + 2 |r0|
+                float lmax = sqrtf(v2.w);                                     // 2 |r0|
                 if (tid >= cnt || !(v1.w > 1e-18f)) ex = __builtin_inff();
-                for (int o = 16; o > 0; o >>= 1) ex = fmaxf(ex, __shfl_xor(ex, o, 32));
+                for (int o = 16; o > 0; o >>= 1) {
+                    ex = fmaxf(ex, __shfl_xor(ex, o, 32));
+                    lmax = fmaxf(lmax, __shfl_xor(lmax, o, 32));
+                }
+                // a group wider than 64 of its longest segments (records that
+                // jump between distant wake rows; a chain of 32 connected
+                // segments spans at most 34) takes the near path, whose split
+                // coordinates do not lose accuracy with the group's width
+                if (ex > 32.0f * lmax) ex = __builtin_inff();
                 if ((tid & 31) == 0 && tid < cnt32) {
                     const float cm = fmaxf(fabsf(cx), fmaxf(fabsf(cy), fabsf(cz)));
                     const float rr = ex * 1.0001f + 1e-5f * cm + 1e-30f;
