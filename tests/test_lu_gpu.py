@@ -82,11 +82,15 @@ def test_triangular_solve_matches_oracle(n):
     np.testing.assert_allclose(x0, xo, rtol=1e-10, atol=1e-12 * np.max(np.abs(xo)))
 
 
-@pytest.mark.parametrize("n", [50, 300])
-def test_lu_zero_pivot_is_an_error(n):
+@pytest.mark.parametrize("n,col", [(50, 16), (300, 100), (1300, 260)])
+def test_lu_zero_pivot_is_an_error(n, col):
+    """50: one-workgroup path; 300: the zero column in a 32-column panel held
+    in registers (masking design); 1300: column 260 lies in the 16-column
+    panel at 256, of 1044 rows (retiring design), where every candidate is 0
+    and retired or empty slots must lose the tie to the live rows."""
     from volcanor_amd import vlm
     A = _matrix(n, 2)
-    A[:, n // 3] = 0.0
+    A[:, col] = 0.0
     with pytest.raises(vlm.VLMError):
         _ctx().lu_factor(A)
 
